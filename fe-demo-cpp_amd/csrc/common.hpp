@@ -1,0 +1,141 @@
+// Shared plumbing of libfemhip: HIP error checks, a device buffer owner, the
+// stream-ordered scratch allocator and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace femhip {
+
+// A HIP runtime call failed (the reference's CUDA_CHECK, include/cuda/cuda-check.hpp:17-22):
+// message "file:line: expr failed: Name (String)". Non-sticky errors are cleared.
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct OutOfMemory : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t err, const char* expr, const char* file, int line) {
+  if (err == hipSuccess) return;
+  (void)hipGetLastError();
+  std::string msg = std::string(file) + ":" + std::to_string(line) + ": " + expr + " failed: " +
+                    hipGetErrorName(err) + " (" + hipGetErrorString(err) + ")";
+  if (err == hipErrorOutOfMemory) throw OutOfMemory(msg);
+  throw DeviceError(msg);
+}
+
+#define HIP_CHECK(expr) ::femhip::hip_check((expr), #expr, __FILE__, __LINE__)
+#define HIP_CHECK_LAUNCH() HIP_CHECK(hipGetLastError())
+
+inline void require(bool ok, const std::string& msg) {
+  if (!ok) throw std::invalid_argument(msg);
+}
+
+// Owner of one device allocation (move-only), the counterpart of the reference's
+// DeviceArray (include/cuda/device-array.hpp:18-95) without the synchronous copies.
+template <class T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { allocate(n); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      n_ = o.n_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  void allocate(size_t n) {
+    release();
+    if (n == 0) return;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)));
+    n_ = n;
+  }
+  void upload(const T* host, size_t n) {
+    allocate(n);
+    if (n) HIP_CHECK(hipMemcpy(p_, host, n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void upload(const std::vector<T>& host) { upload(host.data(), host.size()); }
+  std::vector<T> download() const {
+    std::vector<T> h(n_);
+    if (n_) HIP_CHECK(hipMemcpy(h.data(), p_, n_ * sizeof(T), hipMemcpyDeviceToHost));
+    return h;
+  }
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  size_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+  void release() noexcept {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Stream-ordered temporary (hipMallocAsync/hipFreeAsync on the caller's stream),
+// for the E/Q-vector scratch of the unfused operator path.
+template <class T>
+class StreamTmp {
+ public:
+  StreamTmp(size_t n, hipStream_t s) : s_(s), n_(n) {
+    if (n) HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&p_), n * sizeof(T), s));
+  }
+  ~StreamTmp() {
+    if (p_) (void)hipFreeAsync(p_, s_);
+  }
+  StreamTmp(const StreamTmp&) = delete;
+  StreamTmp& operator=(const StreamTmp&) = delete;
+  T* data() { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  hipStream_t s_;
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+inline int ipow(int b, int e) {
+  int r = 1;
+  while (e-- > 0) r *= b;
+  return r;
+}
+
+inline unsigned grid_for(size_t items, int threads) {
+  const size_t g = (items + threads - 1) / threads;
+  return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+// Fill n elements of p with zero on `s` (a kernel, not hipMemsetAsync, so it is
+// safe inside a captured hipGraph).
+void zero_fill(void* p, size_t bytes, hipStream_t s);
+
+// Restriction / basis / qfunction vocabulary shared by the translation units.
+enum : int { kNoTranspose = 0, kTranspose = 1 };
+enum : int { kEvalNone = 0, kEvalInterp = 1, kEvalGrad = 2, kEvalWeight = 3 };
+enum : int { kScatterAtomic = 0, kScatterDeterministic = 1 };
+
+}  // namespace femhip

@@ -1,0 +1,43 @@
+// Device plumbing: the graph-capture-safe fill kernel, device queries and a
+// launch probe for the host-forced error tests.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace femhip {
+
+namespace {
+
+__global__ __launch_bounds__(256) void zero_kernel(uint4* __restrict__ p, size_t n16, unsigned char* tail,
+                                                   size_t ntail) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n16; i += stride)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+  if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+__global__ void probe_kernel(int* sink) {
+  if (sink) sink[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+void zero_fill(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  // Allocations are at least 256-byte aligned; callers pass whole arrays.
+  const uintptr_t addr = reinterpret_cast<uintptr_t>(p);
+  require(addr % 16 == 0, "zero_fill: pointer must be 16-byte aligned");
+  const size_t n16 = bytes / 16, ntail = bytes % 16;
+  const size_t want = (n16 + 255) / 256;
+  const unsigned grid = static_cast<unsigned>(std::min<size_t>(std::max<size_t>(want, 1), 8192));
+  zero_kernel<<<grid, 256, 0, s>>>(static_cast<uint4*>(p), n16, static_cast<unsigned char*>(p) + n16 * 16, ntail);
+  HIP_CHECK_LAUNCH();
+}
+
+int probe_launch(int blocks, int threads) {
+  hipLaunchKernelGGL(probe_kernel, dim3(blocks), dim3(threads), 0, 0, nullptr);
+  const hipError_t e = hipGetLastError();
+  return static_cast<int>(e);
+}
+
+}  // namespace femhip

@@ -1,0 +1,13 @@
+// Compiled fused-mass shapes with P = 3 (registered at library load).
+#include "fused_mass_kernel.hpp"
+
+namespace femhip {
+namespace {
+
+const FusedShapeReg<3, 4, 1, 16, 16> reg_3_4_1_16;
+const FusedShapeReg<3, 4, 1, 4, 4> reg_3_4_1_4;
+const FusedShapeReg<3, 4, 2, 16, 16> reg_3_4_2_16;
+const FusedShapeReg<3, 4, 3, 16, 16> reg_3_4_3_16;
+
+}  // namespace
+}  // namespace femhip

@@ -1,0 +1,13 @@
+// Compiled fused-mass shapes with P = 8 (registered at library load).
+#include "fused_mass_kernel.hpp"
+
+namespace femhip {
+namespace {
+
+const FusedShapeReg<8, 10, 1, 3, 3> reg_8_10_1_3;
+const FusedShapeReg<8, 10, 1, 2, 2> reg_8_10_1_2;
+const FusedShapeReg<8, 10, 3, 3, 3> reg_8_10_3_3;
+const FusedShapeReg<8, 10, 3, 2, 2> reg_8_10_3_2;
+
+}  // namespace
+}  // namespace femhip

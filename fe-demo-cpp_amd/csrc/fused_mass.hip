@@ -1,0 +1,210 @@
+// Host side of the fused mass operator: registry of compiled shapes, qdata
+// re-layout, dispatch, and the Jacobi diagonal.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "fused_mass.hpp"
+
+namespace femhip {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// Reference backend-strided qdata[q*E + e] (q = (qz*Q + qy)*Q + qx) -> private
+// [e][qy][qx][qz]: each column thread of the fused kernel reads Q contiguous values.
+template <class T>
+__global__ __launch_bounds__(kThreads) void qdata_private_kernel(int E, int Q, const double* __restrict__ ref,
+                                                                 T* __restrict__ priv) {
+  const size_t idx = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const size_t Q2 = static_cast<size_t>(Q) * Q, Q3 = Q2 * Q;
+  if (idx >= Q3 * E) return;
+  // idx walks the reference layout (coalesced reads).
+  const size_t e = idx % E, q = idx / E;
+  const size_t col = q % Q2, qz = q / Q2;
+  priv[(e * Q2 + col) * Q + qz] = static_cast<T>(ref[idx]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void to_f32_kernel(size_t n, const double* __restrict__ a, T* __restrict__ b) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) b[i] = static_cast<T>(a[i]);
+}
+
+// Element diagonal of the mass matrix, one thread per (element, node):
+// d[e][i] = sum_q qdata[e][q] * B2[qz][iz] * B2[qy][iy] * B2[qx][ix], added into
+// diag[offsets[e][i] + c*comp_stride] for every component.
+__global__ __launch_bounds__(kThreads) void mass_diag_kernel(int E, int P, int Q, int NC, int cs,
+                                                             const int* __restrict__ offsets,
+                                                             const double* __restrict__ qpriv,
+                                                             const double* __restrict__ B2,
+                                                             double* __restrict__ diag) {
+  const int P3 = P * P * P, Q2 = Q * Q;
+  const size_t idx = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= static_cast<size_t>(E) * P3) return;
+  const size_t e = idx / P3;
+  const int i = static_cast<int>(idx % P3);
+  const int ix = i % P, iy = (i / P) % P, iz = i / (P * P);
+  const double* qe = qpriv + e * Q2 * Q;
+  double sum = 0.0;
+  for (int qy = 0; qy < Q; ++qy) {
+    for (int qx = 0; qx < Q; ++qx) {
+      const double bxy = B2[qy * P + iy] * B2[qx * P + ix];
+      const double* col = qe + static_cast<size_t>(qy * Q + qx) * Q;
+      double s = 0.0;
+      for (int qz = 0; qz < Q; ++qz) s += B2[qz * P + iz] * col[qz];
+      sum += bxy * s;
+    }
+  }
+  const size_t o = static_cast<size_t>(offsets[idx]);
+  for (int c = 0; c < NC; ++c) unsafeAtomicAdd(&diag[o + static_cast<size_t>(c) * cs], sum);
+}
+
+FusedLauncher find(const FusedKey& k) {
+  auto& reg = fused_registry();
+  auto it = reg.find(k);
+  return it == reg.end() ? nullptr : it->second;
+}
+
+}  // namespace
+
+std::map<FusedKey, FusedLauncher>& fused_registry() {
+  static std::map<FusedKey, FusedLauncher> reg;
+  return reg;
+}
+
+void register_fused(const FusedKey& k, FusedLauncher f) { fused_registry()[k] = f; }
+
+std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int det, int exact) {
+  std::vector<int> out;
+  for (const auto& kv : fused_registry()) {
+    const FusedKey& k = kv.first;
+    if (k.P == P && k.Q == Q && k.NC == NC && k.fp32 == fp32 && k.det == det && k.exact == exact)
+      out.push_back(k.elems);
+  }
+  return out;
+}
+
+int fused_default_elems(int P, int Q, int NC, int fp32) {
+  // The first registered variant is the shape's default (lowest elems that was
+  // compiled as default); the tuner may pick another.
+  auto v = fused_elem_variants(P, Q, NC, fp32, 0, 0);
+  if (v.empty()) return 0;
+  // Prefer the variant registered as the default packing (largest multiple-of-64 fit).
+  int best = v.front();
+  double best_waste = 1.0;
+  const int tpe = std::max({P * P, P * Q, Q * Q});
+  for (int ne : v) {
+    const int thr = ne * tpe, padded = (thr + 63) / 64 * 64;
+    const double waste = 1.0 - static_cast<double>(thr) / padded;
+    if (waste < best_waste - 1e-9 || (waste < best_waste + 1e-9 && ne < best)) {
+      best_waste = waste;
+      best = ne;
+    }
+  }
+  return best;
+}
+
+bool fused_mass_supported(int P, int Q, int NC) { return !fused_elem_variants(P, Q, NC, 0, 0, 0).empty(); }
+
+std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBasis& b, const double* qdata_ref,
+                                             hipStream_t s) {
+  require(b.dim == 3, "fused mass: dim must be 3");
+  require(fused_mass_supported(b.P, b.Q, b.num_comp), "fused mass: shape not compiled");
+  auto f = std::make_unique<FusedMass>();
+  f->P = b.P;
+  f->Q = b.Q;
+  f->NC = b.num_comp;
+  f->num_elem = r.num_elem;
+  f->comp_stride = r.comp_stride;
+  f->l_size = r.l_size;
+  f->rstr = &r;
+  f->scatter = r.scatter_mode;
+  f->B64 = b.interp;
+  f->B32.assign(b.interp.begin(), b.interp.end());
+  f->Bsq.resize(b.interp.size());
+  for (size_t i = 0; i < b.interp.size(); ++i) f->Bsq[i] = b.interp[i] * b.interp[i];
+  const size_t nq = static_cast<size_t>(b.Q) * b.Q * b.Q * r.num_elem;
+  f->qdata64.allocate(nq);
+  qdata_private_kernel<double><<<grid_for(nq, kThreads), kThreads, 0, s>>>(r.num_elem, b.Q, qdata_ref,
+                                                                          f->qdata64.data());
+  HIP_CHECK_LAUNCH();
+  return f;
+}
+
+static FusedKey pick(const FusedMass& f, int fp32) {
+  FusedKey k{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0, fp32 ? 0 : f.exact, 0};
+  k.elems = f.elems > 0 ? f.elems : fused_default_elems(f.P, f.Q, f.NC, fp32);
+  if (!find(k)) {
+    auto v = fused_elem_variants(f.P, f.Q, f.NC, fp32, k.det, k.exact);
+    require(!v.empty(), "fused mass: variant not compiled");
+    k.elems = v.front();
+  }
+  return k;
+}
+
+void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s) {
+  const FusedKey k = pick(f, 0);
+  FusedArgs a{};
+  a.num_elem = f.num_elem;
+  a.comp_stride = f.comp_stride;
+  a.offsets = f.rstr->offsets.data();
+  a.qdata = f.qdata64.data();
+  a.u = u;
+  a.basis = f.B64.data();
+  a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+  if (k.det) {
+    if (f.Y64.size() != a.y_comp_len * f.NC) f.Y64.allocate(a.y_comp_len * f.NC);
+    a.out = f.Y64.data();
+    find(k)(a, s);
+    csr_scatter<double>(*f.rstr, f.Y64.data(), v, true, s);
+  } else {
+    zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(double), s);
+    a.out = v;
+    find(k)(a, s);
+  }
+}
+
+void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s) {
+  if (f.qdata32.empty()) {
+    f.qdata32.allocate(f.qdata64.size());
+    to_f32_kernel<float><<<grid_for(f.qdata64.size(), kThreads), kThreads, 0, s>>>(f.qdata64.size(), f.qdata64.data(),
+                                                                                  f.qdata32.data());
+    HIP_CHECK_LAUNCH();
+  }
+  const FusedKey k = pick(f, 1);
+  FusedArgs a{};
+  a.num_elem = f.num_elem;
+  a.comp_stride = f.comp_stride;
+  a.offsets = f.rstr->offsets.data();
+  a.qdata = f.qdata32.data();
+  a.u = u;
+  a.basis = f.B32.data();
+  a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+  if (k.det) {
+    if (f.Y32.size() != a.y_comp_len * f.NC) f.Y32.allocate(a.y_comp_len * f.NC);
+    a.out = f.Y32.data();
+    find(k)(a, s);
+    csr_scatter<float>(*f.rstr, f.Y32.data(), v, true, s);
+  } else {
+    zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(float), s);
+    a.out = v;
+    find(k)(a, s);
+  }
+}
+
+void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s) {
+  zero_fill(diag, static_cast<size_t>(f.l_size) * sizeof(double), s);
+  StreamTmp<double> B2(f.Bsq.size(), s);
+  HIP_CHECK(hipMemcpyAsync(B2.data(), f.Bsq.data(), f.Bsq.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  const size_t n = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+  mass_diag_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(f.num_elem, f.P, f.Q, f.NC, f.comp_stride,
+                                                               f.rstr->offsets.data(), f.qdata64.data(), B2.data(),
+                                                               diag);
+  HIP_CHECK_LAUNCH();
+  // The host copy of Bsq must outlive the async copy.
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace femhip

@@ -1,0 +1,76 @@
+// Fused per-element mass operator: gather -> interp (x,y,z) -> qdata * u ->
+// interp^T (x,y,z) -> scatter, one launch per apply (plus the transpose-map
+// pass for the deterministic scatter).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "basis.hpp"
+#include "common.hpp"
+#include "restriction.hpp"
+
+namespace femhip {
+
+// What a fused launch computes, chosen per apply.
+struct FusedKey {
+  int P, Q, NC;
+  int fp32;       // 0 FP64, 1 FP32
+  int det;        // 0: atomic scatter-add into v; 1: element outputs to Y, then the transpose-map pass
+  int exact;      // 1: no FMA, reference association (FP64 only)
+  int elems;      // elements per workgroup
+  bool operator<(const FusedKey& o) const {
+    return std::tie(P, Q, NC, fp32, det, exact, elems) < std::tie(o.P, o.Q, o.NC, o.fp32, o.det, o.exact, o.elems);
+  }
+};
+
+// Launch parameters shared by every instantiation.
+struct FusedArgs {
+  int num_elem;
+  int comp_stride;
+  const int* offsets;  // [E][P^3]
+  const void* qdata;   // private layout [E][Q^2][Q] (qz innermost), T
+  const void* u;       // L-vector, T
+  void* out;           // atomic: L-vector v (pre-zeroed); det: Y [NC][E*P^3]
+  size_t y_comp_len;   // E*P^3
+  const void* basis;   // host pointer to Q*P values of T (interp_1d), copied into the kernel arguments
+};
+
+using FusedLauncher = void (*)(const FusedArgs&, hipStream_t);
+
+// Registry of compiled instantiations (filled by the fused_mass_*.hip units).
+std::map<FusedKey, FusedLauncher>& fused_registry();
+void register_fused(const FusedKey& k, FusedLauncher f);
+// Default workgroup packing for a shape (elements per workgroup).
+int fused_default_elems(int P, int Q, int NC, int fp32);
+// Every compiled elements-per-workgroup variant for a shape (for the tuner).
+std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int det, int exact);
+
+// The device state of a fused mass operator.
+struct FusedMass {
+  int P = 0, Q = 0, NC = 0, num_elem = 0, comp_stride = 0, l_size = 0;
+  const Restriction* rstr = nullptr;  // owned by the operator (node restriction)
+  DevBuf<double> qdata64;             // private layout
+  DevBuf<float> qdata32;              // lazily made from qdata64
+  std::vector<double> B64;            // interp_1d [Q][P]
+  std::vector<float> B32;
+  std::vector<double> Bsq;            // interp_1d squared entry-wise (Jacobi diagonal)
+  DevBuf<double> Y64;                 // deterministic element outputs (lazy)
+  DevBuf<float> Y32;
+  int scatter = kScatterAtomic;
+  int exact = 0;
+  int elems = 0;  // 0 = default / tuned
+};
+
+// qdata_ref is the backend-strided layout qdata[q*E + e] (device).
+std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBasis& b, const double* qdata_ref,
+                                             hipStream_t s);
+bool fused_mass_supported(int P, int Q, int NC);
+void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s);
+void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s);
+// Jacobi diagonal: diag = E^T [ (B o B)^T x3 qdata ] (FP64).
+void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);
+
+}  // namespace femhip

@@ -1,0 +1,18 @@
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace femhip {
+
+struct SplitMix64 {
+  uint64_t state;
+  double uniform01();  // (x >> 11) * 2^-53, in [0, 1)
+};
+
+void rng_uniform(uint64_t seed, size_t n, double lo, double hi, double* out);
+void box_offsets(int nx, int ny, int nz, int P, int node_stride, int* offsets);
+void box_coords(int n, double jitter, uint64_t seed, double* coords);
+
+}  // namespace femhip

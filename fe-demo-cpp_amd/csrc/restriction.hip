@@ -1,0 +1,330 @@
+// Element restriction kernels for gfx950 (wave64).
+//
+// Offset gather/scatter use a 64-element x 32-node tile per 256-thread workgroup:
+// the tile's offsets are read row by row (32 consecutive ints of one element),
+// staged in LDS, and the E-vector side is then accessed with 64 consecutive
+// elements per node row, so both the index reads and the element-innermost
+// E-vector accesses are coalesced. (The reference's one-thread-per-E-node
+// mapping, src/cuda/cuda-elem-restriction.cu:37-67, coalesces only one side.)
+//
+// The deterministic scatter is one thread per used L-node summing its CSR row
+// in ascending t, starting from the current L value, so the result equals the
+// CPU loop ((v + c0) + c1) + ... of src/elem-restriction.cpp:73-81 bit for bit
+// for any starting v (the reference kernel starts from 0 and adds the row sum
+// to v once, src/cuda/cuda-elem-restriction.cu:85-92, which matches only for v = 0).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+
+#include "restriction.hpp"
+
+namespace femhip {
+
+namespace {
+
+constexpr int kTileE = 64;
+constexpr int kTileI = 32;
+constexpr int kThreads = 256;
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void offset_gather_kernel(int ne, int es, int nc, int cs,
+                                                                 const int* __restrict__ off,
+                                                                 const T* __restrict__ L, T* __restrict__ E) {
+  __shared__ int s_off[kTileE][kTileI + 1];
+  const int e0 = blockIdx.x * kTileE, i0 = blockIdx.y * kTileI;
+  for (int idx = threadIdx.x; idx < kTileE * kTileI; idx += kThreads) {
+    const int il = idx % kTileI, el = idx / kTileI;
+    const int e = e0 + el, i = i0 + il;
+    s_off[el][il] = (e < ne && i < es) ? off[static_cast<size_t>(e) * es + i] : 0;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < kTileE * kTileI; idx += kThreads) {
+    const int el = idx % kTileE, il = idx / kTileE;
+    const int e = e0 + el, i = i0 + il;
+    if (e < ne && i < es) {
+      const size_t o = static_cast<size_t>(s_off[el][il]);
+      for (int k = 0; k < nc; ++k)
+        E[(static_cast<size_t>(k) * es + i) * ne + e] = L[o + static_cast<size_t>(k) * cs];
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void offset_scatter_atomic_kernel(int ne, int es, int nc, int cs,
+                                                                         const int* __restrict__ off,
+                                                                         const T* __restrict__ E, T* __restrict__ L) {
+  __shared__ int s_off[kTileE][kTileI + 1];
+  const int e0 = blockIdx.x * kTileE, i0 = blockIdx.y * kTileI;
+  for (int idx = threadIdx.x; idx < kTileE * kTileI; idx += kThreads) {
+    const int il = idx % kTileI, el = idx / kTileI;
+    const int e = e0 + el, i = i0 + il;
+    s_off[el][il] = (e < ne && i < es) ? off[static_cast<size_t>(e) * es + i] : 0;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < kTileE * kTileI; idx += kThreads) {
+    const int el = idx % kTileE, il = idx / kTileE;
+    const int e = e0 + el, i = i0 + il;
+    if (e < ne && i < es) {
+      const size_t o = static_cast<size_t>(s_off[el][il]);
+      for (int k = 0; k < nc; ++k)
+        unsafeAtomicAdd(&L[o + static_cast<size_t>(k) * cs], E[(static_cast<size_t>(k) * es + i) * ne + e]);
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void offset_scatter_det_kernel(int rows, int ne, int es, int nc, int cs,
+                                                                      const int* __restrict__ lvec,
+                                                                      const int* __restrict__ toff,
+                                                                      const int* __restrict__ tind,
+                                                                      const T* __restrict__ E, T* __restrict__ L) {
+  const int n = blockIdx.x * kThreads + threadIdx.x;
+  if (n >= rows) return;
+  const size_t l = lvec ? static_cast<size_t>(lvec[n]) : static_cast<size_t>(n);
+  const int t0 = toff[n], t1 = toff[n + 1];
+  for (int k = 0; k < nc; ++k) {
+    T acc = L[l + static_cast<size_t>(k) * cs];
+    for (int t = t0; t < t1; ++t) {
+      const int ti = tind[t];
+      const int e = ti / es, i = ti - e * es;
+      acc += E[(static_cast<size_t>(k) * es + i) * ne + e];
+    }
+    L[l + static_cast<size_t>(k) * cs] = acc;
+  }
+}
+
+// Transpose-map sum of a [comp][t] element-output array (the fused operator's
+// deterministic second pass). Rows ascending in L, t ascending within a row.
+template <class T>
+__global__ __launch_bounds__(kThreads) void csr_scatter_kernel(int rows, size_t comp_len, int nc, int cs,
+                                                               const int* __restrict__ lvec,
+                                                               const int* __restrict__ toff,
+                                                               const int* __restrict__ tind, const T* __restrict__ Y,
+                                                               T* __restrict__ L, int zero_start) {
+  const int n = blockIdx.x * kThreads + threadIdx.x;
+  if (n >= rows) return;
+  const size_t l = lvec ? static_cast<size_t>(lvec[n]) : static_cast<size_t>(n);
+  const int t0 = toff[n], t1 = toff[n + 1];
+  for (int k = 0; k < nc; ++k) {
+    const T* Yk = Y + k * comp_len;
+    T acc = zero_start ? T(0) : L[l + static_cast<size_t>(k) * cs];
+    for (int t = t0; t < t1; ++t) acc += Yk[tind[t]];
+    L[l + static_cast<size_t>(k) * cs] = acc;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void strided_gather_kernel(size_t e_size, int ne, int es, long long s0,
+                                                                  long long s1, long long s2,
+                                                                  const T* __restrict__ L, T* __restrict__ E) {
+  const size_t n = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (n >= e_size) return;
+  const size_t e = n % ne, r = n / ne;
+  const size_t i = r % es, k = r / es;
+  E[n] = L[i * s0 + k * s1 + e * s2];
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void strided_scatter_kernel(size_t e_size, int ne, int es, long long s0,
+                                                                   long long s1, long long s2,
+                                                                   const T* __restrict__ E, T* __restrict__ L) {
+  const size_t n = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (n >= e_size) return;
+  const size_t e = n % ne, r = n / ne;
+  const size_t i = r % es, k = r / es;
+  L[i * s0 + k * s1 + e * s2] += E[n];
+}
+
+}  // namespace
+
+bool Restriction::same_layout(const Restriction& o) const {
+  return type == o.type && num_elem == o.num_elem && elem_size == o.elem_size && num_comp == o.num_comp &&
+         comp_stride == o.comp_stride && l_size == o.l_size && strides == o.strides && h_offsets == o.h_offsets;
+}
+
+void build_transpose_map(int num_elem, int elem_size, int l_size, const int* offsets, TransposeMapHost& map) {
+  const size_t nt = static_cast<size_t>(num_elem) * elem_size;
+  // Count per L-node (a counting sort keyed on the offset; stable in t).
+  std::vector<int> count(static_cast<size_t>(l_size) + 1, 0);
+  for (size_t t = 0; t < nt; ++t) ++count[offsets[t]];
+  std::vector<int> row(l_size, -1);
+  map.l_vec_indices.clear();
+  map.t_offsets.assign(1, 0);
+  for (int l = 0; l < l_size; ++l) {
+    if (count[l] == 0) continue;
+    row[l] = static_cast<int>(map.l_vec_indices.size());
+    map.l_vec_indices.push_back(l);
+    map.t_offsets.push_back(map.t_offsets.back() + count[l]);
+  }
+  map.t_indices.assign(nt, 0);
+  std::vector<int> cursor(map.t_offsets.begin(), map.t_offsets.end() - 1);
+  for (size_t t = 0; t < nt; ++t) map.t_indices[cursor[row[offsets[t]]]++] = static_cast<int>(t);
+}
+
+Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp, int comp_stride, int l_size,
+                                       const int* offsets, int scatter_mode) {
+  // Validation of src/elem-restriction.cpp:102-138.
+  require(num_elem > 0, "elem_restriction_create: num_elem must be positive");
+  require(elem_size > 0, "elem_restriction_create: elem_size must be at least 1");
+  require(num_comp > 0, "elem_restriction_create: num_comp must be at least 1");
+  require(!(num_comp > 1 && comp_stride <= 0),
+          "elem_restriction_create: comp_stride must be at least 1 when num_comp > 1");
+  require(l_size > 0, "elem_restriction_create: l_size must be positive");
+  require(offsets != nullptr, "elem_restriction_create: offsets size must be num_elem * elem_size");
+  require(scatter_mode == kScatterAtomic || scatter_mode == kScatterDeterministic,
+          "elem_restriction_create: unknown scatter mode");
+  const size_t nt = static_cast<size_t>(num_elem) * elem_size;
+  require(nt < (1ull << 31), "elem_restriction_create: num_elem * elem_size must fit in int32");
+  const long long top_comp = static_cast<long long>(num_comp - 1) * comp_stride;
+  for (size_t t = 0; t < nt; ++t) {
+    const int o = offsets[t];
+    require(o >= 0 && o + top_comp < l_size,
+            "elem_restriction_create: offsets (including component offsets) must be in the range [0, l_size - 1]");
+  }
+  auto* r = new Restriction();
+  try {
+    r->type = 0;
+    r->num_elem = num_elem;
+    r->elem_size = elem_size;
+    r->num_comp = num_comp;
+    r->comp_stride = comp_stride;
+    r->l_size = l_size;
+    r->e_size = static_cast<size_t>(num_comp) * nt;
+    r->scatter_mode = scatter_mode;
+    r->h_offsets.assign(offsets, offsets + nt);
+    r->offsets.upload(r->h_offsets);
+    if (scatter_mode == kScatterDeterministic) {
+      TransposeMapHost map;
+      build_transpose_map(num_elem, elem_size, l_size, offsets, map);
+      r->num_nodes = static_cast<int>(map.l_vec_indices.size());
+      r->identity_nodes = (r->num_nodes == l_size);
+      if (!r->identity_nodes) r->l_vec_indices.upload(map.l_vec_indices);
+      r->t_offsets.upload(map.t_offsets);
+      r->t_indices.upload(map.t_indices);
+    }
+  } catch (...) {
+    delete r;
+    throw;
+  }
+  return r;
+}
+
+Restriction* restriction_create_strided(int num_elem, int elem_size, int num_comp, int l_size, const int* strides) {
+  require(num_elem > 0, "elem_restriction_create_strided: num_elem must be positive");
+  require(elem_size > 0, "elem_restriction_create_strided: elem_size must be at least 1");
+  require(num_comp > 0, "elem_restriction_create_strided: num_comp must be at least 1");
+  require(l_size > 0, "elem_restriction_create_strided: l_size must be positive");
+  std::array<int, 3> s;
+  const bool backend = (strides == nullptr);
+  if (backend) {
+    require(static_cast<long long>(elem_size) * num_elem < (1ll << 31),
+            "elem_restriction_create_strided: backend strides exceed int32");
+    s = {num_elem, elem_size * num_elem, 1};
+  } else {
+    s = {strides[0], strides[1], strides[2]};
+  }
+  for (int v : s) require(v >= 0, "elem_restriction_create_strided: strides must be non-negative");
+  const long long top = static_cast<long long>(elem_size - 1) * s[0] + static_cast<long long>(num_comp - 1) * s[1] +
+                        static_cast<long long>(num_elem - 1) * s[2];
+  require(top < l_size, "elem_restriction_create_strided: strided indices must be in the range [0, l_size - 1]");
+  auto* r = new Restriction();
+  r->type = 1;
+  r->num_elem = num_elem;
+  r->elem_size = elem_size;
+  r->num_comp = num_comp;
+  r->comp_stride = 0;
+  r->l_size = l_size;
+  r->e_size = static_cast<size_t>(num_comp) * elem_size * num_elem;
+  r->strides = s;
+  r->backend_strides = backend;
+  return r;
+}
+
+Restriction* restriction_clone(const Restriction& src) {
+  auto* r = new Restriction();
+  try {
+    r->type = src.type;
+    r->num_elem = src.num_elem;
+    r->elem_size = src.elem_size;
+    r->num_comp = src.num_comp;
+    r->comp_stride = src.comp_stride;
+    r->l_size = src.l_size;
+    r->e_size = src.e_size;
+    r->strides = src.strides;
+    r->backend_strides = src.backend_strides;
+    r->scatter_mode = src.scatter_mode;
+    r->h_offsets = src.h_offsets;
+    r->num_nodes = src.num_nodes;
+    r->identity_nodes = src.identity_nodes;
+    auto dup = [](const DevBuf<int>& from, DevBuf<int>& to) {
+      to.allocate(from.size());
+      if (from.size()) HIP_CHECK(hipMemcpy(to.data(), from.data(), from.size() * sizeof(int), hipMemcpyDeviceToDevice));
+    };
+    dup(src.offsets, r->offsets);
+    dup(src.l_vec_indices, r->l_vec_indices);
+    dup(src.t_offsets, r->t_offsets);
+    dup(src.t_indices, r->t_indices);
+  } catch (...) {
+    delete r;
+    throw;
+  }
+  return r;
+}
+
+template <class T>
+void restriction_apply(const Restriction& r, int t_mode, const T* in, size_t in_len, T* out, size_t out_len,
+                       hipStream_t s) {
+  require(t_mode == kNoTranspose || t_mode == kTranspose, "elem_restriction_apply: unknown transpose mode");
+  const bool tr = (t_mode == kTranspose);
+  require(in_len == (tr ? r.e_size : static_cast<size_t>(r.l_size)), "elem_restriction_apply: in has the wrong size");
+  if (tr) {
+    require(out_len == static_cast<size_t>(r.l_size),
+            "elem_restriction_apply: Transpose adds into out, which must already have l_size entries");
+  } else {
+    require(out_len == r.e_size, "elem_restriction_apply: out must have e_size entries");
+  }
+  if (r.e_size == 0) return;
+  require(in != nullptr && out != nullptr, "elem_restriction_apply: null device pointer");
+  if (r.type == 1) {
+    const auto& st = r.strides;
+    if (!tr) {
+      strided_gather_kernel<T><<<grid_for(r.e_size, kThreads), kThreads, 0, s>>>(r.e_size, r.num_elem, r.elem_size,
+                                                                                 st[0], st[1], st[2], in, out);
+    } else {
+      strided_scatter_kernel<T><<<grid_for(r.e_size, kThreads), kThreads, 0, s>>>(r.e_size, r.num_elem, r.elem_size,
+                                                                                  st[0], st[1], st[2], in, out);
+    }
+  } else if (!tr) {
+    dim3 grid((r.num_elem + kTileE - 1) / kTileE, (r.elem_size + kTileI - 1) / kTileI);
+    offset_gather_kernel<T><<<grid, kThreads, 0, s>>>(r.num_elem, r.elem_size, r.num_comp, r.comp_stride,
+                                                      r.offsets.data(), in, out);
+  } else if (r.scatter_mode == kScatterAtomic) {
+    dim3 grid((r.num_elem + kTileE - 1) / kTileE, (r.elem_size + kTileI - 1) / kTileI);
+    offset_scatter_atomic_kernel<T><<<grid, kThreads, 0, s>>>(r.num_elem, r.elem_size, r.num_comp, r.comp_stride,
+                                                              r.offsets.data(), in, out);
+  } else {
+    offset_scatter_det_kernel<T><<<grid_for(r.num_nodes, kThreads), kThreads, 0, s>>>(
+        r.num_nodes, r.num_elem, r.elem_size, r.num_comp, r.comp_stride,
+        r.identity_nodes ? nullptr : r.l_vec_indices.data(), r.t_offsets.data(), r.t_indices.data(), in, out);
+  }
+  HIP_CHECK_LAUNCH();
+}
+
+template <class T>
+void csr_scatter(const Restriction& r, const T* Y, T* L, bool zero_start, hipStream_t s) {
+  require(r.type == 0 && r.scatter_mode == kScatterDeterministic, "csr_scatter: needs a deterministic offset restriction");
+  if (zero_start && !r.identity_nodes) zero_fill(L, static_cast<size_t>(r.l_size) * sizeof(T), s);
+  const size_t comp_len = static_cast<size_t>(r.num_elem) * r.elem_size;
+  csr_scatter_kernel<T><<<grid_for(r.num_nodes, kThreads), kThreads, 0, s>>>(
+      r.num_nodes, comp_len, r.num_comp, r.comp_stride, r.identity_nodes ? nullptr : r.l_vec_indices.data(),
+      r.t_offsets.data(), r.t_indices.data(), Y, L, zero_start ? 1 : 0);
+  HIP_CHECK_LAUNCH();
+}
+
+template void restriction_apply<double>(const Restriction&, int, const double*, size_t, double*, size_t, hipStream_t);
+template void restriction_apply<float>(const Restriction&, int, const float*, size_t, float*, size_t, hipStream_t);
+template void csr_scatter<double>(const Restriction&, const double*, double*, bool, hipStream_t);
+template void csr_scatter<float>(const Restriction&, const float*, float*, bool, hipStream_t);
+
+}  // namespace femhip

@@ -1,0 +1,61 @@
+// Device element restriction: offset (gather / atomic scatter-add /
+// deterministic transpose-map scatter-add) and strided, on the reference layouts
+//   E-vector  E[(k*elem_size + i)*num_elem + e]
+//   L-vector  L[offsets[i + e*elem_size] + k*comp_stride]   (offset)
+//             L[i*s0 + k*s1 + e*s2]                         (strided)
+// (src/elem-restriction.cpp:34-83, src/cuda/cuda-elem-restriction.cu:10-14).
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "common.hpp"
+
+namespace femhip {
+
+struct TransposeMapHost {
+  std::vector<int> l_vec_indices;  // used L-nodes (component 0), ascending
+  std::vector<int> t_offsets;      // CSR row starts, rows + 1 entries
+  std::vector<int> t_indices;      // t = i + e*elem_size, ascending within each row
+};
+
+// CSR inverse of the offsets (src/cuda/cuda-elem-restriction.cu:149-177): rows are
+// the used L-nodes in ascending order, each row the t's that map to it, ascending.
+void build_transpose_map(int num_elem, int elem_size, int l_size, const int* offsets, TransposeMapHost& map);
+
+struct Restriction {
+  int type = 0;  // 0 offset, 1 strided
+  int num_elem = 0, elem_size = 0, num_comp = 0, comp_stride = 0, l_size = 0;
+  size_t e_size = 0;
+  std::array<int, 3> strides{};
+  bool backend_strides = false;
+  int scatter_mode = kScatterAtomic;
+
+  std::vector<int> h_offsets;  // host copy, for operators built on this restriction
+  DevBuf<int> offsets;         // [num_elem][elem_size]
+
+  // Deterministic offset restrictions only.
+  int num_nodes = 0;
+  bool identity_nodes = false;  // every L-node 0..l_size-1 used: l_vec_indices not read
+  DevBuf<int> l_vec_indices, t_offsets, t_indices;
+
+  bool same_layout(const Restriction& o) const;
+};
+
+Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp, int comp_stride, int l_size,
+                                       const int* offsets, int scatter_mode);
+Restriction* restriction_create_strided(int num_elem, int elem_size, int num_comp, int l_size, const int* strides);
+// Deep copy (device arrays duplicated), for operators that store restrictions by value.
+Restriction* restriction_clone(const Restriction& r);
+
+template <class T>
+void restriction_apply(const Restriction& r, int t_mode, const T* in, size_t in_len, T* out, size_t out_len,
+                       hipStream_t s);
+
+// Deterministic transpose-map sum of an element-output array Y laid out
+// [comp][num_elem*elem_size] (index t = i + e*elem_size): for every used L-node,
+// acc = (zero_start ? 0 : L[l]); acc += Y[t] over its row ascending; L[l] = acc.
+template <class T>
+void csr_scatter(const Restriction& r, const T* Y, T* L, bool zero_start, hipStream_t s);
+
+}  // namespace femhip

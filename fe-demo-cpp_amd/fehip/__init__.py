@@ -1,0 +1,561 @@
+"""ctypes binding of libfemhip (include/fem_hip.h) for Python callers, tests and bench.py.
+
+Mirrors the reference's interface names (fe-demo-cpp include/cuda/device-array.hpp,
+include/cuda/cuda-elem-restriction.hpp, include/basis.hpp, include/tensor-contract.hpp,
+include/qfunction.hpp, include/operator.hpp) on device arrays. Errors raise
+InvalidArgument (the reference's std::invalid_argument) or FemHipError (its
+std::runtime_error). Loading fails loudly when the HIP library is not built: there is
+no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfemhip.so")
+
+SUCCESS, ERR_INVALID_ARGUMENT, ERR_DEVICE, ERR_RUNTIME, ERR_UNSUPPORTED, ERR_OUT_OF_MEMORY = range(6)
+NO_TRANSPOSE, TRANSPOSE = 0, 1
+EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 3
+QUAD_GAUSS, QUAD_GAUSS_LOBATTO = 0, 1
+SCATTER_ATOMIC, SCATTER_DETERMINISTIC = 0, 1
+VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
+QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR = 1, 2, 3
+SOLVE_FP64, SOLVE_MIXED = 0, 1
+
+
+class FemHipError(RuntimeError):
+    """A failed call (the reference's std::runtime_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class InvalidArgument(ValueError):
+    """Bad arguments (the reference's std::invalid_argument)."""
+
+    code = ERR_INVALID_ARGUMENT
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_int),
+        ("name", ctypes.c_char * 256),
+        ("arch", ctypes.c_char * 64),
+        ("cu_count", ctypes.c_int),
+        ("wavefront_size", ctypes.c_int),
+        ("max_threads_per_block", ctypes.c_int),
+        ("regs_per_block", ctypes.c_int),
+        ("lds_per_block", ctypes.c_size_t),
+        ("lds_per_cu", ctypes.c_size_t),
+        ("l2_bytes", ctypes.c_size_t),
+        ("total_global_mem", ctypes.c_size_t),
+        ("free_global_mem", ctypes.c_size_t),
+        ("clock_khz", ctypes.c_int),
+        ("memory_clock_khz", ctypes.c_int),
+        ("memory_bus_width", ctypes.c_int),
+    ]
+
+
+class SolveOptions(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int),
+        ("max_iter", ctypes.c_int),
+        ("max_outer", ctypes.c_int),
+        ("rtol", ctypes.c_double),
+        ("inner_rtol", ctypes.c_double),
+    ]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("outer", ctypes.c_int),
+        ("rel_residual", ctypes.c_double),
+        ("ms", ctypes.c_double),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_sz = ctypes.c_size_t
+_d = ctypes.c_double
+_ip = ctypes.POINTER(ctypes.c_int)
+_llp = ctypes.POINTER(ctypes.c_longlong)
+
+# name -> argtypes (every function returns int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "fem_hip_last_error": [],
+    "fem_hip_version": [],
+    "fem_hip_device_count": [_ip],
+    "fem_hip_device_info_get": [_i, ctypes.POINTER(DeviceInfo)],
+    "fem_hip_malloc": [ctypes.POINTER(_vp), _sz],
+    "fem_hip_free": [_vp],
+    "fem_hip_memcpy_htod": [_vp, _vp, _sz],
+    "fem_hip_memcpy_dtoh": [_vp, _vp, _sz],
+    "fem_hip_memcpy_dtod": [_vp, _vp, _sz, _vp],
+    "fem_hip_memset_zero": [_vp, _sz, _vp],
+    "fem_hip_device_synchronize": [],
+    "fem_hip_stream_create": [ctypes.POINTER(_vp)],
+    "fem_hip_stream_destroy": [_vp],
+    "fem_hip_stream_synchronize": [_vp],
+    "fem_hip_event_create": [ctypes.POINTER(_vp)],
+    "fem_hip_event_destroy": [_vp],
+    "fem_hip_event_record": [_vp, _vp],
+    "fem_hip_event_synchronize": [_vp],
+    "fem_hip_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    "fem_hip_probe_launch": [_i, _i],
+    "fem_hip_gauss_legendre": [_i, _vp, _vp],
+    "fem_hip_gauss_lobatto": [_i, _vp, _vp],
+    "fem_hip_lagrange_basis_matrix": [_i, _vp, _i, _vp, _vp, _vp],
+    "fem_hip_restriction_create": [_i, _i, _i, _i, _i, _vp, _i, ctypes.POINTER(_vp)],
+    "fem_hip_restriction_create_strided": [_i, _i, _i, _i, _vp, ctypes.POINTER(_vp)],
+    "fem_hip_restriction_apply": [_vp, _i, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_restriction_apply_f32": [_vp, _i, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_restriction_get_sizes": [_vp, _llp],
+    "fem_hip_restriction_get_transpose_map": [_vp, _vp, _vp, _vp],
+    "fem_hip_restriction_destroy": [_vp],
+    "fem_hip_transpose_map_build": [_i, _i, _i, _vp, _ip, _vp, _vp, _vp],
+    "fem_hip_basis_create_h1_lagrange": [_i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
+    "fem_hip_basis_create_tensor": [_i, _i, _i, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
+    "fem_hip_basis_get": [_vp, _ip, _vp, _vp, _vp, _vp],
+    "fem_hip_basis_apply": [_vp, _i, _i, _i, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_basis_destroy": [_vp],
+    "fem_hip_contract": [_i, _i, _i, _i, _vp, _i, _i, _vp, _vp, _vp],
+    "fem_hip_qfunction_create": [_i, ctypes.POINTER(_vp)],
+    "fem_hip_qfunction_add_input": [_vp, ctypes.c_char_p, _i, _i],
+    "fem_hip_qfunction_add_output": [_vp, ctypes.c_char_p, _i, _i],
+    "fem_hip_qfunction_set_context": [_vp, _vp, _sz],
+    "fem_hip_qfunction_apply": [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],
+    "fem_hip_qfunction_destroy": [_vp],
+    "fem_hip_operator_create": [_vp, ctypes.POINTER(_vp)],
+    "fem_hip_operator_set_field": [_vp, ctypes.c_char_p, _vp, _vp, _i, _vp, _sz, _i],
+    "fem_hip_operator_check_ready": [_vp],
+    "fem_hip_operator_apply": [_vp, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_operator_apply_f32": [_vp, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_operator_get_info": [_vp, _llp],
+    "fem_hip_operator_set_fused_options": [_vp, _i, _i, _i],
+    "fem_hip_operator_mass_diagonal": [_vp, _vp, _sz, _vp],
+    "fem_hip_operator_destroy": [_vp],
+    "fem_hip_operator_create_mass": [_vp, _vp, _vp, ctypes.POINTER(_vp)],
+    "fem_hip_rng_uniform": [ctypes.c_uint64, _sz, _d, _d, _vp],
+    "fem_hip_box_offsets": [_i, _i, _i, _i, _i, _vp],
+    "fem_hip_box_coords": [_i, _d, ctypes.c_uint64, _vp],
+    "fem_hip_mass_solve": [_vp, _vp, _vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveStats), _vp],
+}
+_RESTYPES = {"fem_hip_last_error": ctypes.c_char_p, "fem_hip_version": ctypes.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """The loaded libfemhip; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FemHipError(ERR_RUNTIME, "libfemhip.so is not built (" + LIB_PATH + "): run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != SUCCESS:
+        msg = lib().fem_hip_last_error().decode()
+        if rc == ERR_INVALID_ARGUMENT:
+            raise InvalidArgument(msg)
+        raise FemHipError(rc, msg)
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------- device
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    call("fem_hip_device_count", ctypes.byref(n))
+    return n.value
+
+
+def device_info(device=0):
+    info = DeviceInfo()
+    call("fem_hip_device_info_get", device, ctypes.byref(info))
+    return info
+
+
+def synchronize():
+    call("fem_hip_device_synchronize")
+
+
+class DeviceArray:
+    """Owner of one device allocation (include/cuda/device-array.hpp:18-95)."""
+
+    def __init__(self, n=0, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.size = 0
+        self.ptr = ctypes.c_void_p(None)
+        self._alloc(int(n))
+
+    @classmethod
+    def from_host(cls, host, dtype=None):
+        host = np.ascontiguousarray(host, dtype=dtype if dtype is not None else np.asarray(host).dtype)
+        d = cls(host.size, host.dtype)
+        d.copy_from_host(host)
+        return d
+
+    def _alloc(self, n):
+        if n:
+            p = ctypes.c_void_p()
+            call("fem_hip_malloc", ctypes.byref(p), n * self.dtype.itemsize)
+            self.ptr = p
+        self.size = n
+
+    def release(self):
+        if self.ptr.value:
+            lib().fem_hip_free(self.ptr)
+        self.ptr = ctypes.c_void_p(None)
+        self.size = 0
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    @property
+    def nbytes(self):
+        return self.size * self.dtype.itemsize
+
+    def resize(self, n):
+        if n != self.size:
+            self.release()
+            self._alloc(int(n))
+
+    def zero(self, stream=None):
+        if self.size:
+            call("fem_hip_memset_zero", self.ptr, self.nbytes, stream)
+
+    def copy_from_host(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        if host.size != self.size:
+            raise InvalidArgument("DeviceArray::copy_from_host: size mismatch")
+        if self.size:
+            call("fem_hip_memcpy_htod", self.ptr, _ptr(host), self.nbytes)
+
+    def to_host(self):
+        out = np.empty(self.size, dtype=self.dtype)
+        if self.size:
+            call("fem_hip_memcpy_dtoh", _ptr(out), self.ptr, self.nbytes)
+        return out
+
+
+class Stream:
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        call("fem_hip_stream_create", ctypes.byref(self.h))
+
+    def synchronize(self):
+        call("fem_hip_stream_synchronize", self.h)
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                lib().fem_hip_stream_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        call("fem_hip_event_create", ctypes.byref(self.h))
+
+    def record(self, stream=None):
+        call("fem_hip_event_record", self.h, stream.h if isinstance(stream, Stream) else stream)
+
+    def synchronize(self):
+        call("fem_hip_event_synchronize", self.h)
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        call("fem_hip_event_elapsed_ms", self.h, later.h, ctypes.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                lib().fem_hip_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+def _s(stream):
+    return stream.h if isinstance(stream, Stream) else stream
+
+
+# ---------------------------------------------------------------- host setup
+
+
+def gauss_legendre(Q):
+    x, w = np.zeros(Q), np.zeros(Q)
+    call("fem_hip_gauss_legendre", Q, _ptr(x), _ptr(w))
+    return x, w
+
+
+def gauss_lobatto(Q, weights=True):
+    x = np.zeros(Q)
+    w = np.zeros(Q) if weights else None
+    call("fem_hip_gauss_lobatto", Q, _ptr(x), _ptr(w))
+    return (x, w) if weights else x
+
+
+def lagrange_basis_matrix(nodes, pts):
+    nodes = np.ascontiguousarray(nodes, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64)
+    I = np.zeros(len(pts) * len(nodes))
+    G = np.zeros_like(I)
+    call("fem_hip_lagrange_basis_matrix", len(nodes), _ptr(nodes), len(pts), _ptr(pts), _ptr(I), _ptr(G))
+    return I, G
+
+
+def rng_uniform(seed, n, lo=-1.0, hi=1.0):
+    out = np.empty(n)
+    call("fem_hip_rng_uniform", seed, n, lo, hi, _ptr(out))
+    return out
+
+
+def box_offsets(nx, ny, nz, P, node_stride=1):
+    out = np.empty(nx * ny * nz * P ** 3, dtype=np.int32)
+    call("fem_hip_box_offsets", nx, ny, nz, P, node_stride, _ptr(out))
+    return out
+
+
+def box_coords(n, jitter=0.1, seed=0):
+    out = np.empty(3 * (n + 1) ** 3)
+    call("fem_hip_box_coords", n, jitter, seed, _ptr(out))
+    return out
+
+
+def transpose_map(num_elem, elem_size, l_size, offsets):
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    rows = ctypes.c_int()
+    lvec = np.zeros(l_size, np.int32)
+    toff = np.zeros(l_size + 1, np.int32)
+    tind = np.zeros(num_elem * elem_size, np.int32)
+    call("fem_hip_transpose_map_build", num_elem, elem_size, l_size, _ptr(offsets), ctypes.byref(rows),
+         _ptr(lvec), _ptr(toff), _ptr(tind))
+    n = rows.value
+    return lvec[:n].copy(), toff[: n + 1].copy(), tind
+
+
+# ---------------------------------------------------------------- restriction
+
+
+class ElemRestriction:
+    """Device element restriction (fem::cuda::ElemRestriction counterpart)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        s = (ctypes.c_longlong * 8)()
+        call("fem_hip_restriction_get_sizes", self.h, s)
+        (self.type, self.num_elem, self.elem_size, self.num_comp, self.comp_stride, self.l_size, self.e_size,
+         self.num_nodes) = [int(x) for x in s]
+
+    def __del__(self):
+        try:
+            if self.h and self.h.value:
+                lib().fem_hip_restriction_destroy(self.h)
+        except Exception:
+            pass
+
+    def transpose_map(self):
+        lvec = np.zeros(self.num_nodes, np.int32)
+        toff = np.zeros(self.num_nodes + 1, np.int32)
+        tind = np.zeros(self.num_elem * self.elem_size, np.int32)
+        call("fem_hip_restriction_get_transpose_map", self.h, _ptr(lvec), _ptr(toff), _ptr(tind))
+        return lvec, toff, tind
+
+
+def elem_restriction_create(num_elem, elem_size, num_comp, comp_stride, l_size, offsets,
+                            scatter_mode=SCATTER_ATOMIC):
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    if offsets.size != num_elem * elem_size:
+        raise InvalidArgument("elem_restriction_create: offsets size must be num_elem * elem_size")
+    h = ctypes.c_void_p()
+    call("fem_hip_restriction_create", num_elem, elem_size, num_comp, comp_stride, l_size, _ptr(offsets),
+         scatter_mode, ctypes.byref(h))
+    return ElemRestriction(h)
+
+
+def elem_restriction_create_strided(num_elem, elem_size, num_comp, l_size, strides=None):
+    st = None if strides is None else np.ascontiguousarray(strides, np.int32)
+    h = ctypes.c_void_p()
+    call("fem_hip_restriction_create_strided", num_elem, elem_size, num_comp, l_size, _ptr(st), ctypes.byref(h))
+    return ElemRestriction(h)
+
+
+def elem_restriction_apply(r, t_mode, d_in, d_out, stream=None):
+    name = "fem_hip_restriction_apply_f32" if d_in.dtype == np.float32 else "fem_hip_restriction_apply"
+    call(name, r.h, t_mode, d_in.ptr, d_in.size, d_out.ptr, d_out.size, _s(stream))
+
+
+# ---------------------------------------------------------------- basis
+
+
+class TensorBasis:
+    def __init__(self, handle):
+        self.h = handle
+        dims = (ctypes.c_int * 4)()
+        call("fem_hip_basis_get", self.h, dims, None, None, None, None)
+        self.dim, self.num_comp, self.P_1d, self.Q_1d = [int(x) for x in dims]
+        P, Q = self.P_1d, self.Q_1d
+        self.q_ref_1d, self.q_weight_1d = np.zeros(Q), np.zeros(Q)
+        self.interp_1d, self.grad_1d = np.zeros(Q * P), np.zeros(Q * P)
+        call("fem_hip_basis_get", self.h, None, _ptr(self.q_ref_1d), _ptr(self.q_weight_1d), _ptr(self.interp_1d),
+             _ptr(self.grad_1d))
+
+    def __del__(self):
+        try:
+            if self.h and self.h.value:
+                lib().fem_hip_basis_destroy(self.h)
+        except Exception:
+            pass
+
+    @classmethod
+    def create_tensor_H1_lagrange(cls, dim, num_comp, P_1d, Q_1d, quad_mode=QUAD_GAUSS):
+        h = ctypes.c_void_p()
+        call("fem_hip_basis_create_h1_lagrange", dim, num_comp, P_1d, Q_1d, quad_mode, ctypes.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, dim, num_comp, P_1d, Q_1d, q_ref, q_weight, interp, grad):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (q_ref, q_weight, interp, grad)]
+        h = ctypes.c_void_p()
+        call("fem_hip_basis_create_tensor", dim, num_comp, P_1d, Q_1d, *[_ptr(a) for a in arrs], ctypes.byref(h))
+        return cls(h)
+
+
+def tensor_basis_apply(b, num_elem, t_mode, eval_mode, d_u, d_v, stream=None):
+    u_ptr = d_u.ptr if d_u is not None else None
+    u_len = d_u.size if d_u is not None else 0
+    call("fem_hip_basis_apply", b.h, num_elem, t_mode, eval_mode, u_ptr, u_len, d_v.ptr, d_v.size, _s(stream))
+
+
+def tensor_contract_apply(A, B, C, J, d_t, t_mode, add, d_u, d_v, stream=None):
+    call("fem_hip_contract", A, B, C, J, d_t.ptr, t_mode, int(add), d_u.ptr, d_v.ptr, _s(stream))
+
+
+# ---------------------------------------------------------------- qfunction / operator
+
+
+class QFunction:
+    def __init__(self, kind):
+        self.h = ctypes.c_void_p()
+        call("fem_hip_qfunction_create", kind, ctypes.byref(self.h))
+        self.inputs, self.outputs = [], []
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                lib().fem_hip_qfunction_destroy(self.h)
+        except Exception:
+            pass
+
+    def add_input(self, name, size, eval_mode):
+        call("fem_hip_qfunction_add_input", self.h, name.encode(), size, eval_mode)
+        self.inputs.append((name, size, eval_mode))
+
+    def add_output(self, name, size, eval_mode):
+        call("fem_hip_qfunction_add_output", self.h, name.encode(), size, eval_mode)
+        self.outputs.append((name, size, eval_mode))
+
+    def set_context_int(self, value):
+        v = ctypes.c_int(value)
+        call("fem_hip_qfunction_set_context", self.h, ctypes.byref(v), ctypes.sizeof(v))
+
+    def apply(self, Q, d_in, d_out, stream=None):
+        nin, nout = len(d_in), len(d_out)
+        ip = (ctypes.c_void_p * max(nin, 1))(*[a.ptr.value for a in d_in])
+        il = (ctypes.c_size_t * max(nin, 1))(*[a.size for a in d_in])
+        op = (ctypes.c_void_p * max(nout, 1))(*[a.ptr.value for a in d_out])
+        ol = (ctypes.c_size_t * max(nout, 1))(*[a.size for a in d_out])
+        call("fem_hip_qfunction_apply", self.h, Q, nin, ip, il, nout, op, ol, _s(stream))
+
+
+class Operator:
+    def __init__(self, qf=None, handle=None):
+        self.h = ctypes.c_void_p()
+        if handle is not None:
+            self.h = handle
+        else:
+            call("fem_hip_operator_create", qf.h, ctypes.byref(self.h))
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                lib().fem_hip_operator_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_field(self, name, restriction, basis, vector_kind, passive=None):
+        data, n, on_dev = None, 0, 0
+        if isinstance(passive, DeviceArray):
+            data, n, on_dev = passive.ptr, passive.size, 1
+        elif passive is not None:
+            self._keep = np.ascontiguousarray(passive, np.float64)
+            data, n = _ptr(self._keep), self._keep.size
+        call("fem_hip_operator_set_field", self.h, name.encode(), restriction.h if restriction else None,
+             basis.h if basis else None, vector_kind, data, n, on_dev)
+
+    def check_ready(self):
+        call("fem_hip_operator_check_ready", self.h)
+
+    def info(self):
+        v = (ctypes.c_longlong * 8)()
+        call("fem_hip_operator_get_info", self.h, v)
+        keys = ["num_elem", "num_qpts", "active_in_l_size", "active_out_l_size", "fused", "P", "Q", "num_comp"]
+        return dict(zip(keys, [int(x) for x in v]))
+
+    def set_fused_options(self, scatter_mode=-1, exact=0, elems_per_block=0):
+        call("fem_hip_operator_set_fused_options", self.h, scatter_mode, exact, elems_per_block)
+
+    def apply(self, d_in, d_out, stream=None):
+        in_ptr = d_in.ptr if d_in is not None else None
+        in_len = d_in.size if d_in is not None else 0
+        if d_out.dtype == np.float32:
+            call("fem_hip_operator_apply_f32", self.h, in_ptr, in_len, d_out.ptr, d_out.size, _s(stream))
+        else:
+            call("fem_hip_operator_apply", self.h, in_ptr, in_len, d_out.ptr, d_out.size, _s(stream))
+
+    def mass_diagonal(self, d_diag, stream=None):
+        call("fem_hip_operator_mass_diagonal", self.h, d_diag.ptr, d_diag.size, _s(stream))
+
+    def solve(self, d_b, d_x, mode=SOLVE_FP64, rtol=1e-10, max_iter=2000, max_outer=30, inner_rtol=1e-4,
+              stream=None):
+        opt = SolveOptions(mode, max_iter, max_outer, rtol, inner_rtol)
+        st = SolveStats()
+        call("fem_hip_mass_solve", self.h, d_b.ptr, d_x.ptr, ctypes.byref(opt), ctypes.byref(st), _s(stream))
+        return {"iterations": st.iterations, "outer": st.outer, "rel_residual": st.rel_residual, "ms": st.ms}
+
+
+def operator_create_mass(restriction, basis, d_qdata):
+    h = ctypes.c_void_p()
+    call("fem_hip_operator_create_mass", restriction.h, basis.h, d_qdata.ptr, ctypes.byref(h))
+    return Operator(handle=h)
+
+
+def operator_apply(op, d_in, d_out, stream=None):
+    op.apply(d_in, d_out, stream)

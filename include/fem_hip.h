@@ -1,0 +1,246 @@
+/*
+ * fem_hip.h -- C ABI of the MI355X (gfx950) matrix-free mass-operator library
+ * (libfemhip.so). Plain pointers and sizes only; every device pointer is a HIP
+ * device allocation (from fem_hip_malloc or any other hipMalloc), every stream
+ * a hipStream_t (NULL = the default stream).
+ *
+ * Each entry point returns FEM_HIP_SUCCESS (0) or an error code; the message of
+ * the last failure on the calling thread is fem_hip_last_error(). The C++ layer
+ * in fe-demo-cpp_amd/cpp/fem_hip.hpp turns the codes back into the reference's
+ * exceptions (std::invalid_argument / std::runtime_error).
+ *
+ * Reference interfaces replaced (paths relative to the fe-demo-cpp tree):
+ *   include/cuda/cuda-check.hpp:17-32          -> status codes + fem_hip_last_error
+ *   include/cuda/device-array.hpp:18-95        -> fem_hip_malloc/free/memcpy/memset
+ *   include/cuda/device-info.hpp:8-25          -> fem_hip_device_count/_info
+ *   include/cuda/cuda-elem-restriction.hpp:21-79 -> fem_hip_restriction_*
+ *   include/elem-restriction.hpp:43-57         -> fem_hip_restriction_* (same semantics)
+ *   include/basis.hpp:7-40                     -> fem_hip_gauss_*, fem_hip_lagrange_*, fem_hip_basis_create_*
+ *   include/tensor-contract.hpp:9-53           -> fem_hip_contract, fem_hip_basis_apply
+ *   include/qfunction.hpp:20-61, qfunctions/mass.hpp:8-51 -> fem_hip_qfunction_*
+ *   include/operator.hpp:15-64                 -> fem_hip_operator_*
+ */
+#ifndef FEM_HIP_H
+#define FEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define FEM_HIP_SUCCESS 0
+#define FEM_HIP_ERR_INVALID_ARGUMENT 1 /* reference: std::invalid_argument */
+#define FEM_HIP_ERR_DEVICE 2           /* a HIP runtime call failed (reference: CUDA_CHECK runtime_error) */
+#define FEM_HIP_ERR_RUNTIME 3          /* any other std::runtime_error */
+#define FEM_HIP_ERR_UNSUPPORTED 4      /* e.g. a QFunction with no device implementation */
+#define FEM_HIP_ERR_OUT_OF_MEMORY 5    /* device allocation failed, or admission refused */
+
+/* ---- vocabulary (include/transpose-mode.hpp:13, include/eval-mode.hpp:11) --- */
+#define FEM_HIP_NO_TRANSPOSE 0
+#define FEM_HIP_TRANSPOSE 1
+#define FEM_HIP_EVAL_NONE 0
+#define FEM_HIP_EVAL_INTERP 1
+#define FEM_HIP_EVAL_GRAD 2
+#define FEM_HIP_EVAL_WEIGHT 3
+#define FEM_HIP_QUAD_GAUSS 0 /* include/basis.hpp:23 QuadMode */
+#define FEM_HIP_QUAD_GAUSS_LOBATTO 1
+#define FEM_HIP_SCATTER_ATOMIC 0 /* include/cuda/cuda-elem-restriction.hpp:21 ScatterMode */
+#define FEM_HIP_SCATTER_DETERMINISTIC 1
+#define FEM_HIP_VECTOR_ACTIVE 0 /* include/operator.hpp:15 VectorKind */
+#define FEM_HIP_VECTOR_PASSIVE 1
+#define FEM_HIP_VECTOR_NONE 2
+/* QFunctions with a device implementation (include/qfunctions/mass.hpp). */
+#define FEM_HIP_QF_BUILD_MASS 1        /* ctx: int dim */
+#define FEM_HIP_QF_APPLY_MASS 2        /* size-1 fields */
+#define FEM_HIP_QF_APPLY_MASS_VECTOR 3 /* ctx: int num_comp; v[c][p] = qdata[p] u[c][p] */
+/* Arithmetic of the fused mass apply. */
+#define FEM_HIP_FP64 0
+#define FEM_HIP_FP32 1
+
+const char* fem_hip_last_error(void);
+const char* fem_hip_version(void);
+
+/* ---- device and memory (include/cuda/device-info.hpp, device-array.hpp) ---- */
+typedef struct {
+  int id;
+  char name[256];
+  char arch[64];         /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+  int cu_count;          /* compute units */
+  int wavefront_size;    /* 64 on CDNA */
+  int max_threads_per_block;
+  int regs_per_block;
+  size_t lds_per_block;  /* bytes */
+  size_t lds_per_cu;     /* bytes */
+  size_t l2_bytes;
+  size_t total_global_mem;
+  size_t free_global_mem; /* at the time of the query (hipMemGetInfo) */
+  int clock_khz;
+  int memory_clock_khz;
+  int memory_bus_width;
+} fem_hip_device_info;
+
+int fem_hip_device_count(int* count); /* 0 devices is not an error */
+int fem_hip_device_info_get(int device, fem_hip_device_info* info);
+int fem_hip_malloc(void** ptr, size_t bytes); /* bytes == 0 -> *ptr = NULL */
+int fem_hip_free(void* ptr);
+int fem_hip_memcpy_htod(void* dst, const void* src, size_t bytes);
+int fem_hip_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int fem_hip_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int fem_hip_memset_zero(void* dst, size_t bytes, void* stream); /* a fill kernel, graph-capture safe */
+int fem_hip_device_synchronize(void);
+int fem_hip_stream_create(void** stream);
+int fem_hip_stream_destroy(void* stream);
+int fem_hip_stream_synchronize(void* stream);
+int fem_hip_event_create(void** event);
+int fem_hip_event_destroy(void* event);
+int fem_hip_event_record(void* event, void* stream);
+int fem_hip_event_synchronize(void* event);
+int fem_hip_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Error paths forced from the host (tests/cuda/test-device.cu:104-122): launch
+ * a trivial kernel with `threads` threads per block; returns the launch status. */
+int fem_hip_probe_launch(int blocks, int threads);
+
+/* ---- 1D basis setup, host only (src/basis.cpp:16-228) --------------------- */
+int fem_hip_gauss_legendre(int Q, double* q_ref_1d, double* q_weight_1d);
+int fem_hip_gauss_lobatto(int Q, double* q_ref_1d, double* q_weight_1d /* nullable */);
+int fem_hip_lagrange_basis_matrix(int P, const double* nodes, int M, const double* eval_points, double* interp_1d,
+                                  double* grad_1d);
+
+/* ---- element restriction (include/cuda/cuda-elem-restriction.hpp:46-79) ---- */
+typedef struct fem_hip_restriction_s* fem_hip_restriction;
+
+/* Offset restriction; offsets is a host array [num_elem][elem_size], validated
+ * like src/elem-restriction.cpp:102-138 and copied to the device once. With
+ * FEM_HIP_SCATTER_DETERMINISTIC the transpose map is built here too. */
+int fem_hip_restriction_create(int num_elem, int elem_size, int num_comp, int comp_stride, int l_size,
+                               const int* offsets, int scatter_mode, fem_hip_restriction* out);
+/* Strided restriction; strides == NULL means backend strides {num_elem, elem_size*num_elem, 1}
+ * (src/elem-restriction.cpp:155-204). */
+int fem_hip_restriction_create_strided(int num_elem, int elem_size, int num_comp, int l_size, const int* strides,
+                                       fem_hip_restriction* out);
+/* NoTranspose: in = L-vector (in_len == l_size), out = E-vector overwritten (out_len == e_size).
+ * Transpose:   in = E-vector (in_len == e_size), out = L-vector ADDED INTO (out_len == l_size).
+ * Asynchronous on `stream`. */
+int fem_hip_restriction_apply(fem_hip_restriction r, int t_mode, const double* in, size_t in_len, double* out,
+                              size_t out_len, void* stream);
+int fem_hip_restriction_apply_f32(fem_hip_restriction r, int t_mode, const float* in, size_t in_len, float* out,
+                                  size_t out_len, void* stream);
+/* sizes[8] = {type(0 offset,1 strided), num_elem, elem_size, num_comp, comp_stride, l_size, e_size, num_nodes} */
+int fem_hip_restriction_get_sizes(fem_hip_restriction r, long long* sizes);
+/* Copies the device transpose map back (deterministic offset restrictions only):
+ * l_vec_indices[num_nodes], t_offsets[num_nodes+1], t_indices[num_elem*elem_size]. */
+int fem_hip_restriction_get_transpose_map(fem_hip_restriction r, int* l_vec_indices, int* t_offsets, int* t_indices);
+int fem_hip_restriction_destroy(fem_hip_restriction r);
+/* Host-only transpose map (src/cuda/cuda-elem-restriction.cu:149-177). Arrays must
+ * hold l_size, l_size + 1 and num_elem*elem_size ints; *num_nodes receives the row count. */
+int fem_hip_transpose_map_build(int num_elem, int elem_size, int l_size, const int* offsets, int* num_nodes,
+                                int* l_vec_indices, int* t_offsets, int* t_indices);
+
+/* ---- tensor basis (include/basis.hpp:28-40, include/tensor-contract.hpp) ---- */
+typedef struct fem_hip_basis_s* fem_hip_basis;
+
+int fem_hip_basis_create_h1_lagrange(int dim, int num_comp, int P_1d, int Q_1d, int quad_mode, fem_hip_basis* out);
+/* From an existing host TensorBasis: interp_1d/grad_1d are [Q_1d][P_1d] row-major. */
+int fem_hip_basis_create_tensor(int dim, int num_comp, int P_1d, int Q_1d, const double* q_ref_1d,
+                                const double* q_weight_1d, const double* interp_1d, const double* grad_1d,
+                                fem_hip_basis* out);
+/* dims[4] = {dim, num_comp, P_1d, Q_1d}; any of the arrays may be NULL. */
+int fem_hip_basis_get(fem_hip_basis b, int* dims, double* q_ref_1d, double* q_weight_1d, double* interp_1d,
+                      double* grad_1d);
+/* tensor_basis_apply (src/tensor-contract.cpp:207-222) on device E/Q-vectors in the
+ * reference layout [comp][node][elem] / [dim][comp][qpt][elem]. v_len must be the
+ * exact output size. Grad with Transpose overwrites v with the sum of the dim
+ * blocks (the reference accumulates into whatever v held). */
+int fem_hip_basis_apply(fem_hip_basis b, int num_elem, int t_mode, int eval_mode, const double* u, size_t u_len,
+                        double* v, size_t v_len, void* stream);
+int fem_hip_basis_destroy(fem_hip_basis b);
+/* tensor_contract_apply (src/tensor-contract.cpp:35-59) on device arrays; t is a
+ * device [J][B] (NoTranspose) or [B][J] (Transpose) matrix. */
+int fem_hip_contract(int A, int B, int C, int J, const double* t, int t_mode, int add, const double* u, double* v,
+                     void* stream);
+
+/* ---- QFunction (include/qfunction.hpp:20-61) -------------------------------- */
+typedef struct fem_hip_qfunction_s* fem_hip_qfunction;
+
+int fem_hip_qfunction_create(int qf_kind, fem_hip_qfunction* out);
+int fem_hip_qfunction_add_input(fem_hip_qfunction qf, const char* name, int size, int eval_mode);
+int fem_hip_qfunction_add_output(fem_hip_qfunction qf, const char* name, int size, int eval_mode);
+int fem_hip_qfunction_set_context(fem_hip_qfunction qf, const void* data, size_t bytes);
+/* in[f] / out[f] are device arrays of inputs[f].size*Q / outputs[f].size*Q doubles. */
+int fem_hip_qfunction_apply(fem_hip_qfunction qf, int Q, int num_in, const double* const* in, const size_t* in_lens,
+                            int num_out, double* const* out, const size_t* out_lens, void* stream);
+int fem_hip_qfunction_destroy(fem_hip_qfunction qf);
+
+/* ---- operator (include/operator.hpp:34-64) ------------------------------------ */
+typedef struct fem_hip_operator_s* fem_hip_operator;
+
+/* The QFunction, restrictions and bases are copied (by value, as the reference
+ * stores them); the handles may be destroyed afterwards. */
+int fem_hip_operator_create(fem_hip_qfunction qf, fem_hip_operator* out);
+/* restriction / basis may be NULL ("std::nullopt"). For FEM_HIP_VECTOR_PASSIVE,
+ * passive_data holds passive_len doubles (host memory if passive_on_device == 0). */
+int fem_hip_operator_set_field(fem_hip_operator op, const char* name, fem_hip_restriction restriction,
+                               fem_hip_basis basis, int vector_kind, const double* passive_data, size_t passive_len,
+                               int passive_on_device);
+int fem_hip_operator_check_ready(fem_hip_operator op);
+/* out = A(in): out (out_len == active_out_l_size) is zeroed, then summed into.
+ * Recognised mass operators run the fused per-element kernel. */
+int fem_hip_operator_apply(fem_hip_operator op, const double* in, size_t in_len, double* out, size_t out_len,
+                           void* stream);
+/* info[8] = {num_elem, num_qpts, active_in_l_size, active_out_l_size, fused(0 none,1 mass,2 build),
+ *            P_1d, Q_1d, num_comp} */
+int fem_hip_operator_get_info(fem_hip_operator op, long long* info);
+/* Fused mass apply options: scatter (FEM_HIP_SCATTER_*), exact (1 = no fused
+ * multiply-add, element sums in the reference order; 0 = FMA), elems_per_block
+ * (0 = tuned/default). Takes effect on the next apply. */
+int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, int exact, int elems_per_block);
+/* FP32 fused mass apply on FP32 L-vectors (qdata and basis rounded to FP32 once). */
+int fem_hip_operator_apply_f32(fem_hip_operator op, const float* in, size_t in_len, float* out, size_t out_len,
+                               void* stream);
+/* Jacobi diagonal of a fused mass operator (FP64 L-vector of active_out_l_size). */
+int fem_hip_operator_mass_diagonal(fem_hip_operator op, double* diag, size_t len, void* stream);
+int fem_hip_operator_destroy(fem_hip_operator op);
+
+/* Convenience: the reference mass operator (tests/test-operator.cpp:17-24,75-82)
+ * from a node restriction, a basis and device qdata in the backend-strided layout
+ * qdata[q*num_elem + e]. */
+int fem_hip_operator_create_mass(fem_hip_restriction r, fem_hip_basis b, const double* qdata_device,
+                                 fem_hip_operator* out);
+
+/* ---- synthetic structured meshes (host only) ---------------------------------- */
+/* splitmix64 stream: out[i] = lo + (hi-lo) * ((x_i >> 11) * 2^-53). */
+int fem_hip_rng_uniform(uint64_t seed, size_t n, double lo, double hi, double* out);
+/* Lexicographic offsets of an nx*ny*nz box of P^3-node hexes on the (nx(P-1)+1)x.. grid,
+ * offsets (gx + Nx*(gy + Ny*gz)) * node_stride (tests/cuda/test-cuda-elem-restriction.cu:24-42). */
+int fem_hip_box_offsets(int nx, int ny, int nz, int P, int node_stride, int* offsets);
+/* Vertex coordinates of an n^3 unit-cube mesh, blocked [x...,y...,z...] over the (n+1)^3
+ * vertices, interior vertices moved by jitter*h*U[-1,1) per axis, drawn x,y,z per vertex
+ * in lexicographic order from splitmix64(seed). */
+int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords);
+
+/* ---- mass solve ------------------------------------------------------------ */
+#define FEM_HIP_SOLVE_FP64 0  /* Jacobi-PCG, FP64 throughout */
+#define FEM_HIP_SOLVE_MIXED 1 /* FP32 inner Jacobi-PCG, FP64 iterative refinement */
+typedef struct {
+  int mode;          /* FEM_HIP_SOLVE_* */
+  int max_iter;      /* CG iterations (FP64) / per inner solve (mixed) */
+  int max_outer;     /* refinement steps (mixed) */
+  double rtol;       /* target ||b - M x|| / ||b|| (FP64 residual) */
+  double inner_rtol; /* inner FP32 solve reduction per refinement step */
+} fem_hip_solve_options;
+typedef struct {
+  int iterations;    /* total CG iterations (inner ones for mixed) */
+  int outer;         /* refinement steps (mixed), 0 for FP64 */
+  double rel_residual; /* final ||b - M x|| / ||b|| computed in FP64 */
+  double ms;         /* wall time of the solve */
+} fem_hip_solve_stats;
+int fem_hip_mass_solve(fem_hip_operator op, const double* b, double* x, const fem_hip_solve_options* opt,
+                       fem_hip_solve_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEM_HIP_H */

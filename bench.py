@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Mass-operator benchmark on one MI355X.
+
+Workload (the headline configuration): FP64 fused mass apply on a 64^3 hex mesh of
+the unit cube with +-0.1h vertex jitter (seed 0), P=5 GLL nodes, Q=6 Gauss points,
+num_comp=1: 257^3 = 16,974,593 DoF, u ~ U[-1,1] (seed 1). A step is one
+operator_apply (output zeroing + fused gather/interp/qdata/interp^T/atomic
+scatter kernel) with u, v, qdata and offsets resident in HBM.
+
+Printed (one JSON line): GDoF/s of that apply, the fused kernel's HBM roofline
+(algorithmic bytes per launch / its HIP-event duration), and beside it the
+deterministic-scatter and FP32 applies, the mixed-precision vs FP64 mass solves on
+the 16^3, P=3, Q=4 mesh (p50/p99 latency), and the CPU oracle timed on a bounded
+slab of the same workload.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fe-demo-cpp_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_applies(fehip, op, d_u, d_v, steps, warmup, stream):
+    for _ in range(warmup):
+        op.apply(d_u, d_v, stream)
+    stream.synchronize()
+    e0, e1 = fehip.Event(), fehip.Event()
+    op.timing_begin()
+    fehip.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        op.apply(d_u, d_v, stream)
+    e1.record(stream)
+    e1.synchronize()
+    fehip.synchronize()
+    total_ms = e0.elapsed_ms(e1)
+    k_ms, aux_ms, n = op.timing_end()
+    return total_ms / steps, k_ms / max(n, 1), aux_ms / max(n, 1)
+
+
+def traffic_from_profile():
+    """HBM bytes per fused-kernel launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if present."""
+    path = os.path.join(ROOT, "profiles", "fused_mass_p5q6_64cube_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(problem, slab_layers=8, reps=3):
+    """The oracle's operator_apply (CPU restatement of the reference, one core) on a
+    z-slab of the same mesh: the first `slab_layers` element layers of 64^3."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import oracle
+
+    p = problem
+    n, P, Q = p.n, p.P, p.Q
+    E_s = n * n * slab_layers
+    N1 = n * (P - 1) + 1
+    l_s = N1 * N1 * (slab_layers * (P - 1) + 1)
+    off = p.offsets[: E_s * P ** 3]
+    qd = p.qdata().reshape(Q ** 3, p.num_elem)[:, :E_s].copy().ravel()
+    op = oracle().mass(3, E_s, P, Q, 1, l_s, l_s, off, qd)
+    import fehip
+
+    u = fehip.rng_uniform(1, l_s)
+    op.apply(u)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        op.apply(u)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": l_s / t / 1e9, "unit": "GDoF/s", "cores": 1, "kind": "port",
+            "sample": "oracle operator_apply (single-threaded CPU restatement of the reference, -O2 "
+                      "-ffp-contract=off) on a 64x64x%d-element z-slab of the 64^3 P=5 Q=6 mesh (%d DoF), "
+                      "median of %d applies: %.3f s/apply; host %s, nproc %d" %
+                      (slab_layers, l_s, reps, t, platform.processor() or platform.machine(), os.cpu_count())}
+
+
+def solve_latency(fehip, reps):
+    """Config 3: M x = b on the 16^3, P=3, Q=4 mesh, b = M x*, x* ~ U[-1,1] seed 2."""
+    from fehip.problems import MassProblem
+
+    p = MassProblem(16, 3, 4, 1)
+    x_star = fehip.DeviceArray.from_host(fehip.rng_uniform(2, p.l_size))
+    d_b, d_x = fehip.DeviceArray(p.l_size), fehip.DeviceArray(p.l_size)
+    p.op.apply(x_star, d_b)
+    out = {}
+    for name, mode in (("fp64", fehip.SOLVE_FP64), ("mixed", fehip.SOLVE_MIXED)):
+        p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+        lat, st = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+            lat.append((time.perf_counter() - t0) * 1e3)
+        x = d_x.to_host()
+        err = float(np.max(np.abs(x - x_star.to_host())) / np.max(np.abs(x_star.to_host())))
+        out[name] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "iterations": st["iterations"], "outer": st["outer"], "rel_residual": st["rel_residual"],
+                     "err_vs_xstar": err}
+    out["speedup_mixed_vs_fp64_p50"] = out["fp64"]["p50_ms"] / out["mixed"]["p50_ms"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import fehip
+    from fehip.problems import MassProblem
+
+    info = fehip.device_info(0)
+    log("device:", info.name.decode(), info.arch.decode(), "CUs", info.cu_count)
+    stream = fehip.Stream()
+    n, P, Q = args.n, 5, 6
+    t0 = time.perf_counter()
+    p = MassProblem(n, P, Q, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+    fehip.synchronize()
+    log("setup (mesh, build_mass on device): %.2f s" % (time.perf_counter() - t0))
+    L = p.l_size
+    d_u = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L))
+    d_v = fehip.DeviceArray(L)
+
+    # Headline: FP64 fused apply, atomic scatter.
+    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
+    ms, k_ms, aux_ms = time_applies(fehip, p.op, d_u, d_v, args.steps, args.warmup, stream)
+    bytes64 = p.apply_bytes()
+    log("fp64 atomic: %.4f ms/apply (kernel %.4f, zeroing %.4f)" % (ms, k_ms, aux_ms))
+
+    # Deterministic scatter, exact arithmetic (bit-equal to the reference CPU path).
+    extra = {}
+    for label, scatter, exact in (("fp64_deterministic", fehip.SCATTER_DETERMINISTIC, 0),
+                                  ("fp64_deterministic_exact", fehip.SCATTER_DETERMINISTIC, 1),
+                                  ("fp64_atomic_exact", fehip.SCATTER_ATOMIC, 1)):
+        p.op.set_fused_options(scatter, exact=exact)
+        m2, k2, a2 = time_applies(fehip, p.op, d_u, d_v, max(args.steps // 2, 5), 2, stream)
+        extra[label] = {"ms_per_apply": m2, "gdofs": L / m2 / 1e6, "kernel_ms": k2, "aux_ms": a2,
+                        "kernel_frac_hbm": bytes64 / (k2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(label, extra[label])
+    # FP32 apply.
+    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
+    d_u32 = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L).astype(np.float32))
+    d_v32 = fehip.DeviceArray(L, np.float32)
+    m3, k3, a3 = time_applies(fehip, p.op, d_u32, d_v32, args.steps, args.warmup, stream)
+    bytes32 = p.apply_bytes(4, 4, 4)
+    extra["fp32_atomic"] = {"ms_per_apply": m3, "gdofs": L / m3 / 1e6, "kernel_ms": k3, "aux_ms": a3,
+                            "kernel_frac_hbm": bytes32 / (k3 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log("fp32", extra["fp32_atomic"])
+
+    solve = None
+    if not args.no_solve:
+        solve = solve_latency(fehip, 20)
+        log("solve", solve)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p)
+        log("cpu", cpu)
+
+    achieved = bytes64 / (k_ms * 1e-3) / 1e9
+    line = {
+        "metric": "mass-apply GDoF/s & %HBM roofline, FP64 vs mixed; p50/p99 solve latency",
+        "value": L / ms / 1e6,
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (jittered structured hex mesh, u ~ U[-1,1] splitmix64 seed 1)",
+        "config": {"workload": "fused FP64 mass apply, %d^3 hexes, P=5 GLL, Q=6 Gauss, num_comp=1, %d DoF, "
+                               "atomic scatter" % (n, L),
+                   "num_elem": p.num_elem, "P": P, "Q": Q, "num_comp": 1, "l_size": L},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "kernel": "fused_mass_kernel<5,6,1,double> (atomic)", "kernel_ms": k_ms,
+                     "algorithmic_bytes_per_launch": bytes64, "zeroing_ms": aux_ms},
+        "cpu_baseline": cpu,
+        "variants": extra,
+        "solve_c3": solve,
+        "device": {"name": info.name.decode(), "arch": info.arch.decode(), "cu": info.cu_count},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -82,6 +82,7 @@ void contract(int A, int B, int C, int J, const double* t, int t_mode, bool add,
               hipStream_t s) {
   require(A > 0 && B > 0 && C > 0 && J > 0, "tensor_contract_apply: sizes must be positive");
   require(B <= 32, "tensor_contract_apply: B > 32 is not supported on the device");
+  require(t != nullptr && u != nullptr && v != nullptr, "tensor_contract_apply: null device pointer");
   const int sj = (t_mode == kNoTranspose) ? B : 1;
   const int sb = (t_mode == kNoTranspose) ? 1 : J;
   const unsigned grid = grid_for(static_cast<size_t>(A) * C, kThreads);
@@ -139,7 +140,9 @@ void basis_apply(const Basis& bas, int ne, int t_mode, int eval_mode, const doub
       post *= out1;
     }
   }
-  StreamTmp<double> t0(dim > 1 ? tmp_len : 0, s), t1(dim > 2 ? tmp_len : 0, s);
+  // Pass d writes tmp[(d+1)%2] (the last pass writes v) and pass d+1 reads it:
+  // dim 2 uses tmp[1]; dim 3 uses tmp[1] then tmp[0].
+  StreamTmp<double> t0(dim > 2 ? tmp_len : 0, s), t1(dim > 1 ? tmp_len : 0, s);
   double* tmp[2] = {t0.data(), t1.data()};
 
   // One sum-factorized pass sequence with matrix `mat_for(d)` per axis d.

@@ -516,6 +516,18 @@ int fem_hip_operator_mass_diagonal(fem_hip_operator h, double* diag, size_t len,
     fused_mass_diagonal(f, diag, st(s));
   });
 }
+int fem_hip_operator_timing_begin(fem_hip_operator h) {
+  return guard([&] {
+    need(h, "operator");
+    fused_timing_begin(operator_fused_mass(*h->op, nullptr));
+  });
+}
+int fem_hip_operator_timing_end(fem_hip_operator h, double* kernel_ms, double* aux_ms, int* count) {
+  return guard([&] {
+    need(h, "operator");
+    fused_timing_end(operator_fused_mass(*h->op, nullptr), kernel_ms, aux_ms, count);
+  });
+}
 int fem_hip_operator_destroy(fem_hip_operator op) {
   return guard([&] { delete op; });
 }

@@ -133,6 +133,52 @@ std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBas
   return f;
 }
 
+FusedMass::~FusedMass() {
+  for (auto& e : events) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+    (void)hipEventDestroy(e.c);
+    (void)hipEventDestroy(e.d);
+  }
+}
+
+static FusedMass::StageEvents* next_events(FusedMass& f) {
+  if (!f.timing) return nullptr;
+  if (f.events_used == f.events.size()) {
+    FusedMass::StageEvents e;
+    HIP_CHECK(hipEventCreate(&e.a));
+    HIP_CHECK(hipEventCreate(&e.b));
+    HIP_CHECK(hipEventCreate(&e.c));
+    HIP_CHECK(hipEventCreate(&e.d));
+    f.events.push_back(e);
+  }
+  return &f.events[f.events_used++];
+}
+
+void fused_timing_begin(FusedMass& f) {
+  f.timing = true;
+  f.events_used = 0;
+}
+
+void fused_timing_end(FusedMass& f, double* kernel_ms, double* aux_ms, int* count) {
+  double k = 0.0, x = 0.0;
+  for (size_t i = 0; i < f.events_used; ++i) {
+    const auto& e = f.events[i];
+    HIP_CHECK(hipEventSynchronize(e.d));
+    float t1 = 0.f, t2 = 0.f, t3 = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&t1, e.a, e.b));
+    HIP_CHECK(hipEventElapsedTime(&t2, e.b, e.c));
+    HIP_CHECK(hipEventElapsedTime(&t3, e.c, e.d));
+    k += t2;
+    x += t1 + t3;
+  }
+  if (kernel_ms) *kernel_ms = k;
+  if (aux_ms) *aux_ms = x;
+  if (count) *count = static_cast<int>(f.events_used);
+  f.timing = false;
+  f.events_used = 0;
+}
+
 static FusedKey pick(const FusedMass& f, int fp32) {
   FusedKey k{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0, fp32 ? 0 : f.exact, 0};
   k.elems = f.elems > 0 ? f.elems : fused_default_elems(f.P, f.Q, f.NC, fp32);
@@ -154,16 +200,23 @@ void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s)
   a.u = u;
   a.basis = f.B64.data();
   a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+  FusedMass::StageEvents* ev = next_events(f);
+  if (ev) HIP_CHECK(hipEventRecord(ev->a, s));
   if (k.det) {
     if (f.Y64.size() != a.y_comp_len * f.NC) f.Y64.allocate(a.y_comp_len * f.NC);
     a.out = f.Y64.data();
+    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     find(k)(a, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
     csr_scatter<double>(*f.rstr, f.Y64.data(), v, true, s);
   } else {
     zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(double), s);
     a.out = v;
+    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     find(k)(a, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
   }
+  if (ev) HIP_CHECK(hipEventRecord(ev->d, s));
 }
 
 void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s) {
@@ -182,16 +235,23 @@ void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s) {
   a.u = u;
   a.basis = f.B32.data();
   a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+  FusedMass::StageEvents* ev = next_events(f);
+  if (ev) HIP_CHECK(hipEventRecord(ev->a, s));
   if (k.det) {
     if (f.Y32.size() != a.y_comp_len * f.NC) f.Y32.allocate(a.y_comp_len * f.NC);
     a.out = f.Y32.data();
+    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     find(k)(a, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
     csr_scatter<float>(*f.rstr, f.Y32.data(), v, true, s);
   } else {
     zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(float), s);
     a.out = v;
+    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     find(k)(a, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
   }
+  if (ev) HIP_CHECK(hipEventRecord(ev->d, s));
 }
 
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s) {

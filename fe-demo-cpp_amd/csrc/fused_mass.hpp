@@ -62,7 +62,22 @@ struct FusedMass {
   int scatter = kScatterAtomic;
   int exact = 0;
   int elems = 0;  // 0 = default / tuned
+
+  // Optional per-apply stage timing with HIP events on the apply's stream:
+  // a..b = pre-pass (output zeroing), b..c = the fused kernel, c..d = post-pass
+  // (transpose-map sum). Events are pooled and read back by timing_end.
+  struct StageEvents {
+    hipEvent_t a, b, c, d;
+  };
+  bool timing = false;
+  std::vector<StageEvents> events;
+  size_t events_used = 0;
+  ~FusedMass();
 };
+
+void fused_timing_begin(FusedMass& f);
+// Sums of the stage times (ms) over the applies since timing_begin.
+void fused_timing_end(FusedMass& f, double* kernel_ms, double* aux_ms, int* count);
 
 // qdata_ref is the backend-strided layout qdata[q*E + e] (device).
 std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBasis& b, const double* qdata_ref,

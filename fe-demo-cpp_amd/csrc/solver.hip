@@ -230,6 +230,10 @@ struct Pcg {
       if (graph) HIP_CHECK(hipGraphDestroy(graph));
       exec = nullptr;
       graph = nullptr;
+      // One apply outside the capture, so lazily made operator state (element
+      // output buffers, FP32 copies) exists before recording.
+      apply(p.data(), Ap.data(), s);
+      HIP_CHECK(hipStreamSynchronize(s));
       HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
         for (int k = 0; k < batch_iters; ++k) iteration(apply, x, max_iter, s);

@@ -138,6 +138,8 @@ SIGNATURES = {
     "fem_hip_operator_get_info": [_vp, _llp],
     "fem_hip_operator_set_fused_options": [_vp, _i, _i, _i],
     "fem_hip_operator_mass_diagonal": [_vp, _vp, _sz, _vp],
+    "fem_hip_operator_timing_begin": [_vp],
+    "fem_hip_operator_timing_end": [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _ip],
     "fem_hip_operator_destroy": [_vp],
     "fem_hip_operator_create_mass": [_vp, _vp, _vp, ctypes.POINTER(_vp)],
     "fem_hip_rng_uniform": [ctypes.c_uint64, _sz, _d, _d, _vp],
@@ -539,6 +541,14 @@ class Operator:
             call("fem_hip_operator_apply_f32", self.h, in_ptr, in_len, d_out.ptr, d_out.size, _s(stream))
         else:
             call("fem_hip_operator_apply", self.h, in_ptr, in_len, d_out.ptr, d_out.size, _s(stream))
+
+    def timing_begin(self):
+        call("fem_hip_operator_timing_begin", self.h)
+
+    def timing_end(self):
+        k, a, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        call("fem_hip_operator_timing_end", self.h, ctypes.byref(k), ctypes.byref(a), ctypes.byref(n))
+        return k.value, a.value, n.value
 
     def mass_diagonal(self, d_diag, stream=None):
         call("fem_hip_operator_mass_diagonal", self.h, d_diag.ptr, d_diag.size, _s(stream))

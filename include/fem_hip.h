@@ -202,6 +202,12 @@ int fem_hip_operator_apply_f32(fem_hip_operator op, const float* in, size_t in_l
                                void* stream);
 /* Jacobi diagonal of a fused mass operator (FP64 L-vector of active_out_l_size). */
 int fem_hip_operator_mass_diagonal(fem_hip_operator op, double* diag, size_t len, void* stream);
+/* Kernel-level timing of the fused path with HIP events on the apply's stream:
+ * between begin and end every fused apply records events around its stages;
+ * end synchronizes and returns the summed fused-kernel time, the summed time of
+ * the other stages (output zeroing or the transpose-map pass) and the count. */
+int fem_hip_operator_timing_begin(fem_hip_operator op);
+int fem_hip_operator_timing_end(fem_hip_operator op, double* kernel_ms, double* aux_ms, int* count);
 int fem_hip_operator_destroy(fem_hip_operator op);
 
 /* Convenience: the reference mass operator (tests/test-operator.cpp:17-24,75-82)

@@ -1,0 +1,51 @@
+"""Device plumbing (tests/cuda/test-device.cu:26-138 restated for HIP/gfx950)."""
+import numpy as np
+import pytest
+
+import fehip
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_limits(gpu):
+    info = fehip.device_info(0)
+    assert info.name or info.arch.startswith(b"gfx950")  # the name can be blank on virtualized boxes
+    assert info.cu_count >= 1
+    assert info.wavefront_size == 64  # CDNA wave64 (the reference asserts 32 for NVIDIA)
+    assert info.max_threads_per_block >= 1024
+    assert info.lds_per_block >= 64 * 1024
+    assert 0 < info.free_global_mem <= info.total_global_mem
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.device_info(-1)
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.device_info(fehip.device_count())
+
+
+def test_device_array_round_trip(gpu):
+    host = np.array([1.5, -2.0, 3.25, 0.0, 1e-300, 7.0])
+    d = fehip.DeviceArray.from_host(host)
+    assert d.size == 6 and d.nbytes == 48
+    assert d.to_host().tobytes() == host.tobytes()
+    d.zero()
+    assert (d.to_host() == 0).all()
+    with pytest.raises(fehip.InvalidArgument):
+        d.copy_from_host(np.zeros(4))
+    d.resize(8)
+    d.copy_from_host(np.ones(8))
+    assert (d.to_host() == 1).all()
+    ints = np.array([0, 1, 1, 2], np.int32)
+    assert (fehip.DeviceArray.from_host(ints).to_host() == ints).all()
+    assert fehip.DeviceArray(0).to_host().size == 0
+
+
+def test_errors_forced_from_host(gpu):
+    """An oversize allocation and a 4096-thread block fail cleanly, and the error
+    does not leak into the next call (tests/cuda/test-device.cu:104-122)."""
+    with pytest.raises(fehip.FemHipError) as e:
+        fehip.DeviceArray(1 << 50)
+    assert e.value.code == fehip.ERR_OUT_OF_MEMORY
+    assert "hipMalloc" in str(e.value)
+    fehip.call("fem_hip_probe_launch", 1, 64)  # no stale error
+    with pytest.raises(fehip.FemHipError):
+        fehip.call("fem_hip_probe_launch", 1, 4096)
+    fehip.call("fem_hip_probe_launch", 64, 256)

@@ -1,0 +1,48 @@
+"""Mass solves: FP64 Jacobi-PCG and FP32-inner / FP64-refined PCG on the fused
+operator, checked against the oracle's FP64 CG on the same operator (the
+reference has no solver; the oracle CG runs on the restated operator_apply)."""
+import numpy as np
+import pytest
+
+import fehip
+from fehip.problems import MassProblem
+from helpers import rel_err
+from oracle_lib import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("scatter", [fehip.SCATTER_ATOMIC, fehip.SCATTER_DETERMINISTIC])
+def test_c1_solves(gpu, scatter):
+    print("c1 solves, scatter", scatter, flush=True)
+    p = MassProblem(16, 3, 4, 1, scatter=scatter)
+    orc = oracle().mass(3, p.num_elem, 3, 4, 1, p.l_size, p.num_nodes, p.offsets, p.qdata())
+    x_star = fehip.rng_uniform(2, p.l_size)
+    b = orc.apply(x_star)
+    diag = orc.diagonal()
+    x_cpu, it_cpu, rel_cpu = orc.cg(b, diag, rtol=1e-10)
+    assert rel_cpu <= 1e-10
+    d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
+    s64 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_FP64, rtol=1e-10)
+    x64 = d_x.to_host()
+    assert s64["rel_residual"] <= 1e-10, s64
+    assert abs(s64["iterations"] - it_cpu) <= 2, (s64, it_cpu)
+    s_mx = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED, rtol=1e-10, inner_rtol=1e-4)
+    xm = d_x.to_host()
+    assert s_mx["rel_residual"] <= 1e-10, s_mx
+    # Both agree with the oracle's solution and the manufactured x* to solver accuracy.
+    assert rel_err(x64, x_cpu) < 1e-8
+    assert rel_err(xm, x_cpu) < 1e-8
+    assert rel_err(xm, x_star) < 1e-8
+    print("fp64", s64, "mixed", s_mx, "cpu iters", it_cpu, flush=True)
+
+
+def test_solve_repeatable(gpu):
+    p = MassProblem(8, 4, 5, 2, scatter=fehip.SCATTER_DETERMINISTIC)
+    b = fehip.rng_uniform(5, p.l_size)
+    d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
+    p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC)
+    p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED)
+    x1 = d_x.to_host()
+    p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED)
+    assert d_x.to_host().tobytes() == x1.tobytes()

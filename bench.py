@@ -155,15 +155,22 @@ def main():
 
     # Deterministic scatter, exact arithmetic (bit-equal to the reference CPU path).
     extra = {}
-    for label, scatter, exact in (("fp64_deterministic", fehip.SCATTER_DETERMINISTIC, 0),
-                                  ("fp64_deterministic_exact", fehip.SCATTER_DETERMINISTIC, 1),
-                                  ("fp64_atomic_exact", fehip.SCATTER_ATOMIC, 1)):
-        p.op.set_fused_options(scatter, exact=exact)
+    for label, scatter, exact, elems in (("fp64_deterministic", fehip.SCATTER_DETERMINISTIC, 0, 0),
+                                         ("fp64_deterministic_exact", fehip.SCATTER_DETERMINISTIC, 1, 0),
+                                         ("fp64_atomic_per_node", fehip.SCATTER_ATOMIC, 0, 16),
+                                         ("fp64_atomic_exact", fehip.SCATTER_ATOMIC, 1, 0)):
+        p.op.set_fused_options(scatter, exact=exact, elems_per_block=elems)
         m2, k2, a2 = time_applies(fehip, p.op, d_u, d_v, max(args.steps // 2, 5), 2, stream)
         extra[label] = {"ms_per_apply": m2, "gdofs": L / m2 / 1e6, "kernel_ms": k2, "aux_ms": a2,
                         "kernel_frac_hbm": bytes64 / (k2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log(label, extra[label])
     # FP32 apply.
+    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
+    d_u32 = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L).astype(np.float32))
+    d_v32 = fehip.DeviceArray(L, np.float32)
+    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0, elems_per_block=16)
+    m4, k4, a4 = time_applies(fehip, p.op, d_u32, d_v32, args.steps, args.warmup, stream)
+    extra["fp32_atomic_per_node"] = {"ms_per_apply": m4, "gdofs": L / m4 / 1e6, "kernel_ms": k4, "aux_ms": a4}
     p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
     d_u32 = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L).astype(np.float32))
     d_v32 = fehip.DeviceArray(L, np.float32)

@@ -11,6 +11,7 @@
 #include "../../include/fem_hip.h"
 #include "basis.hpp"
 #include "basis_setup.hpp"
+#include "block_plan.hpp"
 #include "common.hpp"
 #include "mesh.hpp"
 #include "operator.hpp"
@@ -577,6 +578,43 @@ int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords) {
   return guard([&] {
     need(coords, "coords");
     box_coords(n, jitter, seed, coords);
+  });
+}
+
+int fem_hip_block_plan_stats(int ne, int es, int l_size, const int* offsets, int nb, long long* stats) {
+  return guard([&] {
+    need(offsets, "offsets");
+    need(stats, "stats");
+    require(ne > 0 && es > 0 && l_size > 0 && nb > 0, "block_plan_stats: bad sizes");
+    const size_t nt = static_cast<size_t>(ne) * es;
+    for (size_t t = 0; t < nt; ++t)
+      require(offsets[t] >= 0 && offsets[t] < l_size, "block_plan_stats: offsets must be in [0, l_size - 1]");
+    BlockPlan p = build_block_plan(offsets, ne, es, l_size, nb);
+    std::vector<int> seen(ne, 0);
+    long long uniq = 0, inter = 0;
+    for (int b = 0; b < p.nblk; ++b) {
+      const int n0 = p.node_ptr[b], nn = p.node_ptr[b + 1] - n0;
+      uniq += nn;
+      inter += p.ninterior[b];
+      for (int j = 0; j < nb; ++j) {
+        const int e = p.elem[static_cast<size_t>(b) * nb + j];
+        if (j >= p.nelem[b]) {
+          if (e != -1) throw std::runtime_error("block plan: padding slot holds an element");
+          continue;
+        }
+        if (e < 0 || e >= ne) throw std::runtime_error("block plan: bad element id");
+        ++seen[e];
+        for (int i = 0; i < es; ++i) {
+          const int l = p.loc[(static_cast<size_t>(b) * nb + j) * es + i];
+          if (l >= nn || p.nodes[n0 + l] != offsets[static_cast<size_t>(e) * es + i])
+            throw std::runtime_error("block plan: local index does not map back to the offset");
+        }
+      }
+    }
+    for (int e = 0; e < ne; ++e)
+      if (seen[e] != 1) throw std::runtime_error("block plan: element not in exactly one block");
+    const long long v[6] = {p.nblk, p.max_nodes, uniq, inter, uniq - inter, static_cast<long long>(nt)};
+    std::memcpy(stats, v, sizeof(v));
   });
 }
 

@@ -4,9 +4,13 @@
 namespace femhip {
 namespace {
 
-const FusedShapeReg<2, 3, 1, 7, 7> reg_2_3_1_7;
-const FusedShapeReg<2, 3, 2, 7, 7> reg_2_3_2_7;
-const FusedShapeReg<2, 3, 3, 7, 7> reg_2_3_3_7;
+const FusedShapeReg<2, 3, 1, 64, 64> reg_2_3_1_64;
+const FusedShapeReg<2, 3, 2, 64, 64> reg_2_3_2_64;
+const FusedShapeReg<2, 3, 3, 64, 64> reg_2_3_3_64;
+
+const FusedBlockShapeReg<2, 3, 1, 32, 32> blk_2_3_1;
+const FusedBlockShapeReg<2, 3, 2, 32, 32> blk_2_3_2;
+const FusedBlockShapeReg<2, 3, 3, 32, 32> blk_2_3_3;
 
 }  // namespace
 }  // namespace femhip

@@ -5,9 +5,13 @@ namespace femhip {
 namespace {
 
 const FusedShapeReg<3, 4, 1, 16, 16> reg_3_4_1_16;
-const FusedShapeReg<3, 4, 1, 4, 4> reg_3_4_1_4;
+const FusedShapeReg<3, 4, 1, 32, 32> reg_3_4_1_32;
 const FusedShapeReg<3, 4, 2, 16, 16> reg_3_4_2_16;
 const FusedShapeReg<3, 4, 3, 16, 16> reg_3_4_3_16;
+
+const FusedBlockShapeReg<3, 4, 1, 16, 16> blk_3_4_1;
+const FusedBlockShapeReg<3, 4, 2, 16, 16> blk_3_4_2;
+const FusedBlockShapeReg<3, 4, 3, 16, 16> blk_3_4_3;
 
 }  // namespace
 }  // namespace femhip

@@ -4,9 +4,14 @@
 namespace femhip {
 namespace {
 
+const FusedShapeReg<4, 5, 1, 23, 23> reg_4_5_1_23;
 const FusedShapeReg<4, 5, 1, 10, 10> reg_4_5_1_10;
-const FusedShapeReg<4, 5, 2, 10, 10> reg_4_5_2_10;
-const FusedShapeReg<4, 5, 3, 10, 10> reg_4_5_3_10;
+const FusedShapeReg<4, 5, 2, 23, 23> reg_4_5_2_23;
+const FusedShapeReg<4, 5, 3, 23, 23> reg_4_5_3_23;
+
+const FusedBlockShapeReg<4, 5, 1, 16, 16> blk_4_5_1;
+const FusedBlockShapeReg<4, 5, 2, 8, 8> blk_4_5_2;
+const FusedBlockShapeReg<4, 5, 3, 8, 8> blk_4_5_3;
 
 }  // namespace
 }  // namespace femhip

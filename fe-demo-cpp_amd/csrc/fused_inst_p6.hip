@@ -4,9 +4,14 @@
 namespace femhip {
 namespace {
 
+const FusedShapeReg<6, 7, 1, 9, 9> reg_6_7_1_9;
 const FusedShapeReg<6, 7, 1, 5, 5> reg_6_7_1_5;
-const FusedShapeReg<6, 7, 2, 5, 5> reg_6_7_2_5;
-const FusedShapeReg<6, 7, 3, 5, 5> reg_6_7_3_5;
+const FusedShapeReg<6, 7, 2, 9, 9> reg_6_7_2_9;
+const FusedShapeReg<6, 7, 3, 9, 9> reg_6_7_3_9;
+
+const FusedBlockShapeReg<6, 7, 1, 8, 8> blk_6_7_1;
+const FusedBlockShapeReg<6, 7, 2, 4, 4> blk_6_7_2;
+const FusedBlockShapeReg<6, 7, 3, 4, 4> blk_6_7_3;
 
 }  // namespace
 }  // namespace femhip

@@ -9,5 +9,8 @@ const FusedShapeReg<8, 10, 1, 2, 2> reg_8_10_1_2;
 const FusedShapeReg<8, 10, 3, 3, 3> reg_8_10_3_3;
 const FusedShapeReg<8, 10, 3, 2, 2> reg_8_10_3_2;
 
+const FusedBlockShapeReg<8, 10, 1, 3, 3> blk_8_10_1;
+const FusedBlockShapeReg<8, 10, 3, 2, 2> blk_8_10_3;
+
 }  // namespace
 }  // namespace femhip

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 
+#include "block_plan.hpp"
 #include "fused_mass.hpp"
 
 namespace femhip {
@@ -12,18 +13,18 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Reference backend-strided qdata[q*E + e] (q = (qz*Q + qy)*Q + qx) -> private
-// [e][qy][qx][qz]: each column thread of the fused kernel reads Q contiguous values.
+// Reference backend-strided qdata[q*E + e] -> private per-element tiles
+// [e][q]: the fused kernel's z-line threads read qdata[e][qz*Q^2 + t], so lanes
+// of one load instruction read consecutive words.
 template <class T>
 __global__ __launch_bounds__(kThreads) void qdata_private_kernel(int E, int Q, const double* __restrict__ ref,
                                                                  T* __restrict__ priv) {
   const size_t idx = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
-  const size_t Q2 = static_cast<size_t>(Q) * Q, Q3 = Q2 * Q;
+  const size_t Q3 = static_cast<size_t>(Q) * Q * Q;
   if (idx >= Q3 * E) return;
-  // idx walks the reference layout (coalesced reads).
-  const size_t e = idx % E, q = idx / E;
-  const size_t col = q % Q2, qz = q / Q2;
-  priv[(e * Q2 + col) * Q + qz] = static_cast<T>(ref[idx]);
+  // idx walks the private layout (coalesced writes); reads stride E.
+  const size_t e = idx / Q3, q = idx % Q3;
+  priv[idx] = static_cast<T>(ref[q * E + e]);
 }
 
 template <class T>
@@ -46,19 +47,29 @@ __global__ __launch_bounds__(kThreads) void mass_diag_kernel(int E, int P, int Q
   const size_t e = idx / P3;
   const int i = static_cast<int>(idx % P3);
   const int ix = i % P, iy = (i / P) % P, iz = i / (P * P);
-  const double* qe = qpriv + e * Q2 * Q;
+  const double* qe = qpriv + e * Q2 * Q;  // [qz][qy][qx]
   double sum = 0.0;
   for (int qy = 0; qy < Q; ++qy) {
     for (int qx = 0; qx < Q; ++qx) {
       const double bxy = B2[qy * P + iy] * B2[qx * P + ix];
-      const double* col = qe + static_cast<size_t>(qy * Q + qx) * Q;
       double s = 0.0;
-      for (int qz = 0; qz < Q; ++qz) s += B2[qz * P + iz] * col[qz];
+      for (int qz = 0; qz < Q; ++qz) s += B2[qz * P + iz] * qe[(qz * Q + qy) * Q + qx];
       sum += bxy * s;
     }
   }
   const size_t o = static_cast<size_t>(offsets[idx]);
   for (int c = 0; c < NC; ++c) unsafeAtomicAdd(&diag[o + static_cast<size_t>(c) * cs], sum);
+}
+
+// Block-slot copy of the private qdata: dst[slot][q] = src[elem[slot]][q] (0 for padding slots).
+template <class T>
+__global__ __launch_bounds__(kThreads) void qdata_block_kernel(size_t n, int Q3, const int* __restrict__ elem,
+                                                               const double* __restrict__ src, T* __restrict__ dst) {
+  const size_t idx = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= n) return;
+  const size_t slot = idx / Q3, q = idx % Q3;
+  const int e = elem[slot];
+  dst[idx] = e >= 0 ? static_cast<T>(src[static_cast<size_t>(e) * Q3 + q]) : T(0);
 }
 
 FusedLauncher find(const FusedKey& k) {
@@ -75,6 +86,81 @@ std::map<FusedKey, FusedLauncher>& fused_registry() {
 }
 
 void register_fused(const FusedKey& k, FusedLauncher f) { fused_registry()[k] = f; }
+
+std::map<FusedKey, FusedBlockLauncher>& fused_block_registry() {
+  static std::map<FusedKey, FusedBlockLauncher> reg;
+  return reg;
+}
+
+void register_fused_block(const FusedKey& k, FusedBlockLauncher f) { fused_block_registry()[k] = f; }
+
+// The block-accumulated launcher for this operator's shape and precision, or null.
+static FusedBlockLauncher find_block(const FusedMass& f, int fp32, int* nb) {
+  for (const auto& kv : fused_block_registry()) {
+    const FusedKey& k = kv.first;
+    if (k.P == f.P && k.Q == f.Q && k.NC == f.NC && k.fp32 == fp32) {
+      *nb = k.elems;
+      return kv.second;
+    }
+  }
+  return nullptr;
+}
+
+// Builds the block plan on first use (host clustering + upload); false if the shape has no block kernel.
+static bool ensure_block_plan(FusedMass& f, hipStream_t s) {
+  if (f.block_tried) return f.block_nb > 0;
+  f.block_tried = true;
+  int nb = 0;
+  if (!find_block(f, 0, &nb)) return false;
+  int nb32 = 0;
+  if (!find_block(f, 1, &nb32) || nb32 != nb) return false;
+  const int P3 = f.P * f.P * f.P, Q3 = f.Q * f.Q * f.Q;
+  BlockPlan plan = build_block_plan(f.rstr->h_offsets.data(), f.num_elem, P3, f.l_size, nb);
+  f.b_node_ptr.upload(plan.node_ptr);
+  f.b_nodes.upload(plan.nodes);
+  f.b_ninterior.upload(plan.ninterior);
+  f.b_nelem.upload(plan.nelem);
+  f.b_loc.upload(plan.loc);
+  DevBuf<int> elem;
+  elem.upload(plan.elem);
+  const size_t n = static_cast<size_t>(plan.nblk) * nb * Q3;
+  f.b_q64.allocate(n);
+  qdata_block_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, Q3, elem.data(), f.qdata64.data(),
+                                                                       f.b_q64.data());
+  HIP_CHECK_LAUNCH();
+  f.b_q32.allocate(n);
+  qdata_block_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, Q3, elem.data(), f.qdata64.data(),
+                                                                      f.b_q32.data());
+  HIP_CHECK_LAUNCH();
+  HIP_CHECK(hipStreamSynchronize(s));  // `elem` is freed on return
+  f.nblk = plan.nblk;
+  f.block_nb = nb;
+  return true;
+}
+
+template <class T>
+static void launch_block(FusedMass& f, const T* u, T* v, hipStream_t s) {
+  const int fp32 = std::is_same<T, float>::value ? 1 : 0;
+  int nb = 0;
+  FusedBlockLauncher L = find_block(f, fp32, &nb);
+  FusedBlockArgs a{};
+  a.nblk = f.nblk;
+  a.comp_stride = f.comp_stride;
+  a.node_ptr = f.b_node_ptr.data();
+  a.nodes = f.b_nodes.data();
+  a.ninterior = f.b_ninterior.data();
+  a.nelem = f.b_nelem.data();
+  a.loc = f.b_loc.data();
+  a.qdata = fp32 ? static_cast<const void*>(f.b_q32.data()) : static_cast<const void*>(f.b_q64.data());
+  a.u = u;
+  a.v = v;
+  a.basis = fp32 ? static_cast<const void*>(f.B32.data()) : static_cast<const void*>(f.B64.data());
+  L(a, s);
+}
+
+void fused_prepare(FusedMass& f, hipStream_t s) {
+  if (f.scatter == kScatterAtomic && f.elems == 0) ensure_block_plan(f, s);
+}
 
 std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int det, int exact) {
   std::vector<int> out;
@@ -125,6 +211,16 @@ std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBas
   f->B32.assign(b.interp.begin(), b.interp.end());
   f->Bsq.resize(b.interp.size());
   for (size_t i = 0; i < b.interp.size(); ++i) f->Bsq[i] = b.interp[i] * b.interp[i];
+  {
+    // Single-contributor nodes: the fast atomic kernel stores them instead of adding.
+    const size_t nt = r.h_offsets.size();
+    std::vector<int> mult(static_cast<size_t>(r.l_size), 0);
+    for (size_t t = 0; t < nt; ++t) ++mult[r.h_offsets[t]];
+    std::vector<int> flagged(r.h_offsets);
+    for (size_t t = 0; t < nt; ++t)
+      if (mult[flagged[t]] == 1) flagged[t] = static_cast<int>(static_cast<unsigned>(flagged[t]) | 0x80000000u);
+    f->flagged_offsets.upload(flagged);
+  }
   const size_t nq = static_cast<size_t>(b.Q) * b.Q * b.Q * r.num_elem;
   f->qdata64.allocate(nq);
   qdata_private_kernel<double><<<grid_for(nq, kThreads), kThreads, 0, s>>>(r.num_elem, b.Q, qdata_ref,
@@ -196,6 +292,7 @@ void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s)
   a.num_elem = f.num_elem;
   a.comp_stride = f.comp_stride;
   a.offsets = f.rstr->offsets.data();
+  a.flagged_offsets = f.flagged_offsets.data();
   a.qdata = f.qdata64.data();
   a.u = u;
   a.basis = f.B64.data();
@@ -213,7 +310,11 @@ void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s)
     zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(double), s);
     a.out = v;
     if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
-    find(k)(a, s);
+    if (f.elems == 0 && ensure_block_plan(f, s)) {
+      launch_block<double>(f, u, v, s);
+    } else {
+      find(k)(a, s);
+    }
     if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
   }
   if (ev) HIP_CHECK(hipEventRecord(ev->d, s));
@@ -231,6 +332,7 @@ void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s) {
   a.num_elem = f.num_elem;
   a.comp_stride = f.comp_stride;
   a.offsets = f.rstr->offsets.data();
+  a.flagged_offsets = f.flagged_offsets.data();
   a.qdata = f.qdata32.data();
   a.u = u;
   a.basis = f.B32.data();
@@ -248,7 +350,11 @@ void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s) {
     zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(float), s);
     a.out = v;
     if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
-    find(k)(a, s);
+    if (f.elems == 0 && ensure_block_plan(f, s)) {
+      launch_block<float>(f, u, v, s);
+    } else {
+      find(k)(a, s);
+    }
     if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
   }
   if (ev) HIP_CHECK(hipEventRecord(ev->d, s));

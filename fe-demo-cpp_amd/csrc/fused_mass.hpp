@@ -3,6 +3,7 @@
 // pass for the deterministic scatter).
 #pragma once
 
+#include <cstdint>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -30,7 +31,8 @@ struct FusedKey {
 struct FusedArgs {
   int num_elem;
   int comp_stride;
-  const int* offsets;  // [E][P^3]
+  const int* offsets;          // [E][P^3]
+  const int* flagged_offsets;  // the same with bit 31 set on single-contributor nodes
   const void* qdata;   // private layout [E][Q^2][Q] (qz innermost), T
   const void* u;       // L-vector, T
   void* out;           // atomic: L-vector v (pre-zeroed); det: Y [NC][E*P^3]
@@ -39,6 +41,24 @@ struct FusedArgs {
 };
 
 using FusedLauncher = void (*)(const FusedArgs&, hipStream_t);
+
+// Launch parameters of the block-accumulated kernel (atomic scatter mode).
+struct FusedBlockArgs {
+  int nblk;
+  int comp_stride;
+  const int* node_ptr;   // [nblk + 1]
+  const int* nodes;      // block node lists (comp-0 L indices)
+  const int* ninterior;  // [nblk]
+  const int* nelem;      // [nblk]
+  const uint16_t* loc;   // [nblk*NB][P^3]
+  const void* qdata;     // [nblk*NB][Q^3], T
+  const void* u;
+  void* v;               // pre-zeroed L-vector
+  const void* basis;     // host [Q][P] of T
+};
+using FusedBlockLauncher = void (*)(const FusedBlockArgs&, hipStream_t);
+std::map<FusedKey, FusedBlockLauncher>& fused_block_registry();
+void register_fused_block(const FusedKey& k, FusedBlockLauncher f);
 
 // Registry of compiled instantiations (filled by the fused_mass_*.hip units).
 std::map<FusedKey, FusedLauncher>& fused_registry();
@@ -52,11 +72,19 @@ std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int det, in
 struct FusedMass {
   int P = 0, Q = 0, NC = 0, num_elem = 0, comp_stride = 0, l_size = 0;
   const Restriction* rstr = nullptr;  // owned by the operator (node restriction)
+  DevBuf<int> flagged_offsets;        // offsets with bit 31 set where the node has one contributor
   DevBuf<double> qdata64;             // private layout
   DevBuf<float> qdata32;              // lazily made from qdata64
   std::vector<double> B64;            // interp_1d [Q][P]
   std::vector<float> B32;
   std::vector<double> Bsq;            // interp_1d squared entry-wise (Jacobi diagonal)
+  // Block plan of the block-accumulated atomic kernel (built on first use).
+  int block_nb = 0, nblk = 0;
+  bool block_tried = false;
+  DevBuf<int> b_node_ptr, b_nodes, b_ninterior, b_nelem;
+  DevBuf<uint16_t> b_loc;
+  DevBuf<double> b_q64;  // qdata in block-slot order [slot][Q^3]
+  DevBuf<float> b_q32;
   DevBuf<double> Y64;                 // deterministic element outputs (lazy)
   DevBuf<float> Y32;
   int scatter = kScatterAtomic;
@@ -84,6 +112,8 @@ std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBas
                                              hipStream_t s);
 bool fused_mass_supported(int P, int Q, int NC);
 void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s);
+// Builds lazily made state (block plan) so later applies allocate nothing (graph capture).
+void fused_prepare(FusedMass& f, hipStream_t s);
 void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s);
 // Jacobi diagonal: diag = E^T [ (B o B)^T x3 qdata ] (FP64).
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);

@@ -106,10 +106,24 @@ __global__ __launch_bounds__(kThreads) void csr_scatter_kernel(int rows, size_t 
   if (n >= rows) return;
   const size_t l = lvec ? static_cast<size_t>(lvec[n]) : static_cast<size_t>(n);
   const int t0 = toff[n], t1 = toff[n + 1];
+  // Rows of up to 8 entries (every node of a conforming hex mesh) issue all
+  // index loads, then all value loads, before the in-order sum; longer rows
+  // continue in chunks of 8.
+  constexpr int kChunk = 8;
   for (int k = 0; k < nc; ++k) {
     const T* Yk = Y + k * comp_len;
     T acc = zero_start ? T(0) : L[l + static_cast<size_t>(k) * cs];
-    for (int t = t0; t < t1; ++t) acc += Yk[tind[t]];
+    for (int tb = t0; tb < t1; tb += kChunk) {
+      int idx[kChunk];
+      T val[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) idx[j] = (tb + j < t1) ? tind[tb + j] : -1;
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) val[j] = (idx[j] >= 0) ? Yk[idx[j]] : T(0);
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j)
+        if (idx[j] >= 0) acc += val[j];
+    }
     L[l + static_cast<size_t>(k) * cs] = acc;
   }
 }

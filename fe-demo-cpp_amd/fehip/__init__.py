@@ -145,6 +145,7 @@ SIGNATURES = {
     "fem_hip_rng_uniform": [ctypes.c_uint64, _sz, _d, _d, _vp],
     "fem_hip_box_offsets": [_i, _i, _i, _i, _i, _vp],
     "fem_hip_box_coords": [_i, _d, ctypes.c_uint64, _vp],
+    "fem_hip_block_plan_stats": [_i, _i, _i, _vp, _i, _llp],
     "fem_hip_mass_solve": [_vp, _vp, _vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveStats), _vp],
 }
 _RESTYPES = {"fem_hip_last_error": ctypes.c_char_p, "fem_hip_version": ctypes.c_char_p}
@@ -349,6 +350,14 @@ def box_coords(n, jitter=0.1, seed=0):
     out = np.empty(3 * (n + 1) ** 3)
     call("fem_hip_box_coords", n, jitter, seed, _ptr(out))
     return out
+
+
+def block_plan_stats(num_elem, elem_size, l_size, offsets, nb):
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    st = (ctypes.c_longlong * 6)()
+    call("fem_hip_block_plan_stats", num_elem, elem_size, l_size, _ptr(offsets), nb, st)
+    keys = ["blocks", "max_nodes", "unique_nodes", "interior_nodes", "boundary_nodes", "element_nodes"]
+    return dict(zip(keys, [int(x) for x in st]))
 
 
 def transpose_map(num_elem, elem_size, l_size, offsets):

@@ -227,6 +227,15 @@ int fem_hip_box_offsets(int nx, int ny, int nz, int P, int node_stride, int* off
  * in lexicographic order from splitmix64(seed). */
 int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords);
 
+/* Host-only query of the element blocking used by the fused kernel's
+ * block-accumulated scatter: elements grouped into blocks of nb (greedy
+ * clustering on shared nodes). stats[6] = {blocks, max unique nodes in a block,
+ * total unique nodes over blocks, total block-interior nodes (plain stores),
+ * total block-boundary nodes (atomic adds), element nodes (num_elem*elem_size)}.
+ * Also checks the plan: every element in exactly one block, local indices
+ * consistent with the offsets. */
+int fem_hip_block_plan_stats(int num_elem, int elem_size, int l_size, const int* offsets, int nb, long long* stats);
+
 /* ---- mass solve ------------------------------------------------------------ */
 #define FEM_HIP_SOLVE_FP64 0  /* Jacobi-PCG, FP64 throughout */
 #define FEM_HIP_SOLVE_MIXED 1 /* FP32 inner Jacobi-PCG, FP64 iterative refinement */

@@ -168,15 +168,18 @@ def test_fused_mass_matches_oracle(gpu, n, P, Q, nc):
     p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC, exact=1)
     p.op.apply(d_u, d_v)
     assert d_v.to_host().tobytes() == want.tobytes(), rel_err(d_v.to_host(), want)
-    for scatter in (fehip.SCATTER_ATOMIC, fehip.SCATTER_DETERMINISTIC):
-        p.op.set_fused_options(scatter, exact=0)
+    # elems_per_block 0: block-accumulated kernel for the atomic mode; 1: the
+    # one-atomic-per-element-node kernel (first compiled packing).
+    for scatter, elems in ((fehip.SCATTER_ATOMIC, 0), (fehip.SCATTER_ATOMIC, 1), (fehip.SCATTER_DETERMINISTIC, 0)):
+        p.op.set_fused_options(scatter, exact=0, elems_per_block=elems)
         p.op.apply(d_u, d_v)
-        assert rel_err(d_v.to_host(), want) < FP64_RTOL
+        assert rel_err(d_v.to_host(), want) < FP64_RTOL, (scatter, elems)
     # FP32 apply: within single-precision rounding of the FP64 result.
-    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
     d_u32, d_v32 = fehip.DeviceArray.from_host(u.astype(np.float32)), fehip.DeviceArray(p.l_size, np.float32)
-    p.op.apply(d_u32, d_v32)
-    assert rel_err(d_v32.to_host().astype(np.float64), want) < 2e-6
+    for scatter, elems in ((fehip.SCATTER_ATOMIC, 0), (fehip.SCATTER_ATOMIC, 1), (fehip.SCATTER_DETERMINISTIC, 0)):
+        p.op.set_fused_options(scatter, exact=0, elems_per_block=elems)
+        p.op.apply(d_u32, d_v32)
+        assert rel_err(d_v32.to_host().astype(np.float64), want) < 2e-6, (scatter, elems)
 
 
 def test_unfused_generic_path_matches_fused(gpu):

@@ -1,0 +1,267 @@
+// C++ tests of the fem::hip drop-in (fe-demo-cpp_amd/cpp/fem_hip.hpp), written
+// like the reference's tests/cuda/test-cuda-elem-restriction.cu and
+// tests/cuda/test-device.cu: the CPU fem:: implementation (here the oracle's
+// restatement, linked from oracle/src) is the oracle for the device path.
+// A minimal runner stands in for Catch2 (not available offline).
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fem_hip.hpp"
+#include "tensor-contract.hpp"
+
+namespace {
+
+struct TestCase {
+  const char* name;
+  std::function<void()> fn;
+};
+std::vector<TestCase>& registry() {
+  static std::vector<TestCase> r;
+  return r;
+}
+struct Reg {
+  Reg(const char* n, std::function<void()> f) { registry().push_back({n, std::move(f)}); }
+};
+#define TEST_CASE(name) \
+  static void name();   \
+  static Reg reg_##name(#name, name); \
+  static void name()
+#define REQUIRE(cond)                                                                                  \
+  do {                                                                                                 \
+    if (!(cond)) throw std::runtime_error(std::string(__FILE__) + ":" + std::to_string(__LINE__) + \
+                                          ": REQUIRE(" #cond ") failed");                             \
+  } while (0)
+#define REQUIRE_THROWS_AS(expr, type)                                                      \
+  do {                                                                                     \
+    bool thrown = false;                                                                   \
+    try {                                                                                  \
+      expr;                                                                                \
+    } catch (const type&) {                                                                \
+      thrown = true;                                                                       \
+    }                                                                                      \
+    if (!thrown) throw std::runtime_error(std::string("line ") + std::to_string(__LINE__) + \
+                                          ": expected " #type " from " #expr);             \
+  } while (0)
+
+using fem::ContractMode;
+using fem::hip::DeviceArray;
+using fem::hip::ScatterMode;
+
+// tests/cuda/test-cuda-elem-restriction.cu:24-42
+fem::ElemRestriction box_restriction(int nx, int ny, int nz, int P, int num_comp, bool interleaved) {
+  const int Nx = nx * (P - 1) + 1, Ny = ny * (P - 1) + 1, Nz = nz * (P - 1) + 1;
+  const int num_nodes = Nx * Ny * Nz;
+  const int node_stride = interleaved ? num_comp : 1;
+  const int comp_stride = interleaved ? 1 : num_nodes;
+  std::vector<int> offsets;
+  for (int ez = 0; ez < nz; ++ez)
+    for (int ey = 0; ey < ny; ++ey)
+      for (int ex = 0; ex < nx; ++ex)
+        for (int iz = 0; iz < P; ++iz)
+          for (int iy = 0; iy < P; ++iy)
+            for (int ix = 0; ix < P; ++ix)
+              offsets.push_back(((ex * (P - 1) + ix) + Nx * ((ey * (P - 1) + iy) + Ny * (ez * (P - 1) + iz))) *
+                                node_stride);
+  fem::ElemRestriction r;
+  fem::elem_restriction_create(nx * ny * nz, P * P * P, num_comp, comp_stride, num_nodes * num_comp, offsets, r);
+  return r;
+}
+
+std::vector<double> wiggly(int n, double phase) {
+  std::vector<double> v(n);
+  for (int i = 0; i < n; ++i) v[i] = std::sin(0.37 * i + phase) + 1e-3 * std::cos(11.0 * i);
+  return v;
+}
+
+// tests/cuda/test-cuda-elem-restriction.cu:60-79, with bit equality for gather and
+// for the deterministic scatter (which starts from the current L value).
+void check_against_cpu(const fem::ElemRestriction& host, ScatterMode mode) {
+  fem::hip::ElemRestriction dev;
+  fem::hip::elem_restriction_create(host, mode, dev);
+  const std::vector<double> l_in = wiggly(host.l_size, 0.1);
+  std::vector<double> e_cpu;
+  fem::elem_restriction_apply(host, ContractMode::NoTranspose, l_in, e_cpu);
+  DeviceArray<double> d_l(l_in), d_e;
+  fem::hip::elem_restriction_apply(dev, ContractMode::NoTranspose, d_l, d_e);
+  REQUIRE(d_e.to_host() == e_cpu);
+
+  const std::vector<double> e_in = wiggly(host.e_size, 0.7);
+  std::vector<double> l_cpu(host.l_size, 1.0);
+  fem::elem_restriction_apply(host, ContractMode::Transpose, e_in, l_cpu);
+  DeviceArray<double> d_ein(e_in), d_lout(std::vector<double>(host.l_size, 1.0));
+  fem::hip::elem_restriction_apply(dev, ContractMode::Transpose, d_ein, d_lout);
+  const std::vector<double> got = d_lout.to_host();
+  if (mode == ScatterMode::Deterministic) {
+    REQUIRE(got == l_cpu);
+  } else {
+    for (size_t i = 0; i < got.size(); ++i) REQUIRE(std::fabs(got[i] - l_cpu[i]) <= 1e-13);
+  }
+}
+
+}  // namespace
+
+TEST_CASE(device_visible_with_sane_limits) {
+  REQUIRE(fem::hip::device_count() >= 1);
+  const auto info = fem::hip::device_info(0);
+  REQUIRE(info.arch.rfind("gfx950", 0) == 0);
+  REQUIRE(info.wavefront_size == 64);
+  REQUIRE(info.cu_count >= 1);
+  REQUIRE(info.free_global_mem <= info.total_global_mem);
+  REQUIRE_THROWS_AS(fem::hip::device_info(-1), std::invalid_argument);
+}
+
+TEST_CASE(device_array_round_trip_and_errors) {
+  const std::vector<double> host = {1.5, -2.0, 3.25, 0.0, 1e-300, 7.0};
+  DeviceArray<double> d(host);
+  REQUIRE(d.to_host() == host);
+  d.zero();
+  REQUIRE(d.to_host() == std::vector<double>(6, 0.0));
+  REQUIRE_THROWS_AS(d.copy_from_host(std::vector<double>(4)), std::invalid_argument);
+  DeviceArray<double> moved(std::move(d));
+  REQUIRE(d.data() == nullptr && moved.size() == 6);  // NOLINT(bugprone-use-after-move)
+  bool threw = false;
+  try {
+    DeviceArray<double> huge(std::size_t(1) << 50);
+  } catch (const std::runtime_error& e) {
+    threw = std::string(e.what()).find("hipMalloc") != std::string::npos;
+  }
+  REQUIRE(threw);
+}
+
+TEST_CASE(transpose_map_known_answers) {
+  fem::ElemRestriction r;
+  fem::elem_restriction_create(2, 2, 1, 1, 6, {2, 4, 5, 2}, r);
+  fem::hip::TransposeMap map;
+  fem::hip::build_transpose_map(r, map);
+  REQUIRE((map.l_vec_indices == std::vector<int>{2, 4, 5}));
+  REQUIRE((map.t_offsets == std::vector<int>{0, 2, 3, 4}));
+  REQUIRE((map.t_indices == std::vector<int>{0, 3, 1, 2}));
+}
+
+TEST_CASE(restriction_matches_cpu_small_and_3d) {
+  for (ScatterMode mode : {ScatterMode::Atomic, ScatterMode::Deterministic}) {
+    fem::ElemRestriction r;
+    fem::elem_restriction_create(2, 2, 1, 1, 3, {0, 1, 1, 2}, r);
+    check_against_cpu(r, mode);
+    fem::elem_restriction_create(2, 4, 2, 6, 12, {0, 1, 3, 4, 1, 2, 4, 5}, r);
+    check_against_cpu(r, mode);
+    for (bool interleaved : {false, true}) {
+      check_against_cpu(box_restriction(4, 3, 2, 2, 1, interleaved), mode);
+      check_against_cpu(box_restriction(3, 3, 3, 3, 3, interleaved), mode);
+    }
+  }
+}
+
+TEST_CASE(apply_checks_sizes) {
+  fem::ElemRestriction host;
+  fem::elem_restriction_create(2, 2, 1, 1, 3, {0, 1, 1, 2}, host);
+  fem::hip::ElemRestriction dev;
+  fem::hip::elem_restriction_create(host, ScatterMode::Atomic, dev);
+  DeviceArray<double> out;
+  REQUIRE_THROWS_AS(fem::hip::elem_restriction_apply(dev, ContractMode::NoTranspose, DeviceArray<double>(4), out),
+                    std::invalid_argument);
+  REQUIRE_THROWS_AS(fem::hip::elem_restriction_apply(dev, ContractMode::Transpose, DeviceArray<double>(3), out),
+                    std::invalid_argument);
+  REQUIRE_THROWS_AS(fem::hip::elem_restriction_apply(dev, ContractMode::Transpose, DeviceArray<double>(4), out),
+                    std::invalid_argument);
+}
+
+TEST_CASE(operator_apply_mass_matches_cpu) {
+  // A 3^3 mesh of P=5 hexes with Q=6 (the shape of the headline workload), built
+  // with the reference's host API and moved to the device with operator_create.
+  const int n = 3, P = 5, Q = 6, E = n * n * n;
+  const fem::ElemRestriction rstr = box_restriction(n, n, n, P, 1, false);
+  const fem::ElemRestriction coords_r = box_restriction(n, n, n, 2, 3, false);
+  const int nv = (n + 1) * (n + 1) * (n + 1);
+  std::vector<double> coords(3 * nv);
+  for (int gz = 0, v = 0; gz <= n; ++gz)
+    for (int gy = 0; gy <= n; ++gy)
+      for (int gx = 0; gx <= n; ++gx, ++v) {
+        const bool interior = gx > 0 && gx < n && gy > 0 && gy < n && gz > 0 && gz < n;
+        coords[v] = gx / double(n) + (interior ? 0.03 * std::sin(v) : 0.0);
+        coords[nv + v] = gy / double(n) + (interior ? 0.03 * std::cos(v) : 0.0);
+        coords[2 * nv + v] = gz / double(n);
+      }
+  fem::ElemRestriction q_rstr;
+  fem::elem_restriction_create_strided(E, Q * Q * Q, 1, E * Q * Q * Q, q_rstr);
+  auto mesh_basis = fem::TensorBasis::create_tensor_H1_lagrange(3, 3, 2, Q);
+  auto basis = fem::TensorBasis::create_tensor_H1_lagrange(3, 1, P, Q);
+
+  fem::QFunction build_qf;
+  fem::qfunction_create(fem::qfunctions::build_mass, build_qf);
+  fem::qfunction_add_input(build_qf, "dx", 9, fem::EvalMode::Grad);
+  fem::qfunction_add_input(build_qf, "weights", 1, fem::EvalMode::Weight);
+  fem::qfunction_add_output(build_qf, "qdata", 1, fem::EvalMode::None);
+  fem::qfunction_set_context(build_qf, fem::qfunctions::BuildMassContext{3});
+  fem::Operator build;
+  fem::operator_create(build_qf, build);
+  fem::operator_set_field(build, "dx", coords_r, mesh_basis, fem::vector_active());
+  fem::operator_set_field(build, "weights", std::nullopt, mesh_basis, fem::vector_none());
+  fem::operator_set_field(build, "qdata", q_rstr, std::nullopt, fem::vector_active());
+  std::vector<double> qdata;
+  fem::operator_apply(build, coords, qdata);
+
+  fem::hip::Operator dbuild;
+  fem::hip::operator_create(build, dbuild);
+  DeviceArray<double> d_coords(coords), d_qdata;
+  fem::hip::operator_apply(dbuild, d_coords, d_qdata);
+  REQUIRE(d_qdata.to_host() == qdata);  // the unfused device path is the reference's arithmetic
+
+  fem::QFunction mass_qf;
+  fem::qfunction_create(fem::qfunctions::apply_mass, mass_qf);
+  fem::qfunction_add_input(mass_qf, "u", 1, fem::EvalMode::Interp);
+  fem::qfunction_add_input(mass_qf, "qdata", 1, fem::EvalMode::None);
+  fem::qfunction_add_output(mass_qf, "v", 1, fem::EvalMode::Interp);
+  fem::Operator mass;
+  fem::operator_create(mass_qf, mass);
+  fem::operator_set_field(mass, "u", rstr, basis, fem::vector_active());
+  fem::operator_set_field(mass, "qdata", q_rstr, std::nullopt, fem::vector_passive(qdata));
+  fem::operator_set_field(mass, "v", rstr, basis, fem::vector_active());
+  const std::vector<double> u = wiggly(rstr.l_size, 0.4);
+  std::vector<double> v_cpu;
+  fem::operator_apply(mass, u, v_cpu);
+
+  for (ScatterMode mode : {ScatterMode::Atomic, ScatterMode::Deterministic}) {
+    fem::hip::Operator dmass;
+    fem::hip::operator_create(mass, dmass, mode);
+    DeviceArray<double> d_u(u), d_v;
+    fem::hip::operator_apply(dmass, d_u, d_v);
+    const std::vector<double> v = d_v.to_host();
+    double scale = 0.0;
+    for (double x : v_cpu) scale = std::fmax(scale, std::fabs(x));
+    for (size_t i = 0; i < v.size(); ++i) REQUIRE(std::fabs(v[i] - v_cpu[i]) <= 1e-13 * scale);
+  }
+}
+
+TEST_CASE(operator_create_rejects_unknown_qfunction) {
+  fem::QFunction qf;
+  fem::qfunction_create([](void*, int, const double* const*, double* const*) { return 0; }, qf);
+  fem::Operator op;
+  fem::operator_create(qf, op);
+  fem::hip::Operator dev;
+  REQUIRE_THROWS_AS(fem::hip::operator_create(op, dev), std::runtime_error);
+}
+
+int main(int argc, char** argv) {
+  int failed = 0, run = 0;
+  for (const auto& t : registry()) {
+    if (argc > 1 && std::string(argv[1]) != t.name) continue;
+    ++run;
+    std::printf("%s ... ", t.name);
+    std::fflush(stdout);
+    try {
+      t.fn();
+      std::printf("ok\n");
+    } catch (const std::exception& e) {
+      ++failed;
+      std::printf("FAILED: %s\n", e.what());
+    }
+    std::fflush(stdout);
+  }
+  std::printf("%d of %d passed\n", run - failed, run);
+  return failed ? 1 : 0;
+}

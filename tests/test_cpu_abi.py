@@ -1,0 +1,140 @@
+"""CPU-side checks of the product library (no GPU work): it loads, exports every
+entry point include/fem_hip.h declares, and its host-side setup code (quadrature,
+Lagrange matrices, meshes, RNG, transpose map, element blocking) is pinned against
+the oracle and against independent numpy statements."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fehip
+from fehip.problems import host_box_offsets
+from oracle_lib import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "fem_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fem_hip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = fehip.lib()
+    names = header_functions()
+    assert len(names) > 50
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in fehip.SIGNATURES, "binding misses " + n
+    assert set(fehip.SIGNATURES) <= set(names)
+    assert b"gfx950" in fehip.lib().fem_hip_version()
+
+
+def test_no_device_is_not_an_error_for_host_queries():
+    n = fehip.device_count()
+    assert n >= 0
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_host_basis_setup_bitwise_equals_oracle(mode):
+    o = oracle()
+    for P in range(2, 11):
+        for Q in range(1, 13):
+            if mode == 1 and Q < 2:
+                continue
+            ob = o.basis(3, 1, P, Q, mode)
+            nodes = fehip.gauss_lobatto(P, weights=False)
+            x, w = fehip.gauss_legendre(Q) if mode == 0 else fehip.gauss_lobatto(Q)
+            I, G = fehip.lagrange_basis_matrix(nodes, x)
+            for a, b in ((x, ob["q_ref"]), (w, ob["q_weight"]), (I, ob["interp"]), (G, ob["grad"])):
+                assert a.tobytes() == b.tobytes(), (P, Q, mode)
+
+
+def test_host_setup_rejects_bad_sizes():
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.gauss_legendre(0)
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.gauss_lobatto(1)
+
+
+def splitmix64(seed, n):
+    """Independent statement of splitmix64 -> (x >> 11) * 2^-53."""
+    out, s, M = [], seed, (1 << 64) - 1
+    for _ in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out.append((z >> 11) * 2.0 ** -53)
+    return np.array(out)
+
+
+def test_rng_matches_splitmix64():
+    assert (fehip.rng_uniform(1, 1000, 0.0, 1.0) == splitmix64(1, 1000)).all()
+    u = fehip.rng_uniform(7, 100000)
+    assert u.min() >= -1 and u.max() < 1 and abs(u.mean()) < 0.01
+
+
+def test_box_offsets_match_numpy():
+    for args in ((3, 2, 2, 3, 1), (2, 2, 2, 5, 3), (4, 1, 1, 2, 1)):
+        assert (fehip.box_offsets(*args) == host_box_offsets(*args)).all()
+
+
+def test_box_coords_jitter_and_boundary():
+    n, jit = 5, 0.1
+    c = fehip.box_coords(n, jit, 0).reshape(3, n + 1, n + 1, n + 1)  # [axis][z][y][x]
+    g = np.arange(n + 1) / n
+    ref = np.stack(np.meshgrid(g, g, g, indexing="ij")[::-1])  # x, y, z grids in [z][y][x]
+    d = c - ref
+    interior = np.zeros((n + 1,) * 3, bool)
+    interior[1:-1, 1:-1, 1:-1] = True
+    assert (d[:, ~interior] == 0).all()
+    assert np.abs(d[:, interior]).max() <= jit / n
+    r = splitmix64(0, 3 * (n - 1) ** 3).reshape(-1, 3)
+    np.testing.assert_allclose(d[0][interior], jit / n * (2 * r[:, 0] - 1), atol=1e-16)
+
+
+def test_transpose_map_matches_oracle():
+    off = fehip.box_offsets(3, 2, 2, 3, 1)
+    l_size = 7 * 5 * 5
+    for a, b in zip(fehip.transpose_map(12, 27, l_size, off), oracle().transpose_map(12, 27, l_size, off)):
+        assert (a == b).all()
+
+
+def test_block_plan_invariants():
+    # The library checks its own plan (every element once, local indices map back).
+    n, P = 6, 4
+    N = (n * (P - 1) + 1) ** 3
+    off = fehip.box_offsets(n, n, n, P, 1)
+    st = fehip.block_plan_stats(n ** 3, P ** 3, N, off, 16)
+    assert st["blocks"] == -(-n ** 3 // 16)
+    assert st["interior_nodes"] + st["boundary_nodes"] == st["unique_nodes"]
+    assert st["boundary_nodes"] < 0.5 * st["element_nodes"]
+    # Independent count of block-boundary nodes cannot exceed the per-element-node atomics.
+    perm = np.random.default_rng(1).permutation(n ** 3)
+    st2 = fehip.block_plan_stats(n ** 3, P ** 3, N, off.reshape(n ** 3, -1)[perm].ravel(), 16)
+    assert st2["boundary_nodes"] < 0.6 * st2["element_nodes"]
+
+
+def test_restriction_create_validation_before_any_device_work():
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.elem_restriction_create(2, 2, 1, 1, 3, np.array([0, 1, 1, 3]))
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.elem_restriction_create(0, 2, 1, 1, 3, np.array([], np.int32))
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.elem_restriction_create(2, 2, 2, 0, 3, np.array([0, 1, 1, 2]))
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.elem_restriction_create_strided(2, 3, 1, 5)
+
+
+def test_bench_refuses_multi_gpu():
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "one GPU" in r.stderr

@@ -9,13 +9,13 @@ qdata = det(J) w built on the device by the build_mass operator
 """
 import numpy as np
 
-from . import (EVAL_GRAD, EVAL_NONE, EVAL_WEIGHT, QF_BUILD_MASS, SCATTER_ATOMIC, VECTOR_ACTIVE, VECTOR_NONE,
+from . import (EVAL_GRAD, EVAL_NONE, EVAL_WEIGHT, QF_BUILD_MASS, SCATTER_DETERMINISTIC, VECTOR_ACTIVE, VECTOR_NONE,
                DeviceArray, Operator, QFunction, TensorBasis, box_coords, box_offsets, elem_restriction_create,
                elem_restriction_create_strided, operator_create_mass)
 
 
 class MassProblem:
-    def __init__(self, n, P, Q, num_comp=1, jitter=0.1, seed=0, scatter=SCATTER_ATOMIC, stream=None):
+    def __init__(self, n, P, Q, num_comp=1, jitter=0.1, seed=0, scatter=SCATTER_DETERMINISTIC, stream=None):
         self.n, self.P, self.Q, self.nc = n, P, Q, num_comp
         self.num_elem = E = n ** 3
         self.num_nodes = N = (n * (P - 1) + 1) ** 3

@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
 """Mass-operator benchmark on one MI355X.
 
-Workload (the headline configuration): FP64 fused mass apply on a 64^3 hex mesh of
-the unit cube with +-0.1h vertex jitter (seed 0), P=5 GLL nodes, Q=6 Gauss points,
-num_comp=1: 257^3 = 16,974,593 DoF, u ~ U[-1,1] (seed 1). A step is one
-operator_apply (output zeroing + fused gather/interp/qdata/interp^T/atomic
-scatter kernel) with u, v, qdata and offsets resident in HBM.
+Workload (the headline configuration): FP64 fused mass operator_apply on a 64^3
+hex mesh of the unit cube with +-0.1h vertex jitter (seed 0), P=5 GLL nodes, Q=6
+Gauss points, num_comp=1: 257^3 = 16,974,593 DoF, u ~ U[-1,1] (seed 1). A step is
+one operator_apply (every kernel of it: output zeroing or the deterministic
+transpose-map pass, and the fused gather/interp/qdata/interp^T/scatter kernel)
+with u, v, qdata and offsets resident in HBM.
 
-Printed (one JSON line): GDoF/s of that apply, the fused kernel's HBM roofline
-(algorithmic bytes per launch / its HIP-event duration), and beside it the
-deterministic-scatter and FP32 applies, the mixed-precision vs FP64 mass solves on
-the 16^3, P=3, Q=4 mesh (p50/p99 latency), and the CPU oracle timed on a bounded
-slab of the same workload.
+Printed (one JSON line): GDoF/s of the fastest FP64 apply variant, the fused
+kernel's HBM roofline (its algorithmic bytes per launch / its HIP-event duration),
+every other apply variant (atomic, block-accumulated, exact, FP32), the 48^3,
+P=8, Q=10, 3-component apply, the mixed-precision vs FP64 mass solves on the
+16^3, P=3, Q=4 mesh (p50/p99 latency), and the CPU oracle timed on a bounded slab
+of the headline workload.
+
+--variants and --calibrate-copy serve profiler runs: the first restricts the
+apply variants, the second adds K plain copies of one L-vector (a backend-strided
+restriction gather: 8 bytes read and 8 written per DoF) so that counter readings
+can be scaled by a kernel of known traffic.
 """
 import argparse
 import json
@@ -128,6 +135,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 48^3, P=8, Q=10, 3-component line")
+    ap.add_argument("--variants", default="", help="comma-separated subset of the apply variants")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="K",
+                    help="also run K plain copies of one FP64 L-vector")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -151,29 +161,57 @@ def main():
     bytes32 = p.apply_bytes(4, 4, 4)
     E, P3 = p.num_elem, P ** 3
     variants = {}
-    runs = (("fp64_deterministic", np.float64, fehip.SCATTER_DETERMINISTIC, 0, 0),
-            ("fp64_atomic_block", np.float64, fehip.SCATTER_ATOMIC, 0, 0),
-            ("fp64_atomic_per_node", np.float64, fehip.SCATTER_ATOMIC, 0, 16),
-            ("fp64_deterministic_exact", np.float64, fehip.SCATTER_DETERMINISTIC, 1, 0),
-            ("fp32_deterministic", np.float32, fehip.SCATTER_DETERMINISTIC, 0, 0),
-            ("fp32_atomic_block", np.float32, fehip.SCATTER_ATOMIC, 0, 0))
-    for label, dt, scatter, exact, elems in runs:
+    D, A = fehip.SCATTER_DETERMINISTIC, fehip.SCATTER_ATOMIC
+    runs = (("fp64_block_partial", np.float64, D, 0, fehip.FUSED_BLOCK_PARTIAL),
+            ("fp64_elem_tmap", np.float64, D, 0, fehip.FUSED_ELEM_TMAP),
+            ("fp64_block_atomic", np.float64, A, 0, fehip.FUSED_BLOCK_ATOMIC),
+            ("fp64_elem_atomic", np.float64, A, 0, fehip.FUSED_ELEM_ATOMIC),
+            ("fp64_exact_tmap", np.float64, D, 1, fehip.FUSED_ELEM_TMAP),
+            ("fp32_block_partial", np.float32, D, 0, fehip.FUSED_BLOCK_PARTIAL),
+            ("fp32_elem_tmap", np.float32, D, 0, fehip.FUSED_ELEM_TMAP),
+            ("fp32_block_atomic", np.float32, A, 0, fehip.FUSED_BLOCK_ATOMIC))
+    wanted = set(filter(None, args.variants.split(",")))
+    unknown = wanted - {r[0] for r in runs}
+    if unknown:
+        ap.error("unknown variants: %s" % ", ".join(sorted(unknown)))
+    kernel_names = {}
+    for label, dt, scatter, exact, alg in runs:
+        if wanted and label not in wanted:
+            continue
         d_u = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L).astype(dt))
         d_v = fehip.DeviceArray(L, dt)
-        p.op.set_fused_options(scatter, exact=exact, elems_per_block=elems)
-        steps = args.steps if label == "fp64_deterministic" else max(args.steps // 2, 5)
+        p.op.set_fused_options(scatter, exact=exact)
+        p.op.set_fused_algorithm(alg)
+        fp32 = dt == np.float32
+        kernel_names[label] = "%s<%d,%d,1,%s,%d>" % (
+            {0: "fused_mass_fast_kernel", 1: "fused_mass_fast_kernel", 2: "fused_mass_block_kernel",
+             3: "fused_mass_part_kernel"}[alg] if not exact else "fused_mass_exact_kernel",
+            P, Q, "float" if fp32 else "double", p.op.fused_choice(fp32)[1])
+        steps = args.steps if label.startswith("fp64") else max(args.steps // 2, 5)
         m, k, a = time_applies(fehip, p.op, d_u, d_v, steps, args.warmup, stream)
-        s = 4 if dt == np.float32 else 8
-        # The fused kernel's own algorithmic bytes: u, qdata, offsets read; the
-        # deterministic kernel writes the element outputs, the atomic one v.
-        kbytes = s * L + s * E * Q ** 3 + 4 * E * P3 + (s * E * P3 if scatter == fehip.SCATTER_DETERMINISTIC else s * L)
+        s = 4 if fp32 else 8
+        # The fused kernel's own algorithmic bytes: u, qdata and offsets read; v
+        # written, except the element-output (tmap) kernels, which write E*P^3
+        # values and leave the sums to the transpose-map pass.
+        kbytes = s * L + s * E * Q ** 3 + 4 * E * P3 + (s * E * P3 if alg == fehip.FUSED_ELEM_TMAP else s * L)
         variants[label] = {"ms_per_apply": m, "gdofs": L / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
                            "kernel_GBs": kbytes / (k * 1e-3) / 1e9, "kernel_frac_hbm": kbytes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
                            "apply_frac_hbm": (bytes32 if s == 4 else bytes64) / (m * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                           "kernel_bytes": kbytes}
+                           "kernel_bytes": kbytes, "kernel": kernel_names[label]}
         log(label, json.dumps(variants[label]))
-    best = min((v for k2, v in variants.items() if k2.startswith("fp64") and "exact" not in k2),
-               key=lambda v: v["ms_per_apply"])
+    p.op.set_fused_options(D, exact=0)
+    if args.calibrate_copy:
+        r = fehip.elem_restriction_create_strided(L, 1, 1, L)
+        d_a = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L))
+        d_b = fehip.DeviceArray(L)
+        for _ in range(args.calibrate_copy):
+            fehip.elem_restriction_apply(r, fehip.NO_TRANSPOSE, d_a, d_b, stream)
+        stream.synchronize()
+        log("copy calibration: %d copies of %d bytes read + %d written" % (args.calibrate_copy, 8 * L, 8 * L))
+    fp64 = [k2 for k2 in variants if k2.startswith("fp64") and "exact" not in k2]
+    if not fp64:
+        ap.error("--variants must include an FP64 non-exact variant")
+    best = min((variants[k2] for k2 in fp64), key=lambda v: v["ms_per_apply"])
     best_name = [k2 for k2, v in variants.items() if v is best][0]
     ms = best["ms_per_apply"]
 
@@ -186,12 +224,14 @@ def main():
         log("config 5 setup %.1f s" % (time.perf_counter() - t1))
         L5 = p5.l_size
         big = {"workload": "48^3 hexes, P=8, Q=10, num_comp=3, %d DoF" % L5}
-        for label, dt, scatter in (("fp64_deterministic", np.float64, fehip.SCATTER_DETERMINISTIC),
-                                   ("fp64_atomic_block", np.float64, fehip.SCATTER_ATOMIC),
-                                   ("fp32_deterministic", np.float32, fehip.SCATTER_DETERMINISTIC)):
+        for label, dt, scatter, alg in (("fp64_block_partial", np.float64, D, fehip.FUSED_BLOCK_PARTIAL),
+                                        ("fp64_elem_tmap", np.float64, D, fehip.FUSED_ELEM_TMAP),
+                                        ("fp64_block_atomic", np.float64, A, fehip.FUSED_BLOCK_ATOMIC),
+                                        ("fp32_block_partial", np.float32, D, fehip.FUSED_BLOCK_PARTIAL)):
             d_u = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L5).astype(dt))
             d_v = fehip.DeviceArray(L5, dt)
             p5.op.set_fused_options(scatter, exact=0)
+            p5.op.set_fused_algorithm(alg)
             m, k, a = time_applies(fehip, p5.op, d_u, d_v, 10, 2, stream)
             s = 4 if dt == np.float32 else 8
             big[label] = {"ms_per_apply": m, "gdofs": L5 / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
@@ -224,12 +264,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (jittered structured hex mesh, u ~ U[-1,1] splitmix64 seed 1)",
         "config": {"workload": "fused FP64 mass operator_apply, 64^3 hexes, P=5 GLL, Q=6 Gauss, num_comp=1, "
-                               "%d DoF, %s scatter" % (L, best_name.split("_", 1)[1]),
+                               "%d DoF, %s scatter" % (L, best_name.split("_", 1)[1].replace("_", " ")),
                    "num_elem": p.num_elem, "P": P, "Q": Q, "num_comp": 1, "l_size": L},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
-                     "kernel": "fused_mass_fast_kernel<5,6,1,double,16,%s>" % (
-                         "DET" if "deterministic" in best_name else "block-atomic"),
+                     "kernel": best["kernel"],
                      "kernel_ms": k_ms, "algorithmic_bytes_per_launch": best["kernel_bytes"],
                      "apply_algorithmic_bytes": bytes64, "apply_frac": best["apply_frac_hbm"],
                      "other_stages_ms": best["other_stages_ms"]},

@@ -506,8 +506,87 @@ int fem_hip_operator_set_fused_options(fem_hip_operator h, int scatter, int exac
     require(elems >= 0, "set_fused_options: elems_per_block must be >= 0");
     h->op->opt_scatter = scatter;
     h->op->opt_exact = exact ? 1 : 0;
+    h->op->opt_alg = elems > 0 ? -2 : -1;
     h->op->opt_elems = elems;
   });
+}
+int fem_hip_operator_set_fused_algorithm(fem_hip_operator h, int alg, int elems) {
+  return guard([&] {
+    need(h, "operator");
+    require(alg >= -1 && alg < kAlgCount, "set_fused_algorithm: unknown algorithm");
+    require(elems >= 0, "set_fused_algorithm: elems must be >= 0");
+    h->op->opt_alg = alg;
+    h->op->opt_elems = alg < 0 ? 0 : elems;
+  });
+}
+int fem_hip_operator_get_fused_choice(fem_hip_operator h, int fp32, int* alg, int* elems) {
+  return guard([&] {
+    need(h, "operator");
+    need(alg, "alg");
+    need(elems, "elems");
+    const FusedKey k = fused_resolve(operator_fused_mass(*h->op, nullptr), fp32 ? 1 : 0);
+    *alg = k.alg;
+    *elems = k.elems;
+  });
+}
+int fem_hip_operator_autotune(fem_hip_operator h, int fp32, int reps, void* s, int* best_alg, int* best_elems,
+                              double* best_ms, double* results, int max_results, int* n_results) {
+  return guard([&] {
+    need(h, "operator");
+    require(reps > 0, "autotune: reps must be positive");
+    require(max_results >= 0 && (results || max_results == 0), "autotune: bad results buffer");
+    FusedMass& f = operator_fused_mass(*h->op, st(s));
+    std::vector<TuneEntry> all;
+    const TuneEntry best = fused_autotune(f, fp32 ? 1 : 0, reps, st(s), &all);
+    if (best_alg) *best_alg = best.alg;
+    if (best_elems) *best_elems = best.elems;
+    if (best_ms) *best_ms = best.ms;
+    for (int i = 0; i < max_results && i < static_cast<int>(all.size()); ++i) {
+      results[3 * i] = all[i].alg;
+      results[3 * i + 1] = all[i].elems;
+      results[3 * i + 2] = all[i].ms;
+    }
+    if (n_results) *n_results = static_cast<int>(all.size());
+  });
+}
+int fem_hip_tune_table_get(int* rows7, double* ms, long long* num_elem, int max_rows, int* n_rows) {
+  return guard([&] {
+    need(n_rows, "n_rows");
+    require(max_rows >= 0 && (max_rows == 0 || rows7), "tune_table_get: bad rows buffer");
+    int i = 0;
+    for (const auto& kv : tune_table()) {
+      if (i < max_rows) {
+        const TuneKey& k = kv.first;
+        const int r[7] = {k.P, k.Q, k.NC, k.fp32, k.det, kv.second.alg, kv.second.elems};
+        std::memcpy(rows7 + 7 * i, r, sizeof(r));
+        if (ms) ms[i] = kv.second.ms;
+        if (num_elem) num_elem[i] = kv.second.num_elem;
+      }
+      ++i;
+    }
+    *n_rows = i;
+  });
+}
+int fem_hip_tune_table_set(const int* r, double ms, long long num_elem) {
+  return guard([&] {
+    need(r, "row");
+    require(r[3] == 0 || r[3] == 1, "tune_table_set: fp32 must be 0 or 1");
+    require(r[4] == 0 || r[4] == 1, "tune_table_set: deterministic must be 0 or 1");
+    require(r[5] >= 0 && r[5] < kAlgCount, "tune_table_set: unknown algorithm");
+    require(!r[4] || alg_deterministic(r[5]), "tune_table_set: a deterministic row needs a deterministic algorithm");
+    require(!fused_elem_variants(r[0], r[1], r[2], r[3], r[5], 0).empty() &&
+                fused_registry().count(FusedKey{r[0], r[1], r[2], r[3], r[5], 0, r[6]}),
+            "tune_table_set: algorithm/elems not compiled for this shape");
+    TuneEntry e;
+    e.alg = r[5];
+    e.elems = r[6];
+    e.ms = ms;
+    e.num_elem = num_elem;
+    tune_table()[TuneKey{r[0], r[1], r[2], r[3], r[4]}] = e;
+  });
+}
+int fem_hip_tune_table_clear(void) {
+  return guard([&] { tune_table().clear(); });
 }
 int fem_hip_operator_mass_diagonal(fem_hip_operator h, double* diag, size_t len, void* s) {
   return guard([&] {
@@ -581,39 +660,86 @@ int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords) {
   });
 }
 
-int fem_hip_block_plan_stats(int ne, int es, int l_size, const int* offsets, int nb, long long* stats) {
+int fem_hip_block_plan_stats(int ne, int es, int l_size, const int* offsets, int nb, int max_nodes, int num_comp,
+                             int comp_stride, long long* stats) {
   return guard([&] {
     need(offsets, "offsets");
     need(stats, "stats");
-    require(ne > 0 && es > 0 && l_size > 0 && nb > 0, "block_plan_stats: bad sizes");
+    require(ne > 0 && es > 0 && l_size > 0 && nb > 0 && max_nodes >= 0 && num_comp > 0 && comp_stride >= 0,
+            "block_plan_stats: bad sizes");
     const size_t nt = static_cast<size_t>(ne) * es;
     for (size_t t = 0; t < nt; ++t)
       require(offsets[t] >= 0 && offsets[t] < l_size, "block_plan_stats: offsets must be in [0, l_size - 1]");
-    BlockPlan p = build_block_plan(offsets, ne, es, l_size, nb);
-    std::vector<int> seen(ne, 0);
+    BlockPlan p = build_block_plan(offsets, ne, es, l_size, nb, max_nodes);
+    auto fail = [](const char* m) { throw std::runtime_error(std::string("block plan: ") + m); };
+    std::vector<int> seen(ne, 0), mult(l_size, 0), in_blk(l_size, 0), owner(l_size, -1);
+    for (size_t t = 0; t < nt; ++t) ++mult[offsets[t]];
     long long uniq = 0, inter = 0;
     for (int b = 0; b < p.nblk; ++b) {
       const int n0 = p.node_ptr[b], nn = p.node_ptr[b + 1] - n0;
+      if (max_nodes && nn > max_nodes) fail("a block exceeds max_nodes");
+      if (p.part_ptr[b + 1] - p.part_ptr[b] != nn - p.ninterior[b]) fail("partial slots != boundary nodes");
       uniq += nn;
       inter += p.ninterior[b];
+      for (int j = 0; j < nn; ++j) in_blk[p.nodes[n0 + j]] = 0;
       for (int j = 0; j < nb; ++j) {
-        const int e = p.elem[static_cast<size_t>(b) * nb + j];
+        const size_t sl = static_cast<size_t>(b) * nb + j;
+        const int e = p.elem[sl];
         if (j >= p.nelem[b]) {
-          if (e != -1) throw std::runtime_error("block plan: padding slot holds an element");
+          if (e != -1) fail("padding slot holds an element");
           continue;
         }
-        if (e < 0 || e >= ne) throw std::runtime_error("block plan: bad element id");
+        if (e < 0 || e >= ne) fail("bad element id");
+        if (p.color[sl] >= p.ncolor[b]) fail("colour out of range");
         ++seen[e];
         for (int i = 0; i < es; ++i) {
-          const int l = p.loc[(static_cast<size_t>(b) * nb + j) * es + i];
+          const int l = p.loc[sl * es + i];
           if (l >= nn || p.nodes[n0 + l] != offsets[static_cast<size_t>(e) * es + i])
-            throw std::runtime_error("block plan: local index does not map back to the offset");
+            fail("local index does not map back to the offset");
+          ++in_blk[p.nodes[n0 + l]];
         }
       }
+      // Slots of one colour share no node; interior nodes are referenced only here.
+      for (int j = 0; j < nn; ++j) {
+        const int node = p.nodes[n0 + j];
+        if ((j < p.ninterior[b]) != (in_blk[node] == mult[node])) fail("interior/boundary split is wrong");
+        if (j < p.ninterior[b]) {
+          if (owner[node] >= 0) fail("node interior to two blocks");
+          owner[node] = b;
+        }
+      }
+      for (int j = 0; j < p.nelem[b]; ++j)
+        for (int k = j + 1; k < p.nelem[b]; ++k) {
+          const size_t sj = static_cast<size_t>(b) * nb + j, sk = static_cast<size_t>(b) * nb + k;
+          if (p.color[sj] != p.color[sk]) continue;
+          for (int i = 0; i < es; ++i)
+            for (int i2 = 0; i2 < es; ++i2)
+              if (p.loc[sj * es + i] == p.loc[sk * es + i2]) fail("two slots of one colour share a node");
+        }
     }
     for (int e = 0; e < ne; ++e)
-      if (seen[e] != 1) throw std::runtime_error("block plan: element not in exactly one block");
-    const long long v[6] = {p.nblk, p.max_nodes, uniq, inter, uniq - inter, static_cast<long long>(nt)};
+      if (seen[e] != 1) fail("element not in exactly one block");
+    // Second-pass rows: each boundary (block, node) appears once, rows ascending,
+    // partial slots ascending within a row and belonging to that node.
+    PartialRows r = build_partial_rows(p, l_size, num_comp, comp_stride);
+    std::vector<int> part_node(static_cast<size_t>(p.part_ptr.back()), -1);
+    for (int b = 0; b < p.nblk; ++b)
+      for (int j = p.ninterior[b]; j < p.node_ptr[b + 1] - p.node_ptr[b]; ++j)
+        part_node[p.part_ptr[b] + j - p.ninterior[b]] = p.nodes[p.node_ptr[b] + j];
+    std::vector<int> used(part_node.size(), 0);
+    for (size_t m = 0; m < r.node.size(); ++m) {
+      if (m && r.node[m] <= r.node[m - 1]) fail("rows not ascending");
+      if (owner[r.node[m]] >= 0) fail("a row for an interior node");
+      for (int k = r.ptr[m]; k < r.ptr[m + 1]; ++k) {
+        if (k > r.ptr[m] && r.part[k] <= r.part[k - 1]) fail("row partials not ascending");
+        if (part_node[r.part[k]] != r.node[m]) fail("partial of another node in a row");
+        ++used[r.part[k]];
+      }
+    }
+    for (int u : used)
+      if (u != 1) fail("a partial not used exactly once");
+    const long long v[9] = {p.nblk, p.max_nodes, uniq, inter, uniq - inter, static_cast<long long>(nt),
+                            p.max_colors, static_cast<long long>(r.node.size()), r.covers_all ? 1 : 0};
     std::memcpy(stats, v, sizeof(v));
   });
 }

@@ -1,8 +1,10 @@
 // Host side of the fused mass operator: registry of compiled shapes, qdata
-// re-layout, dispatch, and the Jacobi diagonal.
+// re-layout, block plans, algorithm choice and autotuning, dispatch, and the
+// Jacobi diagonal.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "block_plan.hpp"
 #include "fused_mass.hpp"
@@ -72,6 +74,31 @@ __global__ __launch_bounds__(kThreads) void qdata_block_kernel(size_t n, int Q3,
   dst[idx] = e >= 0 ? static_cast<T>(src[static_cast<size_t>(e) * Q3 + q]) : T(0);
 }
 
+// Second pass of the partial-sum scatter: v[node + c*cs] = sum of the node's
+// block partials, added from 0 in ascending block order.
+template <class T>
+__global__ __launch_bounds__(kThreads) void partial_sum_kernel(int nrows, int NC, int cs, const int* __restrict__ rows,
+                                                               const int* __restrict__ ptr,
+                                                               const int* __restrict__ part,
+                                                               const T* __restrict__ partials, size_t pcl,
+                                                               T* __restrict__ v) {
+  const int m = blockIdx.x * kThreads + threadIdx.x;
+  if (m >= nrows) return;
+  const int b = ptr[m], e = ptr[m + 1];
+  const size_t node = static_cast<size_t>(rows[m]);
+  for (int c = 0; c < NC; ++c) {
+    const T* pc = partials + static_cast<size_t>(c) * pcl;
+    T sum = T(0);
+    for (int k = b; k < e; ++k) sum += pc[part[k]];
+    v[node + static_cast<size_t>(c) * cs] = sum;
+  }
+}
+
+std::map<FusedKey, int>& budget_registry() {
+  static std::map<FusedKey, int> reg;
+  return reg;
+}
+
 FusedLauncher find(const FusedKey& k) {
   auto& reg = fused_registry();
   auto it = reg.find(k);
@@ -85,99 +112,44 @@ std::map<FusedKey, FusedLauncher>& fused_registry() {
   return reg;
 }
 
-void register_fused(const FusedKey& k, FusedLauncher f) { fused_registry()[k] = f; }
-
-std::map<FusedKey, FusedBlockLauncher>& fused_block_registry() {
-  static std::map<FusedKey, FusedBlockLauncher> reg;
-  return reg;
+void register_fused(const FusedKey& k, FusedLauncher f, int lds_node_budget) {
+  fused_registry()[k] = f;
+  budget_registry()[k] = lds_node_budget;
 }
 
-void register_fused_block(const FusedKey& k, FusedBlockLauncher f) { fused_block_registry()[k] = f; }
-
-// The block-accumulated launcher for this operator's shape and precision, or null.
-static FusedBlockLauncher find_block(const FusedMass& f, int fp32, int* nb) {
-  for (const auto& kv : fused_block_registry()) {
-    const FusedKey& k = kv.first;
-    if (k.P == f.P && k.Q == f.Q && k.NC == f.NC && k.fp32 == fp32) {
-      *nb = k.elems;
-      return kv.second;
-    }
-  }
-  return nullptr;
+int fused_node_budget(const FusedKey& k) {
+  auto it = budget_registry().find(k);
+  return it == budget_registry().end() ? 0 : it->second;
 }
 
-// Builds the block plan on first use (host clustering + upload); false if the shape has no block kernel.
-static bool ensure_block_plan(FusedMass& f, hipStream_t s) {
-  if (f.block_tried) return f.block_nb > 0;
-  f.block_tried = true;
-  int nb = 0;
-  if (!find_block(f, 0, &nb)) return false;
-  int nb32 = 0;
-  if (!find_block(f, 1, &nb32) || nb32 != nb) return false;
-  const int P3 = f.P * f.P * f.P, Q3 = f.Q * f.Q * f.Q;
-  BlockPlan plan = build_block_plan(f.rstr->h_offsets.data(), f.num_elem, P3, f.l_size, nb);
-  f.b_node_ptr.upload(plan.node_ptr);
-  f.b_nodes.upload(plan.nodes);
-  f.b_ninterior.upload(plan.ninterior);
-  f.b_nelem.upload(plan.nelem);
-  f.b_loc.upload(plan.loc);
-  DevBuf<int> elem;
-  elem.upload(plan.elem);
-  const size_t n = static_cast<size_t>(plan.nblk) * nb * Q3;
-  f.b_q64.allocate(n);
-  qdata_block_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, Q3, elem.data(), f.qdata64.data(),
-                                                                       f.b_q64.data());
-  HIP_CHECK_LAUNCH();
-  f.b_q32.allocate(n);
-  qdata_block_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, Q3, elem.data(), f.qdata64.data(),
-                                                                      f.b_q32.data());
-  HIP_CHECK_LAUNCH();
-  HIP_CHECK(hipStreamSynchronize(s));  // `elem` is freed on return
-  f.nblk = plan.nblk;
-  f.block_nb = nb;
-  return true;
-}
-
-template <class T>
-static void launch_block(FusedMass& f, const T* u, T* v, hipStream_t s) {
-  const int fp32 = std::is_same<T, float>::value ? 1 : 0;
-  int nb = 0;
-  FusedBlockLauncher L = find_block(f, fp32, &nb);
-  FusedBlockArgs a{};
-  a.nblk = f.nblk;
-  a.comp_stride = f.comp_stride;
-  a.node_ptr = f.b_node_ptr.data();
-  a.nodes = f.b_nodes.data();
-  a.ninterior = f.b_ninterior.data();
-  a.nelem = f.b_nelem.data();
-  a.loc = f.b_loc.data();
-  a.qdata = fp32 ? static_cast<const void*>(f.b_q32.data()) : static_cast<const void*>(f.b_q64.data());
-  a.u = u;
-  a.v = v;
-  a.basis = fp32 ? static_cast<const void*>(f.B32.data()) : static_cast<const void*>(f.B64.data());
-  L(a, s);
-}
-
-void fused_prepare(FusedMass& f, hipStream_t s) {
-  if (f.scatter == kScatterAtomic && f.elems == 0) ensure_block_plan(f, s);
-}
-
-std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int det, int exact) {
+std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int alg, int exact) {
   std::vector<int> out;
   for (const auto& kv : fused_registry()) {
     const FusedKey& k = kv.first;
-    if (k.P == P && k.Q == Q && k.NC == NC && k.fp32 == fp32 && k.det == det && k.exact == exact)
+    if (k.P == P && k.Q == Q && k.NC == NC && k.fp32 == fp32 && k.alg == alg && k.exact == exact)
       out.push_back(k.elems);
   }
   return out;
 }
 
-int fused_default_elems(int P, int Q, int NC, int fp32) {
-  // The first registered variant is the shape's default (lowest elems that was
-  // compiled as default); the tuner may pick another.
-  auto v = fused_elem_variants(P, Q, NC, fp32, 0, 0);
+bool fused_mass_supported(int P, int Q, int NC) {
+  return !fused_elem_variants(P, Q, NC, 0, kAlgElemAtomic, 0).empty();
+}
+
+std::map<TuneKey, TuneEntry>& tune_table() {
+  static std::map<TuneKey, TuneEntry> t;
+  return t;
+}
+
+namespace {
+
+// Default packing of an algorithm: for element kernels the variant wasting the
+// fewest threads of its last wavefront (ties: fewer elements); for block
+// kernels the first compiled block size.
+int default_elems(int P, int Q, int NC, int fp32, int alg, int exact) {
+  auto v = fused_elem_variants(P, Q, NC, fp32, alg, exact);
   if (v.empty()) return 0;
-  // Prefer the variant registered as the default packing (largest multiple-of-64 fit).
+  if (alg_blocked(alg)) return v.front();
   int best = v.front();
   double best_waste = 1.0;
   const int tpe = std::max({P * P, P * Q, Q * Q});
@@ -192,7 +164,256 @@ int fused_default_elems(int P, int Q, int NC, int fp32) {
   return best;
 }
 
-bool fused_mass_supported(int P, int Q, int NC) { return !fused_elem_variants(P, Q, NC, 0, 0, 0).empty(); }
+bool valid_for(const FusedMass& f, int alg) {
+  return f.scatter != kScatterDeterministic || alg_deterministic(alg);
+}
+
+DevPlan& ensure_plan(FusedMass& f, int NB, int budget, bool part, hipStream_t s) {
+  auto& slot = f.plans[{NB, budget}];
+  if (!slot) {
+    const int P3 = f.P * f.P * f.P, Q3 = f.Q * f.Q * f.Q;
+    BlockPlan plan = build_block_plan(f.rstr->h_offsets.data(), f.num_elem, P3, f.l_size, NB,
+                                      budget > 0 ? std::min(budget, NB * P3) : 0);
+    auto d = std::make_unique<DevPlan>();
+    d->NB = NB;
+    d->budget = budget;
+    d->nblk = plan.nblk;
+    d->max_nodes = plan.max_nodes;
+    d->node_ptr.upload(plan.node_ptr);
+    d->nodes.upload(plan.nodes);
+    d->ninterior.upload(plan.ninterior);
+    d->nelem.upload(plan.nelem);
+    d->loc.upload(plan.loc);
+    d->color.upload(plan.color);
+    d->ncolor.upload(plan.ncolor);
+    d->part_ptr.upload(plan.part_ptr);
+    d->npart = static_cast<size_t>(plan.part_ptr.back());
+    DevBuf<int> elem;
+    elem.upload(plan.elem);
+    const size_t n = static_cast<size_t>(plan.nblk) * NB * Q3;
+    d->q64.allocate(n);
+    qdata_block_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, Q3, elem.data(), f.qdata64.data(),
+                                                                         d->q64.data());
+    HIP_CHECK_LAUNCH();
+    HIP_CHECK(hipStreamSynchronize(s));  // `elem` is freed on return
+    if (part) {
+      PartialRows r = build_partial_rows(plan, f.l_size, f.NC, f.comp_stride);
+      d->nrows = static_cast<int>(r.node.size());
+      d->rows.upload(r.node);
+      d->row_ptr.upload(r.ptr);
+      d->row_part.upload(r.part);
+      d->covers_all = r.covers_all;
+    }
+    slot = std::move(d);
+  }
+  DevPlan& d = *slot;
+  if (part && d.row_ptr.empty()) {
+    // The plan was made for the atomic kernel with the same budget: add the rows.
+    BlockPlan plan = build_block_plan(f.rstr->h_offsets.data(), f.num_elem, f.P * f.P * f.P, f.l_size, NB,
+                                      budget > 0 ? std::min(budget, NB * f.P * f.P * f.P) : 0);
+    PartialRows r = build_partial_rows(plan, f.l_size, f.NC, f.comp_stride);
+    d.nrows = static_cast<int>(r.node.size());
+    d.rows.upload(r.node);
+    d.row_ptr.upload(r.ptr);
+    d.row_part.upload(r.part);
+    d.covers_all = r.covers_all;
+  }
+  return d;
+}
+
+template <class T>
+DevBuf<T>& plan_qdata(FusedMass& f, DevPlan& d, hipStream_t s);
+template <>
+DevBuf<double>& plan_qdata<double>(FusedMass&, DevPlan& d, hipStream_t) {
+  return d.q64;
+}
+template <>
+DevBuf<float>& plan_qdata<float>(FusedMass&, DevPlan& d, hipStream_t s) {
+  if (d.q32.size() != d.q64.size()) {
+    d.q32.allocate(d.q64.size());
+    to_f32_kernel<float><<<grid_for(d.q64.size(), kThreads), kThreads, 0, s>>>(d.q64.size(), d.q64.data(),
+                                                                              d.q32.data());
+    HIP_CHECK_LAUNCH();
+  }
+  return d.q32;
+}
+
+template <class T>
+DevBuf<T>& plan_partials(DevPlan& d);
+template <>
+DevBuf<double>& plan_partials<double>(DevPlan& d) {
+  return d.part64;
+}
+template <>
+DevBuf<float>& plan_partials<float>(DevPlan& d) {
+  return d.part32;
+}
+
+template <class T>
+DevBuf<T>& elem_qdata(FusedMass& f, hipStream_t s);
+template <>
+DevBuf<double>& elem_qdata<double>(FusedMass& f, hipStream_t) {
+  return f.qdata64;
+}
+template <>
+DevBuf<float>& elem_qdata<float>(FusedMass& f, hipStream_t s) {
+  if (f.qdata32.size() != f.qdata64.size()) {
+    f.qdata32.allocate(f.qdata64.size());
+    to_f32_kernel<float><<<grid_for(f.qdata64.size(), kThreads), kThreads, 0, s>>>(f.qdata64.size(), f.qdata64.data(),
+                                                                                  f.qdata32.data());
+    HIP_CHECK_LAUNCH();
+  }
+  return f.qdata32;
+}
+
+template <class T>
+DevBuf<T>& elem_outputs(FusedMass& f);
+template <>
+DevBuf<double>& elem_outputs<double>(FusedMass& f) {
+  return f.Y64;
+}
+template <>
+DevBuf<float>& elem_outputs<float>(FusedMass& f) {
+  return f.Y32;
+}
+
+FusedMass::StageEvents* next_events(FusedMass& f) {
+  if (!f.timing) return nullptr;
+  if (f.events_used == f.events.size()) {
+    FusedMass::StageEvents e;
+    HIP_CHECK(hipEventCreate(&e.a));
+    HIP_CHECK(hipEventCreate(&e.b));
+    HIP_CHECK(hipEventCreate(&e.c));
+    HIP_CHECK(hipEventCreate(&e.d));
+    f.events.push_back(e);
+  }
+  return &f.events[f.events_used++];
+}
+
+// Makes every buffer the key's algorithm needs (so the apply itself allocates nothing).
+template <class T>
+void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
+  if (alg_blocked(k.alg)) {
+    const bool part = k.alg == kAlgBlockPart;
+    DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), part, s);
+    (void)plan_qdata<T>(f, d, s);
+    if (part) {
+      DevBuf<T>& pb = plan_partials<T>(d);
+      const size_t n = std::max<size_t>(d.npart, 1) * f.NC;
+      if (pb.size() != n) pb.allocate(n);
+    }
+  } else {
+    (void)elem_qdata<T>(f, s);
+    if (k.alg == kAlgElemTmap) {
+      DevBuf<T>& Y = elem_outputs<T>(f);
+      const size_t n = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P * f.NC;
+      if (Y.size() != n) Y.allocate(n);
+    }
+  }
+}
+
+template <class T>
+void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s) {
+  FusedLauncher L = find(k);
+  require(L != nullptr, "fused mass: variant not compiled");
+  prepare_key<T>(f, k, s);
+  const bool f32 = std::is_same<T, float>::value;
+  FusedArgs a;
+  a.comp_stride = f.comp_stride;
+  a.u = u;
+  a.basis = f32 ? static_cast<const void*>(f.B32.data()) : static_cast<const void*>(f.B64.data());
+  FusedMass::StageEvents* ev = next_events(f);
+  if (ev) HIP_CHECK(hipEventRecord(ev->a, s));
+  if (alg_blocked(k.alg)) {
+    DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), k.alg == kAlgBlockPart, s);
+    a.qdata = plan_qdata<T>(f, d, s).data();
+    a.nblk = d.nblk;
+    a.max_nodes = d.max_nodes;
+    a.node_ptr = d.node_ptr.data();
+    a.nodes = d.nodes.data();
+    a.ninterior = d.ninterior.data();
+    a.nelem = d.nelem.data();
+    a.loc = d.loc.data();
+    a.color = d.color.data();
+    a.ncolor = d.ncolor.data();
+    a.part_ptr = d.part_ptr.data();
+    a.v = v;
+    if (k.alg == kAlgBlockAtomic || !d.covers_all) zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
+    if (k.alg == kAlgBlockPart) {
+      a.partials = plan_partials<T>(d).data();
+      a.part_comp_len = std::max<size_t>(d.npart, 1);
+    }
+    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
+    L(a, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
+    if (k.alg == kAlgBlockPart && d.nrows > 0) {
+      partial_sum_kernel<T><<<grid_for(static_cast<size_t>(d.nrows), kThreads), kThreads, 0, s>>>(
+          d.nrows, f.NC, f.comp_stride, d.rows.data(), d.row_ptr.data(), d.row_part.data(),
+          static_cast<const T*>(a.partials), a.part_comp_len, v);
+      HIP_CHECK_LAUNCH();
+    }
+  } else {
+    a.qdata = elem_qdata<T>(f, s).data();
+    a.num_elem = f.num_elem;
+    a.offsets = f.rstr->offsets.data();
+    a.flagged_offsets = f.flagged_offsets.data();
+    a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+    if (k.alg == kAlgElemTmap) {
+      a.out = elem_outputs<T>(f).data();
+      if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
+      L(a, s);
+      if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
+      csr_scatter<T>(*f.rstr, elem_outputs<T>(f).data(), v, true, s);
+    } else {
+      zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
+      a.out = v;
+      if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
+      L(a, s);
+      if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
+    }
+  }
+  if (ev) HIP_CHECK(hipEventRecord(ev->d, s));
+}
+
+}  // namespace
+
+FusedKey fused_resolve(const FusedMass& f, int fp32) {
+  FusedKey k{f.P, f.Q, f.NC, fp32, -1, 0, 0};
+  const bool det = f.scatter == kScatterDeterministic;
+  if (f.exact && !fp32) {
+    // The reference's arithmetic: element kernels only (tmap reproduces the CPU sum order).
+    k.exact = 1;
+    k.alg = det ? kAlgElemTmap : kAlgElemAtomic;
+    k.elems = f.choice.alg == k.alg && f.choice.elems > 0 ? f.choice.elems : default_elems(f.P, f.Q, f.NC, 0, k.alg, 1);
+    require(k.elems > 0 && find(k), "fused mass: exact variant not compiled");
+    return k;
+  }
+  if (f.choice.alg >= 0) {
+    require(valid_for(f, f.choice.alg), "fused mass: a deterministic scatter needs a deterministic algorithm");
+    k.alg = f.choice.alg;
+    k.elems = f.choice.elems > 0 ? f.choice.elems : default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
+    require(k.elems > 0 && find(k), "fused mass: the chosen algorithm is not compiled for this shape");
+    return k;
+  }
+  auto it = tune_table().find(TuneKey{f.P, f.Q, f.NC, fp32, det ? 1 : 0});
+  if (it != tune_table().end()) {
+    FusedKey t = k;
+    t.alg = it->second.alg;
+    t.elems = it->second.elems;
+    if (t.alg >= 0 && valid_for(f, t.alg) && find(t)) return t;
+  }
+  // Default order: partial-sum blocks, then (deterministic) transpose map or (atomic) block atomics, then elements.
+  const int order_det[] = {kAlgBlockPart, kAlgElemTmap};
+  const int order_any[] = {kAlgBlockPart, kAlgBlockAtomic, kAlgElemAtomic, kAlgElemTmap};
+  const int* order = det ? order_det : order_any;
+  const int n = det ? 2 : 4;
+  for (int i = 0; i < n; ++i) {
+    k.alg = order[i];
+    k.elems = default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
+    if (k.elems > 0 && find(k)) return k;
+  }
+  throw Unsupported("fused mass: no compiled variant for this shape");
+}
 
 std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBasis& b, const double* qdata_ref,
                                              hipStream_t s) {
@@ -212,7 +433,7 @@ std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBas
   f->Bsq.resize(b.interp.size());
   for (size_t i = 0; i < b.interp.size(); ++i) f->Bsq[i] = b.interp[i] * b.interp[i];
   {
-    // Single-contributor nodes: the fast atomic kernel stores them instead of adding.
+    // Single-contributor nodes: the element atomic kernel stores them instead of adding.
     const size_t nt = r.h_offsets.size();
     std::vector<int> mult(static_cast<size_t>(r.l_size), 0);
     for (size_t t = 0; t < nt; ++t) ++mult[r.h_offsets[t]];
@@ -236,19 +457,6 @@ FusedMass::~FusedMass() {
     (void)hipEventDestroy(e.c);
     (void)hipEventDestroy(e.d);
   }
-}
-
-static FusedMass::StageEvents* next_events(FusedMass& f) {
-  if (!f.timing) return nullptr;
-  if (f.events_used == f.events.size()) {
-    FusedMass::StageEvents e;
-    HIP_CHECK(hipEventCreate(&e.a));
-    HIP_CHECK(hipEventCreate(&e.b));
-    HIP_CHECK(hipEventCreate(&e.c));
-    HIP_CHECK(hipEventCreate(&e.d));
-    f.events.push_back(e);
-  }
-  return &f.events[f.events_used++];
 }
 
 void fused_timing_begin(FusedMass& f) {
@@ -275,89 +483,90 @@ void fused_timing_end(FusedMass& f, double* kernel_ms, double* aux_ms, int* coun
   f.events_used = 0;
 }
 
-static FusedKey pick(const FusedMass& f, int fp32) {
-  FusedKey k{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0, fp32 ? 0 : f.exact, 0};
-  k.elems = f.elems > 0 ? f.elems : fused_default_elems(f.P, f.Q, f.NC, fp32);
-  if (!find(k)) {
-    auto v = fused_elem_variants(f.P, f.Q, f.NC, fp32, k.det, k.exact);
-    require(!v.empty(), "fused mass: variant not compiled");
-    k.elems = v.front();
+void fused_prepare(FusedMass& f, int fp32, hipStream_t s) {
+  const FusedKey k = fused_resolve(f, fp32);
+  if (fp32) {
+    prepare_key<float>(f, k, s);
+  } else {
+    prepare_key<double>(f, k, s);
   }
-  return k;
 }
 
 void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s) {
-  const FusedKey k = pick(f, 0);
-  FusedArgs a{};
-  a.num_elem = f.num_elem;
-  a.comp_stride = f.comp_stride;
-  a.offsets = f.rstr->offsets.data();
-  a.flagged_offsets = f.flagged_offsets.data();
-  a.qdata = f.qdata64.data();
-  a.u = u;
-  a.basis = f.B64.data();
-  a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
-  FusedMass::StageEvents* ev = next_events(f);
-  if (ev) HIP_CHECK(hipEventRecord(ev->a, s));
-  if (k.det) {
-    if (f.Y64.size() != a.y_comp_len * f.NC) f.Y64.allocate(a.y_comp_len * f.NC);
-    a.out = f.Y64.data();
-    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
-    find(k)(a, s);
-    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-    csr_scatter<double>(*f.rstr, f.Y64.data(), v, true, s);
-  } else {
-    zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(double), s);
-    a.out = v;
-    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
-    if (f.elems == 0 && ensure_block_plan(f, s)) {
-      launch_block<double>(f, u, v, s);
-    } else {
-      find(k)(a, s);
-    }
-    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-  }
-  if (ev) HIP_CHECK(hipEventRecord(ev->d, s));
+  apply_key<double>(f, fused_resolve(f, 0), u, v, s);
 }
 
 void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s) {
-  if (f.qdata32.empty()) {
-    f.qdata32.allocate(f.qdata64.size());
-    to_f32_kernel<float><<<grid_for(f.qdata64.size(), kThreads), kThreads, 0, s>>>(f.qdata64.size(), f.qdata64.data(),
-                                                                                  f.qdata32.data());
-    HIP_CHECK_LAUNCH();
-  }
-  const FusedKey k = pick(f, 1);
-  FusedArgs a{};
-  a.num_elem = f.num_elem;
-  a.comp_stride = f.comp_stride;
-  a.offsets = f.rstr->offsets.data();
-  a.flagged_offsets = f.flagged_offsets.data();
-  a.qdata = f.qdata32.data();
-  a.u = u;
-  a.basis = f.B32.data();
-  a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
-  FusedMass::StageEvents* ev = next_events(f);
-  if (ev) HIP_CHECK(hipEventRecord(ev->a, s));
-  if (k.det) {
-    if (f.Y32.size() != a.y_comp_len * f.NC) f.Y32.allocate(a.y_comp_len * f.NC);
-    a.out = f.Y32.data();
-    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
-    find(k)(a, s);
-    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-    csr_scatter<float>(*f.rstr, f.Y32.data(), v, true, s);
-  } else {
-    zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(float), s);
-    a.out = v;
-    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
-    if (f.elems == 0 && ensure_block_plan(f, s)) {
-      launch_block<float>(f, u, v, s);
-    } else {
-      find(k)(a, s);
+  apply_key<float>(f, fused_resolve(f, 1), u, v, s);
+}
+
+namespace {
+
+template <class T>
+TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneEntry>* all) {
+  const int fp32 = std::is_same<T, float>::value ? 1 : 0;
+  const size_t L = static_cast<size_t>(f.l_size);
+  DevBuf<T> u(L), v(L);
+  // Any input exercises the same memory traffic; ones keep the values finite.
+  std::vector<T> ones(L, T(1));
+  u.upload(ones);
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  TuneEntry best;
+  best.ms = 1e300;
+  const FusedChoice saved = f.choice;
+  const bool saved_timing = f.timing;
+  f.timing = false;
+  try {
+    for (int alg = 0; alg < kAlgCount; ++alg) {
+      if (!valid_for(f, alg)) continue;
+      for (int ne : fused_elem_variants(f.P, f.Q, f.NC, fp32, alg, 0)) {
+        const FusedKey k{f.P, f.Q, f.NC, fp32, alg, 0, ne};
+        apply_key<T>(f, k, u.data(), v.data(), s);  // warm-up, and builds the variant's state
+        HIP_CHECK(hipStreamSynchronize(s));
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) apply_key<T>(f, k, u.data(), v.data(), s);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        TuneEntry t{alg, ne, ms / reps, f.num_elem};
+        if (all) all->push_back(t);
+        if (t.ms < best.ms) best = t;
+      }
     }
-    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f.choice = saved;
+    f.timing = saved_timing;
+    throw;
   }
-  if (ev) HIP_CHECK(hipEventRecord(ev->d, s));
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  f.choice = saved;
+  f.timing = saved_timing;
+  require(best.alg >= 0, "autotune: no candidate");
+  tune_table()[TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0}] = best;
+  // Drop block plans the winner does not use (each holds a slot-ordered qdata copy).
+  const FusedKey wk{f.P, f.Q, f.NC, fp32, best.alg, 0, best.elems};
+  for (auto it = f.plans.begin(); it != f.plans.end();) {
+    const bool keep = alg_blocked(best.alg) && it->first == std::make_pair(best.elems, fused_node_budget(wk));
+    it = keep ? std::next(it) : f.plans.erase(it);
+  }
+  if (best.alg != kAlgElemTmap) {
+    f.Y64.release();
+    f.Y32.release();
+  }
+  return best;
+}
+
+}  // namespace
+
+TuneEntry fused_autotune(FusedMass& f, int fp32, int reps, hipStream_t s, std::vector<TuneEntry>* all) {
+  require(reps > 0, "autotune: reps must be positive");
+  return fp32 ? autotune_impl<float>(f, reps, s, all) : autotune_impl<double>(f, reps, s, all);
 }
 
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s) {

@@ -1,6 +1,6 @@
 // Fused per-element mass operator: gather -> interp (x,y,z) -> qdata * u ->
-// interp^T (x,y,z) -> scatter, one launch per apply (plus the transpose-map
-// pass for the deterministic scatter).
+// interp^T (x,y,z) -> scatter, in one kernel per apply plus, for some scatter
+// algorithms, a second pass.
 #pragma once
 
 #include <cstdint>
@@ -15,85 +15,106 @@
 
 namespace femhip {
 
-// What a fused launch computes, chosen per apply.
+// How the element outputs reach v.
+enum FusedAlg : int {
+  kAlgElemAtomic = 0,  // per element; one atomic add per shared node, plain store for single-contributor nodes
+  kAlgElemTmap = 1,    // per element into Y[E*P^3], then the transpose-map pass (deterministic; CPU order)
+  kAlgBlockAtomic = 2, // per element block; sums in LDS, boundary nodes one atomic each
+  kAlgBlockPart = 3,   // per element block; sums in LDS, boundary nodes one partial each, then a partial-sum pass
+  kAlgCount = 4
+};
+inline bool alg_deterministic(int alg) { return alg == kAlgElemTmap || alg == kAlgBlockPart; }
+inline bool alg_blocked(int alg) { return alg == kAlgBlockAtomic || alg == kAlgBlockPart; }
+
+// One compiled kernel.
 struct FusedKey {
   int P, Q, NC;
-  int fp32;       // 0 FP64, 1 FP32
-  int det;        // 0: atomic scatter-add into v; 1: element outputs to Y, then the transpose-map pass
-  int exact;      // 1: no FMA, reference association (FP64 only)
-  int elems;      // elements per workgroup
+  int fp32;   // 0 FP64, 1 FP32
+  int alg;    // FusedAlg
+  int exact;  // 1: no FMA, reference association (FP64, element algorithms only)
+  int elems;  // elements per workgroup (element algorithms) or per block (block algorithms)
   bool operator<(const FusedKey& o) const {
-    return std::tie(P, Q, NC, fp32, det, exact, elems) < std::tie(o.P, o.Q, o.NC, o.fp32, o.det, o.exact, o.elems);
+    return std::tie(P, Q, NC, fp32, alg, exact, elems) < std::tie(o.P, o.Q, o.NC, o.fp32, o.alg, o.exact, o.elems);
   }
 };
 
-// Launch parameters shared by every instantiation.
+// Launch parameters shared by every instantiation (each kernel reads its part).
 struct FusedArgs {
-  int num_elem;
-  int comp_stride;
-  const int* offsets;          // [E][P^3]
-  const int* flagged_offsets;  // the same with bit 31 set on single-contributor nodes
-  const void* qdata;   // private layout [E][Q^2][Q] (qz innermost), T
-  const void* u;       // L-vector, T
-  void* out;           // atomic: L-vector v (pre-zeroed); det: Y [NC][E*P^3]
-  size_t y_comp_len;   // E*P^3
-  const void* basis;   // host pointer to Q*P values of T (interp_1d), copied into the kernel arguments
+  int comp_stride = 0;
+  const void* qdata = nullptr;  // element algorithms: [E][Q^3]; block algorithms: [nblk*NB][Q^3] (T)
+  const void* u = nullptr;      // L-vector, T
+  const void* basis = nullptr;  // host pointer to Q*P values of T (interp_1d), copied into the kernel arguments
+  // element algorithms
+  int num_elem = 0;
+  const int* offsets = nullptr;          // [E][P^3]
+  const int* flagged_offsets = nullptr;  // the same with bit 31 set on single-contributor nodes
+  void* out = nullptr;                   // atomic: v (pre-zeroed); tmap: Y [NC][E*P^3]
+  size_t y_comp_len = 0;                 // E*P^3
+  // block algorithms
+  int nblk = 0, max_nodes = 0;
+  const int* node_ptr = nullptr;   // [nblk + 1]
+  const int* nodes = nullptr;      // per block: interior then boundary nodes (comp-0 L indices)
+  const int* ninterior = nullptr;  // [nblk]
+  const int* nelem = nullptr;      // [nblk]
+  const uint16_t* loc = nullptr;   // [nblk*NB][P^3]
+  const uint8_t* color = nullptr;  // [nblk*NB]
+  const int* ncolor = nullptr;     // [nblk]
+  const int* part_ptr = nullptr;   // [nblk + 1]
+  void* v = nullptr;               // block atomic: pre-zeroed v; block part: v (interior nodes stored)
+  void* partials = nullptr;        // block part: [NC][npart]
+  size_t part_comp_len = 0;
 };
 
 using FusedLauncher = void (*)(const FusedArgs&, hipStream_t);
-
-// Launch parameters of the block-accumulated kernel (atomic scatter mode).
-struct FusedBlockArgs {
-  int nblk;
-  int comp_stride;
-  const int* node_ptr;   // [nblk + 1]
-  const int* nodes;      // block node lists (comp-0 L indices)
-  const int* ninterior;  // [nblk]
-  const int* nelem;      // [nblk]
-  const uint16_t* loc;   // [nblk*NB][P^3]
-  const void* qdata;     // [nblk*NB][Q^3], T
-  const void* u;
-  void* v;               // pre-zeroed L-vector
-  const void* basis;     // host [Q][P] of T
-};
-using FusedBlockLauncher = void (*)(const FusedBlockArgs&, hipStream_t);
-std::map<FusedKey, FusedBlockLauncher>& fused_block_registry();
-void register_fused_block(const FusedKey& k, FusedBlockLauncher f);
-
-// Registry of compiled instantiations (filled by the fused_mass_*.hip units).
+// Registry of compiled instantiations (filled by the fused_inst_*.hip units).
 std::map<FusedKey, FusedLauncher>& fused_registry();
-void register_fused(const FusedKey& k, FusedLauncher f);
-// Default workgroup packing for a shape (elements per workgroup).
-int fused_default_elems(int P, int Q, int NC, int fp32);
-// Every compiled elements-per-workgroup variant for a shape (for the tuner).
-std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int det, int exact);
+void register_fused(const FusedKey& k, FusedLauncher f, int lds_node_budget);
+// For block kernels: the most unique nodes per block the kernel's LDS is sized for.
+int fused_node_budget(const FusedKey& k);
+// Every compiled elems variant of one algorithm for a shape.
+std::vector<int> fused_elem_variants(int P, int Q, int NC, int fp32, int alg, int exact);
+bool fused_mass_supported(int P, int Q, int NC);
+
+// Device copy of a block plan (one per (NB, node budget) in use).
+struct DevPlan {
+  int NB = 0, budget = 0, nblk = 0, max_nodes = 0, nrows = 0;
+  size_t npart = 0;
+  bool covers_all = false;
+  DevBuf<int> node_ptr, nodes, ninterior, nelem, ncolor, part_ptr, rows, row_ptr, row_part;
+  DevBuf<uint16_t> loc;
+  DevBuf<uint8_t> color;
+  DevBuf<double> q64;  // qdata in slot order [slot][Q^3]
+  DevBuf<float> q32;   // lazily made
+  DevBuf<double> part64;
+  DevBuf<float> part32;
+};
+
+// The selected way to apply: algorithm and its elems (0 = the shape's default).
+struct FusedChoice {
+  int alg = -1;  // -1: automatic (tuned table, else the default order)
+  int elems = 0;
+};
 
 // The device state of a fused mass operator.
 struct FusedMass {
   int P = 0, Q = 0, NC = 0, num_elem = 0, comp_stride = 0, l_size = 0;
   const Restriction* rstr = nullptr;  // owned by the operator (node restriction)
   DevBuf<int> flagged_offsets;        // offsets with bit 31 set where the node has one contributor
-  DevBuf<double> qdata64;             // private layout
+  DevBuf<double> qdata64;             // private layout [E][Q^3]
   DevBuf<float> qdata32;              // lazily made from qdata64
   std::vector<double> B64;            // interp_1d [Q][P]
   std::vector<float> B32;
   std::vector<double> Bsq;            // interp_1d squared entry-wise (Jacobi diagonal)
-  // Block plan of the block-accumulated atomic kernel (built on first use).
-  int block_nb = 0, nblk = 0;
-  bool block_tried = false;
-  DevBuf<int> b_node_ptr, b_nodes, b_ninterior, b_nelem;
-  DevBuf<uint16_t> b_loc;
-  DevBuf<double> b_q64;  // qdata in block-slot order [slot][Q^3]
-  DevBuf<float> b_q32;
-  DevBuf<double> Y64;                 // deterministic element outputs (lazy)
+  std::map<std::pair<int, int>, std::unique_ptr<DevPlan>> plans;  // by (NB, node budget)
+  DevBuf<double> Y64;                 // element outputs of the transpose-map algorithm (lazy)
   DevBuf<float> Y32;
-  int scatter = kScatterAtomic;
-  int exact = 0;
-  int elems = 0;  // 0 = default / tuned
+  int scatter = kScatterAtomic;  // requested: atomic (any order) or deterministic (reproducible)
+  int exact = 0;                 // requested: the reference's arithmetic (bit-equal with deterministic)
+  FusedChoice choice;            // explicit algorithm, or automatic
 
   // Optional per-apply stage timing with HIP events on the apply's stream:
   // a..b = pre-pass (output zeroing), b..c = the fused kernel, c..d = post-pass
-  // (transpose-map sum). Events are pooled and read back by timing_end.
+  // (transpose-map or partial sums). Events are pooled and read back by timing_end.
   struct StageEvents {
     hipEvent_t a, b, c, d;
   };
@@ -110,12 +131,33 @@ void fused_timing_end(FusedMass& f, double* kernel_ms, double* aux_ms, int* coun
 // qdata_ref is the backend-strided layout qdata[q*E + e] (device).
 std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBasis& b, const double* qdata_ref,
                                              hipStream_t s);
-bool fused_mass_supported(int P, int Q, int NC);
 void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s);
-// Builds lazily made state (block plan) so later applies allocate nothing (graph capture).
-void fused_prepare(FusedMass& f, hipStream_t s);
 void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s);
+// The kernel an apply in this precision would launch now (after automatic choice).
+FusedKey fused_resolve(const FusedMass& f, int fp32);
+// Builds lazily made state (block plans, FP32 copies) for the current choice,
+// so later applies allocate nothing (graph capture).
+void fused_prepare(FusedMass& f, int fp32, hipStream_t s);
 // Jacobi diagonal: diag = E^T [ (B o B)^T x3 qdata ] (FP64).
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);
+
+// Autotuning table: the fastest (alg, elems) per (P, Q, NC, fp32, deterministic),
+// measured by fused_autotune and consulted by automatic choice.
+struct TuneKey {
+  int P, Q, NC, fp32, det;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(P, Q, NC, fp32, det) < std::tie(o.P, o.Q, o.NC, o.fp32, o.det);
+  }
+};
+struct TuneEntry {
+  int alg = -1, elems = 0;
+  double ms = 0.0;  // measured time per apply
+  long long num_elem = 0;  // size it was measured on
+};
+std::map<TuneKey, TuneEntry>& tune_table();
+// Times every compiled candidate valid for the operator's requested scatter and
+// precision on its own data (reps applies each, after one warm-up), records the
+// fastest in the table and returns it. Restores the operator's choice (automatic).
+TuneEntry fused_autotune(FusedMass& f, int fp32, int reps, hipStream_t s, std::vector<TuneEntry>* all = nullptr);
 
 }  // namespace femhip

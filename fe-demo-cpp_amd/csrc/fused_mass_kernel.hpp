@@ -542,35 +542,239 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_bloc
   }
 }
 
+// Block-accumulated variant with a deterministic scatter: like the kernel
+// above, but the block's u values are staged in LDS first (one load per unique
+// node instead of one per element node), element outputs are added into the
+// node sums colour by colour (slots of one colour share no node, so the adds
+// need no atomics and their order is fixed), interior nodes are stored and each
+// boundary node's block sum goes to its own partial-sum slot; a second pass
+// (partial_sum_kernel) adds a node's partials in ascending block order. The
+// result is the same bit for bit on every run.
 template <int P, int Q, int NC, class T, int NB>
-void launch_fused_block(const FusedBlockArgs& a, hipStream_t s) {
+__global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part_kernel(
+    int nblk, int comp_stride, const int* __restrict__ node_ptr, const int* __restrict__ nodes,
+    const int* __restrict__ ninterior, const int* __restrict__ nelem, const uint8_t* __restrict__ color,
+    const int* __restrict__ ncolor, const int* __restrict__ part_ptr, const uint16_t* __restrict__ loc,
+    const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
+    size_t part_comp_len, int ldn, MatArg<T, P * Q> Bm) {
   using S = Shape<P, Q>;
+  constexpr int P2 = S::P2, P3 = S::P3, Q2 = S::Q2, Q3 = S::Q3, TPE = S::TPE, RY = S::RY, ZS = S::ZS;
+  constexpr int NT = round64(NB * TPE);
+  constexpr int KN = (NB * P3 + NT - 1) / NT;  // node-list entries per thread (a block has <= NB*P3 nodes)
+  __shared__ T s_b[NB][S::FBUF];
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  T* s_n = reinterpret_cast<T*>(s_dyn);  // [NC][ldn]: the block's u values, then its node sums
+
+  const int per = gridDim.x / 8;
+  const int blk = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (blk >= nblk) return;  // whole workgroup
+  const int n0 = node_ptr[blk], nn = node_ptr[blk + 1] - n0, nint = ninterior[blk], ne_here = nelem[blk];
+  const int ncol = ncolor[blk];
+  const int tid = threadIdx.x;
+
+  int gn[KN];
+#pragma unroll
+  for (int k = 0; k < KN; ++k) {
+    const int j = tid + k * NT;
+    gn[k] = j < nn ? nodes[n0 + j] : 0;
+  }
+  const int el = tid / TPE;
+  const int t = tid - el * TPE;
+  const bool live = el < ne_here;
+  const size_t slot = static_cast<size_t>(blk) * NB + el;
+  T* b = live ? s_b[el] : nullptr;
+  auto B = [&](int q, int p) -> T { return Bm.m[q * P + p]; };
+
+  T qd[Q];
+  if (live && t < Q2) {
+    const T* qp = qdata + slot * Q3 + t;
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) qd[qz] = qp[qz * Q2];
+  }
+  int lo[P];
+  if (live && t < P2) {
+    const uint16_t* lp = loc + slot * P3 + t * P;
+#pragma unroll
+    for (int ix = 0; ix < P; ++ix) lo[ix] = lp[ix];
+  }
+  const int my_color = live ? color[slot] : -1;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      const int j = tid + k * NT;
+      if (j < nn) s_n[c * ldn + j] = u[static_cast<size_t>(gn[k]) + static_cast<size_t>(c) * comp_stride];
+    }
+  }
+  __syncthreads();
+
+#pragma unroll 1
+  for (int c = 0; c < NC; ++c) {
+    T* sn = s_n + c * ldn;
+    // S1: x-rows (iz,iy): gather from the staged node values, contract x.
+    if (live && t < P2) {
+      const int row = (t / P) * ZS + (t % P) * RY;
+      T x[P];
+#pragma unroll
+      for (int ix = 0; ix < P; ++ix) x[ix] = sn[lo[ix]];
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        T acc = T(0);
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) acc = __builtin_fma(B(qx, ix), x[ix], acc);
+        b[row + qx] = acc;
+      }
+    }
+    __syncthreads();
+    // S2: y-columns (iz,qx), in place; the component's u values are consumed, so
+    // its slice of s_n is cleared for the sums.
+    for (int j = tid; j < nn; j += NT) sn[j] = T(0);
+    if (live && t < P * Q) {
+      const int col = (t / Q) * ZS + t % Q;
+      T x[P];
+#pragma unroll
+      for (int iy = 0; iy < P; ++iy) x[iy] = b[col + iy * RY];
+#pragma unroll
+      for (int qy = 0; qy < Q; ++qy) {
+        T acc = T(0);
+#pragma unroll
+        for (int iy = 0; iy < P; ++iy) acc = __builtin_fma(B(qy, iy), x[iy], acc);
+        b[col + qy * RY] = acc;
+      }
+    }
+    __syncthreads();
+    // S3: z-lines (qy,qx): contract z, times qdata, contract z back.
+    if (live && t < Q2) {
+      const int line = (t / Q) * RY + t % Q;
+      T x[P];
+#pragma unroll
+      for (int iz = 0; iz < P; ++iz) x[iz] = b[iz * ZS + line];
+      T d[Q];
+#pragma unroll
+      for (int qz = 0; qz < Q; ++qz) {
+        T acc = T(0);
+#pragma unroll
+        for (int iz = 0; iz < P; ++iz) acc = __builtin_fma(B(qz, iz), x[iz], acc);
+        d[qz] = qd[qz] * acc;
+      }
+#pragma unroll
+      for (int iz = 0; iz < P; ++iz) {
+        T acc = T(0);
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) acc = __builtin_fma(B(qz, iz), d[qz], acc);
+        b[iz * ZS + line] = acc;
+      }
+    }
+    __syncthreads();
+    // S4: y-columns back (iz,qx), in place.
+    if (live && t < P * Q) {
+      const int col = (t / Q) * ZS + t % Q;
+      T x[Q];
+#pragma unroll
+      for (int qy = 0; qy < Q; ++qy) x[qy] = b[col + qy * RY];
+#pragma unroll
+      for (int iy = 0; iy < P; ++iy) {
+        T acc = T(0);
+#pragma unroll
+        for (int qy = 0; qy < Q; ++qy) acc = __builtin_fma(B(qy, iy), x[qy], acc);
+        b[col + iy * RY] = acc;
+      }
+    }
+    __syncthreads();
+    // S5: x-rows back (iz,iy) into registers, then the colour rounds.
+    T out[P];
+    if (live && t < P2) {
+      const int row = (t / P) * ZS + (t % P) * RY;
+      T x[Q];
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) x[qx] = b[row + qx];
+#pragma unroll
+      for (int ix = 0; ix < P; ++ix) {
+        T acc = T(0);
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) acc = __builtin_fma(B(qx, ix), x[qx], acc);
+        out[ix] = acc;
+      }
+    }
+#pragma unroll 1
+    for (int k = 0; k < ncol; ++k) {
+      if (my_color == k && t < P2) {
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) sn[lo[ix]] += out[ix];
+      }
+      __syncthreads();
+    }
+  }
+  // Flush: interior nodes are complete; each boundary node's block sum is a partial.
+  const size_t pb = static_cast<size_t>(part_ptr[blk]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      const int j = tid + k * NT;
+      if (j < nn) {
+        const T val = s_n[c * ldn + j];
+        if (j < nint) {
+          v[static_cast<size_t>(gn[k]) + static_cast<size_t>(c) * comp_stride] = val;
+        } else {
+          partials[static_cast<size_t>(c) * part_comp_len + pb + (j - nint)] = val;
+        }
+      }
+    }
+  }
+}
+
+// Node-sum values a block kernel can keep in LDS (0 = the kernel sizes its own).
+template <int P, int Q, int NC, class T, int NB>
+constexpr int part_node_budget() {
+  using S = Shape<P, Q>;
+  constexpr int fixed = NB * S::FBUF * static_cast<int>(sizeof(T));
+  // Aim for four workgroups per CU (40 KiB each); large elements get 64 KiB.
+  constexpr int target = (40 * 1024 - fixed) / (NC * static_cast<int>(sizeof(T))) >= 2 * S::P3 ? 40 * 1024 : 64 * 1024;
+  return (target - fixed) / (NC * static_cast<int>(sizeof(T)));
+}
+
+template <int P, int Q, int NC, class T>
+MatArg<T, P * Q> basis_arg(const void* basis) {
   MatArg<T, P * Q> Bm;
-  const T* src = static_cast<const T*>(a.basis);
+  const T* src = static_cast<const T*>(basis);
   for (int i = 0; i < P * Q; ++i) Bm.m[i] = src[i];
+  return Bm;
+}
+
+template <int P, int Q, int NC, class T, int NB>
+void launch_block_atomic(const FusedArgs& a, hipStream_t s) {
+  using S = Shape<P, Q>;
   const long long grid = (static_cast<long long>(a.nblk) + 7) / 8 * 8;
   require(grid < (1ll << 31), "fused mass: too many blocks for one launch");
+  require(a.max_nodes <= NB * S::P3, "fused mass: block plan exceeds the kernel's node capacity");
   constexpr int threads = round64(NB * S::TPE);
   fused_mass_block_kernel<P, Q, NC, T, NB><<<static_cast<unsigned>(grid), threads, 0, s>>>(
       a.nblk, a.comp_stride, a.node_ptr, a.nodes, a.ninterior, a.nelem, a.loc, static_cast<const T*>(a.qdata),
-      static_cast<const T*>(a.u), static_cast<T*>(a.v), Bm);
+      static_cast<const T*>(a.u), static_cast<T*>(a.v), basis_arg<P, Q, NC, T>(a.basis));
   HIP_CHECK_LAUNCH();
 }
 
 template <int P, int Q, int NC, class T, int NB>
-struct FusedBlockReg {
-  FusedBlockReg() {
-    register_fused_block(FusedKey{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, 2, 0, NB},
-                         &launch_fused_block<P, Q, NC, T, NB>);
-  }
-};
+void launch_block_part(const FusedArgs& a, hipStream_t s) {
+  using S = Shape<P, Q>;
+  const long long grid = (static_cast<long long>(a.nblk) + 7) / 8 * 8;
+  require(grid < (1ll << 31), "fused mass: too many blocks for one launch");
+  require(a.max_nodes > 0 && a.max_nodes <= part_node_budget<P, Q, NC, T, NB>() && a.max_nodes <= NB * S::P3,
+          "fused mass: block plan exceeds the kernel's LDS node budget");
+  constexpr int threads = round64(NB * S::TPE);
+  const size_t lds = static_cast<size_t>(NC) * a.max_nodes * sizeof(T);
+  fused_mass_part_kernel<P, Q, NC, T, NB><<<static_cast<unsigned>(grid), threads, lds, s>>>(
+      a.nblk, a.comp_stride, a.node_ptr, a.nodes, a.ninterior, a.nelem, a.color, a.ncolor, a.part_ptr, a.loc,
+      static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v), static_cast<T*>(a.partials),
+      a.part_comp_len, a.max_nodes, basis_arg<P, Q, NC, T>(a.basis));
+  HIP_CHECK_LAUNCH();
+}
 
 template <int P, int Q, int NC, class T, int NE, bool EXACT, bool DET>
 void launch_fused(const FusedArgs& a, hipStream_t s) {
   using S = Shape<P, Q>;
-  MatArg<T, P * Q> Bm;
-  const T* src = static_cast<const T*>(a.basis);
-  for (int i = 0; i < P * Q; ++i) Bm.m[i] = src[i];
+  const auto Bm = basis_arg<P, Q, NC, T>(a.basis);
   const long long blocks = (static_cast<long long>(a.num_elem) + NE - 1) / NE;
   const long long grid = (blocks + 7) / 8 * 8;
   require(grid < (1ll << 31), "fused mass: too many elements for one launch");
@@ -591,12 +795,13 @@ void launch_fused(const FusedArgs& a, hipStream_t s) {
 template <int P, int Q, int NC, class T, int NE, bool EXACT, bool DET>
 struct FusedReg {
   FusedReg() {
-    register_fused(FusedKey{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, DET ? 1 : 0, EXACT ? 1 : 0, NE},
-                   &launch_fused<P, Q, NC, T, NE, EXACT, DET>);
+    register_fused(FusedKey{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, DET ? kAlgElemTmap : kAlgElemAtomic,
+                            EXACT ? 1 : 0, NE},
+                   &launch_fused<P, Q, NC, T, NE, EXACT, DET>, 0);
   }
 };
 
-// All variants of one shape: FP64 {atomic, det} x {fast, exact}, FP32 {atomic, det}.
+// All element-algorithm variants of one shape: FP64 {atomic, tmap} x {fast, exact}, FP32 {atomic, tmap}.
 template <int P, int Q, int NC, int NE64, int NE32>
 struct FusedShapeReg {
   FusedReg<P, Q, NC, double, NE64, false, false> a;
@@ -607,7 +812,17 @@ struct FusedShapeReg {
   FusedReg<P, Q, NC, float, NE32, false, true> f;
 };
 
-// Block-accumulated variants (FP64 and FP32).
+template <int P, int Q, int NC, class T, int NB>
+struct FusedBlockReg {
+  FusedBlockReg() {
+    const int fp32 = std::is_same<T, float>::value ? 1 : 0;
+    register_fused(FusedKey{P, Q, NC, fp32, kAlgBlockAtomic, 0, NB}, &launch_block_atomic<P, Q, NC, T, NB>, 0);
+    register_fused(FusedKey{P, Q, NC, fp32, kAlgBlockPart, 0, NB}, &launch_block_part<P, Q, NC, T, NB>,
+                   part_node_budget<P, Q, NC, T, NB>());
+  }
+};
+
+// Block-algorithm variants (atomic and partial-sum; FP64 and FP32).
 template <int P, int Q, int NC, int NB64, int NB32>
 struct FusedBlockShapeReg {
   FusedBlockReg<P, Q, NC, double, NB64> a;

@@ -160,7 +160,8 @@ FusedMass& operator_fused_mass(Operator& op, hipStream_t s) {
   if (f.scatter == kScatterDeterministic && op.inputs[0].rstr->scatter_mode != kScatterDeterministic)
     throw std::invalid_argument("operator: deterministic scatter needs a restriction created with ScatterMode::Deterministic");
   f.exact = op.opt_exact;
-  f.elems = op.opt_elems;
+  f.choice.alg = op.opt_alg == -2 ? (f.scatter == kScatterDeterministic ? kAlgElemTmap : kAlgElemAtomic) : op.opt_alg;
+  f.choice.elems = op.opt_elems;
   return f;
 }
 

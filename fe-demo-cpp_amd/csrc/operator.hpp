@@ -35,7 +35,10 @@ struct Operator {
   bool fused_checked = false;
   int fused_kind = 0;  // 0 none, 1 mass
   std::unique_ptr<FusedMass> fused;
-  int opt_scatter = -1, opt_exact = 0, opt_elems = 0;
+  // Fused options: scatter (-1: the restriction's mode), exact arithmetic, and the
+  // algorithm (-1 automatic, -2 the element kernels of the scatter mode, else a
+  // FusedAlg) with its elements per workgroup/block (0 = default).
+  int opt_scatter = -1, opt_exact = 0, opt_alg = -1, opt_elems = 0;
 };
 
 Operator* operator_create(const QFunctionDev& qf);

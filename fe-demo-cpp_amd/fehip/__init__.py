@@ -20,6 +20,8 @@ NO_TRANSPOSE, TRANSPOSE = 0, 1
 EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 3
 QUAD_GAUSS, QUAD_GAUSS_LOBATTO = 0, 1
 SCATTER_ATOMIC, SCATTER_DETERMINISTIC = 0, 1
+FUSED_AUTO, FUSED_ELEM_ATOMIC, FUSED_ELEM_TMAP, FUSED_BLOCK_ATOMIC, FUSED_BLOCK_PARTIAL = -1, 0, 1, 2, 3
+FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_partial"}
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
 QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR = 1, 2, 3
 SOLVE_FP64, SOLVE_MIXED = 0, 1
@@ -137,6 +139,12 @@ SIGNATURES = {
     "fem_hip_operator_apply_f32": [_vp, _vp, _sz, _vp, _sz, _vp],
     "fem_hip_operator_get_info": [_vp, _llp],
     "fem_hip_operator_set_fused_options": [_vp, _i, _i, _i],
+    "fem_hip_operator_set_fused_algorithm": [_vp, _i, _i],
+    "fem_hip_operator_get_fused_choice": [_vp, _i, _ip, _ip],
+    "fem_hip_operator_autotune": [_vp, _i, _i, _vp, _ip, _ip, ctypes.POINTER(_d), ctypes.POINTER(_d), _i, _ip],
+    "fem_hip_tune_table_get": [_ip, ctypes.POINTER(_d), _llp, _i, _ip],
+    "fem_hip_tune_table_set": [_ip, _d, ctypes.c_longlong],
+    "fem_hip_tune_table_clear": [],
     "fem_hip_operator_mass_diagonal": [_vp, _vp, _sz, _vp],
     "fem_hip_operator_timing_begin": [_vp],
     "fem_hip_operator_timing_end": [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _ip],
@@ -145,7 +153,7 @@ SIGNATURES = {
     "fem_hip_rng_uniform": [ctypes.c_uint64, _sz, _d, _d, _vp],
     "fem_hip_box_offsets": [_i, _i, _i, _i, _i, _vp],
     "fem_hip_box_coords": [_i, _d, ctypes.c_uint64, _vp],
-    "fem_hip_block_plan_stats": [_i, _i, _i, _vp, _i, _llp],
+    "fem_hip_block_plan_stats": [_i, _i, _i, _vp, _i, _i, _i, _i, _llp],
     "fem_hip_mass_solve": [_vp, _vp, _vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveStats), _vp],
 }
 _RESTYPES = {"fem_hip_last_error": ctypes.c_char_p, "fem_hip_version": ctypes.c_char_p}
@@ -352,12 +360,35 @@ def box_coords(n, jitter=0.1, seed=0):
     return out
 
 
-def block_plan_stats(num_elem, elem_size, l_size, offsets, nb):
+def block_plan_stats(num_elem, elem_size, l_size, offsets, nb, max_nodes=0, num_comp=1, comp_stride=None):
     offsets = np.ascontiguousarray(offsets, np.int32)
-    st = (ctypes.c_longlong * 6)()
-    call("fem_hip_block_plan_stats", num_elem, elem_size, l_size, _ptr(offsets), nb, st)
-    keys = ["blocks", "max_nodes", "unique_nodes", "interior_nodes", "boundary_nodes", "element_nodes"]
+    st = (ctypes.c_longlong * 9)()
+    cs = l_size if comp_stride is None else comp_stride
+    call("fem_hip_block_plan_stats", num_elem, elem_size, l_size, _ptr(offsets), nb, max_nodes, num_comp, cs, st)
+    keys = ["blocks", "max_nodes", "unique_nodes", "interior_nodes", "boundary_nodes", "element_nodes", "max_colors",
+            "rows", "covers_all"]
     return dict(zip(keys, [int(x) for x in st]))
+
+
+def tune_table():
+    """Rows (P, Q, num_comp, fp32, deterministic, alg, elems, ms, num_elem) of the process-wide tuning table."""
+    n = ctypes.c_int()
+    call("fem_hip_tune_table_get", None, None, None, 0, ctypes.byref(n))
+    k = n.value
+    rows = (ctypes.c_int * (7 * max(k, 1)))()
+    ms = (_d * max(k, 1))()
+    ne = (ctypes.c_longlong * max(k, 1))()
+    call("fem_hip_tune_table_get", rows, ms, ne, k, ctypes.byref(n))
+    return [tuple(rows[7 * i:7 * i + 7]) + (float(ms[i]), int(ne[i])) for i in range(k)]
+
+
+def tune_table_set(P, Q, num_comp, fp32, deterministic, alg, elems, ms=0.0, num_elem=0):
+    row = (ctypes.c_int * 7)(P, Q, num_comp, int(fp32), int(deterministic), alg, elems)
+    call("fem_hip_tune_table_set", row, float(ms), int(num_elem))
+
+
+def tune_table_clear():
+    call("fem_hip_tune_table_clear")
 
 
 def transpose_map(num_elem, elem_size, l_size, offsets):
@@ -542,6 +573,23 @@ class Operator:
 
     def set_fused_options(self, scatter_mode=-1, exact=0, elems_per_block=0):
         call("fem_hip_operator_set_fused_options", self.h, scatter_mode, exact, elems_per_block)
+
+    def set_fused_algorithm(self, alg, elems=0):
+        call("fem_hip_operator_set_fused_algorithm", self.h, alg, elems)
+
+    def fused_choice(self, fp32=False):
+        a, e = ctypes.c_int(), ctypes.c_int()
+        call("fem_hip_operator_get_fused_choice", self.h, int(fp32), ctypes.byref(a), ctypes.byref(e))
+        return a.value, e.value
+
+    def autotune(self, fp32=False, reps=10, stream=None):
+        """Times every valid algorithm/packing; returns (best, [(alg, elems, ms), ...])."""
+        a, e, ms, n = ctypes.c_int(), ctypes.c_int(), _d(), ctypes.c_int()
+        res = (_d * (3 * 64))()
+        call("fem_hip_operator_autotune", self.h, int(fp32), reps, _s(stream), ctypes.byref(a), ctypes.byref(e),
+             ctypes.byref(ms), res, 64, ctypes.byref(n))
+        rows = [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2])) for i in range(min(n.value, 64))]
+        return (a.value, e.value, ms.value), rows
 
     def apply(self, d_in, d_out, stream=None):
         in_ptr = d_in.ptr if d_in is not None else None

@@ -193,10 +193,45 @@ int fem_hip_operator_apply(fem_hip_operator op, const double* in, size_t in_len,
 /* info[8] = {num_elem, num_qpts, active_in_l_size, active_out_l_size, fused(0 none,1 mass,2 build),
  *            P_1d, Q_1d, num_comp} */
 int fem_hip_operator_get_info(fem_hip_operator op, long long* info);
-/* Fused mass apply options: scatter (FEM_HIP_SCATTER_*), exact (1 = no fused
- * multiply-add, element sums in the reference order; 0 = FMA), elems_per_block
- * (0 = tuned/default). Takes effect on the next apply. */
+/* Fused mass apply options: scatter (FEM_HIP_SCATTER_*; -1 = the restriction's
+ * mode), exact (1 = no fused multiply-add, element sums in the reference order;
+ * 0 = FMA), elems_per_block (0 = automatic algorithm choice; > 0 = the element
+ * kernels of the scatter mode with that many elements per workgroup). Takes
+ * effect on the next apply. Deterministic results are the same bit for bit on
+ * every run; with exact = 1 they equal the reference's CPU operator_apply. */
 int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, int exact, int elems_per_block);
+
+/* How a fused mass apply reaches v (no counterpart in the reference; its CUDA
+ * restriction offers the atomic and transpose-map scatters,
+ * include/cuda/cuda-elem-restriction.hpp:21). */
+#define FEM_HIP_FUSED_AUTO -1         /* tuned table, else the default order */
+#define FEM_HIP_FUSED_ELEM_ATOMIC 0   /* per element, one atomic add per shared node */
+#define FEM_HIP_FUSED_ELEM_TMAP 1     /* per element into element outputs, then the transpose-map sum (deterministic) */
+#define FEM_HIP_FUSED_BLOCK_ATOMIC 2  /* element blocks summed in LDS, boundary nodes one atomic add each */
+#define FEM_HIP_FUSED_BLOCK_PARTIAL 3 /* element blocks summed in LDS, boundary partials summed by a second pass
+                                         (deterministic) */
+/* Explicit algorithm and its elements per workgroup (element kernels) or per
+ * block (block kernels); 0 = that algorithm's default. A deterministic scatter
+ * mode accepts only the deterministic algorithms. exact = 1 overrides. */
+int fem_hip_operator_set_fused_algorithm(fem_hip_operator op, int alg, int elems);
+/* The algorithm and elems the next apply in this precision (fp32 = 0/1) will run. */
+int fem_hip_operator_get_fused_choice(fem_hip_operator op, int fp32, int* alg, int* elems);
+/* Autotuner: times every compiled algorithm/packing valid for the operator's
+ * scatter mode in one precision on the operator's own data (reps applies each,
+ * after a warm-up; the operator's output is not touched), records the fastest
+ * for (P, Q, num_comp, fp32, deterministic) in the process-wide tuning table,
+ * which automatic choice then follows, and returns it. results (may be NULL)
+ * receives up to max_results rows {alg, elems, ms_per_apply} of every candidate;
+ * *n_results is set to the number of candidates. Synchronous. */
+int fem_hip_operator_autotune(fem_hip_operator op, int fp32, int reps, void* stream, int* best_alg, int* best_elems,
+                              double* best_ms, double* results, int max_results, int* n_results);
+/* Tuning table rows {P, Q, num_comp, fp32, deterministic, alg, elems} with the
+ * time per apply and the element count it was measured on. get: copies up to
+ * max_rows rows and sets *n_rows to the table size; set: adds or replaces one
+ * row (alg must be compiled for the shape); clear: empties the table. */
+int fem_hip_tune_table_get(int* rows7, double* ms, long long* num_elem, int max_rows, int* n_rows);
+int fem_hip_tune_table_set(const int* row7, double ms, long long num_elem);
+int fem_hip_tune_table_clear(void);
 /* FP32 fused mass apply on FP32 L-vectors (qdata and basis rounded to FP32 once). */
 int fem_hip_operator_apply_f32(fem_hip_operator op, const float* in, size_t in_len, float* out, size_t out_len,
                                void* stream);
@@ -227,14 +262,18 @@ int fem_hip_box_offsets(int nx, int ny, int nz, int P, int node_stride, int* off
  * in lexicographic order from splitmix64(seed). */
 int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords);
 
-/* Host-only query of the element blocking used by the fused kernel's
- * block-accumulated scatter: elements grouped into blocks of nb (greedy
- * clustering on shared nodes). stats[6] = {blocks, max unique nodes in a block,
- * total unique nodes over blocks, total block-interior nodes (plain stores),
- * total block-boundary nodes (atomic adds), element nodes (num_elem*elem_size)}.
- * Also checks the plan: every element in exactly one block, local indices
- * consistent with the offsets. */
-int fem_hip_block_plan_stats(int num_elem, int elem_size, int l_size, const int* offsets, int nb, long long* stats);
+/* Host-only statistics and invariant checks of the element-block plan used by
+ * the block algorithms: blocks of at most nb elements and max_nodes unique nodes
+ * (0 = no limit), greedy clustering on shared nodes. stats[9] = {blocks, max
+ * unique nodes in a block, total unique nodes over blocks, total block-interior
+ * nodes (plain stores), total block-boundary nodes (atomic adds or partials),
+ * element nodes (num_elem*elem_size), most colours in a block, second-pass rows,
+ * 1 if interior nodes and rows write every L entry}. Checks: every element in
+ * exactly one block, local indices map back to the offsets, slots of one colour
+ * share no node, interior nodes belong to one block, each partial is summed once
+ * by the row of its node, in ascending order. */
+int fem_hip_block_plan_stats(int num_elem, int elem_size, int l_size, const int* offsets, int nb, int max_nodes,
+                             int num_comp, int comp_stride, long long* stats);
 
 /* ---- mass solve ------------------------------------------------------------ */
 #define FEM_HIP_SOLVE_FP64 0  /* Jacobi-PCG, FP64 throughout */

@@ -118,6 +118,29 @@ def test_block_plan_invariants():
     perm = np.random.default_rng(1).permutation(n ** 3)
     st2 = fehip.block_plan_stats(n ** 3, P ** 3, N, off.reshape(n ** 3, -1)[perm].ravel(), 16)
     assert st2["boundary_nodes"] < 0.6 * st2["element_nodes"]
+    # Structured hexes need at most 8 colours (2x2x2 parity); every L entry is written.
+    assert st["max_colors"] <= 8 and st2["max_colors"] <= 8
+    assert st["covers_all"] == 1 and st["rows"] == N - st["interior_nodes"]
+
+
+def test_block_plan_node_cap_and_coverage():
+    n, P = 5, 3
+    N = (n * (P - 1) + 1) ** 3
+    off = fehip.box_offsets(n, n, n, P, 1)
+    free = fehip.block_plan_stats(n ** 3, P ** 3, N, off, 16)
+    capped = fehip.block_plan_stats(n ** 3, P ** 3, N, off, 16, max_nodes=60)
+    assert free["max_nodes"] > 60 >= capped["max_nodes"]
+    assert capped["blocks"] > free["blocks"]
+    # An unreferenced trailing L entry is not written by the block passes.
+    assert fehip.block_plan_stats(n ** 3, P ** 3, N + 1, off, 16)["covers_all"] == 0
+    # Interleaved components (node stride 2, comp_stride 1) still cover the L-vector.
+    off2 = fehip.box_offsets(n, n, n, P, 2)
+    st = fehip.block_plan_stats(n ** 3, P ** 3, 2 * N, off2, 8, num_comp=2, comp_stride=1)
+    assert st["covers_all"] == 1
+    # A single-element cap is allowed; below one element's nodes is refused.
+    assert fehip.block_plan_stats(n ** 3, P ** 3, N, off, 16, max_nodes=P ** 3)["blocks"] == n ** 3
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.block_plan_stats(n ** 3, P ** 3, N, off, 16, max_nodes=P ** 3 - 1)
 
 
 def test_restriction_create_validation_before_any_device_work():

@@ -3,8 +3,11 @@
 * Unfused path: the reference's known answers (tests/test-operator.cpp:256-340) and
   bit equality with the oracle (same arithmetic, same order).
 * Fused mass kernel: with exact arithmetic and the deterministic transpose-map
-  scatter it equals the oracle's operator_apply bit for bit; with FMA and/or the
-  atomic scatter it agrees within 1e-13 relative (the FP64 tolerance).
+  scatter it equals the oracle's operator_apply bit for bit; with FMA and/or any
+  other scatter algorithm (element atomics, block atomics, block partial sums)
+  it agrees within 1e-13 relative (the FP64 tolerance), and the deterministic
+  algorithms repeat bit for bit.
+* The autotuner times every valid algorithm and its choice is the one applied.
 * At the 64^3, P=5, Q=6 size, size-independent properties: 1^T M 1 = volume = 1,
   symmetry, deterministic reproducibility, atomic vs deterministic agreement.
 """
@@ -19,6 +22,8 @@ from oracle_lib import oracle
 pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-13
+ALGS = (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_ATOMIC, fehip.FUSED_BLOCK_PARTIAL)
+DET_ALGS = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL)
 
 
 def mesh1d_qdata(h0, h1):
@@ -168,18 +173,70 @@ def test_fused_mass_matches_oracle(gpu, n, P, Q, nc):
     p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC, exact=1)
     p.op.apply(d_u, d_v)
     assert d_v.to_host().tobytes() == want.tobytes(), rel_err(d_v.to_host(), want)
-    # elems_per_block 0: block-accumulated kernel for the atomic mode; 1: the
-    # one-atomic-per-element-node kernel (first compiled packing).
-    for scatter, elems in ((fehip.SCATTER_ATOMIC, 0), (fehip.SCATTER_ATOMIC, 1), (fehip.SCATTER_DETERMINISTIC, 0)):
-        p.op.set_fused_options(scatter, exact=0, elems_per_block=elems)
+    # Every scatter algorithm, FP64 within 1e-13 and FP32 within single-precision
+    # rounding; outputs start as NaN, so an entry the apply fails to write shows.
+    d_u32 = fehip.DeviceArray.from_host(u.astype(np.float32))
+    nan64, nan32 = np.full(p.l_size, np.nan), np.full(p.l_size, np.nan, np.float32)
+    for alg in ALGS:
+        det = alg in DET_ALGS
+        p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC if det else fehip.SCATTER_ATOMIC, exact=0)
+        p.op.set_fused_algorithm(alg)
+        assert p.op.fused_choice()[0] == alg
+        d_v = fehip.DeviceArray.from_host(nan64)
         p.op.apply(d_u, d_v)
-        assert rel_err(d_v.to_host(), want) < FP64_RTOL, (scatter, elems)
-    # FP32 apply: within single-precision rounding of the FP64 result.
-    d_u32, d_v32 = fehip.DeviceArray.from_host(u.astype(np.float32)), fehip.DeviceArray(p.l_size, np.float32)
-    for scatter, elems in ((fehip.SCATTER_ATOMIC, 0), (fehip.SCATTER_ATOMIC, 1), (fehip.SCATTER_DETERMINISTIC, 0)):
-        p.op.set_fused_options(scatter, exact=0, elems_per_block=elems)
+        got = d_v.to_host()
+        assert rel_err(got, want) < FP64_RTOL, alg
+        if det:
+            d_v = fehip.DeviceArray.from_host(nan64)
+            p.op.apply(d_u, d_v)
+            assert d_v.to_host().tobytes() == got.tobytes(), alg
+        d_v32 = fehip.DeviceArray.from_host(nan32)
         p.op.apply(d_u32, d_v32)
-        assert rel_err(d_v32.to_host().astype(np.float64), want) < 2e-6, (scatter, elems)
+        assert rel_err(d_v32.to_host().astype(np.float64), want) < 2e-6, alg
+    # The element kernels through elems_per_block (first compiled packing).
+    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
+    p.op.set_fused_algorithm(fehip.FUSED_ELEM_ATOMIC)
+    ne = p.op.fused_choice()[1]
+    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0, elems_per_block=ne)
+    assert p.op.fused_choice() == (fehip.FUSED_ELEM_ATOMIC, ne)
+    p.op.apply(d_u, d_v)
+    assert rel_err(d_v.to_host(), want) < FP64_RTOL
+    # A deterministic scatter refuses the atomic algorithms.
+    p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC, exact=0)
+    p.op.set_fused_algorithm(fehip.FUSED_BLOCK_ATOMIC)
+    with pytest.raises(fehip.InvalidArgument):
+        p.op.apply(d_u, d_v)
+    p.op.set_fused_algorithm(fehip.FUSED_AUTO)
+
+
+def test_autotune_records_fastest_valid_choice(gpu):
+    print("autotune", flush=True)
+    p = MassProblem(8, 4, 5, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+    u = fehip.rng_uniform(3, p.l_size)
+    want = oracle_problem(p).apply(u)
+    d_u, d_v = fehip.DeviceArray.from_host(u), fehip.DeviceArray(p.l_size)
+    fehip.tune_table_clear()
+    try:
+        p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
+        (alg, ne, ms), rows = p.op.autotune(reps=3)
+        assert {r[0] for r in rows} == set(ALGS)
+        assert ms == min(r[2] for r in rows) and (alg, ne, ms) in rows
+        assert (4, 5, 1, 0, 0, alg, ne) in [t[:7] for t in fehip.tune_table()]
+        assert p.op.fused_choice() == (alg, ne)
+        p.op.apply(d_u, d_v)
+        assert rel_err(d_v.to_host(), want) < FP64_RTOL
+        # Deterministic mode times only the deterministic algorithms.
+        p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC, exact=0)
+        (alg2, ne2, _), rows2 = p.op.autotune(fp32=True, reps=2)
+        assert {r[0] for r in rows2} == set(DET_ALGS)
+        assert p.op.fused_choice(fp32=True) == (alg2, ne2)
+        # A table row set by hand steers automatic choice.
+        fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_ELEM_TMAP, 23)
+        assert p.op.fused_choice() == (fehip.FUSED_ELEM_TMAP, 23)
+        with pytest.raises(fehip.InvalidArgument):
+            fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_BLOCK_ATOMIC, 16)
+    finally:
+        fehip.tune_table_clear()
 
 
 def test_unfused_generic_path_matches_fused(gpu):
@@ -220,9 +277,12 @@ def test_fused_c2_properties(gpu):
     p.op.apply(d_w, d_Mw)
     a, b = w @ first, u @ d_Mw.to_host()
     assert abs(a - b) <= 1e-12 * abs(a)
-    p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
-    p.op.apply(d_u, d_Mu)
-    assert rel_err(d_Mu.to_host(), first) < FP64_RTOL
+    for alg in ALGS:
+        det = alg in DET_ALGS
+        p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC if det else fehip.SCATTER_ATOMIC, exact=0)
+        p.op.set_fused_algorithm(alg)
+        p.op.apply(d_u, d_Mu)
+        assert rel_err(d_Mu.to_host(), first) < FP64_RTOL, alg
     p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC, exact=1)
     p.op.apply(d_u, d_Mu)
     assert rel_err(d_Mu.to_host(), first) < FP64_RTOL

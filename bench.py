@@ -57,17 +57,15 @@ def time_applies(fehip, op, d_u, d_v, steps, warmup, stream):
     return total_ms / steps, k_ms / max(n, 1), aux_ms / max(n, 1)
 
 
-def traffic_from_profile():
-    """HBM bytes per fused-kernel launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if present."""
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if it holds
+    that kernel."""
     path = os.path.join(ROOT, "profiles", "fused_mass_p5q6_64cube_pmc.json")
     if not os.path.exists(path):
         return None
-    try:
-        with open(path) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
-    except Exception:
-        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch", {}).get(kernel)
 
 
 def cpu_baseline(problem, slab_layers=8, reps=3):
@@ -162,6 +160,21 @@ def main():
     E, P3 = p.num_elem, P ** 3
     variants = {}
     D, A = fehip.SCATTER_DETERMINISTIC, fehip.SCATTER_ATOMIC
+    # The autotuner times every compiled algorithm and packing on this operator;
+    # each variant below then runs at its algorithm's fastest packing.
+    tuning = {}
+    if not args.variants:
+        for tag, scatter, fp32 in (("fp64", A, False), ("fp32", D, True)):
+            p.op.set_fused_options(scatter, exact=0)
+            best_t, rows = p.op.autotune(fp32=fp32, reps=10)
+            tuning[tag] = {"best": {"alg": fehip.FUSED_NAMES[best_t[0]], "elems": best_t[1], "ms": best_t[2]},
+                           "candidates": [{"alg": fehip.FUSED_NAMES[r[0]], "elems": r[1], "ms": r[2]} for r in rows]}
+            log("autotune", tag, json.dumps(tuning[tag]["best"]))
+
+    def tuned_elems(tag, alg):
+        rows = [c for c in tuning.get(tag, {}).get("candidates", []) if c["alg"] == fehip.FUSED_NAMES[alg]]
+        return min(rows, key=lambda c: c["ms"])["elems"] if rows else 0
+
     runs = (("fp64_block_partial", np.float64, D, 0, fehip.FUSED_BLOCK_PARTIAL),
             ("fp64_elem_tmap", np.float64, D, 0, fehip.FUSED_ELEM_TMAP),
             ("fp64_block_atomic", np.float64, A, 0, fehip.FUSED_BLOCK_ATOMIC),
@@ -180,13 +193,18 @@ def main():
             continue
         d_u = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L).astype(dt))
         d_v = fehip.DeviceArray(L, dt)
-        p.op.set_fused_options(scatter, exact=exact)
-        p.op.set_fused_algorithm(alg)
         fp32 = dt == np.float32
-        kernel_names[label] = "%s<%d,%d,1,%s,%d>" % (
-            {0: "fused_mass_fast_kernel", 1: "fused_mass_fast_kernel", 2: "fused_mass_block_kernel",
-             3: "fused_mass_part_kernel"}[alg] if not exact else "fused_mass_exact_kernel",
-            P, Q, "float" if fp32 else "double", p.op.fused_choice(fp32)[1])
+        p.op.set_fused_options(scatter, exact=exact)
+        if not exact:
+            p.op.set_fused_algorithm(alg, tuned_elems("fp32" if fp32 else "fp64", alg))
+        ne = p.op.fused_choice(fp32)[1]
+        tname = "float" if fp32 else "double"
+        if alg in (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP):
+            kernel_names[label] = "fused_mass_%s_kernel<%d, %d, 1, %s, %d, %s>" % (
+                "exact" if exact else "fast", P, Q, tname, ne, "true" if alg == fehip.FUSED_ELEM_TMAP else "false")
+        else:
+            kernel_names[label] = "fused_mass_%s_kernel<%d, %d, 1, %s, %d>" % (
+                "part" if alg == fehip.FUSED_BLOCK_PARTIAL else "block", P, Q, tname, ne)
         steps = args.steps if label.startswith("fp64") else max(args.steps // 2, 5)
         m, k, a = time_applies(fehip, p.op, d_u, d_v, steps, args.warmup, stream)
         s = 4 if fp32 else 8
@@ -267,13 +285,14 @@ def main():
                                "%d DoF, %s scatter" % (L, best_name.split("_", 1)[1].replace("_", " ")),
                    "num_elem": p.num_elem, "P": P, "Q": Q, "num_comp": 1, "l_size": L},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(best["kernel"]),
                      "kernel": best["kernel"],
                      "kernel_ms": k_ms, "algorithmic_bytes_per_launch": best["kernel_bytes"],
                      "apply_algorithmic_bytes": bytes64, "apply_frac": best["apply_frac_hbm"],
                      "other_stages_ms": best["other_stages_ms"]},
         "cpu_baseline": cpu,
         "variants": variants,
+        "tuning": tuning,
         "config5": big,
         "solve_c3": solve,
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "cu": info.cu_count},

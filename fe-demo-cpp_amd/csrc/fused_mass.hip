@@ -74,24 +74,70 @@ __global__ __launch_bounds__(kThreads) void qdata_block_kernel(size_t n, int Q3,
   dst[idx] = e >= 0 ? static_cast<T>(src[static_cast<size_t>(e) * Q3 + q]) : T(0);
 }
 
-// Second pass of the partial-sum scatter: v[node + c*cs] = sum of the node's
-// block partials, added from 0 in ascending block order.
+// Second passes of the deterministic scatters: for each row m (an L-node),
+// v[node + c*cs] = sum over the row's entries k of vals[c*cl + idx[k]], added
+// from 0 in ascending k. The transpose-map pass (entries t = i + e*P^3 in
+// ascending t) reproduces the CPU scatter's order; the partial-sum pass adds a
+// node's block partials in ascending block order. rows == nullptr means row m
+// is node m. Each thread takes kRows rows
+// (kThreads apart, so lanes stay coalesced) and issues their loads together.
+constexpr int kRows = 4, kChunk = 4;
 template <class T>
-__global__ __launch_bounds__(kThreads) void partial_sum_kernel(int nrows, int NC, int cs, const int* __restrict__ rows,
-                                                               const int* __restrict__ ptr,
-                                                               const int* __restrict__ part,
-                                                               const T* __restrict__ partials, size_t pcl,
-                                                               T* __restrict__ v) {
-  const int m = blockIdx.x * kThreads + threadIdx.x;
-  if (m >= nrows) return;
-  const int b = ptr[m], e = ptr[m + 1];
-  const size_t node = static_cast<size_t>(rows[m]);
-  for (int c = 0; c < NC; ++c) {
-    const T* pc = partials + static_cast<size_t>(c) * pcl;
-    T sum = T(0);
-    for (int k = b; k < e; ++k) sum += pc[part[k]];
-    v[node + static_cast<size_t>(c) * cs] = sum;
+__global__ __launch_bounds__(kThreads) void row_sum_kernel(int nrows, int NC, int cs, const int* __restrict__ rows,
+                                                           const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                           const T* __restrict__ vals, size_t cl,
+                                                           T* __restrict__ v) {
+  const int base = blockIdx.x * (kThreads * kRows) + threadIdx.x;
+  int rb[kRows], re[kRows], node[kRows];
+  int longest = 0;
+#pragma unroll
+  for (int j = 0; j < kRows; ++j) {
+    const int m = base + j * kThreads;
+    const bool ok = m < nrows;
+    rb[j] = ok ? ptr[m] : 0;
+    re[j] = ok ? ptr[m + 1] : 0;
+    node[j] = ok ? (rows ? rows[m] : m) : -1;
+    longest = max(longest, re[j] - rb[j]);
   }
+  for (int c = 0; c < NC; ++c) {
+    const T* vc = vals + static_cast<size_t>(c) * cl;
+    T acc[kRows];
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) acc[j] = T(0);
+    for (int off = 0; off < longest; off += kChunk) {
+      int id[kRows][kChunk];
+      T val[kRows][kChunk];
+#pragma unroll
+      for (int j = 0; j < kRows; ++j)
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) {
+          const int k = rb[j] + off + q;
+          id[j][q] = k < re[j] ? idx[k] : -1;
+        }
+#pragma unroll
+      for (int j = 0; j < kRows; ++j)
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) val[j][q] = id[j][q] >= 0 ? vc[id[j][q]] : T(0);
+#pragma unroll
+      for (int j = 0; j < kRows; ++j)
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q)
+          if (id[j][q] >= 0) acc[j] += val[j][q];
+    }
+#pragma unroll
+    for (int j = 0; j < kRows; ++j)
+      if (node[j] >= 0) v[static_cast<size_t>(node[j]) + static_cast<size_t>(c) * cs] = acc[j];
+  }
+}
+
+template <class T>
+void launch_row_sum(int nrows, int NC, int cs, const int* rows, const int* ptr, const int* idx, const T* vals,
+                    size_t cl, T* v, hipStream_t s) {
+  if (nrows <= 0) return;
+  const size_t per = static_cast<size_t>(kThreads) * kRows;
+  row_sum_kernel<T><<<static_cast<unsigned>((nrows + per - 1) / per), kThreads, 0, s>>>(nrows, NC, cs, rows, ptr,
+                                                                                       idx, vals, cl, v);
+  HIP_CHECK_LAUNCH();
 }
 
 std::map<FusedKey, int>& budget_registry() {
@@ -143,21 +189,20 @@ std::map<TuneKey, TuneEntry>& tune_table() {
 
 namespace {
 
-// Default packing of an algorithm: for element kernels the variant wasting the
-// fewest threads of its last wavefront (ties: fewer elements); for block
-// kernels the first compiled block size.
+// Default packing of an algorithm: the variant keeping the most elements
+// resident per CU when the CU holds 32 wavefronts (whole workgroups only);
+// ties go to more elements per workgroup. LDS and registers are not modelled:
+// the autotuner measures.
 int default_elems(int P, int Q, int NC, int fp32, int alg, int exact) {
   auto v = fused_elem_variants(P, Q, NC, fp32, alg, exact);
   if (v.empty()) return 0;
-  if (alg_blocked(alg)) return v.front();
-  int best = v.front();
-  double best_waste = 1.0;
   const int tpe = std::max({P * P, P * Q, Q * Q});
+  int best = 0, best_res = -1;
   for (int ne : v) {
-    const int thr = ne * tpe, padded = (thr + 63) / 64 * 64;
-    const double waste = 1.0 - static_cast<double>(thr) / padded;
-    if (waste < best_waste - 1e-9 || (waste < best_waste + 1e-9 && ne < best)) {
-      best_waste = waste;
+    const int waves = (ne * tpe + 63) / 64;
+    const int resident = (32 / waves) * ne;
+    if (resident > best_res || (resident == best_res && ne > best)) {
+      best_res = resident;
       best = ne;
     }
   }
@@ -219,6 +264,48 @@ DevPlan& ensure_plan(FusedMass& f, int NB, int budget, bool part, hipStream_t s)
     d.covers_all = r.covers_all;
   }
   return d;
+}
+
+// Transpose map of the referenced nodes (host build, once): the element kernel
+// writes every element output to Y, and this pass sums each node's outputs.
+void ensure_tmap(FusedMass& f) {
+  if (f.tm_built) return;
+  const std::vector<int>& off = f.rstr->h_offsets;
+  const size_t nt = off.size();
+  std::vector<int> mult(static_cast<size_t>(f.l_size) + 1, 0);
+  for (size_t t = 0; t < nt; ++t) ++mult[off[t]];
+  std::vector<int> row_of(f.l_size, -1), rows, ptr{0};
+  for (int l = 0; l < f.l_size; ++l) {
+    if (mult[l] >= 1) {
+      row_of[l] = static_cast<int>(rows.size());
+      rows.push_back(l);
+      ptr.push_back(ptr.back() + mult[l]);
+    }
+  }
+  std::vector<int> idx(static_cast<size_t>(ptr.back())), fill(ptr.begin(), ptr.end() - 1);
+  for (size_t t = 0; t < nt; ++t)  // ascending t within each row
+    if (row_of[off[t]] >= 0) idx[fill[row_of[off[t]]]++] = static_cast<int>(t);
+  // Written entries: every referenced node, for every component.
+  std::vector<char> hit(f.l_size, 0);
+  bool ok = true;
+  for (int l = 0; l < f.l_size && ok; ++l) {
+    if (!mult[l]) continue;
+    for (int c = 0; c < f.NC && ok; ++c) {
+      const long long g = static_cast<long long>(l) + static_cast<long long>(c) * f.comp_stride;
+      ok = g < f.l_size;
+      if (ok) hit[g] = 1;
+    }
+  }
+  for (int l = 0; l < f.l_size && ok; ++l) ok = hit[l] != 0;
+  f.tm_nrows = static_cast<int>(rows.size());
+  bool identity = true;
+  for (size_t m = 0; m < rows.size() && identity; ++m) identity = rows[m] == static_cast<int>(m);
+  if (identity) rows.clear();  // row m is node m: the pass reads no node ids
+  f.tm_rows.upload(rows);
+  f.tm_ptr.upload(ptr);
+  f.tm_idx.upload(idx);
+  f.tm_covers_all = ok;
+  f.tm_built = true;
 }
 
 template <class T>
@@ -305,6 +392,7 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
   } else {
     (void)elem_qdata<T>(f, s);
     if (k.alg == kAlgElemTmap) {
+      ensure_tmap(f);
       DevBuf<T>& Y = elem_outputs<T>(f);
       const size_t n = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P * f.NC;
       if (Y.size() != n) Y.allocate(n);
@@ -346,24 +434,24 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
     if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     L(a, s);
     if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-    if (k.alg == kAlgBlockPart && d.nrows > 0) {
-      partial_sum_kernel<T><<<grid_for(static_cast<size_t>(d.nrows), kThreads), kThreads, 0, s>>>(
-          d.nrows, f.NC, f.comp_stride, d.rows.data(), d.row_ptr.data(), d.row_part.data(),
-          static_cast<const T*>(a.partials), a.part_comp_len, v);
-      HIP_CHECK_LAUNCH();
-    }
+    if (k.alg == kAlgBlockPart)
+      launch_row_sum<T>(d.nrows, f.NC, f.comp_stride, d.rows.data(), d.row_ptr.data(), d.row_part.data(),
+                        static_cast<const T*>(a.partials), a.part_comp_len, v, s);
   } else {
     a.qdata = elem_qdata<T>(f, s).data();
     a.num_elem = f.num_elem;
     a.offsets = f.rstr->offsets.data();
     a.flagged_offsets = f.flagged_offsets.data();
     a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+    a.v = v;
     if (k.alg == kAlgElemTmap) {
       a.out = elem_outputs<T>(f).data();
+      if (!f.tm_covers_all) zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
       if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
       L(a, s);
       if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-      csr_scatter<T>(*f.rstr, elem_outputs<T>(f).data(), v, true, s);
+      launch_row_sum<T>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows.data(), f.tm_ptr.data(), f.tm_idx.data(),
+                        elem_outputs<T>(f).data(), a.y_comp_len, v, s);
     } else {
       zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
       a.out = v;

@@ -108,6 +108,11 @@ struct FusedMass {
   std::map<std::pair<int, int>, std::unique_ptr<DevPlan>> plans;  // by (NB, node budget)
   DevBuf<double> Y64;                 // element outputs of the transpose-map algorithm (lazy)
   DevBuf<float> Y32;
+  // Transpose map over the referenced nodes: rows ascending, entries
+  // t = i + e*P^3 ascending (the CPU scatter's order).
+  bool tm_built = false, tm_covers_all = false;
+  int tm_nrows = 0;
+  DevBuf<int> tm_rows, tm_ptr, tm_idx;
   int scatter = kScatterAtomic;  // requested: atomic (any order) or deterministic (reproducible)
   int exact = 0;                 // requested: the reference's arithmetic (bit-equal with deterministic)
   FusedChoice choice;            // explicit algorithm, or automatic

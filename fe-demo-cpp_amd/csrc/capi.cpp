@@ -12,6 +12,7 @@
 #include "basis.hpp"
 #include "basis_setup.hpp"
 #include "block_plan.hpp"
+#include "server.hpp"
 #include "common.hpp"
 #include "mesh.hpp"
 #include "operator.hpp"
@@ -139,21 +140,25 @@ int fem_hip_malloc(void** ptr, size_t bytes) {
   return guard([&] {
     need(ptr, "ptr");
     *ptr = nullptr;
+    SharedCapture lk;
     if (bytes) HIP_CHECK(hipMalloc(ptr, bytes));
   });
 }
 int fem_hip_free(void* ptr) {
   return guard([&] {
+    SharedCapture lk;
     if (ptr) HIP_CHECK(hipFree(ptr));
   });
 }
 int fem_hip_memcpy_htod(void* dst, const void* src, size_t bytes) {
   return guard([&] {
+    SharedCapture lk;
     if (bytes) HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   });
 }
 int fem_hip_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
   return guard([&] {
+    SharedCapture lk;
     if (bytes) HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   });
 }
@@ -554,6 +559,7 @@ int fem_hip_tune_table_get(int* rows7, double* ms, long long* num_elem, int max_
     need(n_rows, "n_rows");
     require(max_rows >= 0 && (max_rows == 0 || rows7), "tune_table_get: bad rows buffer");
     int i = 0;
+    std::lock_guard<std::mutex> lk(tune_mutex());
     for (const auto& kv : tune_table()) {
       if (i < max_rows) {
         const TuneKey& k = kv.first;
@@ -582,11 +588,15 @@ int fem_hip_tune_table_set(const int* r, double ms, long long num_elem) {
     e.elems = r[6];
     e.ms = ms;
     e.num_elem = num_elem;
+    std::lock_guard<std::mutex> lk(tune_mutex());
     tune_table()[TuneKey{r[0], r[1], r[2], r[3], r[4]}] = e;
   });
 }
 int fem_hip_tune_table_clear(void) {
-  return guard([&] { tune_table().clear(); });
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    tune_table().clear();
+  });
 }
 int fem_hip_operator_mass_diagonal(fem_hip_operator h, double* diag, size_t len, void* s) {
   return guard([&] {
@@ -771,3 +781,105 @@ int fem_hip_mass_solve(fem_hip_operator h, const double* b, double* x, const fem
 }
 
 }  // extern "C"
+
+/* ---- serving loop ---- */
+struct fem_hip_server_s {
+  femhip::MassServer* s = nullptr;
+  ~fem_hip_server_s() { femhip::server_destroy(s); }
+};
+
+namespace {
+femhip::ServeRequest to_req(const fem_hip_serve_request& r) {
+  femhip::ServeRequest q;
+  q.n = r.n;
+  q.P = r.P;
+  q.Q = r.Q;
+  q.num_comp = r.num_comp;
+  q.mode = r.mode;
+  q.jitter = r.jitter;
+  q.rtol = r.rtol;
+  q.mesh_seed = r.mesh_seed;
+  q.rhs_seed = r.rhs_seed;
+  return q;
+}
+}  // namespace
+
+int fem_hip_server_create(const fem_hip_server_options* o, fem_hip_server* out) {
+  return guard([&] {
+    need(out, "out");
+    femhip::ServerOptions so;
+    if (o) {
+      so.max_active = o->max_active;
+      so.workers = o->workers;
+      so.hbm_fraction = o->hbm_fraction;
+      so.batch_iters = o->batch_iters;
+      so.autotune = o->autotune != 0;
+      so.keep_x = o->keep_x != 0;
+    }
+    auto h = std::make_unique<fem_hip_server_s>();
+    h->s = femhip::server_create(so);
+    *out = h.release();
+  });
+}
+int fem_hip_server_submit(fem_hip_server s, const fem_hip_serve_request* req, long long* id) {
+  return guard([&] {
+    need(s, "server");
+    need(req, "request");
+    need(id, "id");
+    *id = femhip::server_submit(*s->s, to_req(*req));
+  });
+}
+int fem_hip_server_wait(fem_hip_server s, long long id, double timeout_ms, fem_hip_serve_result* res, double* x,
+                        size_t x_len) {
+  int rc = 0;
+  const int g = guard([&] {
+    need(s, "server");
+    need(res, "result");
+    femhip::ServeResult r;
+    if (!femhip::server_wait(*s->s, id, timeout_ms, &r)) {
+      rc = FEM_HIP_ERR_NOT_READY;
+      return;
+    }
+    res->status = r.status;
+    res->iterations = r.iterations;
+    res->outer = r.outer;
+    res->rel_residual = r.rel_residual;
+    res->queue_ms = r.queue_ms;
+    res->setup_ms = r.setup_ms;
+    res->solve_ms = r.solve_ms;
+    res->latency_ms = r.latency_ms;
+    res->l_size = r.l_size;
+    res->est_bytes = r.est_bytes;
+    if (r.status != 0) {
+      rc = r.status;
+      throw std::runtime_error(r.error);
+    }
+    if (x) {
+      require(x_len == r.x.size(), "server_wait: x_len must equal the solution length (keep_x on)");
+      std::memcpy(x, r.x.data(), x_len * sizeof(double));
+    }
+  });
+  if (rc == FEM_HIP_ERR_NOT_READY) g_last_error = "server_wait: timed out";
+  return rc ? rc : g;
+}
+int fem_hip_server_stats(fem_hip_server s, long long* stats) {
+  return guard([&] {
+    need(s, "server");
+    need(stats, "stats");
+    const femhip::ServerStats st = femhip::server_stats(*s->s);
+    const long long v[9] = {st.submitted, st.completed, st.failed, st.active, st.queued, st.active_peak,
+                            static_cast<long long>(st.budget), static_cast<long long>(st.used),
+                            static_cast<long long>(st.used_peak)};
+    std::memcpy(stats, v, sizeof(v));
+  });
+}
+int fem_hip_server_destroy(fem_hip_server s) {
+  return guard([&] { delete s; });
+}
+int fem_hip_serve_request_bytes(const fem_hip_serve_request* req, size_t* bytes) {
+  return guard([&] {
+    need(req, "request");
+    need(bytes, "bytes");
+    *bytes = femhip::serve_request_bytes(to_req(*req));
+  });
+}

@@ -9,6 +9,8 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <condition_variable>
+#include <mutex>
 #include <vector>
 
 namespace femhip {
@@ -41,6 +43,39 @@ inline void require(bool ok, const std::string& msg) {
   if (!ok) throw std::invalid_argument(msg);
 }
 
+// Shared/exclusive lock between hipGraph stream capture (exclusive) and the
+// device calls that must not run while another thread records a graph
+// (shared): synchronous copies on the legacy stream, allocation and release,
+// stream creation and destruction. Writers (captures) are preferred: once one
+// waits, new readers wait too. Shared holds are re-entrant per thread, and the
+// recording thread passes its own shared requests.
+class CaptureLock {
+ public:
+  void lock_shared();
+  void unlock_shared();
+  void lock();
+  void unlock();
+
+ private:
+  std::mutex m_;
+  std::condition_variable cv_;
+  int readers_ = 0, writers_waiting_ = 0;
+  bool writer_ = false;
+};
+CaptureLock& capture_lock();
+struct SharedCapture {
+  SharedCapture() { capture_lock().lock_shared(); }
+  ~SharedCapture() { capture_lock().unlock_shared(); }
+  SharedCapture(const SharedCapture&) = delete;
+  SharedCapture& operator=(const SharedCapture&) = delete;
+};
+struct ExclusiveCapture {
+  ExclusiveCapture() { capture_lock().lock(); }
+  ~ExclusiveCapture() { capture_lock().unlock(); }
+  ExclusiveCapture(const ExclusiveCapture&) = delete;
+  ExclusiveCapture& operator=(const ExclusiveCapture&) = delete;
+};
+
 // Owner of one device allocation (move-only), the counterpart of the reference's
 // DeviceArray (include/cuda/device-array.hpp:18-95) without the synchronous copies.
 template <class T>
@@ -68,16 +103,19 @@ class DevBuf {
   void allocate(size_t n) {
     release();
     if (n == 0) return;
+    SharedCapture lk;
     HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)));
     n_ = n;
   }
   void upload(const T* host, size_t n) {
     allocate(n);
+    SharedCapture lk;
     if (n) HIP_CHECK(hipMemcpy(p_, host, n * sizeof(T), hipMemcpyHostToDevice));
   }
   void upload(const std::vector<T>& host) { upload(host.data(), host.size()); }
   std::vector<T> download() const {
     std::vector<T> h(n_);
+    SharedCapture lk;
     if (n_) HIP_CHECK(hipMemcpy(h.data(), p_, n_ * sizeof(T), hipMemcpyDeviceToHost));
     return h;
   }
@@ -86,7 +124,10 @@ class DevBuf {
   size_t size() const { return n_; }
   bool empty() const { return n_ == 0; }
   void release() noexcept {
-    if (p_) (void)hipFree(p_);
+    if (p_) {
+      SharedCapture lk;
+      (void)hipFree(p_);
+    }
     p_ = nullptr;
     n_ = 0;
   }
@@ -102,10 +143,14 @@ template <class T>
 class StreamTmp {
  public:
   StreamTmp(size_t n, hipStream_t s) : s_(s), n_(n) {
+    SharedCapture lk;
     if (n) HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&p_), n * sizeof(T), s));
   }
   ~StreamTmp() {
-    if (p_) (void)hipFreeAsync(p_, s_);
+    if (p_) {
+      SharedCapture lk;
+      (void)hipFreeAsync(p_, s_);
+    }
   }
   StreamTmp(const StreamTmp&) = delete;
   StreamTmp& operator=(const StreamTmp&) = delete;

@@ -22,6 +22,42 @@ __global__ void probe_kernel(int* sink) {
 
 }  // namespace
 
+namespace {
+thread_local int t_shared = 0;       // shared holds of this thread (re-entrant)
+thread_local bool t_exclusive = false;  // this thread is recording a graph
+}  // namespace
+
+void CaptureLock::lock_shared() {
+  if (t_exclusive || t_shared++ > 0) return;
+  std::unique_lock<std::mutex> lk(m_);
+  cv_.wait(lk, [&] { return !writer_ && writers_waiting_ == 0; });
+  ++readers_;
+}
+void CaptureLock::unlock_shared() {
+  if (t_exclusive || --t_shared > 0) return;
+  std::lock_guard<std::mutex> lk(m_);
+  if (--readers_ == 0) cv_.notify_all();
+}
+void CaptureLock::lock() {
+  if (t_shared > 0) throw std::logic_error("capture lock: recording while holding device setup");
+  std::unique_lock<std::mutex> lk(m_);
+  ++writers_waiting_;
+  cv_.wait(lk, [&] { return !writer_ && readers_ == 0; });
+  --writers_waiting_;
+  writer_ = true;
+  t_exclusive = true;
+}
+void CaptureLock::unlock() {
+  std::lock_guard<std::mutex> lk(m_);
+  writer_ = false;
+  t_exclusive = false;
+  cv_.notify_all();
+}
+CaptureLock& capture_lock() {
+  static CaptureLock l;
+  return l;
+}
+
 void zero_fill(void* p, size_t bytes, hipStream_t s) {
   if (bytes == 0) return;
   // Allocations are at least 256-byte aligned; callers pass whole arrays.

@@ -36,13 +36,12 @@ __global__ __launch_bounds__(kThreads) void to_f32_kernel(size_t n, const double
 }
 
 // Element diagonal of the mass matrix, one thread per (element, node):
-// d[e][i] = sum_q qdata[e][q] * B2[qz][iz] * B2[qy][iy] * B2[qx][ix], added into
-// diag[offsets[e][i] + c*comp_stride] for every component.
-__global__ __launch_bounds__(kThreads) void mass_diag_kernel(int E, int P, int Q, int NC, int cs,
-                                                             const int* __restrict__ offsets,
-                                                             const double* __restrict__ qpriv,
+// d[e][i] = sum_q qdata[e][q] * B2[qz][iz] * B2[qy][iy] * B2[qx][ix], written to
+// Yd[t] (t = i + e*P^3); the transpose-map pass then sums each node's entries in
+// ascending t, so the diagonal is the same bit for bit on every run.
+__global__ __launch_bounds__(kThreads) void mass_diag_kernel(int E, int P, int Q, const double* __restrict__ qpriv,
                                                              const double* __restrict__ B2,
-                                                             double* __restrict__ diag) {
+                                                             double* __restrict__ Yd) {
   const int P3 = P * P * P, Q2 = Q * Q;
   const size_t idx = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (idx >= static_cast<size_t>(E) * P3) return;
@@ -59,8 +58,7 @@ __global__ __launch_bounds__(kThreads) void mass_diag_kernel(int E, int P, int Q
       sum += bxy * s;
     }
   }
-  const size_t o = static_cast<size_t>(offsets[idx]);
-  for (int c = 0; c < NC; ++c) unsafeAtomicAdd(&diag[o + static_cast<size_t>(c) * cs], sum);
+  Yd[idx] = sum;
 }
 
 // Block-slot copy of the private qdata: dst[slot][q] = src[elem[slot]][q] (0 for padding slots).
@@ -185,6 +183,11 @@ bool fused_mass_supported(int P, int Q, int NC) {
 std::map<TuneKey, TuneEntry>& tune_table() {
   static std::map<TuneKey, TuneEntry> t;
   return t;
+}
+
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
 }
 
 namespace {
@@ -483,12 +486,15 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
     require(k.elems > 0 && find(k), "fused mass: the chosen algorithm is not compiled for this shape");
     return k;
   }
-  auto it = tune_table().find(TuneKey{f.P, f.Q, f.NC, fp32, det ? 1 : 0});
-  if (it != tune_table().end()) {
-    FusedKey t = k;
-    t.alg = it->second.alg;
-    t.elems = it->second.elems;
-    if (t.alg >= 0 && valid_for(f, t.alg) && find(t)) return t;
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_table().find(TuneKey{f.P, f.Q, f.NC, fp32, det ? 1 : 0});
+    if (it != tune_table().end()) {
+      FusedKey t = k;
+      t.alg = it->second.alg;
+      t.elems = it->second.elems;
+      if (t.alg >= 0 && valid_for(f, t.alg) && find(t)) return t;
+    }
   }
   // Default order: partial-sum blocks, then (deterministic) transpose map or (atomic) block atomics, then elements.
   const int order_det[] = {kAlgBlockPart, kAlgElemTmap};
@@ -636,7 +642,10 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
   f.choice = saved;
   f.timing = saved_timing;
   require(best.alg >= 0, "autotune: no candidate");
-  tune_table()[TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0}] = best;
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    tune_table()[TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0}] = best;
+  }
   // Drop block plans the winner does not use (each holds a slot-ordered qdata copy).
   const FusedKey wk{f.P, f.Q, f.NC, fp32, best.alg, 0, best.elems};
   for (auto it = f.plans.begin(); it != f.plans.end();) {
@@ -658,14 +667,18 @@ TuneEntry fused_autotune(FusedMass& f, int fp32, int reps, hipStream_t s, std::v
 }
 
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s) {
-  zero_fill(diag, static_cast<size_t>(f.l_size) * sizeof(double), s);
+  ensure_tmap(f);
+  if (!f.tm_covers_all) zero_fill(diag, static_cast<size_t>(f.l_size) * sizeof(double), s);
   StreamTmp<double> B2(f.Bsq.size(), s);
   HIP_CHECK(hipMemcpyAsync(B2.data(), f.Bsq.data(), f.Bsq.size() * sizeof(double), hipMemcpyHostToDevice, s));
   const size_t n = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
-  mass_diag_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(f.num_elem, f.P, f.Q, f.NC, f.comp_stride,
-                                                               f.rstr->offsets.data(), f.qdata64.data(), B2.data(),
-                                                               diag);
+  StreamTmp<double> Yd(n, s);
+  mass_diag_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(f.num_elem, f.P, f.Q, f.qdata64.data(), B2.data(),
+                                                               Yd.data());
   HIP_CHECK_LAUNCH();
+  // Every component gets the same diagonal (value stride 0 across components).
+  launch_row_sum<double>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows.data(), f.tm_ptr.data(), f.tm_idx.data(),
+                         Yd.data(), 0, diag, s);
   // The host copy of Bsq must outlive the async copy.
   HIP_CHECK(hipStreamSynchronize(s));
 }

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <tuple>
 #include <vector>
@@ -159,7 +160,8 @@ struct TuneEntry {
   double ms = 0.0;  // measured time per apply
   long long num_elem = 0;  // size it was measured on
 };
-std::map<TuneKey, TuneEntry>& tune_table();
+std::map<TuneKey, TuneEntry>& tune_table();  // guard every access with tune_mutex()
+std::mutex& tune_mutex();
 // Times every compiled candidate valid for the operator's requested scatter and
 // precision on its own data (reps applies each, after one warm-up), records the
 // fastest in the table and returns it. Restores the operator's choice (automatic).

@@ -127,6 +127,7 @@ void operator_set_field(Operator& op, const std::string& name, std::shared_ptr<c
   f.vkind = vkind;
   if (vkind == kVectorPassive) {
     f.passive.allocate(passive_len);
+    SharedCapture lk;
     HIP_CHECK(hipMemcpy(f.passive.data(), passive, passive_len * sizeof(double),
                         passive_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   }

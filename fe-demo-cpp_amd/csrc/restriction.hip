@@ -273,6 +273,7 @@ Restriction* restriction_clone(const Restriction& src) {
     r->identity_nodes = src.identity_nodes;
     auto dup = [](const DevBuf<int>& from, DevBuf<int>& to) {
       to.allocate(from.size());
+      SharedCapture lk;
       if (from.size()) HIP_CHECK(hipMemcpy(to.data(), from.data(), from.size() * sizeof(int), hipMemcpyDeviceToDevice));
     };
     dup(src.offsets, r->offsets);

@@ -1,0 +1,475 @@
+// Continuous-batching serving loop for mass solves (see server.hpp).
+#include "server.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <stdexcept>
+#include <thread>
+
+#include "basis.hpp"
+#include "basis_setup.hpp"
+#include "fused_mass.hpp"
+#include "mesh.hpp"
+#include "operator.hpp"
+#include "qfunction.hpp"
+#include "restriction.hpp"
+#include "solver.hpp"
+
+namespace femhip {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+long long ipow3(long long v) { return v * v * v; }
+
+// The C ABI's status code of a failure.
+int status_of(const std::exception& e) {
+  if (dynamic_cast<const std::invalid_argument*>(&e)) return 1;
+  if (dynamic_cast<const DeviceError*>(&e)) return 2;
+  if (dynamic_cast<const Unsupported*>(&e)) return 4;
+  if (dynamic_cast<const OutOfMemory*>(&e)) return 5;
+  return 3;
+}
+
+}  // namespace
+
+size_t serve_request_bytes(const ServeRequest& r) {
+  const long long E = ipow3(r.n), P3 = ipow3(r.P), Q3 = ipow3(r.Q), nc = r.num_comp;
+  const long long N = ipow3(static_cast<long long>(r.n) * (r.P - 1) + 1), L = nc * N, nv = ipow3(r.n + 1);
+  long long b = 0;
+  b += E * P3 * (4 + 4 + 4 + 4);       // offsets, flagged copy, transpose map, multi-row map
+  b += E * P3 * (2 + 12 * nc);         // block-local indices, element outputs (FP64 + FP32)
+  b += E * Q3 * (8 + 8 + 8 + 8 + 4 + 4);  // qdata: built, operator copy, private, block order, FP32 copies
+  b += E * Q3 * 8 * 13;                // build_mass temporaries (dx gradients, weights, output)
+  b += E * (8 * 3 * 8 + 500 * nc);     // mesh E-vector, block node lists and partials
+  b += N * 16 + L * (8 * 10 + 4 * 7);  // node maps; solver vectors (FP64 and FP32)
+  b += nv * 3 * 8 * 2 + E * 8 * 4 * 2;  // coordinates, mesh restriction
+  b += 16ll << 20;                      // small buffers, allocator granularity
+  return static_cast<size_t>(b);
+}
+
+namespace {
+
+struct Job {
+  long long id = 0;
+  ServeRequest req;
+  size_t bytes = 0;
+  Clock::time_point t_submit, t_admit, t_ready, t_done;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Operator> op;
+  DevBuf<double> b, x;
+  MassSolverState* state = nullptr;
+  SolveJob* solve = nullptr;
+  ServeResult res;
+  ~Job() {
+    solve_job_destroy(solve);
+    mass_solver_state_destroy(state);
+    op.reset();
+    b.release();
+    x.release();
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      SharedCapture lk;
+      (void)hipStreamDestroy(stream);
+    }
+  }
+};
+
+void validate(const ServeRequest& r) {
+  require(r.n >= 1 && r.n <= 512, "serve: n must be in [1, 512]");
+  require(r.P >= 2 && r.Q >= 1 && r.num_comp >= 1 && r.num_comp <= 3, "serve: bad P, Q or num_comp");
+  require(fused_mass_supported(r.P, r.Q, r.num_comp), "serve: (P, Q, num_comp) has no compiled fused kernel");
+  require(r.mode == 0 || r.mode == 1, "serve: mode must be 0 (FP64) or 1 (mixed)");
+  require(r.rtol > 0 && r.jitter >= 0 && r.jitter < 0.5, "serve: bad rtol or jitter");
+  require(ipow3(static_cast<long long>(r.n) * (r.P - 1) + 1) * r.num_comp < (1ll << 31), "serve: L-vector too large");
+}
+
+// Mesh, restrictions, qdata (build_mass on the device) and the fused mass
+// operator of a request, as tests/test-operator.cpp:17-24,75-112 set them up.
+std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s) {
+  const int n = r.n, P = r.P, Q = r.Q, nc = r.num_comp;
+  const int E = n * n * n, P3 = P * P * P, Q3 = Q * Q * Q, nv = (n + 1) * (n + 1) * (n + 1);
+  const int N = static_cast<int>(ipow3(static_cast<long long>(n) * (P - 1) + 1));
+  std::vector<int> off(static_cast<size_t>(E) * P3), moff(static_cast<size_t>(E) * 8);
+  box_offsets(n, n, n, P, 1, off.data());
+  box_offsets(n, n, n, 2, 1, moff.data());
+  std::vector<double> coords(static_cast<size_t>(3) * nv);
+  box_coords(n, r.jitter, r.mesh_seed, coords.data());
+
+  std::shared_ptr<const Restriction> rstr(
+      restriction_create_offset(E, P3, nc, N, nc * N, off.data(), kScatterDeterministic));
+  std::shared_ptr<const Restriction> mrstr(restriction_create_offset(E, 8, 3, nv, 3 * nv, moff.data(), kScatterAtomic));
+  std::shared_ptr<const Restriction> qrstr(restriction_create_strided(E, Q3, 1, E * Q3, nullptr));
+  std::shared_ptr<const Basis> basis(basis_create(make_h1_lagrange(3, nc, P, Q, 0)));
+  std::shared_ptr<const Basis> mbasis(basis_create(make_h1_lagrange(3, 3, 2, Q, 0)));
+
+  QFunctionDev bq;
+  bq.kind = kQfBuildMass;
+  qfunction_add_input(bq, "dx", 9, kEvalGrad);
+  qfunction_add_input(bq, "weights", 1, kEvalWeight);
+  qfunction_add_output(bq, "qdata", 1, kEvalNone);
+  const int dim = 3;
+  bq.ctx.resize(sizeof(int));
+  std::memcpy(bq.ctx.data(), &dim, sizeof(int));
+  std::unique_ptr<Operator> bop(operator_create(bq));
+  operator_set_field(*bop, "dx", mrstr, mbasis, kVectorActive, nullptr, 0, false);
+  operator_set_field(*bop, "weights", nullptr, mbasis, kVectorNone, nullptr, 0, false);
+  operator_set_field(*bop, "qdata", qrstr, nullptr, kVectorActive, nullptr, 0, false);
+  DevBuf<double> d_coords, qdata(static_cast<size_t>(E) * Q3);
+  d_coords.upload(coords);
+  operator_apply(*bop, d_coords.data(), d_coords.size(), qdata.data(), qdata.size(), s);
+
+  QFunctionDev mq;
+  mq.kind = nc == 1 ? kQfApplyMass : kQfApplyMassVector;
+  if (nc > 1) {
+    mq.ctx.resize(sizeof(int));
+    std::memcpy(mq.ctx.data(), &nc, sizeof(int));
+  }
+  qfunction_add_input(mq, "u", nc, kEvalInterp);
+  qfunction_add_input(mq, "qdata", 1, kEvalNone);
+  qfunction_add_output(mq, "v", nc, kEvalInterp);
+  std::unique_ptr<Operator> op(operator_create(mq));
+  HIP_CHECK(hipStreamSynchronize(s));  // qdata is complete before the passive copy
+  operator_set_field(*op, "u", rstr, basis, kVectorActive, nullptr, 0, false);
+  operator_set_field(*op, "qdata", qrstr, nullptr, kVectorPassive, qdata.data(), qdata.size(), true);
+  operator_set_field(*op, "v", rstr, basis, kVectorActive, nullptr, 0, false);
+  op->opt_scatter = kScatterDeterministic;  // the same bits alone and in any mix
+  return op;
+}
+
+}  // namespace
+
+class MassServer {
+ public:
+  explicit MassServer(const ServerOptions& o) : opt_(o) {
+    require(o.max_active >= 1 && o.workers >= 1 && o.batch_iters >= 1, "server: bad options");
+    require(o.hbm_fraction > 0 && o.hbm_fraction <= 0.95, "server: hbm_fraction must be in (0, 0.95]");
+    size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    budget_ = static_cast<size_t>(static_cast<double>(free_b) * o.hbm_fraction);
+    HIP_CHECK(hipGetDevice(&device_));
+    scheduler_ = std::thread([this] { scheduler_loop(); });
+    for (int i = 0; i < o.workers; ++i) workers_.emplace_back([this] { worker_loop(); });
+  }
+
+  ~MassServer() {
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      stopping_ = true;
+      cv_.notify_all();
+      work_cv_.notify_all();
+    }
+    scheduler_.join();
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      workers_stop_ = true;
+      work_cv_.notify_all();
+    }
+    for (auto& t : workers_) t.join();
+  }
+
+  long long submit(const ServeRequest& r) {
+    auto j = std::make_unique<Job>();
+    j->req = r;
+    j->t_submit = Clock::now();
+    std::unique_lock<std::mutex> lk(mu_);
+    require(!stopping_, "server: stopping");
+    j->id = next_id_++;
+    ++stats_.submitted;
+    const long long id = j->id;
+    try {
+      validate(r);
+      j->bytes = serve_request_bytes(r);
+      if (j->bytes > budget_) throw OutOfMemory("serve: the request needs more than the server's HBM budget");
+    } catch (const std::exception& e) {
+      j->res.status = status_of(e);
+      j->res.error = e.what();
+      finish_locked(std::move(j));
+      return id;
+    }
+    pending_.push_back(std::move(j));
+    cv_.notify_all();
+    return id;
+  }
+
+  bool wait(long long id, double timeout_ms, ServeResult* out) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto ready = [&] { return done_.count(id) > 0; };
+    if (timeout_ms < 0) {
+      done_cv_.wait(lk, ready);
+    } else if (!done_cv_.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) {
+      return false;
+    }
+    auto it = done_.find(id);
+    *out = std::move(it->second);
+    done_.erase(it);
+    return true;
+  }
+
+  ServerStats stats() {
+    std::unique_lock<std::mutex> lk(mu_);
+    ServerStats s = stats_;
+    s.active = static_cast<long long>(in_setup_ + running_.size() + in_teardown_);
+    s.queued = static_cast<long long>(pending_.size());
+    s.budget = budget_;
+    s.used = used_;
+    return s;
+  }
+
+ private:
+  // Admission: FIFO; the head waits for a slot and for its bytes to fit the budget.
+  void admit_locked() {
+    while (!pending_.empty()) {
+      const long long active = static_cast<long long>(in_setup_ + running_.size() + in_teardown_);
+      Job& h = *pending_.front();
+      if (active >= opt_.max_active || used_ + h.bytes > budget_) break;
+      used_ += h.bytes;
+      stats_.used_peak = std::max(stats_.used_peak, used_);
+      h.t_admit = Clock::now();
+      ++in_setup_;
+      stats_.active_peak = std::max(stats_.active_peak, active + 1);
+      setup_q_.push_back(std::move(pending_.front()));
+      pending_.pop_front();
+      work_cv_.notify_one();
+    }
+  }
+
+  void finish_locked(std::unique_ptr<Job> j) {
+    j->t_done = Clock::now();
+    ServeResult& r = j->res;
+    r.latency_ms = ms_between(j->t_submit, j->t_done);
+    if (r.status == 0) {
+      ++stats_.completed;
+    } else {
+      ++stats_.failed;
+    }
+    r.est_bytes = j->bytes;
+    done_[j->id] = std::move(r);
+    done_cv_.notify_all();
+  }
+
+  void scheduler_loop() {
+    (void)hipSetDevice(device_);
+    std::vector<std::unique_ptr<Job>> mine;  // running jobs owned by this thread
+    while (true) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (auto& j : ready_q_) mine.push_back(std::move(j));
+        ready_q_.clear();
+        admit_locked();
+        const bool idle = mine.empty();
+        if (idle) {
+          if (stopping_ && pending_.empty() && in_setup_ == 0) break;
+          cv_.wait_for(lk, std::chrono::milliseconds(2));
+          continue;
+        }
+      }
+      // Advance every running solve; hand finished ones to the workers.
+      bool progressed = false;
+      for (auto& j : mine) {
+        try {
+          if (solve_job_step(*j->solve)) {
+            progressed = true;
+            const SolveStats& st = solve_job_stats(*j->solve);
+            j->res.iterations = st.iterations;
+            j->res.outer = st.outer;
+            j->res.rel_residual = st.rel_residual;
+            j->res.solve_ms = ms_between(j->t_ready, Clock::now());
+            std::unique_lock<std::mutex> lk(mu_);
+            running_.erase(j.get());
+            ++in_teardown_;
+            teardown_q_.push_back(std::move(j));
+            work_cv_.notify_one();
+          }
+        } catch (const std::exception& e) {
+          j->res.status = status_of(e);
+          j->res.error = e.what();
+          std::unique_lock<std::mutex> lk(mu_);
+          running_.erase(j.get());
+          ++in_teardown_;
+          teardown_q_.push_back(std::move(j));
+          work_cv_.notify_one();
+          progressed = true;
+        }
+      }
+      mine.erase(std::remove(mine.begin(), mine.end(), nullptr), mine.end());
+      if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  }
+
+  void worker_loop() {
+    (void)hipSetDevice(device_);
+    while (true) {
+      std::unique_ptr<Job> j;
+      bool setup = false;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        work_cv_.wait(lk, [&] { return !teardown_q_.empty() || !setup_q_.empty() || workers_stop_; });
+        if (!teardown_q_.empty()) {  // teardown first: it frees budget
+          j = std::move(teardown_q_.front());
+          teardown_q_.pop_front();
+        } else if (!setup_q_.empty()) {
+          j = std::move(setup_q_.front());
+          setup_q_.pop_front();
+          setup = true;
+        } else {
+          return;  // workers_stop_ and nothing left
+        }
+      }
+      if (setup) {
+        do_setup(std::move(j));
+      } else {
+        do_teardown(std::move(j));
+      }
+    }
+  }
+
+  void do_setup(std::unique_ptr<Job> j) {
+    bool ok = true;
+    try {
+      {
+        {
+          SharedCapture lk;
+          HIP_CHECK(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+        }
+        j->op = build_problem(j->req, j->stream);
+        const size_t L = static_cast<size_t>(j->op->active_in_l_size);
+        j->res.l_size = static_cast<long long>(L);
+        std::vector<double> rhs(L);
+        rng_uniform(j->req.rhs_seed, L, -1.0, 1.0, rhs.data());
+        j->b.upload(rhs);
+        j->x.allocate(L);
+        FusedMass& f = operator_fused_mass(*j->op, j->stream);
+        if (opt_.autotune) maybe_tune(f, j->req.mode == 1, j->stream);
+        if (!tuned_shape(f, j->req.mode == 1)) {
+          // No measured choice: the transpose-map algorithm, whose setup is a
+          // counting sort (the block algorithms cluster elements on the host first).
+          j->op->opt_alg = kAlgElemTmap;
+          f.choice.alg = kAlgElemTmap;
+        }
+        SolveOptions so;
+        so.mode = j->req.mode;
+        so.rtol = j->req.rtol;
+        so.max_iter = 2000;
+        so.max_outer = 30;
+        so.batch = opt_.batch_iters;
+        j->solve = solve_job_create(*j->op, j->state, j->b.data(), j->x.data(), so, j->stream);
+        HIP_CHECK(hipStreamSynchronize(j->stream));
+      }
+      solve_job_prepare_graphs(*j->solve);  // exclusive: other threads' allocations and copies wait
+    } catch (const std::exception& e) {
+      ok = false;
+      j->res.status = status_of(e);
+      j->res.error = e.what();
+    }
+    std::unique_lock<std::mutex> lk(mu_);
+    --in_setup_;
+    j->t_ready = Clock::now();
+    j->res.queue_ms = ms_between(j->t_submit, j->t_admit);
+    j->res.setup_ms = ms_between(j->t_admit, j->t_ready);
+    if (ok) {  // the scheduler thread takes it over
+      running_.insert(j.get());
+      ready_q_.push_back(std::move(j));
+      cv_.notify_all();
+    } else {
+      ++in_teardown_;
+      teardown_q_.push_back(std::move(j));
+      work_cv_.notify_one();
+    }
+  }
+
+  void do_teardown(std::unique_ptr<Job> j) {
+    try {
+      if (j->res.status == 0 && opt_.keep_x) {
+        j->res.x.resize(j->x.size());
+        if (!j->res.x.empty())
+          HIP_CHECK(hipMemcpyAsync(j->res.x.data(), j->x.data(), j->res.x.size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, j->stream));
+        HIP_CHECK(hipStreamSynchronize(j->stream));
+      }
+    } catch (const std::exception& e) {
+      j->res.status = 3;
+      j->res.error = e.what();
+    }
+    const size_t bytes = j->bytes;
+    ServeResult res = std::move(j->res);
+    j->res = ServeResult();
+    const long long id = j->id;
+    const Clock::time_point t_submit = j->t_submit;
+    j.reset();  // frees the device state
+    auto done = std::make_unique<Job>();
+    done->id = id;
+    done->bytes = bytes;
+    done->t_submit = t_submit;
+    done->res = std::move(res);
+    std::unique_lock<std::mutex> lk(mu_);
+    --in_teardown_;
+    used_ -= bytes;
+    finish_locked(std::move(done));
+    cv_.notify_all();
+  }
+
+  // Whether the tuning table holds every precision this solve applies in.
+  static bool tuned_shape(const FusedMass& f, bool mixed) {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    for (int fp32 : {0, 1}) {
+      if (fp32 && !mixed) continue;
+      if (!tune_table().count(TuneKey{f.P, f.Q, f.NC, fp32, 1})) return false;
+    }
+    return true;
+  }
+
+  // Autotunes (P, Q, num_comp, precision, deterministic) once per server.
+  void maybe_tune(FusedMass& f, bool mixed, hipStream_t s) {
+    for (int fp32 : {0, 1}) {
+      if (fp32 && !mixed) continue;
+      const TuneKey k{f.P, f.Q, f.NC, fp32, 1};
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        if (tuned_.count(k)) continue;
+        tuned_.insert(k);
+      }
+      fused_autotune(f, fp32, 5, s);
+    }
+  }
+
+  ServerOptions opt_;
+  int device_ = 0;
+  size_t budget_ = 0, used_ = 0;
+  std::mutex mu_;
+  std::condition_variable cv_, work_cv_, done_cv_;
+  bool stopping_ = false, workers_stop_ = false;
+  long long next_id_ = 1;
+  size_t in_setup_ = 0, in_teardown_ = 0;
+  std::deque<std::unique_ptr<Job>> pending_, setup_q_, teardown_q_;
+  std::vector<std::unique_ptr<Job>> ready_q_;
+  std::set<Job*> running_;
+  std::map<long long, ServeResult> done_;
+  std::set<TuneKey> tuned_;
+  ServerStats stats_;
+  std::thread scheduler_;
+  std::vector<std::thread> workers_;
+};
+
+MassServer* server_create(const ServerOptions& o) { return new MassServer(o); }
+long long server_submit(MassServer& s, const ServeRequest& r) { return s.submit(r); }
+bool server_wait(MassServer& s, long long id, double timeout_ms, ServeResult* out) {
+  return s.wait(id, timeout_ms, out);
+}
+ServerStats server_stats(MassServer& s) { return s.stats(); }
+void server_destroy(MassServer* s) { delete s; }
+
+}  // namespace femhip

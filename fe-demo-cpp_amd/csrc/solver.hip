@@ -8,9 +8,15 @@
 //
 // All CG scalars live on the device (no host round trip per iteration); a
 // batch of iterations is captured once into a hipGraph and replayed, and the
-// host reads the convergence flag once per batch. Once the residual test
-// passes, the flag freezes x, so extra iterations of a batch change nothing.
-// Reductions use a fixed grid and a fixed tree, so a solve is reproducible.
+// host reads the convergence flag once per batch (an asynchronous copy into
+// pinned memory followed by an event). Once the residual test passes, the flag
+// freezes x, so extra iterations of a batch change nothing. Reductions use a
+// fixed grid and a fixed tree, so a solve is reproducible.
+//
+// A solve is a state machine (SolveJob) advanced by solve_job_step, which never
+// blocks: it looks at the event of the work it enqueued last and, when that has
+// finished, enqueues the next piece. mass_solve drives one job to the end; the
+// serving loop drives many, each on its own stream.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -198,6 +204,8 @@ struct Pcg {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   int batch = 0;
+  const T* b_bound = nullptr;  // the b and x the graph was captured with
+  T* x_bound = nullptr;
 
   explicit Pcg(size_t n_) : n(n_), r(n_), z(n_), p(n_), Ap(n_), dinv(n_), part(2 * kRedBlocks), sc(kNSCAL) {}
   ~Pcg() {
@@ -216,24 +224,24 @@ struct Pcg {
     cg_p_kernel<T><<<kRedBlocks, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
   }
 
-  // Solve A x = b from x = 0 until ||r|| <= rtol ||b|| or max_iter iterations.
-  // The graph of `batch_iters` iterations is captured on first use (x and b
-  // addresses must stay fixed between solves on this instance).
-  template <class Apply>
-  int solve(Apply&& apply, const T* b, T* x, double rtol, int max_iter, int batch_iters, hipStream_t s) {
+  // x = 0 and the first residual, for A x = b with ||r|| <= rtol ||b|| as the goal.
+  void init(const T* b, T* x, double rtol, hipStream_t s) {
     cg_init_kernel<T><<<kRedBlocks, kThreads, 0, s>>>(n, b, dinv.data(), x, r.data(), z.data(), p.data(),
                                                       part.data());
     cg_init_scalars_kernel<<<1, 1, 0, s>>>(part.data(), sc.data(), rtol * rtol);
     HIP_CHECK_LAUNCH();
-    if (!exec || batch != batch_iters) {
+  }
+
+  // Records the graph of `batch_iters` iterations for this b and x (if it is not
+  // recorded yet). No other thread does device setup or teardown meanwhile.
+  template <class Apply>
+  void prepare_graph(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, hipStream_t s) {
+    if (!exec || batch != batch_iters || b_bound != b || x_bound != x) {
       if (exec) HIP_CHECK(hipGraphExecDestroy(exec));
       if (graph) HIP_CHECK(hipGraphDestroy(graph));
       exec = nullptr;
       graph = nullptr;
-      // One apply outside the capture, so lazily made operator state (element
-      // output buffers, FP32 copies) exists before recording.
-      apply(p.data(), Ap.data(), s);
-      HIP_CHECK(hipStreamSynchronize(s));
+      ExclusiveCapture lk;
       HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
         for (int k = 0; k < batch_iters; ++k) iteration(apply, x, max_iter, s);
@@ -246,17 +254,17 @@ struct Pcg {
       HIP_CHECK(hipStreamEndCapture(s, &graph));
       HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       batch = batch_iters;
+      b_bound = b;
+      x_bound = x;
     }
-    double host_sc[kNSCAL];
-    for (int done_iters = 0; done_iters < max_iter; done_iters += batch_iters) {
-      HIP_CHECK(hipGraphLaunch(exec, s));
-      HIP_CHECK(hipMemcpyAsync(host_sc, sc.data(), sizeof(host_sc), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      if (host_sc[kDONE] != 0.0) break;
-    }
-    HIP_CHECK(hipMemcpyAsync(host_sc, sc.data(), sizeof(host_sc), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    return static_cast<int>(host_sc[kITERS]);
+  }
+
+  // Enqueues `batch_iters` iterations (one graph launch) and the copy of the scalars into host_sc.
+  template <class Apply>
+  void launch_batch(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, double* host_sc, hipStream_t s) {
+    prepare_graph(apply, b, x, max_iter, batch_iters, s);
+    HIP_CHECK(hipGraphLaunch(exec, s));
+    HIP_CHECK(hipMemcpyAsync(host_sc, sc.data(), kNSCAL * sizeof(double), hipMemcpyDeviceToHost, s));
   }
 };
 
@@ -268,31 +276,242 @@ struct MassSolverState {
   std::unique_ptr<Pcg<double>> pcg64;
   std::unique_ptr<Pcg<float>> pcg32;
   DevBuf<float> r32, d32;
-  const double* b_bound = nullptr;  // addresses the captured graphs were built with
-  double* x_bound = nullptr;
 };
 
 void mass_solver_state_destroy(MassSolverState* st) { delete st; }
 
-static double true_residual(Operator& op, FusedMass& f, MassSolverState& st, const double* b, const double* x,
-                            double bnorm, float* r32, hipStream_t s) {
-  fused_mass_apply64(f, x, st.Mx.data(), s);
-  residual_kernel<<<kRedBlocks, kThreads, 0, s>>>(st.n, b, st.Mx.data(), st.r64.data(), r32, st.part.data());
-  sum_to_scalar_kernel<<<1, 1, 0, s>>>(st.part.data(), st.scal.data());
+namespace {
+
+// Pinned host words the device reports into: the CG scalars, ||b||^2, ||b - Mx||^2.
+enum { kHostB = kNSCAL, kHostRes, kHostN };
+
+}  // namespace
+
+struct SolveJob {
+  enum Phase { kStart, kIter, kFinal, kCheck, kInner, kDone };
+  Operator* op = nullptr;
+  FusedMass* f = nullptr;
+  MassSolverState* st = nullptr;
+  const double* b = nullptr;
+  double* x = nullptr;
+  SolveOptions opt;
+  hipStream_t s = nullptr;
+  hipEvent_t ev = nullptr;
+  double* hs = nullptr;  // pinned [kHostN]
+  Phase phase = kStart;
+  bool in_flight = false;
+  int batch = 8, outer = 0, inner_total = 0;
+  double bnorm = 0.0;
+  SolveStats stats;
+  std::chrono::steady_clock::time_point t0;
+  ~SolveJob() {
+    if (ev) (void)hipEventDestroy(ev);
+    if (hs) {
+      SharedCapture lk;
+      (void)hipHostFree(hs);
+    }
+  }
+};
+
+namespace {
+
+// Enqueues r = b - M x (FP64 fused apply), ||r||^2 into hs[kHostRes] and, if
+// r32 is given, r rounded to FP32.
+void enqueue_residual(SolveJob& j, float* r32) {
+  MassSolverState& st = *j.st;
+  fused_mass_apply64(*j.f, j.x, st.Mx.data(), j.s);
+  residual_kernel<<<kRedBlocks, kThreads, 0, j.s>>>(st.n, j.b, st.Mx.data(), st.r64.data(), r32, st.part.data());
+  sum_to_scalar_kernel<<<1, 1, 0, j.s>>>(st.part.data(), st.scal.data());
   HIP_CHECK_LAUNCH();
-  double rr = 0.0;
-  HIP_CHECK(hipMemcpyAsync(&rr, st.scal.data(), sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  (void)op;
-  return bnorm > 0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+  HIP_CHECK(hipMemcpyAsync(j.hs + kHostRes, st.scal.data(), sizeof(double), hipMemcpyDeviceToHost, j.s));
+}
+
+void enqueue_bnorm(SolveJob& j) {
+  MassSolverState& st = *j.st;
+  dot_partial_kernel<double><<<kRedBlocks, kThreads, 0, j.s>>>(st.n, j.b, j.b, st.part.data());
+  sum_to_scalar_kernel<<<1, 1, 0, j.s>>>(st.part.data(), st.scal.data());
+  HIP_CHECK_LAUNCH();
+  HIP_CHECK(hipMemcpyAsync(j.hs + kHostB, st.scal.data(), sizeof(double), hipMemcpyDeviceToHost, j.s));
+}
+
+double rel_of(const SolveJob& j) {
+  const double rr = std::sqrt(std::max(j.hs[kHostRes], 0.0));
+  return j.bnorm > 0 ? rr / j.bnorm : rr;
+}
+
+// Enqueues the next piece of work for the current phase (the previous piece has finished).
+void advance(SolveJob& j) {
+  MassSolverState& st = *j.st;
+  FusedMass& f = *j.f;
+  auto apply64 = [&f](const double* in, double* out, hipStream_t ss) { fused_mass_apply64(f, in, out, ss); };
+  auto apply32 = [&f](const float* in, float* out, hipStream_t ss) { fused_mass_apply32(f, in, out, ss); };
+  switch (j.phase) {
+    case SolveJob::kStart:
+      j.t0 = std::chrono::steady_clock::now();
+      enqueue_bnorm(j);
+      if (j.opt.mode == 0) {
+        st.pcg64->init(j.b, j.x, j.opt.rtol, j.s);
+        st.pcg64->launch_batch(apply64, j.b, j.x, j.opt.max_iter, j.batch, j.hs, j.s);
+        j.phase = SolveJob::kIter;
+      } else {
+        zero_fill(j.x, st.n * sizeof(double), j.s);
+        enqueue_residual(j, st.r32.data());
+        j.phase = SolveJob::kCheck;
+      }
+      break;
+    case SolveJob::kIter:  // FP64 PCG batches
+      j.bnorm = std::sqrt(j.hs[kHostB]);
+      if (j.hs[kDONE] != 0.0) {
+        j.stats.iterations = static_cast<int>(j.hs[kITERS]);
+        enqueue_residual(j, nullptr);
+        j.phase = SolveJob::kFinal;
+      } else {
+        st.pcg64->launch_batch(apply64, j.b, j.x, j.opt.max_iter, j.batch, j.hs, j.s);
+      }
+      break;
+    case SolveJob::kFinal:
+      j.stats.rel_residual = rel_of(j);
+      j.phase = SolveJob::kDone;
+      break;
+    case SolveJob::kCheck: {  // mixed: FP64 residual of the current x is in hs
+      j.bnorm = std::sqrt(j.hs[kHostB]);
+      const double rel = rel_of(j);
+      if (rel <= j.opt.rtol || j.outer >= j.opt.max_outer) {
+        j.stats.rel_residual = rel;
+        j.stats.iterations = j.inner_total;
+        j.stats.outer = j.outer;
+        j.phase = SolveJob::kDone;
+      } else {
+        st.pcg32->init(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
+        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
+        j.phase = SolveJob::kInner;
+      }
+      break;
+    }
+    case SolveJob::kInner:
+      if (j.hs[kDONE] != 0.0) {
+        j.inner_total += static_cast<int>(j.hs[kITERS]);
+        add_correction_kernel<<<grid_for(st.n, kThreads), kThreads, 0, j.s>>>(st.n, st.d32.data(), j.x);
+        HIP_CHECK_LAUNCH();
+        ++j.outer;
+        enqueue_residual(j, st.r32.data());
+        j.phase = SolveJob::kCheck;
+      } else {
+        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
+      }
+      break;
+    case SolveJob::kDone:
+      return;
+  }
+  if (j.phase != SolveJob::kDone) {
+    HIP_CHECK(hipEventRecord(j.ev, j.s));
+    j.in_flight = true;
+  } else {
+    j.stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - j.t0).count();
+  }
+}
+
+}  // namespace
+
+SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* b, double* x, const SolveOptions& opt,
+                           hipStream_t s) {
+  require(s != nullptr, "solve job: needs a non-default stream (graph capture)");
+  require(opt.mode == 0 || opt.mode == 1, "mass_solve: unknown mode");
+  require(opt.rtol > 0 && opt.max_iter > 0, "mass_solve: rtol and max_iter must be positive");
+  require(b != nullptr && x != nullptr, "mass_solve: b and x must be device arrays");
+  FusedMass& f = operator_fused_mass(op, s);
+  const size_t n = static_cast<size_t>(f.l_size);
+  if (!state || state->n != n) {
+    delete state;
+    state = nullptr;
+    auto ns = std::make_unique<MassSolverState>();
+    ns->n = n;
+    ns->diag.allocate(n);
+    ns->Mx.allocate(n);
+    ns->r64.allocate(n);
+    ns->part.allocate(2 * kRedBlocks);
+    ns->scal.allocate(4);
+    fused_mass_diagonal(f, ns->diag.data(), s);
+    state = ns.release();
+  }
+  MassSolverState& st = *state;
+  // Everything the captured graphs use is made here, outside capture: the PCG
+  // vectors, the Jacobi inverse, and the operator's lazily made state (one apply).
+  if (opt.mode == 0 && !st.pcg64) {
+    st.pcg64 = std::make_unique<Pcg<double>>(n);
+    recip_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg64->dinv.data());
+    HIP_CHECK_LAUNCH();
+  }
+  if (opt.mode == 1 && !st.pcg32) {
+    st.pcg32 = std::make_unique<Pcg<float>>(n);
+    st.r32.allocate(n);
+    st.d32.allocate(n);
+    recip_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg32->dinv.data());
+    HIP_CHECK_LAUNCH();
+  }
+  fused_mass_apply64(f, x, st.Mx.data(), s);
+  if (opt.mode == 1) fused_mass_apply32(f, st.d32.data(), st.r32.data(), s);
+  auto j = std::make_unique<SolveJob>();
+  j->op = &op;
+  j->f = &f;
+  j->st = &st;
+  j->b = b;
+  j->x = x;
+  j->opt = opt;
+  j->s = s;
+  j->batch = opt.batch > 0 ? opt.batch : 8;
+  HIP_CHECK(hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));
+  {
+    SharedCapture lk;
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&j->hs), kHostN * sizeof(double), hipHostMallocDefault));
+  }
+  for (int i = 0; i < kHostN; ++i) j->hs[i] = 0.0;
+  return j.release();
+}
+
+void solve_job_prepare_graphs(SolveJob& j) {
+  MassSolverState& st = *j.st;
+  FusedMass& f = *j.f;
+  if (j.opt.mode == 0) {
+    auto apply64 = [&f](const double* in, double* out, hipStream_t ss) { fused_mass_apply64(f, in, out, ss); };
+    st.pcg64->prepare_graph(apply64, j.b, j.x, j.opt.max_iter, j.batch, j.s);
+  } else {
+    auto apply32 = [&f](const float* in, float* out, hipStream_t ss) { fused_mass_apply32(f, in, out, ss); };
+    st.pcg32->prepare_graph(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.s);
+  }
+}
+
+bool solve_job_step(SolveJob& j) {
+  if (j.phase == SolveJob::kDone) return true;
+  if (j.in_flight) {
+    const hipError_t e = hipEventQuery(j.ev);
+    if (e == hipErrorNotReady) return false;
+    HIP_CHECK(e);
+    j.in_flight = false;
+  }
+  advance(j);
+  return j.phase == SolveJob::kDone;
+}
+
+void solve_job_wait(SolveJob& j) {
+  while (j.phase != SolveJob::kDone) {
+    if (j.in_flight) {
+      HIP_CHECK(hipEventSynchronize(j.ev));
+      j.in_flight = false;
+    }
+    advance(j);
+  }
+}
+
+const SolveStats& solve_job_stats(const SolveJob& j) { return j.stats; }
+bool solve_job_done(const SolveJob& j) { return j.phase == SolveJob::kDone; }
+void solve_job_destroy(SolveJob* j) {
+  if (j && j->in_flight) (void)hipEventSynchronize(j->ev);
+  delete j;
 }
 
 void mass_solve(Operator& op, MassSolverState*& state, const double* b, double* x, const SolveOptions& opt,
                 SolveStats& stats, hipStream_t user_stream) {
-  FusedMass& f = operator_fused_mass(op, user_stream);
-  const size_t n = static_cast<size_t>(f.l_size);
-  require(opt.mode == 0 || opt.mode == 1, "mass_solve: unknown mode");
-  require(opt.rtol > 0 && opt.max_iter > 0, "mass_solve: rtol and max_iter must be positive");
   // Graph capture needs a non-default stream.
   hipStream_t s = user_stream;
   hipStream_t own = nullptr;
@@ -309,73 +528,9 @@ void mass_solve(Operator& op, MassSolverState*& state, const double* b, double* 
       }
     }
   } guard{own};
-
-  if (!state || state->n != n) {
-    delete state;
-    state = new MassSolverState();
-    state->n = n;
-    state->diag.allocate(n);
-    state->Mx.allocate(n);
-    state->r64.allocate(n);
-    state->part.allocate(2 * kRedBlocks);
-    state->scal.allocate(4);
-    fused_mass_diagonal(f, state->diag.data(), s);
-  }
-  MassSolverState& st = *state;
-  if (st.b_bound != b || st.x_bound != x) {
-    st.pcg64.reset();
-    st.pcg32.reset();
-    st.b_bound = b;
-    st.x_bound = x;
-  }
-  // ||b||
-  dot_partial_kernel<double><<<kRedBlocks, kThreads, 0, s>>>(n, b, b, st.part.data());
-  sum_to_scalar_kernel<<<1, 1, 0, s>>>(st.part.data(), st.scal.data());
-  double bb = 0.0;
-  HIP_CHECK(hipMemcpyAsync(&bb, st.scal.data(), sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  const double bnorm = std::sqrt(bb);
-
-  const auto t0 = std::chrono::steady_clock::now();
-  stats = SolveStats{};
-  const int batch = opt.batch > 0 ? opt.batch : 8;
-  if (opt.mode == 0) {
-    if (!st.pcg64) {
-      st.pcg64 = std::make_unique<Pcg<double>>(n);
-      recip_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg64->dinv.data());
-      HIP_CHECK_LAUNCH();
-    }
-    auto apply = [&](const double* in, double* out, hipStream_t ss) { fused_mass_apply64(f, in, out, ss); };
-    stats.iterations = st.pcg64->solve(apply, b, x, opt.rtol, opt.max_iter, batch, s);
-    stats.outer = 0;
-  } else {
-    if (!st.pcg32) {
-      st.pcg32 = std::make_unique<Pcg<float>>(n);
-      st.r32.allocate(n);
-      st.d32.allocate(n);
-      recip_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg32->dinv.data());
-      HIP_CHECK_LAUNCH();
-      // Make the FP32 qdata copy now, outside any graph capture.
-      fused_mass_apply32(f, st.d32.data(), st.r32.data(), s);
-    }
-    auto apply32 = [&](const float* in, float* out, hipStream_t ss) { fused_mass_apply32(f, in, out, ss); };
-    zero_fill(x, n * sizeof(double), s);
-    int outer = 0, inner_total = 0;
-    double rel = true_residual(op, f, st, b, x, bnorm, st.r32.data(), s);
-    while (rel > opt.rtol && outer < opt.max_outer) {
-      inner_total += st.pcg32->solve(apply32, st.r32.data(), st.d32.data(), opt.inner_rtol, opt.max_iter, batch, s);
-      add_correction_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.d32.data(), x);
-      HIP_CHECK_LAUNCH();
-      ++outer;
-      rel = true_residual(op, f, st, b, x, bnorm, st.r32.data(), s);
-    }
-    stats.iterations = inner_total;
-    stats.outer = outer;
-  }
-  HIP_CHECK(hipStreamSynchronize(s));
-  const auto t1 = std::chrono::steady_clock::now();
-  stats.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  stats.rel_residual = true_residual(op, f, st, b, x, bnorm, nullptr, s);
+  std::unique_ptr<SolveJob, void (*)(SolveJob*)> j(solve_job_create(op, state, b, x, opt, s), solve_job_destroy);
+  solve_job_wait(*j);
+  stats = j->stats;
 }
 
 }  // namespace femhip

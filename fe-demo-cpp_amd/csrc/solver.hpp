@@ -22,6 +22,22 @@ struct SolveStats {
 
 struct MassSolverState;
 void mass_solver_state_destroy(MassSolverState* st);
+
+// One solve as a non-blocking state machine on a stream of its own (not the
+// default stream): create prepares everything (may synchronize the stream),
+// step enqueues the next piece of work once the previous one has finished and
+// returns true when the solve is complete, wait drives it to the end.
+struct SolveJob;
+SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* b, double* x, const SolveOptions& opt,
+                           hipStream_t s);
+// Records the job's CG graphs now (else the first step does): under the
+// exclusive capture lock, so call it holding no shared one.
+void solve_job_prepare_graphs(SolveJob& j);
+bool solve_job_step(SolveJob& j);
+void solve_job_wait(SolveJob& j);
+bool solve_job_done(const SolveJob& j);
+const SolveStats& solve_job_stats(const SolveJob& j);
+void solve_job_destroy(SolveJob* j);
 // Solves M x = b on the fused mass operator. `state` caches the diagonal, work
 // vectors and captured graphs between solves on the same operator.
 void mass_solve(Operator& op, MassSolverState*& state, const double* b, double* x, const SolveOptions& opt,

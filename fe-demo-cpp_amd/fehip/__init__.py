@@ -27,6 +27,9 @@ QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR = 1, 2, 3
 SOLVE_FP64, SOLVE_MIXED = 0, 1
 
 
+ERR_NOT_READY = 6
+
+
 class FemHipError(RuntimeError):
     """A failed call (the reference's std::runtime_error)."""
 
@@ -145,6 +148,12 @@ SIGNATURES = {
     "fem_hip_tune_table_get": [_ip, ctypes.POINTER(_d), _llp, _i, _ip],
     "fem_hip_tune_table_set": [_ip, _d, ctypes.c_longlong],
     "fem_hip_tune_table_clear": [],
+    "fem_hip_server_create": [ctypes.c_void_p, ctypes.POINTER(_vp)],
+    "fem_hip_server_submit": [_vp, ctypes.c_void_p, _llp],
+    "fem_hip_server_wait": [_vp, ctypes.c_longlong, _d, ctypes.c_void_p, _vp, _sz],
+    "fem_hip_server_stats": [_vp, _llp],
+    "fem_hip_server_destroy": [_vp],
+    "fem_hip_serve_request_bytes": [ctypes.c_void_p, ctypes.POINTER(_sz)],
     "fem_hip_operator_mass_diagonal": [_vp, _vp, _sz, _vp],
     "fem_hip_operator_timing_begin": [_vp],
     "fem_hip_operator_timing_end": [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _ip],
@@ -626,3 +635,84 @@ def operator_create_mass(restriction, basis, d_qdata):
 
 def operator_apply(op, d_in, d_out, stream=None):
     op.apply(d_in, d_out, stream)
+
+
+# ---------------------------------------------------------------- serving loop
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("max_active", ctypes.c_int), ("workers", ctypes.c_int), ("hbm_fraction", ctypes.c_double),
+                ("batch_iters", ctypes.c_int), ("autotune", ctypes.c_int), ("keep_x", ctypes.c_int)]
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("P", ctypes.c_int), ("Q", ctypes.c_int), ("num_comp", ctypes.c_int),
+                ("mode", ctypes.c_int), ("jitter", ctypes.c_double), ("rtol", ctypes.c_double),
+                ("mesh_seed", ctypes.c_uint64), ("rhs_seed", ctypes.c_uint64)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("outer", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("queue_ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("l_size", ctypes.c_longlong),
+                ("est_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def serve_request(n, P, Q=None, num_comp=1, mode=SOLVE_FP64, jitter=0.1, rtol=1e-10, mesh_seed=0, rhs_seed=1):
+    return ServeRequest(n, P, P + 1 if Q is None else Q, num_comp, mode, jitter, rtol, mesh_seed, rhs_seed)
+
+
+def serve_l_size(req):
+    return req.num_comp * (req.n * (req.P - 1) + 1) ** 3
+
+
+def serve_request_bytes(req):
+    b = _sz()
+    call("fem_hip_serve_request_bytes", ctypes.byref(req), ctypes.byref(b))
+    return b.value
+
+
+class Server:
+    """Continuous-batching mass-solve server (fem_hip_server_*)."""
+
+    def __init__(self, max_active=16, workers=8, hbm_fraction=0.5, batch_iters=8, autotune=False, keep_x=True):
+        o = ServerOptions(max_active, workers, hbm_fraction, batch_iters, int(autotune), int(keep_x))
+        self.h = ctypes.c_void_p()
+        call("fem_hip_server_create", ctypes.byref(o), ctypes.byref(self.h))
+        self.keep_x = keep_x
+
+    def submit(self, req):
+        i = ctypes.c_longlong()
+        call("fem_hip_server_submit", self.h, ctypes.byref(req), ctypes.byref(i))
+        return i.value
+
+    def wait(self, rid, l_size=0, timeout_ms=-1.0):
+        """(result dict, solution or None) once request `rid` is done, None on a
+        timeout; raises if the request failed. l_size > 0 (server made with
+        keep_x) also returns the solution of that length."""
+        res = ServeResult()
+        x = np.empty(l_size) if l_size else None
+        rc = lib().fem_hip_server_wait(self.h, rid, float(timeout_ms), ctypes.byref(res),
+                                       x.ctypes.data_as(_vp) if x is not None else None, l_size)
+        if rc == ERR_NOT_READY:
+            return None
+        check(rc)
+        return res.as_dict(), x
+
+    def stats(self):
+        st = (ctypes.c_longlong * 9)()
+        call("fem_hip_server_stats", self.h, st)
+        keys = ["submitted", "completed", "failed", "active", "queued", "active_peak", "budget", "used", "used_peak"]
+        return dict(zip(keys, [int(v) for v in st]))
+
+    def close(self):
+        if self.h:
+            call("fem_hip_server_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -37,6 +37,7 @@ extern "C" {
 #define FEM_HIP_ERR_RUNTIME 3          /* any other std::runtime_error */
 #define FEM_HIP_ERR_UNSUPPORTED 4      /* e.g. a QFunction with no device implementation */
 #define FEM_HIP_ERR_OUT_OF_MEMORY 5    /* device allocation failed, or admission refused */
+#define FEM_HIP_ERR_NOT_READY 6        /* a wait timed out (the request is still in flight) */
 
 /* ---- vocabulary (include/transpose-mode.hpp:13, include/eval-mode.hpp:11) --- */
 #define FEM_HIP_NO_TRANSPOSE 0
@@ -293,6 +294,52 @@ typedef struct {
 } fem_hip_solve_stats;
 int fem_hip_mass_solve(fem_hip_operator op, const double* b, double* x, const fem_hip_solve_options* opt,
                        fem_hip_solve_stats* stats, void* stream);
+
+/* ---- serving loop ------------------------------------------------------------
+ * No counterpart in the reference (it solves nothing): continuous batching of
+ * mass-solve requests on one GPU. Requests are admitted while a slot is free and
+ * their estimated device bytes fit the budget (hbm_fraction of the HBM free when
+ * the server starts); they are set up on worker threads and their CG iterations
+ * (graph-captured batches, one stream per request) overlap on the device; a
+ * request arriving meanwhile starts as soon as a slot and memory free up. Every
+ * solve uses the deterministic scatter, so a request's solution is the same bit
+ * for bit whether it runs alone or in a mix (as long as the tuning table, which
+ * picks the algorithm, is the same). */
+typedef struct fem_hip_server_s* fem_hip_server;
+typedef struct {
+  int max_active;      /* requests set up or solving at once */
+  int workers;         /* host threads for setup and teardown */
+  double hbm_fraction; /* (0, 0.95] */
+  int batch_iters;     /* CG iterations per graph launch */
+  int autotune;        /* 1: tune each new (P, Q, num_comp, precision) on first sight */
+  int keep_x;          /* 1: keep solutions for fem_hip_server_wait */
+} fem_hip_server_options;
+typedef struct {
+  int n, P, Q, num_comp; /* n^3 hexes of the unit cube, order P, Q Gauss points */
+  int mode;              /* FEM_HIP_SOLVE_FP64 or FEM_HIP_SOLVE_MIXED */
+  double jitter, rtol;   /* vertex jitter (fraction of h), relative residual target */
+  uint64_t mesh_seed, rhs_seed; /* b ~ U[-1,1] from rhs_seed (splitmix64) */
+} fem_hip_serve_request;
+typedef struct {
+  int status; /* 0 or the FEM_HIP_ERR_* code of the failure */
+  int iterations, outer;
+  double rel_residual;
+  double queue_ms, setup_ms, solve_ms, latency_ms; /* submit->admit, admit->ready, solve, submit->done */
+  long long l_size;
+  size_t est_bytes;
+} fem_hip_serve_result;
+int fem_hip_server_create(const fem_hip_server_options* opt, fem_hip_server* out);
+int fem_hip_server_submit(fem_hip_server s, const fem_hip_serve_request* req, long long* id);
+/* Waits up to timeout_ms (< 0: forever); FEM_HIP_ERR_NOT_READY on timeout. On
+ * completion fills res and, if x is not NULL, copies the solution (x_len must
+ * equal res->l_size). Returns the request's own status; a request is reported once. */
+int fem_hip_server_wait(fem_hip_server s, long long id, double timeout_ms, fem_hip_serve_result* res, double* x,
+                        size_t x_len);
+/* stats[9] = {submitted, completed, failed, active, queued, active_peak, budget bytes, used bytes, used peak} */
+int fem_hip_server_stats(fem_hip_server s, long long* stats);
+/* Finishes every submitted request, then stops. */
+int fem_hip_server_destroy(fem_hip_server s);
+int fem_hip_serve_request_bytes(const fem_hip_serve_request* req, size_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -161,3 +161,13 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_serve_request_bytes_estimate():
+    """The serving admission estimate grows with the mesh, the order and the components."""
+    est = lambda *a: fehip.serve_request_bytes(fehip.serve_request(*a))
+    assert est(8, 3) < est(16, 3) < est(16, 5) < est(16, 5, 6, 3)
+    # At least the solver's FP64 work vectors and the qdata copies.
+    n, P, Q = 20, 4, 5
+    L = (n * (P - 1) + 1) ** 3
+    assert est(n, P, Q) > 10 * 8 * L + 4 * 8 * n ** 3 * Q ** 3

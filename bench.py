@@ -12,8 +12,10 @@ Printed (one JSON line): GDoF/s of the fastest FP64 apply variant, the fused
 kernel's HBM roofline (its algorithmic bytes per launch / its HIP-event duration),
 every other apply variant (atomic, block-accumulated, exact, FP32), the 48^3,
 P=8, Q=10, 3-component apply, the mixed-precision vs FP64 mass solves on the
-16^3, P=3, Q=4 mesh (p50/p99 latency), and the CPU oracle timed on a bounded slab
-of the headline workload.
+16^3, P=3, Q=4 mesh (p50/p99 latency), the serving workload (2000 mass-solve
+requests with Poisson arrivals through the continuous-batching server:
+throughput and p50/p99 latency), and the CPU oracle timed on a bounded slab of
+the headline workload.
 
 --variants and --calibrate-copy serve profiler runs: the first restricts the
 apply variants, the second adds K plain copies of one L-vector (a backend-strided
@@ -133,6 +135,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 48^3, P=8, Q=10, 3-component line")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=60.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--variants", default="", help="comma-separated subset of the apply variants")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="K",
                     help="also run K plain copies of one FP64 L-vector")
@@ -146,6 +151,7 @@ def main():
 
     info = fehip.device_info(0)
     log("device:", info.name.decode(), info.arch.decode(), "CUs", info.cu_count)
+    log("tuning table rows loaded:", fehip.load_default_tuning())
     stream = fehip.Stream()
     n, P, Q = args.n, 5, 6
     t0 = time.perf_counter()
@@ -261,6 +267,19 @@ def main():
     if not args.no_solve:
         solve = solve_latency(fehip, 20)
         log("solve", solve)
+    serve = None
+    if not args.no_serve:
+        # Config 4: mass-solve requests (n ~ U{4..40}, P ~ U{2..6}, Q = P+1,
+        # num_comp ~ U{1..3}, half FP64 and half mixed) with Poisson arrivals.
+        from fehip.serving import make_requests, run_workload
+
+        t1 = time.perf_counter()
+        serve, _ = run_workload(make_requests(args.serve_requests, seed=0), args.serve_rate, max_active=16,
+                                workers=8, hbm_fraction=0.5)
+        serve["workload"] = ("%d mass-solve requests, Poisson arrivals at %.0f/s, n ~ U{4..40}, P ~ U{2..6}, "
+                             "Q = P+1, num_comp ~ U{1..3}, FP64 or mixed (1:1), rtol 1e-10"
+                             % (args.serve_requests, args.serve_rate))
+        log("serve (%.1f s)" % (time.perf_counter() - t1), json.dumps(serve))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p)
@@ -295,6 +314,7 @@ def main():
         "tuning": tuning,
         "config5": big,
         "solve_c3": solve,
+        "serving_c4": serve,
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "cu": info.cu_count},
     }
     print(json.dumps(line), flush=True)

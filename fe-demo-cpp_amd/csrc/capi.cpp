@@ -596,6 +596,20 @@ int fem_hip_tune_table_clear(void) {
   return guard([&] {
     std::lock_guard<std::mutex> lk(tune_mutex());
     tune_table().clear();
+    tune_alg_table().clear();
+  });
+}
+int fem_hip_tune_table_save(const char* path) {
+  return guard([&] {
+    need(path, "path");
+    tune_table_save(path);
+  });
+}
+int fem_hip_tune_table_load(const char* path, int* rows_taken) {
+  return guard([&] {
+    need(path, "path");
+    const int n = tune_table_load(path);
+    if (rows_taken) *rows_taken = n;
   });
 }
 int fem_hip_operator_mass_diagonal(fem_hip_operator h, double* diag, size_t len, void* s) {
@@ -815,6 +829,7 @@ int fem_hip_server_create(const fem_hip_server_options* o, fem_hip_server* out) 
       so.batch_iters = o->batch_iters;
       so.autotune = o->autotune != 0;
       so.keep_x = o->keep_x != 0;
+      so.cheap_setup = o->cheap_setup != 0;
     }
     auto h = std::make_unique<fem_hip_server_s>();
     h->s = femhip::server_create(so);

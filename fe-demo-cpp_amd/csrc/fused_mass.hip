@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 
 #include "block_plan.hpp"
 #include "fused_mass.hpp"
@@ -188,6 +190,67 @@ std::map<TuneKey, TuneEntry>& tune_table() {
 std::mutex& tune_mutex() {
   static std::mutex m;
   return m;
+}
+
+std::map<TuneAlgKey, TuneEntry>& tune_alg_table() {
+  static std::map<TuneAlgKey, TuneEntry> t;
+  return t;
+}
+
+void tune_table_save(const std::string& path) {
+  std::ofstream out(path);
+  require(static_cast<bool>(out), "tune_table_save: cannot write " + path);
+  out.precision(6);
+  std::lock_guard<std::mutex> lk(tune_mutex());
+  for (const auto& kv : tune_table()) {
+    const TuneKey& k = kv.first;
+    const TuneEntry& e = kv.second;
+    out << "best " << k.P << ' ' << k.Q << ' ' << k.NC << ' ' << k.fp32 << ' ' << k.det << ' ' << e.alg << ' '
+        << e.elems << ' ' << e.ms << ' ' << e.num_elem << '\n';
+  }
+  for (const auto& kv : tune_alg_table()) {
+    const TuneAlgKey& k = kv.first;
+    const TuneEntry& e = kv.second;
+    out << "alg " << k.P << ' ' << k.Q << ' ' << k.NC << ' ' << k.fp32 << ' ' << k.alg << ' ' << e.elems << ' '
+        << e.ms << ' ' << e.num_elem << '\n';
+  }
+  require(static_cast<bool>(out), "tune_table_save: write failed: " + path);
+}
+
+int tune_table_load(const std::string& path) {
+  std::ifstream in(path);
+  require(static_cast<bool>(in), "tune_table_load: cannot read " + path);
+  std::string line;
+  int taken = 0, lineno = 0;
+  std::lock_guard<std::mutex> lk(tune_mutex());
+  while (std::getline(in, line)) {
+    ++lineno;
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream ls(line);
+    std::string kind;
+    ls >> kind;
+    TuneEntry e;
+    if (kind == "best") {
+      TuneKey k;
+      ls >> k.P >> k.Q >> k.NC >> k.fp32 >> k.det >> e.alg >> e.elems >> e.ms >> e.num_elem;
+      require(!ls.fail(), "tune_table_load: bad row at line " + std::to_string(lineno));
+      if (!find(FusedKey{k.P, k.Q, k.NC, k.fp32, e.alg, 0, e.elems})) continue;
+      if (k.det && !alg_deterministic(e.alg)) continue;
+      tune_table()[k] = e;
+      ++taken;
+    } else if (kind == "alg") {
+      TuneAlgKey k;
+      ls >> k.P >> k.Q >> k.NC >> k.fp32 >> k.alg >> e.elems >> e.ms >> e.num_elem;
+      require(!ls.fail(), "tune_table_load: bad row at line " + std::to_string(lineno));
+      e.alg = k.alg;
+      if (!find(FusedKey{k.P, k.Q, k.NC, k.fp32, k.alg, 0, e.elems})) continue;
+      tune_alg_table()[k] = e;
+      ++taken;
+    } else {
+      throw std::invalid_argument("tune_table_load: unknown row kind at line " + std::to_string(lineno));
+    }
+  }
+  return taken;
 }
 
 namespace {
@@ -482,7 +545,12 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
   if (f.choice.alg >= 0) {
     require(valid_for(f, f.choice.alg), "fused mass: a deterministic scatter needs a deterministic algorithm");
     k.alg = f.choice.alg;
-    k.elems = f.choice.elems > 0 ? f.choice.elems : default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
+    k.elems = f.choice.elems;
+    if (k.elems <= 0) {
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      auto it = tune_alg_table().find(TuneAlgKey{f.P, f.Q, f.NC, fp32, k.alg});
+      k.elems = it != tune_alg_table().end() ? it->second.elems : default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
+    }
     require(k.elems > 0 && find(k), "fused mass: the chosen algorithm is not compiled for this shape");
     return k;
   }
@@ -609,6 +677,7 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
   HIP_CHECK(hipEventCreate(&e1));
   TuneEntry best;
   best.ms = 1e300;
+  std::vector<TuneEntry> local;
   const FusedChoice saved = f.choice;
   const bool saved_timing = f.timing;
   f.timing = false;
@@ -626,6 +695,7 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
         float ms = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
         TuneEntry t{alg, ne, ms / reps, f.num_elem};
+        local.push_back(t);
         if (all) all->push_back(t);
         if (t.ms < best.ms) best = t;
       }
@@ -645,6 +715,10 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
   {
     std::lock_guard<std::mutex> lk(tune_mutex());
     tune_table()[TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0}] = best;
+    for (const TuneEntry& t : local) {
+      TuneEntry& slot = tune_alg_table()[TuneAlgKey{f.P, f.Q, f.NC, fp32, t.alg}];
+      if (slot.alg < 0 || t.ms < slot.ms || slot.num_elem != t.num_elem) slot = t;
+    }
   }
   // Drop block plans the winner does not use (each holds a slot-ordered qdata copy).
   const FusedKey wk{f.P, f.Q, f.NC, fp32, best.alg, 0, best.elems};

@@ -7,6 +7,7 @@
 #include <map>
 #include <mutex>
 #include <memory>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -162,6 +163,22 @@ struct TuneEntry {
 };
 std::map<TuneKey, TuneEntry>& tune_table();  // guard every access with tune_mutex()
 std::mutex& tune_mutex();
+// The fastest packing of each algorithm: (P, Q, NC, fp32, alg) -> (elems, ms).
+// An explicit algorithm with elems 0 uses it (else the default packing).
+struct TuneAlgKey {
+  int P, Q, NC, fp32, alg;
+  bool operator<(const TuneAlgKey& o) const {
+    return std::tie(P, Q, NC, fp32, alg) < std::tie(o.P, o.Q, o.NC, o.fp32, o.alg);
+  }
+};
+std::map<TuneAlgKey, TuneEntry>& tune_alg_table();  // guarded by tune_mutex()
+// Text form of both tables: one row per line,
+//   best P Q NC fp32 det alg elems ms num_elem
+//   alg  P Q NC fp32 alg elems ms num_elem
+// load replaces rows with the same key, skips rows not compiled in this
+// library, and returns the rows taken.
+void tune_table_save(const std::string& path);
+int tune_table_load(const std::string& path);
 // Times every compiled candidate valid for the operator's requested scatter and
 // precision on its own data (reps applies each, after one warm-up), records the
 // fastest in the table and returns it. Restores the operator's choice (automatic).

@@ -354,9 +354,9 @@ class MassServer {
         j->x.allocate(L);
         FusedMass& f = operator_fused_mass(*j->op, j->stream);
         if (opt_.autotune) maybe_tune(f, j->req.mode == 1, j->stream);
-        if (!tuned_shape(f, j->req.mode == 1)) {
-          // No measured choice: the transpose-map algorithm, whose setup is a
-          // counting sort (the block algorithms cluster elements on the host first).
+        if (opt_.cheap_setup || !tuned_shape(f, j->req.mode == 1)) {
+          // The transpose-map algorithm (at its tuned packing, if any): its setup is
+          // a counting sort, where the block algorithms cluster elements on the host.
           j->op->opt_alg = kAlgElemTmap;
           f.choice.alg = kAlgElemTmap;
         }

@@ -48,6 +48,9 @@ struct ServerOptions {
   int batch_iters = 8;        // CG iterations per graph launch
   bool autotune = false;      // tune each new (P, Q, num_comp, precision) on first sight
   bool keep_x = true;         // return the solution vectors
+  // true: every request uses the transpose-map scatter (cheapest setup) at its
+  // tuned packing; false: the tuning table's fastest algorithm where it has one.
+  bool cheap_setup = true;
 };
 
 struct ServerStats {

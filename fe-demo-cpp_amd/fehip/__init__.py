@@ -148,6 +148,8 @@ SIGNATURES = {
     "fem_hip_tune_table_get": [_ip, ctypes.POINTER(_d), _llp, _i, _ip],
     "fem_hip_tune_table_set": [_ip, _d, ctypes.c_longlong],
     "fem_hip_tune_table_clear": [],
+    "fem_hip_tune_table_save": [ctypes.c_char_p],
+    "fem_hip_tune_table_load": [ctypes.c_char_p, _ip],
     "fem_hip_server_create": [ctypes.c_void_p, ctypes.POINTER(_vp)],
     "fem_hip_server_submit": [_vp, ctypes.c_void_p, _llp],
     "fem_hip_server_wait": [_vp, ctypes.c_longlong, _d, ctypes.c_void_p, _vp, _sz],
@@ -400,6 +402,24 @@ def tune_table_clear():
     call("fem_hip_tune_table_clear")
 
 
+def tune_table_save(path):
+    call("fem_hip_tune_table_save", os.fsencode(path))
+
+
+def tune_table_load(path):
+    n = ctypes.c_int()
+    call("fem_hip_tune_table_load", os.fsencode(path), ctypes.byref(n))
+    return n.value
+
+
+TUNE_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning", "mi355x_gfx950.txt")
+
+
+def load_default_tuning():
+    """Loads the committed MI355X tuning table if present; returns the rows taken."""
+    return tune_table_load(TUNE_FILE) if os.path.exists(TUNE_FILE) else 0
+
+
 def transpose_map(num_elem, elem_size, l_size, offsets):
     offsets = np.ascontiguousarray(offsets, np.int32)
     rows = ctypes.c_int()
@@ -640,7 +660,8 @@ def operator_apply(op, d_in, d_out, stream=None):
 # ---------------------------------------------------------------- serving loop
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_active", ctypes.c_int), ("workers", ctypes.c_int), ("hbm_fraction", ctypes.c_double),
-                ("batch_iters", ctypes.c_int), ("autotune", ctypes.c_int), ("keep_x", ctypes.c_int)]
+                ("batch_iters", ctypes.c_int), ("autotune", ctypes.c_int), ("keep_x", ctypes.c_int),
+                ("cheap_setup", ctypes.c_int)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -676,8 +697,10 @@ def serve_request_bytes(req):
 class Server:
     """Continuous-batching mass-solve server (fem_hip_server_*)."""
 
-    def __init__(self, max_active=16, workers=8, hbm_fraction=0.5, batch_iters=8, autotune=False, keep_x=True):
-        o = ServerOptions(max_active, workers, hbm_fraction, batch_iters, int(autotune), int(keep_x))
+    def __init__(self, max_active=16, workers=8, hbm_fraction=0.5, batch_iters=8, autotune=False, keep_x=True,
+                 cheap_setup=True):
+        o = ServerOptions(max_active, workers, hbm_fraction, batch_iters, int(autotune), int(keep_x),
+                          int(cheap_setup))
         self.h = ctypes.c_void_p()
         call("fem_hip_server_create", ctypes.byref(o), ctypes.byref(self.h))
         self.keep_x = keep_x

@@ -1,0 +1,73 @@
+#!/usr/bin/env python3
+"""Autotunes every compiled fused-mass shape on this GPU and writes the table.
+
+For each (P, Q, num_comp) with a compiled fused kernel, a jittered structured
+hex mesh of about 24 million element nodes (at most 96^3 hexes) is built, so
+the vectors are far larger than the caches, and the autotuner times every
+compiled algorithm and packing: FP64 with the atomic scatter allowed (all four
+algorithms; the fastest deterministic one of them is recorded as the FP64
+deterministic choice) and FP32 with the deterministic scatter (the mixed
+solve's inner apply). The fastest choices go to the tuning table,
+written as text (default: mi355x_gfx950.txt beside this script), which
+fehip.load_default_tuning() and fem_hip_tune_table_load() read.
+
+Usage: python fe-demo-cpp_amd/tuning/tune_all.py [--out PATH] [--reps N]
+"""
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import fehip  # noqa: E402
+from fehip.problems import MassProblem  # noqa: E402
+
+SHAPES = [(2, 3, (1, 2, 3)), (3, 4, (1, 2, 3)), (4, 5, (1, 2, 3)), (5, 6, (1, 2, 3)), (6, 7, (1, 2, 3)),
+          (8, 10, (1, 3))]
+
+
+def mesh_size(P, target=24_000_000, n_max=96):
+    n = 2
+    while n < n_max and (n + 1) ** 3 * P ** 3 <= target:
+        n += 1
+    return n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "mi355x_gfx950.txt"))
+    ap.add_argument("--reps", type=int, default=8)
+    args = ap.parse_args()
+    info = fehip.device_info(0)
+    fehip.tune_table_clear()
+    for P, Q, ncs in SHAPES:
+        for nc in ncs:
+            n = mesh_size(P)
+            t0 = time.perf_counter()
+            p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
+            p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
+            (alg, ne, ms), rows = p.op.autotune(fp32=False, reps=args.reps)
+            det = min((r for r in rows if r[0] in (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL)),
+                      key=lambda r: r[2])
+            fehip.tune_table_set(P, Q, nc, 0, 1, det[0], det[1], det[2], n ** 3)
+            p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC, exact=0)
+            (alg32, ne32, ms32), _ = p.op.autotune(fp32=True, reps=args.reps)
+            print("P=%d Q=%d nc=%d n=%d  fp64: %s/%d %.4f ms (det %s/%d %.4f ms)  fp32 det: %s/%d %.4f ms" % (
+                P, Q, nc, n, fehip.FUSED_NAMES[alg], ne, ms, fehip.FUSED_NAMES[det[0]], det[1], det[2],
+                fehip.FUSED_NAMES[alg32], ne32, ms32), flush=True)
+            del p
+            print("  (%.1f s)" % (time.perf_counter() - t0), flush=True)
+    fehip.tune_table_save(args.out)
+    with open(args.out) as f:
+        body = f.read()
+    with open(args.out, "w") as f:
+        f.write("# fused-mass tuning table, %s (%s, %d CUs); written by tune_all.py\n" % (
+            info.arch.decode(), info.name.decode() or "MI355X", info.cu_count))
+        f.write(body)
+    print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

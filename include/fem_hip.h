@@ -233,6 +233,12 @@ int fem_hip_operator_autotune(fem_hip_operator op, int fp32, int reps, void* str
 int fem_hip_tune_table_get(int* rows7, double* ms, long long* num_elem, int max_rows, int* n_rows);
 int fem_hip_tune_table_set(const int* row7, double ms, long long num_elem);
 int fem_hip_tune_table_clear(void);
+/* Save / load the tuning tables as text (the best algorithm per shape and
+ * determinism, and the fastest packing per algorithm). Load replaces rows with
+ * the same key, skips rows for kernels this library lacks, and reports the rows
+ * taken. */
+int fem_hip_tune_table_save(const char* path);
+int fem_hip_tune_table_load(const char* path, int* rows_taken);
 /* FP32 fused mass apply on FP32 L-vectors (qdata and basis rounded to FP32 once). */
 int fem_hip_operator_apply_f32(fem_hip_operator op, const float* in, size_t in_len, float* out, size_t out_len,
                                void* stream);
@@ -313,6 +319,8 @@ typedef struct {
   int batch_iters;     /* CG iterations per graph launch */
   int autotune;        /* 1: tune each new (P, Q, num_comp, precision) on first sight */
   int keep_x;          /* 1: keep solutions for fem_hip_server_wait */
+  int cheap_setup;     /* 1: transpose-map scatter for every request (no host element clustering);
+                          0: the tuning table's fastest algorithm where it has one */
 } fem_hip_server_options;
 typedef struct {
   int n, P, Q, num_comp; /* n^3 hexes of the unit cube, order P, Q Gauss points */

@@ -171,3 +171,27 @@ def test_serve_request_bytes_estimate():
     n, P, Q = 20, 4, 5
     L = (n * (P - 1) + 1) ** 3
     assert est(n, P, Q) > 10 * 8 * L + 4 * 8 * n ** 3 * Q ** 3
+
+
+def test_tune_table_save_load_roundtrip(tmp_path):
+    fehip.tune_table_clear()
+    try:
+        fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BLOCK_PARTIAL, 14, 0.39, 262144)
+        fehip.tune_table_set(3, 4, 2, 1, 0, fehip.FUSED_ELEM_ATOMIC, 16, 0.05, 4096)
+        path = str(tmp_path / "t.txt")
+        fehip.tune_table_save(path)
+        rows = sorted(fehip.tune_table())
+        fehip.tune_table_clear()
+        assert fehip.tune_table() == []
+        with open(path, "a") as f:
+            f.write("# comment\nbest 7 7 1 0 0 0 8 0.1 10\n")  # no (7, 7) kernel: skipped
+        assert fehip.tune_table_load(path) == 2
+        got = sorted(fehip.tune_table())
+        assert [r[:7] for r in got] == [r[:7] for r in rows]
+        np.testing.assert_allclose([r[7] for r in got], [r[7] for r in rows], rtol=1e-5)
+        with open(path, "a") as f:
+            f.write("bogus 1 2 3\n")
+        with pytest.raises(fehip.InvalidArgument):
+            fehip.tune_table_load(path)
+    finally:
+        fehip.tune_table_clear()

@@ -183,11 +183,14 @@ def main():
 
     runs = (("fp64_block_partial", np.float64, D, 0, fehip.FUSED_BLOCK_PARTIAL),
             ("fp64_elem_tmap", np.float64, D, 0, fehip.FUSED_ELEM_TMAP),
+            ("fp64_elem_tmap_pipe", np.float64, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
+            ("fp64_block_partial_lds", np.float64, A, 0, fehip.FUSED_BLOCK_PARTIAL_LDS),
             ("fp64_block_atomic", np.float64, A, 0, fehip.FUSED_BLOCK_ATOMIC),
             ("fp64_elem_atomic", np.float64, A, 0, fehip.FUSED_ELEM_ATOMIC),
             ("fp64_exact_tmap", np.float64, D, 1, fehip.FUSED_ELEM_TMAP),
             ("fp32_block_partial", np.float32, D, 0, fehip.FUSED_BLOCK_PARTIAL),
             ("fp32_elem_tmap", np.float32, D, 0, fehip.FUSED_ELEM_TMAP),
+            ("fp32_elem_tmap_pipe", np.float32, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
             ("fp32_block_atomic", np.float32, A, 0, fehip.FUSED_BLOCK_ATOMIC))
     wanted = set(filter(None, args.variants.split(",")))
     unknown = wanted - {r[0] for r in runs}
@@ -208,16 +211,21 @@ def main():
         if alg in (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP):
             kernel_names[label] = "fused_mass_%s_kernel<%d, %d, 1, %s, %d, %s>" % (
                 "exact" if exact else "fast", P, Q, tname, ne, "true" if alg == fehip.FUSED_ELEM_TMAP else "false")
+        elif alg == fehip.FUSED_ELEM_TMAP_PIPE:
+            kernel_names[label] = "fused_mass_pipe_kernel<%d, %d, 1, %s, %d>" % (P, Q, tname, ne)
         else:
-            kernel_names[label] = "fused_mass_%s_kernel<%d, %d, 1, %s, %d>" % (
-                "part" if alg == fehip.FUSED_BLOCK_PARTIAL else "block", P, Q, tname, ne)
+            kernel_names[label] = {
+                fehip.FUSED_BLOCK_PARTIAL: "fused_mass_part_kernel<%d, %d, 1, %s, %d, false>",
+                fehip.FUSED_BLOCK_PARTIAL_LDS: "fused_mass_part_kernel<%d, %d, 1, %s, %d, true>",
+                fehip.FUSED_BLOCK_ATOMIC: "fused_mass_block_kernel<%d, %d, 1, %s, %d>"}[alg] % (P, Q, tname, ne)
         steps = args.steps if label.startswith("fp64") else max(args.steps // 2, 5)
         m, k, a = time_applies(fehip, p.op, d_u, d_v, steps, args.warmup, stream)
         s = 4 if fp32 else 8
         # The fused kernel's own algorithmic bytes: u, qdata and offsets read; v
         # written, except the element-output (tmap) kernels, which write E*P^3
         # values and leave the sums to the transpose-map pass.
-        kbytes = s * L + s * E * Q ** 3 + 4 * E * P3 + (s * E * P3 if alg == fehip.FUSED_ELEM_TMAP else s * L)
+        tmap = alg in (fehip.FUSED_ELEM_TMAP, fehip.FUSED_ELEM_TMAP_PIPE)
+        kbytes = s * L + s * E * Q ** 3 + 4 * E * P3 + (s * E * P3 if tmap else s * L)
         variants[label] = {"ms_per_apply": m, "gdofs": L / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
                            "kernel_GBs": kbytes / (k * 1e-3) / 1e9, "kernel_frac_hbm": kbytes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
                            "apply_frac_hbm": (bytes32 if s == 4 else bytes64) / (m * 1e-3) / 1e9 / HBM_PEAK_GBS,

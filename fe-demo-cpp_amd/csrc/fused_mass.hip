@@ -447,7 +447,7 @@ FusedMass::StageEvents* next_events(FusedMass& f) {
 template <class T>
 void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
   if (alg_blocked(k.alg)) {
-    const bool part = k.alg == kAlgBlockPart;
+    const bool part = alg_partial(k.alg);
     DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), part, s);
     (void)plan_qdata<T>(f, d, s);
     if (part) {
@@ -457,7 +457,7 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
     }
   } else {
     (void)elem_qdata<T>(f, s);
-    if (k.alg == kAlgElemTmap) {
+    if (alg_tmap(k.alg)) {
       ensure_tmap(f);
       DevBuf<T>& Y = elem_outputs<T>(f);
       const size_t n = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P * f.NC;
@@ -479,7 +479,7 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
   FusedMass::StageEvents* ev = next_events(f);
   if (ev) HIP_CHECK(hipEventRecord(ev->a, s));
   if (alg_blocked(k.alg)) {
-    DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), k.alg == kAlgBlockPart, s);
+    DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), alg_partial(k.alg), s);
     a.qdata = plan_qdata<T>(f, d, s).data();
     a.nblk = d.nblk;
     a.max_nodes = d.max_nodes;
@@ -493,14 +493,14 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
     a.part_ptr = d.part_ptr.data();
     a.v = v;
     if (k.alg == kAlgBlockAtomic || !d.covers_all) zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
-    if (k.alg == kAlgBlockPart) {
+    if (alg_partial(k.alg)) {
       a.partials = plan_partials<T>(d).data();
       a.part_comp_len = std::max<size_t>(d.npart, 1);
     }
     if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     L(a, s);
     if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-    if (k.alg == kAlgBlockPart)
+    if (alg_partial(k.alg))
       launch_row_sum<T>(d.nrows, f.NC, f.comp_stride, d.rows.data(), d.row_ptr.data(), d.row_part.data(),
                         static_cast<const T*>(a.partials), a.part_comp_len, v, s);
   } else {
@@ -510,7 +510,7 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
     a.flagged_offsets = f.flagged_offsets.data();
     a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
     a.v = v;
-    if (k.alg == kAlgElemTmap) {
+    if (alg_tmap(k.alg)) {
       a.out = elem_outputs<T>(f).data();
       if (!f.tm_covers_all) zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
       if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
@@ -565,10 +565,11 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
     }
   }
   // Default order: partial-sum blocks, then (deterministic) transpose map or (atomic) block atomics, then elements.
-  const int order_det[] = {kAlgBlockPart, kAlgElemTmap};
-  const int order_any[] = {kAlgBlockPart, kAlgBlockAtomic, kAlgElemAtomic, kAlgElemTmap};
+  const int order_det[] = {kAlgBlockPart, kAlgElemTmapPipe, kAlgElemTmap};
+  const int order_any[] = {kAlgBlockPartLds, kAlgBlockPart, kAlgBlockAtomic, kAlgElemAtomic, kAlgElemTmapPipe,
+                           kAlgElemTmap};
   const int* order = det ? order_det : order_any;
-  const int n = det ? 2 : 4;
+  const int n = det ? 3 : 6;
   for (int i = 0; i < n; ++i) {
     k.alg = order[i];
     k.elems = default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
@@ -726,7 +727,7 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
     const bool keep = alg_blocked(best.alg) && it->first == std::make_pair(best.elems, fused_node_budget(wk));
     it = keep ? std::next(it) : f.plans.erase(it);
   }
-  if (best.alg != kAlgElemTmap) {
+  if (!alg_tmap(best.alg)) {
     f.Y64.release();
     f.Y32.release();
   }

@@ -23,10 +23,16 @@ enum FusedAlg : int {
   kAlgElemTmap = 1,    // per element into Y[E*P^3], then the transpose-map pass (deterministic; CPU order)
   kAlgBlockAtomic = 2, // per element block; sums in LDS, boundary nodes one atomic each
   kAlgBlockPart = 3,   // per element block; sums in LDS, boundary nodes one partial each, then a partial-sum pass
-  kAlgCount = 4
+  kAlgElemTmapPipe = 4,  // kAlgElemTmap with a persistent, software-pipelined element kernel
+  kAlgBlockPartLds = 5,  // kAlgBlockPart with LDS atomic sums in place of the colour rounds (any order)
+  kAlgCount = 6
 };
-inline bool alg_deterministic(int alg) { return alg == kAlgElemTmap || alg == kAlgBlockPart; }
-inline bool alg_blocked(int alg) { return alg == kAlgBlockAtomic || alg == kAlgBlockPart; }
+inline bool alg_deterministic(int alg) {
+  return alg == kAlgElemTmap || alg == kAlgBlockPart || alg == kAlgElemTmapPipe;
+}
+inline bool alg_tmap(int alg) { return alg == kAlgElemTmap || alg == kAlgElemTmapPipe; }
+inline bool alg_blocked(int alg) { return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds; }
+inline bool alg_partial(int alg) { return alg == kAlgBlockPart || alg == kAlgBlockPartLds; }
 
 // One compiled kernel.
 struct FusedKey {

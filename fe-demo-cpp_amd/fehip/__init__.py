@@ -21,7 +21,9 @@ EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 3
 QUAD_GAUSS, QUAD_GAUSS_LOBATTO = 0, 1
 SCATTER_ATOMIC, SCATTER_DETERMINISTIC = 0, 1
 FUSED_AUTO, FUSED_ELEM_ATOMIC, FUSED_ELEM_TMAP, FUSED_BLOCK_ATOMIC, FUSED_BLOCK_PARTIAL = -1, 0, 1, 2, 3
-FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_partial"}
+FUSED_ELEM_TMAP_PIPE, FUSED_BLOCK_PARTIAL_LDS = 4, 5
+FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_partial", 4: "elem_tmap_pipe",
+               5: "block_partial_lds"}
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
 QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR = 1, 2, 3
 SOLVE_FP64, SOLVE_MIXED = 0, 1

@@ -211,6 +211,8 @@ int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, in
 #define FEM_HIP_FUSED_BLOCK_ATOMIC 2  /* element blocks summed in LDS, boundary nodes one atomic add each */
 #define FEM_HIP_FUSED_BLOCK_PARTIAL 3 /* element blocks summed in LDS, boundary partials summed by a second pass
                                          (deterministic) */
+#define FEM_HIP_FUSED_ELEM_TMAP_PIPE 4 /* ELEM_TMAP with a persistent, software-pipelined element kernel */
+#define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS 5 /* BLOCK_PARTIAL summing in LDS with atomics (any order) */
 /* Explicit algorithm and its elements per workgroup (element kernels) or per
  * block (block kernels); 0 = that algorithm's default. A deterministic scatter
  * mode accepts only the deterministic algorithms. exact = 1 overrides. */

@@ -4,7 +4,8 @@
   bit equality with the oracle (same arithmetic, same order).
 * Fused mass kernel: with exact arithmetic and the deterministic transpose-map
   scatter it equals the oracle's operator_apply bit for bit; with FMA and/or any
-  other scatter algorithm (element atomics, block atomics, block partial sums)
+  other scatter algorithm (element atomics, block atomics, block partial sums
+  with colour rounds or LDS atomics, the pipelined transpose-map kernel)
   it agrees within 1e-13 relative (the FP64 tolerance), and the deterministic
   algorithms repeat bit for bit.
 * The autotuner times every valid algorithm and its choice is the one applied.
@@ -22,8 +23,9 @@ from oracle_lib import oracle
 pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-13
-ALGS = (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_ATOMIC, fehip.FUSED_BLOCK_PARTIAL)
-DET_ALGS = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL)
+ALGS = (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_ATOMIC, fehip.FUSED_BLOCK_PARTIAL,
+        fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BLOCK_PARTIAL_LDS)
+DET_ALGS = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP_PIPE)
 
 
 def mesh1d_qdata(h0, h1):

@@ -87,17 +87,23 @@ def cpu_baseline(problem, slab_layers=8, reps=3):
     import fehip
 
     u = fehip.rng_uniform(1, l_s)
-    op.apply(u)
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
+    # One core, as the reference runs (single-threaded): pin to the first allowed CPU.
+    allowed = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(allowed)})
+    try:
         op.apply(u)
-        times.append(time.perf_counter() - t0)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            op.apply(u)
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, allowed)
     t = float(np.median(times))
     return {"value": l_s / t / 1e9, "unit": "GDoF/s", "cores": 1, "kind": "port",
             "sample": "oracle operator_apply (single-threaded CPU restatement of the reference, -O2 "
                       "-ffp-contract=off) on a 64x64x%d-element z-slab of the 64^3 P=5 Q=6 mesh (%d DoF), "
-                      "median of %d applies: %.3f s/apply; host %s, nproc %d" %
+                      "median of %d applies pinned to one core: %.3f s/apply; host %s, nproc %d" %
                       (slab_layers, l_s, reps, t, platform.processor() or platform.machine(), os.cpu_count())}
 
 
@@ -137,7 +143,8 @@ def main():
     ap.add_argument("--no-big", action="store_true", help="skip the 48^3, P=8, Q=10, 3-component line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-rate", type=float, default=60.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="Poisson arrival rate, requests/s (default: 0.8 x the measured service capacity)")
     ap.add_argument("--variants", default="", help="comma-separated subset of the apply variants")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="K",
                     help="also run K plain copies of one FP64 L-vector")
@@ -279,14 +286,21 @@ def main():
     if not args.no_serve:
         # Config 4: mass-solve requests (n ~ U{4..40}, P ~ U{2..6}, Q = P+1,
         # num_comp ~ U{1..3}, half FP64 and half mixed) with Poisson arrivals.
-        from fehip.serving import make_requests, run_workload
+        from fehip.serving import make_requests, measure_capacity, run_workload
 
         t1 = time.perf_counter()
-        serve, _ = run_workload(make_requests(args.serve_requests, seed=0), args.serve_rate, max_active=16,
-                                workers=8, hbm_fraction=0.5)
-        serve["workload"] = ("%d mass-solve requests, Poisson arrivals at %.0f/s, n ~ U{4..40}, P ~ U{2..6}, "
-                             "Q = P+1, num_comp ~ U{1..3}, FP64 or mixed (1:1), rtol 1e-10"
-                             % (args.serve_requests, args.serve_rate))
+        opts = dict(max_active=16, workers=8, hbm_fraction=0.5)
+        capacity = None
+        rate = args.serve_rate
+        if rate <= 0:
+            capacity, _ = measure_capacity(300, **opts)
+            rate = 0.8 * capacity
+            log("serving capacity %.1f requests/s -> arrival rate %.1f/s" % (capacity, rate))
+        serve, _ = run_workload(make_requests(args.serve_requests, seed=0), rate, **opts)
+        serve["measured_capacity_req_per_s"] = capacity
+        serve["workload"] = ("%d mass-solve requests, Poisson arrivals at %.1f/s (0.8 x measured capacity), "
+                             "n ~ U{4..40}, P ~ U{2..6}, Q = P+1, num_comp ~ U{1..3}, FP64 or mixed (1:1), "
+                             "rtol 1e-10, request r seeded 1000+r" % (args.serve_requests, rate))
         log("serve (%.1f s)" % (time.perf_counter() - t1), json.dumps(serve))
     cpu = None
     if not args.no_cpu:

@@ -41,6 +41,84 @@ __global__ __launch_bounds__(kThreads) void apply_mass_kernel(size_t n, size_t Q
   v[i] = qd[i % Q] * u[i];
 }
 
+// Diffusion QFunctions (no reference counterpart): gradient fields [d][comp][Q],
+// J(d,c) = dx[(d*dim + c)*Q + p] as in build_mass. build: qdata = (w / det) C C^T
+// with C the cofactor matrix of J, det expanded along row 0 as in build_mass,
+// stored symmetric as (00),(11),(22),(12),(02),(01) (dim 3), (00),(11),(01)
+// (dim 2), (00) (dim 1). apply: dv[a][c] = sum_b qdata(a,b) du[b][c], b
+// ascending. Sums start from 0 and add left to right, without fused
+// multiply-add, like the CPU statement of the same functions.
+__device__ __forceinline__ int sym_index(int dim, int a, int b) {
+  if (a == b) return a;
+  if (dim == 2) return 2;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo == 1 ? 3 : (hi == 2 ? 4 : 5);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void build_diffusion_kernel(int Q, const double* __restrict__ J,
+                                                                   const double* __restrict__ w,
+                                                                   double* __restrict__ qdata) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= Q) return;
+  auto A = [&](int d, int c) { return J[static_cast<size_t>(d * DIM + c) * Q + p]; };
+  double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double det;
+  if constexpr (DIM == 1) {
+    det = A(0, 0);
+    C[0][0] = 1.0;
+  } else if constexpr (DIM == 2) {
+    det = A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0);
+    C[0][0] = A(1, 1);
+    C[0][1] = -A(1, 0);
+    C[1][0] = -A(0, 1);
+    C[1][1] = A(0, 0);
+  } else {
+    const double m0 = A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1);
+    const double m1 = A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0);
+    const double m2 = A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0);
+    det = A(0, 0) * m0 - A(0, 1) * m1 + A(0, 2) * m2;
+    C[0][0] = m0;
+    C[0][1] = -m1;
+    C[0][2] = m2;
+    C[1][0] = A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2);
+    C[1][1] = A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0);
+    C[1][2] = A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1);
+    C[2][0] = A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1);
+    C[2][1] = A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2);
+    C[2][2] = A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0);
+  }
+  const double scale = w[p] / det;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+#pragma unroll
+    for (int b = a; b < DIM; ++b) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) s = s + C[a][k] * C[b][k];
+      qdata[static_cast<size_t>(sym_index(DIM, a, b)) * Q + p] = scale * s;
+    }
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void apply_diffusion_kernel(int Q, int nc, const double* __restrict__ du,
+                                                                   const double* __restrict__ qd,
+                                                                   double* __restrict__ dv) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= Q) return;
+  for (int c = 0; c < nc; ++c) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < DIM; ++b)
+        s = s + qd[static_cast<size_t>(sym_index(DIM, a, b)) * Q + p] * du[static_cast<size_t>(b * nc + c) * Q + p];
+      dv[static_cast<size_t>(a * nc + c) * Q + p] = s;
+    }
+  }
+}
+
 int ctx_int(const QFunctionDev& qf, const char* who) {
   if (qf.ctx.size() < sizeof(int)) throw std::invalid_argument(std::string(who) + ": context missing");
   int v;
@@ -51,7 +129,7 @@ int ctx_int(const QFunctionDev& qf, const char* who) {
 }  // namespace
 
 QFunctionDev* qfunction_create(int kind) {
-  if (kind != kQfBuildMass && kind != kQfApplyMass && kind != kQfApplyMassVector)
+  if (kind < kQfBuildMass || kind > kQfApplyDiffusion)
     throw Unsupported("qfunction_create: no device implementation for this QFunction");
   auto* q = new QFunctionDev();
   q->kind = kind;
@@ -117,6 +195,30 @@ void qfunction_apply(const QFunctionDev& qf, int Q, int num_in, const double* co
         throw std::runtime_error("qfunction_apply: user function returned non-zero error code");
       const size_t n = static_cast<size_t>(nc) * Q;
       apply_mass_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(n, static_cast<size_t>(Q), in[0], in[1], out[0]);
+      break;
+    }
+    case kQfBuildDiffusion: {
+      require(num_in >= 2 && num_out >= 1, "build_diffusion: needs inputs dx, weights and output qdata");
+      const int dim = ctx_int(qf, "build_diffusion");
+      if (dim < 1 || dim > 3 || qf.inputs[0].size != dim * dim || qf.outputs[0].size != dim * (dim + 1) / 2)
+        throw std::runtime_error("qfunction_apply: user function returned non-zero error code");
+      if (dim == 1) build_diffusion_kernel<1><<<grid, kThreads, 0, s>>>(Q, in[0], in[1], out[0]);
+      if (dim == 2) build_diffusion_kernel<2><<<grid, kThreads, 0, s>>>(Q, in[0], in[1], out[0]);
+      if (dim == 3) build_diffusion_kernel<3><<<grid, kThreads, 0, s>>>(Q, in[0], in[1], out[0]);
+      break;
+    }
+    case kQfApplyDiffusion: {
+      require(num_in >= 2 && num_out >= 1, "apply_diffusion: needs inputs du, qdata and output dv");
+      require(qf.ctx.size() >= 2 * sizeof(int), "apply_diffusion: context {dim, num_comp} missing");
+      int dn[2];
+      std::memcpy(dn, qf.ctx.data(), sizeof(dn));
+      const int dim = dn[0], nc = dn[1];
+      if (dim < 1 || dim > 3 || nc < 1 || qf.inputs[0].size != dim * nc || qf.outputs[0].size != dim * nc ||
+          qf.inputs[1].size != dim * (dim + 1) / 2)
+        throw std::runtime_error("qfunction_apply: user function returned non-zero error code");
+      if (dim == 1) apply_diffusion_kernel<1><<<grid, kThreads, 0, s>>>(Q, nc, in[0], in[1], out[0]);
+      if (dim == 2) apply_diffusion_kernel<2><<<grid, kThreads, 0, s>>>(Q, nc, in[0], in[1], out[0]);
+      if (dim == 3) apply_diffusion_kernel<3><<<grid, kThreads, 0, s>>>(Q, nc, in[0], in[1], out[0]);
       break;
     }
     default:

@@ -9,7 +9,7 @@
 
 namespace femhip {
 
-enum : int { kQfBuildMass = 1, kQfApplyMass = 2, kQfApplyMassVector = 3 };
+enum : int { kQfBuildMass = 1, kQfApplyMass = 2, kQfApplyMassVector = 3, kQfBuildDiffusion = 4, kQfApplyDiffusion = 5 };
 
 struct QFField {
   std::string name;

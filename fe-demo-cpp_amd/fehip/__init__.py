@@ -25,7 +25,7 @@ FUSED_ELEM_TMAP_PIPE, FUSED_BLOCK_PARTIAL_LDS = 4, 5
 FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_partial", 4: "elem_tmap_pipe",
                5: "block_partial_lds"}
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
-QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR = 1, 2, 3
+QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR, QF_BUILD_DIFFUSION, QF_APPLY_DIFFUSION = 1, 2, 3, 4, 5
 SOLVE_FP64, SOLVE_MIXED = 0, 1
 
 
@@ -558,6 +558,10 @@ class QFunction:
     def set_context_int(self, value):
         v = ctypes.c_int(value)
         call("fem_hip_qfunction_set_context", self.h, ctypes.byref(v), ctypes.sizeof(v))
+
+    def set_context_ints(self, *values):
+        v = (ctypes.c_int * len(values))(*values)
+        call("fem_hip_qfunction_set_context", self.h, v, ctypes.sizeof(v))
 
     def apply(self, Q, d_in, d_out, stream=None):
         nin, nout = len(d_in), len(d_out)

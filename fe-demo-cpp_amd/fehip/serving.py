@@ -16,7 +16,9 @@ import numpy as np
 from . import SOLVE_FP64, SOLVE_MIXED, Server, serve_l_size, serve_request
 
 
-def make_requests(count, seed=0, n_range=(4, 40), p_range=(2, 6), mixed_share=0.5, rtol=1e-10):
+def make_requests(count, seed=0, n_range=(4, 40), p_range=(2, 6), mixed_share=0.5, rtol=1e-10, first_seed=1000):
+    """Request r: its shape and mode from numpy's generator (seed), its vertex
+    jitter and right-hand side from splitmix64 seed first_seed + r."""
     rng = np.random.default_rng(seed)
     reqs = []
     for i in range(count):
@@ -24,8 +26,16 @@ def make_requests(count, seed=0, n_range=(4, 40), p_range=(2, 6), mixed_share=0.
         P = int(rng.integers(p_range[0], p_range[1] + 1))
         nc = int(rng.integers(1, 4))
         mode = SOLVE_MIXED if rng.random() < mixed_share else SOLVE_FP64
-        reqs.append(serve_request(n, P, P + 1, nc, mode, 0.1, rtol, mesh_seed=10_000 + i, rhs_seed=20_000 + i))
+        reqs.append(serve_request(n, P, P + 1, nc, mode, 0.1, rtol, mesh_seed=first_seed + i,
+                                  rhs_seed=first_seed + i))
     return reqs
+
+
+def measure_capacity(count=300, seed=7, **server_opts):
+    """Service capacity (requests/s): `count` requests of the same distribution
+    all submitted at once, completions over wall time."""
+    summary, _ = run_workload(make_requests(count, seed=seed, first_seed=500_000), rate_per_s=1e6, **server_opts)
+    return summary["throughput_req_per_s"], summary
 
 
 def run_workload(reqs, rate_per_s, seed=1, server=None, keep_x=False, **server_opts):

@@ -57,6 +57,14 @@ extern "C" {
 #define FEM_HIP_QF_BUILD_MASS 1        /* ctx: int dim */
 #define FEM_HIP_QF_APPLY_MASS 2        /* size-1 fields */
 #define FEM_HIP_QF_APPLY_MASS_VECTOR 3 /* ctx: int num_comp; v[c][p] = qdata[p] u[c][p] */
+/* Diffusion (Poisson), no reference counterpart; gradient fields [d][comp][Q]:
+ * build: inputs dx (dim*dim, Grad), weights (1, Weight); output qdata
+ *   (dim*(dim+1)/2, None) = w det(J) J^-1 J^-T, symmetric, stored
+ *   (00),(11),(22),(12),(02),(01) for dim 3; (00),(11),(01) for dim 2. ctx: int dim.
+ * apply: inputs du (dim*num_comp, Grad), qdata; output dv (dim*num_comp, Grad),
+ *   dv[a][c] = sum_b qdata(a,b) du[b][c]. ctx: int dim, int num_comp. */
+#define FEM_HIP_QF_BUILD_DIFFUSION 4
+#define FEM_HIP_QF_APPLY_DIFFUSION 5
 /* Arithmetic of the fused mass apply. */
 #define FEM_HIP_FP64 0
 #define FEM_HIP_FP32 1

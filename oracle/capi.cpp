@@ -4,7 +4,9 @@
 //   * against oracle/include + oracle/src (the restatement)  -> oracle/liboracle.so
 //   * against the reference's own include/ + src/*.cpp       -> oracle/_ref/libfemref.so
 // and only uses the interface both share (include/basis.hpp, elem-restriction.hpp,
-// tensor-contract.hpp, qfunction.hpp, qfunctions/mass.hpp, operator.hpp).
+// tensor-contract.hpp, qfunction.hpp, qfunctions/mass.hpp, operator.hpp). The
+// vector-mass and diffusion QFunctions, which the reference lacks, are defined
+// here, so both builds run the same code for them.
 // Nothing in the product links or loads this.
 #include <cmath>
 #include <cstdint>
@@ -64,6 +66,87 @@ int vector_mass(void* ctx, int Q, const double* const* in, double* const* out) {
   const int nc = static_cast<VecMassCtx*>(ctx)->num_comp;
   for (int c = 0; c < nc; ++c)
     for (int p = 0; p < Q; ++p) out[0][c * Q + p] = in[1][p] * in[0][c * Q + p];
+  return 0;
+}
+
+// Diffusion (Poisson) QFunctions: no reference counterpart. They follow the
+// reference's QFunction signature (include/qfunction.hpp:17-20) and layouts:
+// gradient fields are [d][comp][Q] (src/tensor-contract.cpp:115,153), J is read
+// as in build_mass, J(d,c) = dx[(d*dim + c)*Q + p] = dx_c/dX_d
+// (include/qfunctions/mass.hpp:13-14).
+//   build_diffusion: qdata = w det(J) J^-1 J^-T in reference coordinates, with
+//     C the cofactor matrix of A(d,c) = J(d,c): qdata = (w / det) C C^T, stored
+//     symmetric in the order (00), (11), (22), (12), (02), (01) for dim 3,
+//     (00), (11), (01) for dim 2, (00) for dim 1. det is expanded along row 0
+//     with the association of include/qfunctions/mass.hpp:31-33.
+//   apply_diffusion: dv[a][c] = sum_b qdata(a,b) du[b][c], b ascending.
+// The device QFunctions perform the same operations in the same order.
+struct DiffusionCtx {
+  int dim, num_comp;
+};
+inline int sym_index(int dim, int a, int b) {
+  if (a == b) return a;
+  if (dim == 2) return 2;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo == 1 ? 3 : (hi == 2 ? 4 : 5);  // (1,2) -> 3, (0,2) -> 4, (0,1) -> 5
+}
+int build_diffusion(void* ctx, int Q, const double* const* in, double* const* out) {
+  const int dim = static_cast<DiffusionCtx*>(ctx)->dim;
+  if (dim < 1 || dim > 3) return 1;
+  const int ns = dim * (dim + 1) / 2;
+  for (int p = 0; p < Q; ++p) {
+    auto A = [&](int d, int c) { return in[0][(d * dim + c) * Q + p]; };
+    const double w = in[1][p];
+    double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double det;
+    if (dim == 1) {
+      det = A(0, 0);
+      C[0][0] = 1.0;
+    } else if (dim == 2) {
+      det = A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0);
+      C[0][0] = A(1, 1);
+      C[0][1] = -A(1, 0);
+      C[1][0] = -A(0, 1);
+      C[1][1] = A(0, 0);
+    } else {
+      const double m0 = A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1);
+      const double m1 = A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0);
+      const double m2 = A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0);
+      det = A(0, 0) * m0 - A(0, 1) * m1 + A(0, 2) * m2;
+      C[0][0] = m0;
+      C[0][1] = -m1;
+      C[0][2] = m2;
+      C[1][0] = A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2);
+      C[1][1] = A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0);
+      C[1][2] = A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1);
+      C[2][0] = A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1);
+      C[2][1] = A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2);
+      C[2][2] = A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0);
+    }
+    const double scale = w / det;
+    for (int a = 0; a < dim; ++a) {
+      for (int b = a; b < dim; ++b) {
+        double s = 0.0;
+        for (int k = 0; k < dim; ++k) s = s + C[a][k] * C[b][k];
+        out[0][sym_index(dim, a, b) * Q + p] = scale * s;
+      }
+    }
+    (void)ns;
+  }
+  return 0;
+}
+int apply_diffusion(void* ctx, int Q, const double* const* in, double* const* out) {
+  const int dim = static_cast<DiffusionCtx*>(ctx)->dim, nc = static_cast<DiffusionCtx*>(ctx)->num_comp;
+  if (dim < 1 || dim > 3) return 1;
+  for (int p = 0; p < Q; ++p) {
+    for (int c = 0; c < nc; ++c) {
+      for (int a = 0; a < dim; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < dim; ++b) s = s + in[1][sym_index(dim, a, b) * Q + p] * in[0][(b * nc + c) * Q + p];
+        out[0][(a * nc + c) * Q + p] = s;
+      }
+    }
+  }
   return 0;
 }
 
@@ -377,5 +460,63 @@ int orc_transpose_map(int ne, int es, int l_size, const int* offsets, int* rows,
   });
 }
 #endif
+
+// qdata of the diffusion operator on a mesh given by coordinates (as orc_build_qdata):
+// ne * Q^dim points, dim*(dim+1)/2 values each, field-major [s][q*ne + e].
+int orc_build_diffusion_qdata(int dim, int ne, int mesh_P, int Q, int quad_mode, int coord_l_size,
+                              int coord_comp_stride, const int* mesh_offsets, const double* coords, double* qdata) {
+  return guarded([&] {
+    const int es = ipow(mesh_P, dim), nq = ipow(Q, dim), ns = dim * (dim + 1) / 2;
+    fem::TensorBasis mb = fem::TensorBasis::create_tensor_H1_lagrange(dim, dim, mesh_P, Q, qmode(quad_mode));
+    fem::ElemRestriction cr, qr;
+    fem::elem_restriction_create(ne, es, dim, coord_comp_stride, coord_l_size,
+                                 to_vec(mesh_offsets, static_cast<size_t>(ne) * es), cr);
+    fem::elem_restriction_create_strided(ne, nq, ns, ns * ne * nq, qr);
+    fem::QFunction qf;
+    fem::qfunction_create(build_diffusion, qf);
+    fem::qfunction_add_input(qf, "dx", dim * dim, fem::EvalMode::Grad);
+    fem::qfunction_add_input(qf, "weights", 1, fem::EvalMode::Weight);
+    fem::qfunction_add_output(qf, "qdata", ns, fem::EvalMode::None);
+    fem::qfunction_set_context(qf, DiffusionCtx{dim, 1});
+    fem::Operator op;
+    fem::operator_create(qf, op);
+    fem::operator_set_field(op, "dx", cr, mb, fem::vector_active());
+    fem::operator_set_field(op, "weights", std::nullopt, mb, fem::vector_none());
+    fem::operator_set_field(op, "qdata", qr, std::nullopt, fem::vector_active());
+    std::vector<double> out;
+    fem::operator_apply(op, to_vec(coords, coord_l_size), out);
+    std::memcpy(qdata, out.data(), out.size() * sizeof(double));
+  });
+}
+
+// Diffusion apply v = K u, composed from the interface calls: gather, gradient,
+// apply_diffusion, transposed gradient into a zeroed E-vector, scatter-add into a
+// zeroed v. (operator_apply would add the transposed gradient to the gathered
+// E-values it reuses as scratch: src/operator.cpp:178 with
+// src/tensor-contract.cpp:115-122,142.)
+int orc_diffusion_apply(int dim, int ne, int P, int Q, int nc, int quad_mode, int l_size, int comp_stride,
+                        const int* offsets, const double* qdata, const double* u, double* v) {
+  return guarded([&] {
+    const int es = ipow(P, dim), nq = ipow(Q, dim), ns = dim * (dim + 1) / 2;
+    fem::TensorBasis b = fem::TensorBasis::create_tensor_H1_lagrange(dim, nc, P, Q, qmode(quad_mode));
+    fem::ElemRestriction r;
+    fem::elem_restriction_create(ne, es, nc, comp_stride, l_size, to_vec(offsets, static_cast<size_t>(ne) * es), r);
+    std::vector<double> e, du, dv, ev;
+    fem::elem_restriction_apply(r, fem::ContractMode::NoTranspose, to_vec(u, l_size), e);
+    fem::tensor_basis_apply(b, ne, fem::ContractMode::NoTranspose, fem::EvalMode::Grad, e, du);
+    const int Qt = ne * nq;
+    dv.assign(static_cast<size_t>(dim) * nc * Qt, 0.0);
+    DiffusionCtx ctx{dim, nc};
+    const double* ins[2] = {du.data(), qdata};
+    double* outs[1] = {dv.data()};
+    if (apply_diffusion(&ctx, Qt, ins, outs) != 0) throw std::runtime_error("apply_diffusion failed");
+    (void)ns;
+    ev.assign(static_cast<size_t>(nc) * es * ne, 0.0);
+    fem::tensor_basis_apply(b, ne, fem::ContractMode::Transpose, fem::EvalMode::Grad, dv, ev);
+    std::vector<double> out(l_size, 0.0);
+    fem::elem_restriction_apply(r, fem::ContractMode::Transpose, ev, out);
+    std::memcpy(v, out.data(), out.size() * sizeof(double));
+  });
+}
 
 }  // extern "C"

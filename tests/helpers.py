@@ -31,3 +31,24 @@ def ramp(n, scale):
 def rel_err(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def node_coords(n, P, coords, gll):
+    """Physical coordinates [3][N^3] (node gx + N*(gy + N*gz)) of the order-P nodes
+    of the jittered n^3 box: the trilinear map of each element's 8 vertices
+    (coords [3][(n+1)^3], x fastest) evaluated at the GLL points `gll`."""
+    c = np.asarray(coords).reshape(3, n + 1, n + 1, n + 1)  # [axis][z][y][x]
+    N = n * (P - 1) + 1
+    g = np.arange(N)
+    e = np.minimum(g // (P - 1), n - 1)
+    t = (np.asarray(gll)[g - e * (P - 1)] + 1) / 2
+    out = np.zeros((3, N, N, N))
+    for dz in (0, 1):
+        wz = (t if dz else 1 - t)[:, None, None]
+        for dy in (0, 1):
+            wy = (t if dy else 1 - t)[None, :, None]
+            for dx in (0, 1):
+                wx = (t if dx else 1 - t)[None, None, :]
+                out += (wz * wy * wx)[None] * c[:, (e + dz)[:, None, None], (e + dy)[None, :, None],
+                                                (e + dx)[None, None, :]]
+    return out.reshape(3, -1)

@@ -34,6 +34,8 @@ _SIGS = {
     "orc_mass_diagonal": ([_vp, _vp], _i),
     "orc_mass_cg": ([_vp, _vp, _vp, _vp, _i, _d, ctypes.POINTER(_i), ctypes.POINTER(_d)], _i),
     "orc_transpose_map": ([_i, _i, _i, _vp, ctypes.POINTER(_i), _vp, _vp, _vp], _i),
+    "orc_build_diffusion_qdata": ([_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp], _i),
+    "orc_diffusion_apply": ([_i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp], _i),
 }
 
 
@@ -142,6 +144,24 @@ class Oracle:
         out = np.zeros(ne * Q ** dim)
         self._chk(self.L.orc_build_qdata(dim, ne, mesh_P, Q, quad_mode, coord_l_size, coord_comp_stride, _p(mo),
                                          _p(co), _p(out)))
+        return out
+
+    def build_diffusion_qdata(self, dim, ne, mesh_P, Q, coord_l_size, coord_comp_stride, mesh_offsets, coords,
+                              quad_mode=0):
+        mo = np.ascontiguousarray(mesh_offsets, np.int32)
+        co = np.ascontiguousarray(coords, np.float64)
+        out = np.zeros(dim * (dim + 1) // 2 * ne * Q ** dim)
+        self._chk(self.L.orc_build_diffusion_qdata(dim, ne, mesh_P, Q, quad_mode, coord_l_size, coord_comp_stride,
+                                                   _p(mo), _p(co), _p(out)))
+        return out
+
+    def diffusion_apply(self, dim, ne, P, Q, nc, l_size, comp_stride, offsets, qdata, u, quad_mode=0):
+        off = np.ascontiguousarray(offsets, np.int32)
+        qd = np.ascontiguousarray(qdata, np.float64)
+        uu = np.ascontiguousarray(u, np.float64)
+        out = np.zeros(l_size)
+        self._chk(self.L.orc_diffusion_apply(dim, ne, P, Q, nc, quad_mode, l_size, comp_stride, _p(off), _p(qd),
+                                             _p(uu), _p(out)))
         return out
 
     def mass(self, dim, ne, P, Q, nc, l_size, comp_stride, offsets, qdata, quad_mode=0):

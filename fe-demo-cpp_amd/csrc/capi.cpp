@@ -12,6 +12,7 @@
 #include "basis.hpp"
 #include "basis_setup.hpp"
 #include "block_plan.hpp"
+#include "mesh_io.hpp"
 #include "server.hpp"
 #include "common.hpp"
 #include "mesh.hpp"
@@ -896,5 +897,49 @@ int fem_hip_serve_request_bytes(const fem_hip_serve_request* req, size_t* bytes)
     need(req, "request");
     need(bytes, "bytes");
     *bytes = femhip::serve_request_bytes(to_req(*req));
+  });
+}
+
+/* ---- on-disk meshes ---- */
+int fem_hip_mesh_read(const char* path, fem_hip_mesh_info* info, int* offsets, int* vertex_offsets, double* coords) {
+  return guard([&] {
+    need(path, "path");
+    need(info, "info");
+    const femhip::MeshData m = femhip::mesh_read(path);
+    info->dim = m.dim;
+    info->num_elem = m.num_elem;
+    info->P = m.P;
+    info->num_nodes = m.num_nodes;
+    info->num_vertices = m.num_vertices;
+    if (offsets) std::memcpy(offsets, m.offsets.data(), m.offsets.size() * sizeof(int));
+    if (vertex_offsets) std::memcpy(vertex_offsets, m.vertex_offsets.data(), m.vertex_offsets.size() * sizeof(int));
+    if (coords) std::memcpy(coords, m.coords.data(), m.coords.size() * sizeof(double));
+  });
+}
+int fem_hip_mesh_write(const char* path, const fem_hip_mesh_info* info, const int* offsets,
+                       const int* vertex_offsets, const double* coords) {
+  return guard([&] {
+    need(path, "path");
+    need(info, "info");
+    need(offsets, "offsets");
+    need(vertex_offsets, "vertex_offsets");
+    need(coords, "coords");
+    require(info->dim >= 1 && info->dim <= 3 && info->num_elem > 0 && info->P >= 2 && info->num_vertices > 0,
+            "mesh_write: bad sizes");
+    femhip::MeshData m;
+    m.dim = info->dim;
+    m.num_elem = info->num_elem;
+    m.P = info->P;
+    m.num_nodes = info->num_nodes;
+    m.num_vertices = info->num_vertices;
+    long long es = 1, vs = 1;
+    for (int d = 0; d < m.dim; ++d) {
+      es *= m.P;
+      vs *= 2;
+    }
+    m.offsets.assign(offsets, offsets + m.num_elem * es);
+    m.vertex_offsets.assign(vertex_offsets, vertex_offsets + m.num_elem * vs);
+    m.coords.assign(coords, coords + static_cast<size_t>(m.dim) * m.num_vertices);
+    femhip::mesh_write(path, m);
   });
 }

@@ -152,6 +152,8 @@ SIGNATURES = {
     "fem_hip_tune_table_clear": [],
     "fem_hip_tune_table_save": [ctypes.c_char_p],
     "fem_hip_tune_table_load": [ctypes.c_char_p, _ip],
+    "fem_hip_mesh_read": [ctypes.c_char_p, ctypes.c_void_p, _vp, _vp, _vp],
+    "fem_hip_mesh_write": [ctypes.c_char_p, ctypes.c_void_p, _vp, _vp, _vp],
     "fem_hip_server_create": [ctypes.c_void_p, ctypes.POINTER(_vp)],
     "fem_hip_server_submit": [_vp, ctypes.c_void_p, _llp],
     "fem_hip_server_wait": [_vp, ctypes.c_longlong, _d, ctypes.c_void_p, _vp, _sz],
@@ -661,6 +663,33 @@ def operator_create_mass(restriction, basis, d_qdata):
 
 def operator_apply(op, d_in, d_out, stream=None):
     op.apply(d_in, d_out, stream)
+
+
+# ---------------------------------------------------------------- on-disk meshes
+class MeshInfo(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("num_elem", ctypes.c_int), ("P", ctypes.c_int), ("num_nodes", ctypes.c_int),
+                ("num_vertices", ctypes.c_int)]
+
+
+def mesh_read(path):
+    """A FEMHMSH1 mesh file as a dict (dim, num_elem, P, num_nodes, num_vertices,
+    offsets [E*P^dim], vertex_offsets [E*2^dim], coords [dim*num_vertices])."""
+    info = MeshInfo()
+    call("fem_hip_mesh_read", os.fsencode(path), ctypes.byref(info), None, None, None)
+    off = np.empty(info.num_elem * info.P ** info.dim, np.int32)
+    voff = np.empty(info.num_elem * 2 ** info.dim, np.int32)
+    coords = np.empty(info.dim * info.num_vertices)
+    call("fem_hip_mesh_read", os.fsencode(path), ctypes.byref(info), _ptr(off), _ptr(voff), _ptr(coords))
+    return {"dim": info.dim, "num_elem": info.num_elem, "P": info.P, "num_nodes": info.num_nodes,
+            "num_vertices": info.num_vertices, "offsets": off, "vertex_offsets": voff, "coords": coords}
+
+
+def mesh_write(path, mesh):
+    info = MeshInfo(mesh["dim"], mesh["num_elem"], mesh["P"], mesh["num_nodes"], mesh["num_vertices"])
+    off = np.ascontiguousarray(mesh["offsets"], np.int32)
+    voff = np.ascontiguousarray(mesh["vertex_offsets"], np.int32)
+    coords = np.ascontiguousarray(mesh["coords"], np.float64)
+    call("fem_hip_mesh_write", os.fsencode(path), ctypes.byref(info), _ptr(off), _ptr(voff), _ptr(coords))
 
 
 # ---------------------------------------------------------------- serving loop

@@ -311,6 +311,21 @@ typedef struct {
 int fem_hip_mass_solve(fem_hip_operator op, const double* b, double* x, const fem_hip_solve_options* opt,
                        fem_hip_solve_stats* stats, void* stream);
 
+/* ---- on-disk meshes ------------------------------------------------------------
+ * No reference counterpart (its meshes are built in code: tests/test-operator.cpp:90-121,
+ * tests/cuda/test-cuda-elem-restriction.cu:24-42). A FEMHMSH1 file holds a hex
+ * (quad, line) mesh: the node restriction offsets [num_elem][P^dim] in any
+ * numbering, the element vertices [num_elem][2^dim] and the vertex coordinates
+ * [dim][num_vertices], with an FNV-1a checksum. read fills info and, where the
+ * pointers are not NULL, the arrays (call once with NULL arrays for the sizes);
+ * it rejects bad headers, sizes, out-of-range indices and checksum mismatches. */
+typedef struct {
+  int dim, num_elem, P, num_nodes, num_vertices;
+} fem_hip_mesh_info;
+int fem_hip_mesh_read(const char* path, fem_hip_mesh_info* info, int* offsets, int* vertex_offsets, double* coords);
+int fem_hip_mesh_write(const char* path, const fem_hip_mesh_info* info, const int* offsets,
+                       const int* vertex_offsets, const double* coords);
+
 /* ---- serving loop ------------------------------------------------------------
  * No counterpart in the reference (it solves nothing): continuous batching of
  * mass-solve requests on one GPU. Requests are admitted while a slot is free and

@@ -192,6 +192,7 @@ def main():
             ("fp64_elem_tmap", np.float64, D, 0, fehip.FUSED_ELEM_TMAP),
             ("fp64_elem_tmap_pipe", np.float64, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
             ("fp64_block_partial_lds", np.float64, A, 0, fehip.FUSED_BLOCK_PARTIAL_LDS),
+            ("fp64_block_partial_lds_pipe", np.float64, A, 0, fehip.FUSED_BLOCK_PARTIAL_LDS_PIPE),
             ("fp64_block_atomic", np.float64, A, 0, fehip.FUSED_BLOCK_ATOMIC),
             ("fp64_elem_atomic", np.float64, A, 0, fehip.FUSED_ELEM_ATOMIC),
             ("fp64_exact_tmap", np.float64, D, 1, fehip.FUSED_ELEM_TMAP),
@@ -224,6 +225,7 @@ def main():
             kernel_names[label] = {
                 fehip.FUSED_BLOCK_PARTIAL: "fused_mass_part_kernel<%d, %d, 1, %s, %d, false>",
                 fehip.FUSED_BLOCK_PARTIAL_LDS: "fused_mass_part_kernel<%d, %d, 1, %s, %d, true>",
+                fehip.FUSED_BLOCK_PARTIAL_LDS_PIPE: "fused_mass_part_pipe_kernel<%d, %d, 1, %s, %d>",
                 fehip.FUSED_BLOCK_ATOMIC: "fused_mass_block_kernel<%d, %d, 1, %s, %d>"}[alg] % (P, Q, tname, ne)
         steps = args.steps if label.startswith("fp64") else max(args.steps // 2, 5)
         m, k, a = time_applies(fehip, p.op, d_u, d_v, steps, args.warmup, stream)

@@ -13,6 +13,7 @@ const FusedShapeReg<5, 6, 3, 16, 16> reg_5_6_3_16;
 const FusedBlockShapeReg<5, 6, 1, 16, 16> blk_5_6_1;
 const FusedBlockShapeReg<5, 6, 1, 14, 14> blk_5_6_1_14;
 const FusedBlockShapeReg<5, 6, 1, 8, 8> blk_5_6_1_8;
+const FusedBlockShapeReg<5, 6, 1, 7, 7> blk_5_6_1_7;
 const FusedBlockShapeReg<5, 6, 2, 8, 8> blk_5_6_2;
 const FusedBlockShapeReg<5, 6, 3, 8, 8> blk_5_6_3;
 

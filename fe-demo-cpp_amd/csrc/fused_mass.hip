@@ -566,10 +566,10 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
   }
   // Default order: partial-sum blocks, then (deterministic) transpose map or (atomic) block atomics, then elements.
   const int order_det[] = {kAlgBlockPart, kAlgElemTmapPipe, kAlgElemTmap};
-  const int order_any[] = {kAlgBlockPartLds, kAlgBlockPart, kAlgBlockAtomic, kAlgElemAtomic, kAlgElemTmapPipe,
-                           kAlgElemTmap};
+  const int order_any[] = {kAlgBlockPartLds, kAlgBlockPartLdsPipe, kAlgBlockPart, kAlgBlockAtomic, kAlgElemAtomic,
+                           kAlgElemTmapPipe, kAlgElemTmap};
   const int* order = det ? order_det : order_any;
-  const int n = det ? 3 : 6;
+  const int n = det ? 3 : 7;
   for (int i = 0; i < n; ++i) {
     k.alg = order[i];
     k.elems = default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);

@@ -25,14 +25,19 @@ enum FusedAlg : int {
   kAlgBlockPart = 3,   // per element block; sums in LDS, boundary nodes one partial each, then a partial-sum pass
   kAlgElemTmapPipe = 4,  // kAlgElemTmap with a persistent, software-pipelined element kernel
   kAlgBlockPartLds = 5,  // kAlgBlockPart with LDS atomic sums in place of the colour rounds (any order)
-  kAlgCount = 6
+  kAlgBlockPartLdsPipe = 6,  // kAlgBlockPartLds as a persistent, software-pipelined kernel
+  kAlgCount = 7
 };
 inline bool alg_deterministic(int alg) {
   return alg == kAlgElemTmap || alg == kAlgBlockPart || alg == kAlgElemTmapPipe;
 }
 inline bool alg_tmap(int alg) { return alg == kAlgElemTmap || alg == kAlgElemTmapPipe; }
-inline bool alg_blocked(int alg) { return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds; }
-inline bool alg_partial(int alg) { return alg == kAlgBlockPart || alg == kAlgBlockPartLds; }
+inline bool alg_blocked(int alg) {
+  return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds || alg == kAlgBlockPartLdsPipe;
+}
+inline bool alg_partial(int alg) {
+  return alg == kAlgBlockPart || alg == kAlgBlockPartLds || alg == kAlgBlockPartLdsPipe;
+}
 
 // One compiled kernel.
 struct FusedKey {

@@ -19,6 +19,7 @@
 // serving loop drives many, each on its own stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -29,7 +30,16 @@ namespace femhip {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRedBlocks = 240;  // fixed reduction grid
+constexpr int kMaxRed = 2048;  // most reduction blocks (partials per dot product)
+
+// Reduction grid for vectors of n entries: about 2 entries per thread (small
+// vectors are latency-bound), at most kMaxRed blocks (large ones stream with
+// every CU busy). Fixed for a given n, so every solve of that size sums in the
+// same order.
+int red_blocks(size_t n) {
+  const size_t per = static_cast<size_t>(kThreads) * 2;
+  return static_cast<int>(std::min<size_t>(kMaxRed, std::max<size_t>(1, (n + per - 1) / per)));
+}
 
 // Scalar slots.
 enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kNSCAL };
@@ -57,10 +67,12 @@ __global__ __launch_bounds__(kThreads) void dot_partial_kernel(size_t n, const T
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__device__ double sum_partials(const double* part, int nb) {
+// Sum of nb partials by one block: thread t adds partials t, t + kThreads, ...
+// in order, then the fixed tree of block_sum. The result is valid in thread 0.
+__device__ double sum_partials(const double* part, int nb, double* sh) {
   double s = 0.0;
-  for (int i = 0; i < nb; ++i) s += part[i];
-  return s;
+  for (int i = threadIdx.x; i < nb; i += kThreads) s += part[i];
+  return block_sum(s, sh);
 }
 
 // r = b, z = dinv * r, p = z; partials of r.z and r.r
@@ -86,13 +98,15 @@ __global__ __launch_bounds__(kThreads) void cg_init_kernel(size_t n, const T* __
   const double c = block_sum(srr, sh[1]);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
-    part[kRedBlocks + blockIdx.x] = c;
+    part[kMaxRed + blockIdx.x] = c;
   }
 }
 
-__global__ void cg_init_scalars_kernel(const double* __restrict__ part, double* __restrict__ sc, double rtol2) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double rz = sum_partials(part, kRedBlocks), rr = sum_partials(part + kRedBlocks, kRedBlocks);
+__global__ __launch_bounds__(kThreads) void cg_init_scalars_kernel(const double* __restrict__ part,
+                                                                   double* __restrict__ sc, double rtol2, int nb) {
+  __shared__ double sh[2][kThreads / 64];
+  const double rz = sum_partials(part, nb, sh[0]), rr = sum_partials(part + kMaxRed, nb, sh[1]);
+  if (threadIdx.x != 0) return;
   sc[kRZ] = rz;
   sc[kRR] = rr;
   sc[kTHRESH] = rtol2 * rr;
@@ -100,9 +114,11 @@ __global__ void cg_init_scalars_kernel(const double* __restrict__ part, double* 
   sc[kITERS] = 0.0;
 }
 
-__global__ void cg_alpha_kernel(const double* __restrict__ part, double* __restrict__ sc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double pap = sum_partials(part, kRedBlocks);
+__global__ __launch_bounds__(kThreads) void cg_alpha_kernel(const double* __restrict__ part, double* __restrict__ sc,
+                                                            int nb) {
+  __shared__ double sh[kThreads / 64];
+  const double pap = sum_partials(part, nb, sh);
+  if (threadIdx.x != 0) return;
   sc[kPAP] = pap;
   sc[kALPHA] = (sc[kDONE] != 0.0 || pap == 0.0) ? 0.0 : sc[kRZ] / pap;
 }
@@ -134,17 +150,19 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(size_t n, const dou
   const double c = block_sum(srr, sh[1]);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
-    part[kRedBlocks + blockIdx.x] = c;
+    part[kMaxRed + blockIdx.x] = c;
   }
 }
 
-__global__ void cg_beta_kernel(const double* __restrict__ part, double* __restrict__ sc, int max_iter) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(kThreads) void cg_beta_kernel(const double* __restrict__ part, double* __restrict__ sc,
+                                                           int max_iter, int nb) {
+  __shared__ double sh[2][kThreads / 64];
+  const double rz_new = sum_partials(part, nb, sh[0]), rr = sum_partials(part + kMaxRed, nb, sh[1]);
+  if (threadIdx.x != 0) return;
   if (sc[kDONE] != 0.0) {
     sc[kBETA] = 0.0;
     return;
   }
-  const double rz_new = sum_partials(part, kRedBlocks), rr = sum_partials(part + kRedBlocks, kRedBlocks);
   sc[kITERS] += 1.0;
   sc[kRR] = rr;
   sc[kBETA] = (sc[kRZ] == 0.0) ? 0.0 : rz_new / sc[kRZ];
@@ -191,8 +209,11 @@ __global__ __launch_bounds__(kThreads) void add_correction_kernel(size_t n, cons
   if (i < n) x[i] += static_cast<double>(d[i]);
 }
 
-__global__ void sum_to_scalar_kernel(const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = sum_partials(part, kRedBlocks);
+__global__ __launch_bounds__(kThreads) void sum_to_scalar_kernel(const double* __restrict__ part,
+                                                                 double* __restrict__ out, int nb) {
+  __shared__ double sh[kThreads / 64];
+  const double v = sum_partials(part, nb, sh);
+  if (threadIdx.x == 0) *out = v;
 }
 
 // One PCG instance (vectors of T) over an apply callback.
@@ -207,7 +228,10 @@ struct Pcg {
   const T* b_bound = nullptr;  // the b and x the graph was captured with
   T* x_bound = nullptr;
 
-  explicit Pcg(size_t n_) : n(n_), r(n_), z(n_), p(n_), Ap(n_), dinv(n_), part(2 * kRedBlocks), sc(kNSCAL) {}
+  int nb;  // reduction blocks
+
+  explicit Pcg(size_t n_)
+      : n(n_), r(n_), z(n_), p(n_), Ap(n_), dinv(n_), part(2 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)) {}
   ~Pcg() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -216,19 +240,18 @@ struct Pcg {
   template <class Apply>
   void iteration(Apply&& apply, T* x, int max_iter, hipStream_t s) {
     apply(p.data(), Ap.data(), s);
-    dot_partial_kernel<T><<<kRedBlocks, kThreads, 0, s>>>(n, p.data(), Ap.data(), part.data());
-    cg_alpha_kernel<<<1, 1, 0, s>>>(part.data(), sc.data());
-    cg_update_kernel<T><<<kRedBlocks, kThreads, 0, s>>>(n, sc.data(), p.data(), Ap.data(), dinv.data(), x, r.data(),
-                                                        z.data(), part.data());
-    cg_beta_kernel<<<1, 1, 0, s>>>(part.data(), sc.data(), max_iter);
-    cg_p_kernel<T><<<kRedBlocks, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
+    dot_partial_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), part.data());
+    cg_alpha_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), nb);
+    cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), p.data(), Ap.data(), dinv.data(), x, r.data(), z.data(),
+                                                part.data());
+    cg_beta_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), max_iter, nb);
+    cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
   }
 
   // x = 0 and the first residual, for A x = b with ||r|| <= rtol ||b|| as the goal.
   void init(const T* b, T* x, double rtol, hipStream_t s) {
-    cg_init_kernel<T><<<kRedBlocks, kThreads, 0, s>>>(n, b, dinv.data(), x, r.data(), z.data(), p.data(),
-                                                      part.data());
-    cg_init_scalars_kernel<<<1, 1, 0, s>>>(part.data(), sc.data(), rtol * rtol);
+    cg_init_kernel<T><<<nb, kThreads, 0, s>>>(n, b, dinv.data(), x, r.data(), z.data(), p.data(), part.data());
+    cg_init_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb);
     HIP_CHECK_LAUNCH();
   }
 
@@ -320,16 +343,18 @@ namespace {
 void enqueue_residual(SolveJob& j, float* r32) {
   MassSolverState& st = *j.st;
   fused_mass_apply64(*j.f, j.x, st.Mx.data(), j.s);
-  residual_kernel<<<kRedBlocks, kThreads, 0, j.s>>>(st.n, j.b, st.Mx.data(), st.r64.data(), r32, st.part.data());
-  sum_to_scalar_kernel<<<1, 1, 0, j.s>>>(st.part.data(), st.scal.data());
+  const int nb = red_blocks(st.n);
+  residual_kernel<<<nb, kThreads, 0, j.s>>>(st.n, j.b, st.Mx.data(), st.r64.data(), r32, st.part.data());
+  sum_to_scalar_kernel<<<1, kThreads, 0, j.s>>>(st.part.data(), st.scal.data(), nb);
   HIP_CHECK_LAUNCH();
   HIP_CHECK(hipMemcpyAsync(j.hs + kHostRes, st.scal.data(), sizeof(double), hipMemcpyDeviceToHost, j.s));
 }
 
 void enqueue_bnorm(SolveJob& j) {
   MassSolverState& st = *j.st;
-  dot_partial_kernel<double><<<kRedBlocks, kThreads, 0, j.s>>>(st.n, j.b, j.b, st.part.data());
-  sum_to_scalar_kernel<<<1, 1, 0, j.s>>>(st.part.data(), st.scal.data());
+  const int nb = red_blocks(st.n);
+  dot_partial_kernel<double><<<nb, kThreads, 0, j.s>>>(st.n, j.b, j.b, st.part.data());
+  sum_to_scalar_kernel<<<1, kThreads, 0, j.s>>>(st.part.data(), st.scal.data(), nb);
   HIP_CHECK_LAUNCH();
   HIP_CHECK(hipMemcpyAsync(j.hs + kHostB, st.scal.data(), sizeof(double), hipMemcpyDeviceToHost, j.s));
 }
@@ -429,7 +454,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
     ns->diag.allocate(n);
     ns->Mx.allocate(n);
     ns->r64.allocate(n);
-    ns->part.allocate(2 * kRedBlocks);
+    ns->part.allocate(2 * kMaxRed);
     ns->scal.allocate(4);
     fused_mass_diagonal(f, ns->diag.data(), s);
     state = ns.release();

@@ -221,6 +221,7 @@ int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, in
                                          (deterministic) */
 #define FEM_HIP_FUSED_ELEM_TMAP_PIPE 4 /* ELEM_TMAP with a persistent, software-pipelined element kernel */
 #define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS 5 /* BLOCK_PARTIAL summing in LDS with atomics (any order) */
+#define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS_PIPE 6 /* BLOCK_PARTIAL_LDS as a persistent, software-pipelined kernel */
 /* Explicit algorithm and its elements per workgroup (element kernels) or per
  * block (block kernels); 0 = that algorithm's default. A deterministic scatter
  * mode accepts only the deterministic algorithms. exact = 1 overrides. */

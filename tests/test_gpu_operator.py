@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-13
 ALGS = (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_ATOMIC, fehip.FUSED_BLOCK_PARTIAL,
-        fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BLOCK_PARTIAL_LDS)
+        fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BLOCK_PARTIAL_LDS, fehip.FUSED_BLOCK_PARTIAL_LDS_PIPE)
 DET_ALGS = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP_PIPE)
 
 

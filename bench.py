@@ -188,7 +188,8 @@ def main():
         rows = [c for c in tuning.get(tag, {}).get("candidates", []) if c["alg"] == fehip.FUSED_NAMES[alg]]
         return min(rows, key=lambda c: c["ms"])["elems"] if rows else 0
 
-    runs = (("fp64_block_partial", np.float64, D, 0, fehip.FUSED_BLOCK_PARTIAL),
+    runs = (("fp64_box_column", np.float64, D, 0, fehip.FUSED_BOX_COLUMN),
+            ("fp64_block_partial", np.float64, D, 0, fehip.FUSED_BLOCK_PARTIAL),
             ("fp64_elem_tmap", np.float64, D, 0, fehip.FUSED_ELEM_TMAP),
             ("fp64_elem_tmap_pipe", np.float64, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
             ("fp64_block_partial_lds", np.float64, A, 0, fehip.FUSED_BLOCK_PARTIAL_LDS),
@@ -196,6 +197,7 @@ def main():
             ("fp64_block_atomic", np.float64, A, 0, fehip.FUSED_BLOCK_ATOMIC),
             ("fp64_elem_atomic", np.float64, A, 0, fehip.FUSED_ELEM_ATOMIC),
             ("fp64_exact_tmap", np.float64, D, 1, fehip.FUSED_ELEM_TMAP),
+            ("fp32_box_column", np.float32, D, 0, fehip.FUSED_BOX_COLUMN),
             ("fp32_block_partial", np.float32, D, 0, fehip.FUSED_BLOCK_PARTIAL),
             ("fp32_elem_tmap", np.float32, D, 0, fehip.FUSED_ELEM_TMAP),
             ("fp32_elem_tmap_pipe", np.float32, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
@@ -221,6 +223,9 @@ def main():
                 "exact" if exact else "fast", P, Q, tname, ne, "true" if alg == fehip.FUSED_ELEM_TMAP else "false")
         elif alg == fehip.FUSED_ELEM_TMAP_PIPE:
             kernel_names[label] = "fused_mass_pipe_kernel<%d, %d, 1, %s, %d>" % (P, Q, tname, ne)
+        elif alg == fehip.FUSED_BOX_COLUMN:
+            bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne]
+            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d>" % (P, Q, tname, bx, by)
         else:
             kernel_names[label] = {
                 fehip.FUSED_BLOCK_PARTIAL: "fused_mass_part_kernel<%d, %d, 1, %s, %d, false>",
@@ -232,9 +237,11 @@ def main():
         s = 4 if fp32 else 8
         # The fused kernel's own algorithmic bytes: u, qdata and offsets read; v
         # written, except the element-output (tmap) kernels, which write E*P^3
-        # values and leave the sums to the transpose-map pass.
+        # values and leave the sums to the transpose-map pass, and the box-column
+        # kernel, which reads no offsets (the box formula gives every node).
         tmap = alg in (fehip.FUSED_ELEM_TMAP, fehip.FUSED_ELEM_TMAP_PIPE)
-        kbytes = s * L + s * E * Q ** 3 + 4 * E * P3 + (s * E * P3 if tmap else s * L)
+        offs = 0 if alg == fehip.FUSED_BOX_COLUMN else 4 * E * P3
+        kbytes = s * L + s * E * Q ** 3 + offs + (s * E * P3 if tmap else s * L)
         variants[label] = {"ms_per_apply": m, "gdofs": L / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
                            "kernel_GBs": kbytes / (k * 1e-3) / 1e9, "kernel_frac_hbm": kbytes / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
                            "apply_frac_hbm": (bytes32 if s == 4 else bytes64) / (m * 1e-3) / 1e9 / HBM_PEAK_GBS,

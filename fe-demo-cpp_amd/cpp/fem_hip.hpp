@@ -339,6 +339,136 @@ inline fem_hip_solve_stats mass_solve(Operator& op, const DeviceArray<double>& b
   return st;
 }
 
+// Jacobi diagonal of a fused mass operator (deterministic).
+inline void mass_diagonal(Operator& op, DeviceArray<double>& diag, void* stream = nullptr) {
+  diag.resize(op.active_out_l_size);
+  FEM_HIP_CHECK(fem_hip_operator_mass_diagonal(op.handle.get(), diag.data(), diag.size(), stream));
+}
+
+// How a fused mass apply reaches the output (values of FEM_HIP_FUSED_*).
+enum class FusedAlgorithm {
+  Auto = FEM_HIP_FUSED_AUTO,
+  ElemAtomic = FEM_HIP_FUSED_ELEM_ATOMIC,
+  ElemTmap = FEM_HIP_FUSED_ELEM_TMAP,
+  BlockAtomic = FEM_HIP_FUSED_BLOCK_ATOMIC,
+  BlockPartial = FEM_HIP_FUSED_BLOCK_PARTIAL,
+  ElemTmapPipe = FEM_HIP_FUSED_ELEM_TMAP_PIPE,
+  BlockPartialLds = FEM_HIP_FUSED_BLOCK_PARTIAL_LDS,
+  BlockPartialLdsPipe = FEM_HIP_FUSED_BLOCK_PARTIAL_LDS_PIPE,
+};
+struct FusedChoice {
+  FusedAlgorithm alg = FusedAlgorithm::Auto;
+  int elems = 0;
+  double ms = 0.0;  // time per apply (autotune only)
+};
+
+// Fixes the algorithm (and elements per workgroup or block; 0 = its default) of
+// the next fused applies; Auto returns to the tuning table's choice.
+inline void operator_set_fused_algorithm(Operator& op, FusedAlgorithm alg, int elems = 0) {
+  FEM_HIP_CHECK(fem_hip_operator_set_fused_algorithm(op.handle.get(), static_cast<int>(alg), elems));
+}
+
+inline FusedChoice operator_fused_choice(Operator& op, bool fp32 = false) {
+  int alg = 0, elems = 0;
+  FEM_HIP_CHECK(fem_hip_operator_get_fused_choice(op.handle.get(), fp32 ? 1 : 0, &alg, &elems));
+  return {static_cast<FusedAlgorithm>(alg), elems, 0.0};
+}
+
+// Times every compiled algorithm and packing on the operator's own data and
+// records the fastest in the process-wide tuning table. Synchronous.
+inline FusedChoice operator_autotune(Operator& op, bool fp32 = false, int reps = 10, void* stream = nullptr) {
+  int alg = 0, elems = 0, n = 0;
+  double ms = 0.0;
+  FEM_HIP_CHECK(fem_hip_operator_autotune(op.handle.get(), fp32 ? 1 : 0, reps, stream, &alg, &elems, &ms, nullptr, 0,
+                                          &n));
+  return {static_cast<FusedAlgorithm>(alg), elems, ms};
+}
+
+// Tuning tables as text; load returns the rows taken.
+inline int tune_table_load(const std::string& path) {
+  int taken = 0;
+  FEM_HIP_CHECK(fem_hip_tune_table_load(path.c_str(), &taken));
+  return taken;
+}
+inline void tune_table_save(const std::string& path) { FEM_HIP_CHECK(fem_hip_tune_table_save(path.c_str())); }
+
+// A FEMHMSH1 mesh file: node restriction offsets [num_elem][P^dim], element
+// vertices [num_elem][2^dim], coordinates [dim][num_vertices].
+struct Mesh {
+  fem_hip_mesh_info info{};
+  std::vector<int> offsets, vertex_offsets;
+  std::vector<double> coords;
+};
+
+inline Mesh mesh_read(const std::string& path) {
+  Mesh m;
+  FEM_HIP_CHECK(fem_hip_mesh_read(path.c_str(), &m.info, nullptr, nullptr, nullptr));
+  size_t es = 1, vs = 1;
+  for (int d = 0; d < m.info.dim; ++d) {
+    es *= static_cast<size_t>(m.info.P);
+    vs *= 2;
+  }
+  m.offsets.resize(es * m.info.num_elem);
+  m.vertex_offsets.resize(vs * m.info.num_elem);
+  m.coords.resize(static_cast<size_t>(m.info.dim) * m.info.num_vertices);
+  FEM_HIP_CHECK(fem_hip_mesh_read(path.c_str(), &m.info, m.offsets.data(), m.vertex_offsets.data(), m.coords.data()));
+  return m;
+}
+
+inline void mesh_write(const std::string& path, const Mesh& m) {
+  FEM_HIP_CHECK(fem_hip_mesh_write(path.c_str(), &m.info, m.offsets.data(), m.vertex_offsets.data(), m.coords.data()));
+}
+
+// The node restriction of a mesh as the reference's host type, with num_comp
+// blocked components (comp_stride = num_nodes).
+inline fem::ElemRestriction mesh_restriction(const Mesh& m, int num_comp = 1) {
+  fem::ElemRestriction r;
+  const int es = static_cast<int>(m.offsets.size() / m.info.num_elem);
+  fem::elem_restriction_create(m.info.num_elem, es, num_comp, m.info.num_nodes, num_comp * m.info.num_nodes,
+                               m.offsets, r);
+  return r;
+}
+
+// Continuous-batching mass-solve server (fem_hip_server_*): submit requests,
+// wait for their results. Destruction finishes every submitted request.
+class Server {
+ public:
+  explicit Server(const fem_hip_server_options& opt) {
+    fem_hip_server s = nullptr;
+    FEM_HIP_CHECK(fem_hip_server_create(&opt, &s));
+    handle_.reset(s);
+  }
+  long long submit(const fem_hip_serve_request& req) {
+    long long id = 0;
+    FEM_HIP_CHECK(fem_hip_server_submit(handle_.get(), &req, &id));
+    return id;
+  }
+  // Blocks until request `id` is done; throws if the request failed.
+  fem_hip_serve_result wait(long long id) {
+    fem_hip_serve_result res{};
+    FEM_HIP_CHECK(fem_hip_server_wait(handle_.get(), id, -1.0, &res, nullptr, 0));
+    return res;
+  }
+  // Same, and copies the solution (l_size = num_comp * (n*(P-1)+1)^3 entries;
+  // the server must keep solutions).
+  fem_hip_serve_result wait(long long id, long long l_size, std::vector<double>& x) {
+    fem_hip_serve_result res{};
+    x.resize(static_cast<size_t>(l_size));
+    FEM_HIP_CHECK(fem_hip_server_wait(handle_.get(), id, -1.0, &res, x.data(), x.size()));
+    return res;
+  }
+
+ private:
+  struct Destroy {
+    void operator()(fem_hip_server s) const { (void)fem_hip_server_destroy(s); }
+  };
+  std::unique_ptr<fem_hip_server_s, Destroy> handle_;
+};
+
+// Defaults: 16 requests at once, 8 setup workers, half the free HBM, 8 CG
+// iterations per graph launch, no autotuning, solutions kept, transpose-map scatter.
+inline fem_hip_server_options server_options() { return fem_hip_server_options{16, 8, 0.5, 8, 0, 1, 1}; }
+
 inline void synchronize() { FEM_HIP_CHECK(fem_hip_device_synchronize()); }
 
 }  // namespace fem::hip

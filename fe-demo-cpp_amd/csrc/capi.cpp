@@ -12,6 +12,7 @@
 #include "basis.hpp"
 #include "basis_setup.hpp"
 #include "block_plan.hpp"
+#include "fused_box_kernel.hpp"
 #include "mesh_io.hpp"
 #include "server.hpp"
 #include "common.hpp"
@@ -525,6 +526,13 @@ int fem_hip_operator_set_fused_algorithm(fem_hip_operator h, int alg, int elems)
     h->op->opt_elems = alg < 0 ? 0 : elems;
   });
 }
+int fem_hip_operator_set_box_layers(fem_hip_operator h, int layers) {
+  return guard([&] {
+    need(h, "operator");
+    require(layers >= 0, "set_box_layers: layers must be >= 0");
+    h->op->opt_box_lz = layers;
+  });
+}
 int fem_hip_operator_get_fused_choice(fem_hip_operator h, int fp32, int* alg, int* elems) {
   return guard([&] {
     need(h, "operator");
@@ -682,6 +690,109 @@ int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords) {
   return guard([&] {
     need(coords, "coords");
     box_coords(n, jitter, seed, coords);
+  });
+}
+
+int fem_hip_box_detect(int num_elem, int P, const int* offsets, int num_comp, int comp_stride, int l_size,
+                       int* n3, int* is_box) {
+  return guard([&] {
+    need(offsets, "offsets");
+    need(n3, "n3");
+    need(is_box, "is_box");
+    require(num_elem > 0 && P >= 2 && num_comp > 0 && l_size > 0, "box_detect: bad sizes");
+    *is_box = detect_box(offsets, num_elem, P, num_comp, comp_stride, l_size, n3) ? 1 : 0;
+  });
+}
+
+int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layers, long long* stats) {
+  return guard([&] {
+    need(stats, "stats");
+    int BX = 0, BY = 0;
+    require(box_columns(elems, &BX, &BY), "box_column_check: no box-column variant with these elems");
+    require(nx > 0 && ny > 0 && nz > 0 && P >= 2 && nx % BX == 0 && ny % BY == 0, "box_column_check: bad sizes");
+    BoxGeom g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.Nx = nx * (P - 1) + 1;
+    g.Ny = ny * (P - 1) + 1;
+    g.Nz = nz * (P - 1) + 1;
+    g.ncx = nx / BX;
+    g.ncy = ny / BY;
+    require(layers >= 0 && (layers == 0 || nz % layers == 0), "box_column_check: layers must divide nz");
+    g.lz = layers > 0 ? layers : box_layers(nz, static_cast<long long>(g.ncx) * g.ncy, 2048, P);
+    require(g.lz > 0, "box_column_check: no layer count fits");
+    g.nsz = nz / g.lz;
+    g.cs = g.Nx * g.Ny * g.Nz;
+    const int LXn = BX * (P - 1) + 1, LYn = BY * (P - 1) + 1, LZn = g.lz * (P - 1) + 1;
+    require(LXn >= 3 && LYn >= 3 && LZn >= 3, "box_column_check: columns too thin");
+    const long long nseg = static_cast<long long>(g.ncx) * g.ncy * g.nsz;
+    const long long face = box_face_size(LXn, LYn, LZn);
+    const long long N = static_cast<long long>(g.Nx) * g.Ny * g.Nz;
+    require(N * 2 + nseg * face < (1ll << 31), "box_column_check: too large to check");
+    auto fail = [](const char* m) { throw std::runtime_error(std::string("box columns: ") + m); };
+    // What the column kernel writes: each region node once, to v or to a slot.
+    std::vector<int> direct(N, 0);
+    std::vector<char> slot(static_cast<size_t>(nseg * face), 0);
+    long long slots_written = 0, direct_nodes = 0;
+    for (long long w = 0; w < nseg; ++w) {
+      const int cx = static_cast<int>(w % g.ncx), cy = static_cast<int>((w / g.ncx) % g.ncy),
+                sz = static_cast<int>(w / (static_cast<long long>(g.ncx) * g.ncy));
+      for (int lz = 0; lz < LZn; ++lz)
+        for (int ly = 0; ly < LYn; ++ly)
+          for (int lx = 0; lx < LXn; ++lx) {
+            const bool xs = (lx == 0 && cx > 0) || (lx == LXn - 1 && cx < g.ncx - 1);
+            const bool ys = (ly == 0 && cy > 0) || (ly == LYn - 1 && cy < g.ncy - 1);
+            const bool zs = (lz == 0 && sz > 0) || (lz == LZn - 1 && sz < g.nsz - 1);
+            const long long node = (cx * (LXn - 1) + lx) +
+                                   static_cast<long long>(g.Nx) * ((cy * (LYn - 1) + ly) +
+                                                                   static_cast<long long>(g.Ny) * (sz * (LZn - 1) + lz));
+            if (!(xs || ys || zs)) {
+              ++direct[node];
+              ++direct_nodes;
+            } else {
+              const int sl = box_slot(lx, ly, lz, xs, ys, LXn, LYn, LZn);
+              if (sl < 0 || sl >= face) fail("slot outside the segment's area");
+              char& c = slot[static_cast<size_t>(w * face + sl)];
+              if (c) fail("two nodes of a segment share a slot");
+              c = 1;
+              ++slots_written;
+            }
+          }
+    }
+    // What the shared-node pass reads: each shared node once, each written slot once.
+    const long long count = box_shared_count(g, P, BX, BY);
+    std::vector<char> shared(N, 0);
+    int max_owners = 0;
+    long long slots_read = 0;
+    for (long long id = 0; id < count; ++id) {
+      int gx, gy, gz, seg[8], sl[8];
+      box_shared_node(g, P, BX, BY, id, &gx, &gy, &gz);
+      if (gx < 0 || gx >= g.Nx || gy < 0 || gy >= g.Ny || gz < 0 || gz >= g.Nz) fail("shared node out of the box");
+      const long long node = gx + static_cast<long long>(g.Nx) * (gy + static_cast<long long>(g.Ny) * gz);
+      if (shared[node]) fail("a shared node is enumerated twice");
+      shared[node] = 1;
+      const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, sl);
+      if (n < 2) fail("a shared node has fewer than 2 segments");
+      max_owners = std::max(max_owners, n);
+      for (int i = 0; i < n; ++i) {
+        if (i > 0 && seg[i] <= seg[i - 1]) fail("segments not ascending");
+        if (seg[i] < 0 || seg[i] >= nseg || sl[i] < 0 || sl[i] >= face) fail("owner out of range");
+        char& c = slot[static_cast<size_t>(static_cast<long long>(seg[i]) * face + sl[i])];
+        if (c != 1) fail("a shared node reads a slot no segment wrote for it, or reads it twice");
+        c = 2;
+        ++slots_read;
+      }
+    }
+    for (long long node = 0; node < N; ++node)
+      if (direct[node] + shared[node] != 1) fail("a node is not written exactly once");
+    if (slots_read != slots_written) fail("a written slot is never read");
+    stats[0] = nseg;
+    stats[1] = g.lz;
+    stats[2] = count;
+    stats[3] = slots_written;
+    stats[4] = direct_nodes;
+    stats[5] = max_owners;
   });
 }
 

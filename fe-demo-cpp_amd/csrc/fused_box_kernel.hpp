@@ -1,0 +1,513 @@
+// Box-column algorithm of the fused mass apply, for node restrictions that are
+// a structured nx x ny x nz box of P^3-node hexes with lexicographic node
+// numbering (offset of local node (ix,iy,iz) of element (ex,ey,ez) is
+// gx + Nx*(gy + Ny*gz), g = e*(P-1) + i; e = ex + nx*(ey + ny*ez)). Nothing
+// about the mesh is read from memory: only qdata, u and v move.
+//
+// The elements are cut into columns of BX x BY elements in x and y, and each
+// column into segments of lz element layers in z; one workgroup owns one
+// segment and marches up its layers. Per layer:
+//   * each of the BX*BY elements runs the fast kernel's five in-place stages
+//     in its own LDS tile (u gathered straight from its x-rows), each 1D
+//     product in the even-odd form of the symmetric basis (eo_forward);
+//   * the element outputs are added into a ring of P node planes in LDS in 4
+//     colour rounds ((ex&1) | (ey&1)<<1): elements of one colour share no node,
+//     so every node sums its contributions in a fixed order;
+//   * the layer's lower P-1 planes are then complete (the top one waits for the
+//     next layer, whose bottom plane it is): their nodes go straight to v,
+//     except nodes on a face the segment shares with another segment (a
+//     column face inside the box, or a segment end inside the box), which go
+//     to the segment's partial area.
+// A second pass gives each shared node the sum of its 2, 4 or 8 partials in
+// ascending segment order. Every node of v is written exactly once, the
+// result is the same bit for bit on every run, and no atomics are used.
+//
+// Partial area of segment w: face_size = 2*LYn*LZn + 2*LXn*LZn + 2*LXn*LYn
+// slots (LXn = BX*(P-1)+1, LYn = BY*(P-1)+1, LZn = lz*(P-1)+1 nodes), slot by
+// the first shared direction of the node (x, else y, else z): see box_slot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fused_mass_kernel.hpp"
+
+namespace femhip {
+
+// Geometry of one launch (all sizes in elements unless noted).
+struct BoxGeom {
+  int nx, ny, nz;     // elements per direction
+  int Nx, Ny, Nz;     // nodes per direction
+  int ncx, ncy, nsz;  // columns in x, y; segments per column
+  int lz;             // element layers per segment
+  int cs;             // component stride of u and v (= Nx*Ny*Nz)
+};
+
+__host__ __device__ inline long long box_face_size(int LXn, int LYn, int LZn) {
+  return 2LL * LYn * LZn + 2LL * LXn * LZn + 2LL * LXn * LYn;
+}
+
+// Slot of local node (lx, ly, lz) in its segment's partial area; xs / ys say
+// whether the node's x / y plane is a column face inside the box (else the
+// node lies on a shared segment end).
+__host__ __device__ inline int box_slot(int lx, int ly, int lz, bool xs, bool ys, int LXn, int LYn, int LZn) {
+  if (xs) return (lx == 0 ? 0 : 1) * LYn * LZn + lz * LYn + ly;
+  int base = 2 * LYn * LZn;
+  if (ys) return base + (ly == 0 ? 0 : 1) * LXn * LZn + lz * LXn + lx;
+  base += 2 * LXn * LZn;
+  return base + (lz == 0 ? 0 : 1) * LXn * LYn + ly * LXn + lx;
+}
+
+// The q-th node coordinate, in ascending order, that is not an interior cut
+// (cuts at multiples of S strictly inside [0, N-1], nb of them).
+__host__ __device__ inline int box_uncut(int q, int S, int nb) {
+  if (q == 0) return 0;
+  const int b = (q - 1) / (S - 1);
+  return q + (b < nb ? b : nb);
+}
+
+// Shared nodes: interior column faces in x (all y, z), then in y (x not a
+// cut), then interior segment ends (x and y not cuts).
+__host__ __device__ inline long long box_shared_count(const BoxGeom& g, int P, int BX, int BY) {
+  const long long nbx = g.ncx - 1, nby = g.ncy - 1, nbz = g.nsz - 1;
+  return nbx * g.Ny * g.Nz + nby * (g.Nx - nbx) * g.Nz + nbz * (g.Nx - nbx) * (g.Ny - nby);
+}
+
+// Coordinates of shared node `id` (0 <= id < box_shared_count).
+__host__ __device__ inline void box_shared_node(const BoxGeom& g, int P, int BX, int BY, long long id, int* gx,
+                                                int* gy, int* gz) {
+  const int SX = BX * (P - 1), SY = BY * (P - 1), SZ = g.lz * (P - 1);
+  const int nbx = g.ncx - 1, nby = g.ncy - 1;
+  const long long cx = static_cast<long long>(nbx) * g.Ny * g.Nz;
+  if (id < cx) {
+    *gy = static_cast<int>(id % g.Ny);
+    const long long r = id / g.Ny;
+    *gz = static_cast<int>(r % g.Nz);
+    *gx = static_cast<int>(r / g.Nz + 1) * SX;
+    return;
+  }
+  id -= cx;
+  const int Nxu = g.Nx - nbx, Nyu = g.Ny - nby;
+  const long long cy = static_cast<long long>(nby) * Nxu * g.Nz;
+  if (id < cy) {
+    *gx = box_uncut(static_cast<int>(id % Nxu), SX, nbx);
+    const long long r = id / Nxu;
+    *gz = static_cast<int>(r % g.Nz);
+    *gy = static_cast<int>(r / g.Nz + 1) * SY;
+    return;
+  }
+  id -= cy;
+  *gx = box_uncut(static_cast<int>(id % Nxu), SX, nbx);
+  const long long r = id / Nxu;
+  *gy = box_uncut(static_cast<int>(r % Nyu), SY, nby);
+  *gz = static_cast<int>(r / Nyu + 1) * SZ;
+}
+
+// The segments (ascending) whose node regions hold node (gx, gy, gz), with the
+// node's slot in each; returns how many (1, 2, 4 or 8).
+__host__ __device__ inline int box_owners(const BoxGeom& g, int P, int BX, int BY, int gx, int gy, int gz, int* seg,
+                                          int* slot) {
+  const int SX = BX * (P - 1), SY = BY * (P - 1), SZ = g.lz * (P - 1);
+  const int LXn = SX + 1, LYn = SY + 1, LZn = SZ + 1;
+  const bool xs = gx % SX == 0 && gx > 0 && gx < g.Nx - 1;
+  const bool ys = gy % SY == 0 && gy > 0 && gy < g.Ny - 1;
+  const bool zs = gz % SZ == 0 && gz > 0 && gz < g.Nz - 1;
+  int cx1 = gx / SX, cy1 = gy / SY, sz1 = gz / SZ;
+  if (cx1 > g.ncx - 1) cx1 = g.ncx - 1;
+  if (cy1 > g.ncy - 1) cy1 = g.ncy - 1;
+  if (sz1 > g.nsz - 1) sz1 = g.nsz - 1;
+  const int cx0 = xs ? cx1 - 1 : cx1, cy0 = ys ? cy1 - 1 : cy1, sz0 = zs ? sz1 - 1 : sz1;
+  int n = 0;
+  for (int s = sz0; s <= sz1; ++s)
+    for (int cy = cy0; cy <= cy1; ++cy)
+      for (int cx = cx0; cx <= cx1; ++cx) {
+        seg[n] = (s * g.ncy + cy) * g.ncx + cx;
+        slot[n] = box_slot(gx - cx * SX, gy - cy * SY, gz - s * SZ, xs, ys, LXn, LYn, LZn);
+        ++n;
+      }
+  return n;
+}
+
+// Even-odd form of a symmetric 1D interpolation matrix B [Q][P]
+// (B(Q-1-q, P-1-p) = B(q, p), true for Lagrange bases on symmetric nodes at
+// symmetric points): with x_e = x_j + x_(P-1-j), x_o = x_j - x_(P-1-j),
+//   out_q, out_(Q-1-q) = sum_j Be(q,j) x_e[j] (+ Bm(q) x_mid) +- sum_j Bo(q,j) x_o[j],
+//   Be(q,j) = (B(q,j) + B(q,P-1-j))/2, Bo(q,j) = (B(q,j) - B(q,P-1-j))/2, Bm(q) = B(q,(P-1)/2),
+// and the same values give the transposed product. Half the basis values and
+// about half the multiply-adds of the plain product (Kronbichler & Kormann,
+// "Fast matrix-free evaluation of discontinuous Galerkin finite element
+// operators", ACM TOMS 45(3), 2019, sec. 4.2).
+template <int P, int Q>
+struct EvenOdd {
+  static constexpr int HP = P / 2, HQ = (Q + 1) / 2, QF = Q / 2, PM = P & 1, QM = Q & 1;
+  static constexpr int BE = 0, BM = HQ * HP, BO = HQ * HP + PM * HQ, N = BO + QF * HP;
+};
+
+// Packs Be, Bm, Bo (EvenOdd layout) from B [Q][P].
+template <int P, int Q, class T>
+MatArg<T, EvenOdd<P, Q>::N> even_odd_arg(const void* basis) {
+  using E = EvenOdd<P, Q>;
+  const T* B = static_cast<const T*>(basis);
+  MatArg<T, E::N> M;
+  for (int q = 0; q < E::HQ; ++q)
+    for (int j = 0; j < E::HP; ++j) M.m[E::BE + q * E::HP + j] = (B[q * P + j] + B[q * P + P - 1 - j]) / T(2);
+  if (E::PM)
+    for (int q = 0; q < E::HQ; ++q) M.m[E::BM + q] = B[q * P + P / 2];
+  for (int q = 0; q < E::QF; ++q)
+    for (int j = 0; j < E::HP; ++j) M.m[E::BO + q * E::HP + j] = (B[q * P + j] - B[q * P + P - 1 - j]) / T(2);
+  return M;
+}
+
+// out[Q] = B x[P]
+template <int P, int Q, class T>
+__device__ __forceinline__ void eo_forward(const T (&x)[P], T (&out)[Q], const MatArg<T, EvenOdd<P, Q>::N>& M) {
+  using E = EvenOdd<P, Q>;
+  T xe[E::HP > 0 ? E::HP : 1], xo[E::HP > 0 ? E::HP : 1];
+#pragma unroll
+  for (int j = 0; j < E::HP; ++j) {
+    xe[j] = x[j] + x[P - 1 - j];
+    xo[j] = x[j] - x[P - 1 - j];
+  }
+#pragma unroll
+  for (int q = 0; q < E::HQ; ++q) {
+    T e = E::PM ? M.m[E::BM + q] * x[P / 2] : T(0);
+#pragma unroll
+    for (int j = 0; j < E::HP; ++j) e = fma_t(M.m[E::BE + q * E::HP + j], xe[j], e);
+    if (q < E::QF) {
+      T o = T(0);
+#pragma unroll
+      for (int j = 0; j < E::HP; ++j) o = fma_t(M.m[E::BO + q * E::HP + j], xo[j], o);
+      out[q] = e + o;
+      out[Q - 1 - q] = e - o;
+    } else {
+      out[q] = e;  // the middle point (Q odd)
+    }
+  }
+}
+
+// out[P] = B^T y[Q]
+template <int P, int Q, class T>
+__device__ __forceinline__ void eo_transpose(const T (&y)[Q], T (&out)[P], const MatArg<T, EvenOdd<P, Q>::N>& M) {
+  using E = EvenOdd<P, Q>;
+  T ye[E::QF > 0 ? E::QF : 1], yo[E::QF > 0 ? E::QF : 1];
+#pragma unroll
+  for (int k = 0; k < E::QF; ++k) {
+    ye[k] = y[k] + y[Q - 1 - k];
+    yo[k] = y[k] - y[Q - 1 - k];
+  }
+#pragma unroll
+  for (int p = 0; p < E::HP; ++p) {
+    T e = E::QM ? M.m[E::BE + E::QF * E::HP + p] * y[E::QF] : T(0);
+#pragma unroll
+    for (int k = 0; k < E::QF; ++k) e = fma_t(M.m[E::BE + k * E::HP + p], ye[k], e);
+    T o = T(0);
+#pragma unroll
+    for (int k = 0; k < E::QF; ++k) o = fma_t(M.m[E::BO + k * E::HP + p], yo[k], o);
+    out[p] = e + o;
+    out[P - 1 - p] = e - o;
+  }
+  if (E::PM) {
+    T m = E::QM ? M.m[E::BM + E::QF] * y[E::QF] : T(0);
+#pragma unroll
+    for (int k = 0; k < E::QF; ++k) m = fma_t(M.m[E::BM + k], ye[k], m);
+    out[P / 2] = m;
+  }
+}
+
+template <int P, int Q, int NC, class T, int BX, int BY>
+__global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_kernel(
+    BoxGeom g, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
+    size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em) {
+  using S = Shape<P, Q>;
+  constexpr int NE = BX * BY;
+  constexpr int P2 = S::P2, Q2 = S::Q2, Q3 = S::Q3, TPE = S::TPE, RY = S::RY, ZS = S::ZS;
+  constexpr int LXn = BX * (P - 1) + 1, LYn = BY * (P - 1) + 1, PLANE = LXn * LYn;
+  __shared__ T s_b[NE][S::FBUF];
+  __shared__ T s_top[NC][PLANE];  // top node plane of the last layer, completed by the next one
+  static_assert(PLANE <= NE * S::TPE, "one node position per thread");
+
+  // Segment of this workgroup; XCD x (which receives workgroups h with
+  // h % 8 == x) gets the contiguous range [x*per, (x+1)*per) when 8 divides the grid.
+  const int nwg = gridDim.x;
+  int w = blockIdx.x;
+  if (nwg % 8 == 0) w = (blockIdx.x % 8) * (nwg / 8) + blockIdx.x / 8;
+  const int cx = w % g.ncx, cy = (w / g.ncx) % g.ncy, sz = w / (g.ncx * g.ncy);
+  const int LZn = g.lz * (P - 1) + 1;
+  const int gx0 = cx * BX * (P - 1), gy0 = cy * BY * (P - 1), gz0 = sz * g.lz * (P - 1);
+  const size_t Nxy = static_cast<size_t>(g.Nx) * g.Ny;
+  T* my_part = partials + static_cast<size_t>(w) * box_face_size(LXn, LYn, LZn);
+
+  const int el = threadIdx.x / TPE;
+  const int t = threadIdx.x - el * TPE;
+  const bool live = el < NE;  // the workgroup is rounded up to whole wavefronts
+  const int lex = live ? el % BX : 0, ley = live ? el / BX : 0;
+  const int ex = cx * BX + lex, ey = cy * BY + ley;
+  T* b = live ? s_b[el] : nullptr;
+
+  // Thread `pos` owns node position (plx, ply) of the column cross-section in
+  // every plane: the layer's elements holding it (1, 2 or 4, ascending) and
+  // the offsets of (ix, iy) in their tiles, its v address (less the plane)
+  // and whether it lies on a column face shared with another column.
+  const int pos = threadIdx.x;
+  const bool pos_thread = pos < PLANE;
+  const int ply = pos_thread ? pos / LXn : 0, plx = pos_thread ? pos - (pos / LXn) * LXn : 0;
+  int toff[4] = {0, 0, 0, 0};
+  int ncon = 0;
+  if (pos_thread) {
+    int xl[2], xi[2], yl[2], yi[2], nxl = 0, nyl = 0;
+    const int qx = plx / (P - 1), rx = plx - qx * (P - 1), qy = ply / (P - 1), ry = ply - qy * (P - 1);
+    if (rx == 0) {
+      if (qx > 0) { xl[nxl] = qx - 1; xi[nxl++] = P - 1; }
+      if (qx < BX) { xl[nxl] = qx; xi[nxl++] = 0; }
+    } else {
+      xl[nxl] = qx; xi[nxl++] = rx;
+    }
+    if (ry == 0) {
+      if (qy > 0) { yl[nyl] = qy - 1; yi[nyl++] = P - 1; }
+      if (qy < BY) { yl[nyl] = qy; yi[nyl++] = 0; }
+    } else {
+      yl[nyl] = qy; yi[nyl++] = ry;
+    }
+    for (int a = 0; a < nyl; ++a)
+      for (int c2 = 0; c2 < nxl; ++c2) toff[ncon++] = (yl[a] * BX + xl[c2]) * S::FBUF + yi[a] * RY + xi[c2];
+  }
+  const bool pxs = (plx == 0 && cx > 0) || (plx == LXn - 1 && cx < g.ncx - 1);
+  const bool pys = (ply == 0 && cy > 0) || (ply == LYn - 1 && cy < g.ncy - 1);
+  T* const pv = v + static_cast<size_t>(gx0 + plx) + static_cast<size_t>(g.Nx) * (gy0 + ply);
+
+  // Software pipeline over the (layer, component) steps: the u row of the
+  // next step and the qdata of the next layer are loaded while the current
+  // step contracts, so HBM latency overlaps the LDS work.
+  // u row of thread (iz,iy) = t for element layer ez, component c:
+  const int riz = t / P, riy = t - (t / P) * P;
+  const bool row_thread = live && t < P2, line_thread = live && t < Q2;
+  auto row_ptr = [&](int ez, int c) -> const T* {
+    return u + static_cast<size_t>(c) * g.cs + static_cast<size_t>(ex * (P - 1)) +
+           static_cast<size_t>(g.Nx) * (ey * (P - 1) + riy) + Nxy * (ez * (P - 1) + riz);
+  };
+  auto qd_ptr = [&](int ez) -> const T* {
+    const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ez);
+    return qdata + e * Q3 + t;
+  };
+  T xc[P] = {}, qc[Q] = {};
+  if (row_thread) {
+    const T* r = row_ptr(sz * g.lz, 0);
+#pragma unroll
+    for (int ix = 0; ix < P; ++ix) xc[ix] = r[ix];
+  }
+  if (line_thread) {
+    const T* r = qd_ptr(sz * g.lz);
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) qc[qz] = r[qz * Q2];
+  }
+
+#pragma unroll 1
+  for (int k = 0; k < g.lz; ++k) {
+    const int ez = sz * g.lz + k;
+    T qn[Q] = {};
+    if (line_thread && k + 1 < g.lz) {
+      const T* r = qd_ptr(ez + 1);
+#pragma unroll
+      for (int qz = 0; qz < Q; ++qz) qn[qz] = r[qz * Q2];
+    }
+    T qd[Q];
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) qd[qz] = qc[qz];
+#pragma unroll 1
+    for (int c = 0; c < NC; ++c) {
+      // The next step's u row.
+      T xn[P] = {};
+      const int cn = c + 1 < NC ? c + 1 : 0, kn = c + 1 < NC ? k : k + 1;
+      if (row_thread && kn < g.lz) {
+        const T* r = row_ptr(sz * g.lz + kn, cn);
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) xn[ix] = r[ix];
+      }
+      // S1: x-rows (iz,iy): contract x.
+      if (row_thread) {
+        const int iz = riz, iy = riy;
+        T x[P], y[Q];
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) x[ix] = xc[ix];
+        eo_forward<P, Q, T>(x, y, Em);
+        const int row = iz * ZS + iy * RY;
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) b[row + qx] = y[qx];
+      }
+      __syncthreads();
+      // S2: y-columns (iz,qx).
+      if (live && t < P * Q) {
+        const int col = (t / Q) * ZS + t % Q;
+        T x[P], y[Q];
+#pragma unroll
+        for (int iy = 0; iy < P; ++iy) x[iy] = b[col + iy * RY];
+        eo_forward<P, Q, T>(x, y, Em);
+#pragma unroll
+        for (int qy = 0; qy < Q; ++qy) b[col + qy * RY] = y[qy];
+      }
+      __syncthreads();
+      // S3: z-lines (qy,qx): contract z, times qdata, contract z back.
+      if (live && t < Q2) {
+        const int line = (t / Q) * RY + t % Q;
+        T x[P], d[Q];
+#pragma unroll
+        for (int iz = 0; iz < P; ++iz) x[iz] = b[iz * ZS + line];
+        eo_forward<P, Q, T>(x, d, Em);
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) d[qz] *= qd[qz];
+        eo_transpose<P, Q, T>(d, x, Em);
+#pragma unroll
+        for (int iz = 0; iz < P; ++iz) b[iz * ZS + line] = x[iz];
+      }
+      __syncthreads();
+      // S4: y-columns back.
+      if (live && t < P * Q) {
+        const int col = (t / Q) * ZS + t % Q;
+        T y[Q], x[P];
+#pragma unroll
+        for (int qy = 0; qy < Q; ++qy) y[qy] = b[col + qy * RY];
+        eo_transpose<P, Q, T>(y, x, Em);
+#pragma unroll
+        for (int iy = 0; iy < P; ++iy) b[col + iy * RY] = x[iy];
+      }
+      __syncthreads();
+      // S5: x-rows back.
+      if (live && t < P2) {
+        const int row = (t / P) * ZS + (t % P) * RY;
+        T y[Q], x[P];
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) y[qx] = b[row + qx];
+        eo_transpose<P, Q, T>(y, x, Em);
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) b[row + ix] = x[ix];
+      }
+      __syncthreads();
+      // Node sums: each position adds its elements' outputs in ascending
+      // element order (the bottom plane after what the layer below left in
+      // s_top), then the plane goes to v, to the partial area (shared faces),
+      // or, for the top plane below the segment's last layer, to s_top.
+      if (pos_thread) {
+        const T* tb = &s_b[0][0];
+        T* vc = pv + static_cast<size_t>(c) * g.cs;
+        T* pc = my_part + static_cast<size_t>(c) * part_comp_len;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          T sum;
+          int m0 = 1;
+          if (j == 0 && k > 0) {
+            sum = s_top[c][pos];
+            m0 = 0;
+          } else {
+            sum = tb[toff[0] + j * ZS];
+          }
+#pragma unroll
+          for (int m = 0; m < 4; ++m)
+            if (m >= m0 && m < ncon) sum += tb[toff[m] + j * ZS];
+          if (j == P - 1 && k + 1 < g.lz) {
+            s_top[c][pos] = sum;
+          } else {
+            const int gz = ez * (P - 1) + j, lzl = gz - gz0;
+            const bool zs = (lzl == 0 && sz > 0) || (lzl == LZn - 1 && sz < g.nsz - 1);
+            if (!(pxs || pys || zs)) {
+              vc[Nxy * gz] = sum;
+            } else {
+              pc[box_slot(plx, ply, lzl, pxs, pys, LXn, LYn, LZn)] = sum;
+            }
+          }
+        }
+      }
+      __syncthreads();  // the next step's S1 overwrites the tiles
+#pragma unroll
+      for (int ix = 0; ix < P; ++ix) xc[ix] = xn[ix];
+    }
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) qc[qz] = qn[qz];
+  }
+}
+
+// v at every shared node = its partials summed in ascending segment order.
+template <class T, int P, int BX, int BY>
+__global__ __launch_bounds__(256) void box_shared_sum_kernel(BoxGeom g, int nc, long long count,
+                                                             const T* __restrict__ partials, size_t part_comp_len,
+                                                             T* __restrict__ v) {
+  const long long id = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= count) return;
+  int gx, gy, gz, seg[8], slot[8];
+  box_shared_node(g, P, BX, BY, id, &gx, &gy, &gz);
+  const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, slot);
+  const int LZn = g.lz * (P - 1) + 1;
+  const size_t face = static_cast<size_t>(box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1, LZn));
+  const size_t node = static_cast<size_t>(gx) + static_cast<size_t>(g.Nx) * (gy + static_cast<size_t>(g.Ny) * gz);
+  for (int c = 0; c < nc; ++c) {
+    const T* pc = partials + static_cast<size_t>(c) * part_comp_len;
+    T s = pc[seg[0] * face + slot[0]];
+    for (int i = 1; i < n; ++i) s += pc[seg[i] * face + slot[i]];
+    v[static_cast<size_t>(c) * g.cs + node] = s;
+  }
+}
+
+inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
+  BoxGeom g;
+  g.nx = a.box_n[0];
+  g.ny = a.box_n[1];
+  g.nz = a.box_n[2];
+  g.Nx = g.nx * (P - 1) + 1;
+  g.Ny = g.ny * (P - 1) + 1;
+  g.Nz = g.nz * (P - 1) + 1;
+  g.ncx = g.nx / BX;
+  g.ncy = g.ny / BY;
+  g.lz = a.box_lz;
+  g.nsz = g.lz > 0 ? g.nz / g.lz : 0;
+  g.cs = g.Nx * g.Ny * g.Nz;  // blocked components (checked when the box was detected)
+  return g;
+}
+
+template <int P, int Q, int NC, class T, int BX, int BY>
+void launch_box(const FusedArgs& a, hipStream_t s) {
+  using S = Shape<P, Q>;
+  if (a.box_pass == 2) {  // report how many workgroups fit the GPU at once
+    *a.box_slots = persistent_grid(fused_box_kernel<P, Q, NC, T, BX, BY>, round64(BX * BY * S::TPE), 0);
+    return;
+  }
+  const BoxGeom g = box_geom(a, P, BX, BY);
+  require(g.nx % BX == 0 && g.ny % BY == 0 && g.lz > 0 && g.nz % g.lz == 0 && BX * (P - 1) >= 2 &&
+              BY * (P - 1) >= 2 && g.lz * (P - 1) >= 2,
+          "fused mass: box does not divide into the kernel's columns");
+  const long long nseg = static_cast<long long>(g.ncx) * g.ncy * g.nsz;
+  require(nseg < (1ll << 31) && static_cast<long long>(g.Nx) * g.Ny * g.Nz < (1ll << 31), "fused mass: box too large");
+  require(a.part_comp_len >= static_cast<size_t>(nseg * box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1,
+                                                                      g.lz * (P - 1) + 1)),
+          "fused mass: box partial buffer too small");
+  if (a.box_pass == 0) {
+    constexpr int threads = round64(BX * BY * S::TPE);
+    fused_box_kernel<P, Q, NC, T, BX, BY><<<static_cast<unsigned>(nseg), threads, 0, s>>>(
+        g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
+        static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
+  } else {
+    const long long count = box_shared_count(g, P, BX, BY);
+    if (count == 0) return;
+    box_shared_sum_kernel<T, P, BX, BY><<<static_cast<unsigned>((count + 255) / 256), 256, 0, s>>>(
+        g, NC, count, static_cast<const T*>(a.partials), a.part_comp_len, static_cast<T*>(a.v));
+  }
+  HIP_CHECK_LAUNCH();
+}
+
+// Key elems of a box-column variant: BX * BY elements per layer.
+template <int P, int Q, int NC, int BX, int BY>
+struct FusedBoxReg {
+  FusedBoxReg() {
+    register_fused(FusedKey{P, Q, NC, 0, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, double, BX, BY>, 0);
+    register_fused(FusedKey{P, Q, NC, 1, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, float, BX, BY>, 0);
+  }
+};
+
+// Column cross-section (BX, BY) of each compiled elems value.
+inline bool box_columns(int elems, int* BX, int* BY) {
+  switch (elems) {
+    case 16: *BX = 4; *BY = 4; return true;
+    case 8: *BX = 4; *BY = 2; return true;
+    case 4: *BX = 2; *BY = 2; return true;
+    default: return false;
+  }
+}
+
+}  // namespace femhip

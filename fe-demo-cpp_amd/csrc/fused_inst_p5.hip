@@ -1,4 +1,5 @@
 // Compiled fused-mass shapes with P = 5 (registered at library load).
+#include "fused_box_kernel.hpp"
 #include "fused_mass_kernel.hpp"
 
 namespace femhip {
@@ -16,6 +17,11 @@ const FusedBlockShapeReg<5, 6, 1, 8, 8> blk_5_6_1_8;
 const FusedBlockShapeReg<5, 6, 1, 7, 7> blk_5_6_1_7;
 const FusedBlockShapeReg<5, 6, 2, 8, 8> blk_5_6_2;
 const FusedBlockShapeReg<5, 6, 3, 8, 8> blk_5_6_3;
+
+const FusedBoxReg<5, 6, 1, 4, 4> box_5_6_1_16;
+const FusedBoxReg<5, 6, 1, 4, 2> box_5_6_1_8;
+const FusedBoxReg<5, 6, 1, 2, 2> box_5_6_1_4;
+const FusedBoxReg<5, 6, 3, 4, 2> box_5_6_3_8;
 
 }  // namespace
 }  // namespace femhip

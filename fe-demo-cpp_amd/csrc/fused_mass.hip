@@ -4,11 +4,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <fstream>
 #include <sstream>
 
 #include "block_plan.hpp"
+#include "fused_box_kernel.hpp"
 #include "fused_mass.hpp"
 
 namespace femhip {
@@ -276,7 +278,59 @@ int default_elems(int P, int Q, int NC, int fp32, int alg, int exact) {
 }
 
 bool valid_for(const FusedMass& f, int alg) {
-  return f.scatter != kScatterDeterministic || alg_deterministic(alg);
+  return (f.scatter != kScatterDeterministic || alg_deterministic(alg)) && (!alg_box(alg) || f.box);
+}
+
+// Workgroups of a box-column kernel resident on the GPU at once (asked of the
+// kernel's launcher once per key).
+long long box_slots(const FusedKey& k) {
+  static std::mutex m;
+  static std::map<FusedKey, int> cache;
+  std::lock_guard<std::mutex> lk(m);
+  auto it = cache.find(k);
+  if (it != cache.end()) return it->second;
+  FusedLauncher L = find(k);
+  require(L != nullptr, "fused mass: box-column variant not compiled");
+  int slots = 0;
+  FusedArgs a;
+  a.box_pass = 2;
+  a.box_slots = &slots;
+  L(a, nullptr);
+  cache[k] = slots;
+  return slots;
+}
+
+// Element layers per segment: the operator's setting, else box_layers for the
+// kernel's residency.
+int box_lz_of(const FusedMass& f, const FusedKey& k) {
+  if (f.box_lz > 0) return f.box_lz;
+  int BX = 0, BY = 0;
+  box_columns(k.elems, &BX, &BY);
+  return box_layers(f.box_n[2], static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY), box_slots(k), f.P);
+}
+
+// Whether the box-column variant with `elems` elements per layer tiles f's box
+// (without asking the GPU: an automatic layer count exists when some divisor
+// lz of nz has lz*(P-1) >= 2).
+bool box_fits(const FusedMass& f, int elems) {
+  int BX = 0, BY = 0;
+  if (!f.box || !f.basis_symmetric || !box_columns(elems, &BX, &BY)) return false;
+  if (f.box_n[0] % BX || f.box_n[1] % BY || BX * (f.P - 1) < 2 || BY * (f.P - 1) < 2) return false;
+  if (f.box_lz > 0) return f.box_n[2] % f.box_lz == 0 && f.box_lz * (f.P - 1) >= 2;
+  return box_layers(f.box_n[2], 1, 1, f.P) > 0;
+}
+
+bool key_valid(const FusedMass& f, const FusedKey& k) {
+  return valid_for(f, k.alg) && (!alg_box(k.alg) || box_fits(f, k.elems));
+}
+
+// The first compiled box-column variant that tiles the box (most elements per layer first).
+int box_default_elems(const FusedMass& f, int fp32) {
+  std::vector<int> v = fused_elem_variants(f.P, f.Q, f.NC, fp32, kAlgBoxColumn, 0);
+  std::sort(v.rbegin(), v.rend());
+  for (int ne : v)
+    if (box_fits(f, ne)) return ne;
+  return 0;
 }
 
 DevPlan& ensure_plan(FusedMass& f, int NB, int budget, bool part, hipStream_t s) {
@@ -420,6 +474,27 @@ DevBuf<float>& elem_qdata<float>(FusedMass& f, hipStream_t s) {
 }
 
 template <class T>
+DevBuf<T>& box_partials(FusedMass& f);
+template <>
+DevBuf<double>& box_partials<double>(FusedMass& f) {
+  return f.box_part64;
+}
+template <>
+DevBuf<float>& box_partials<float>(FusedMass& f) {
+  return f.box_part32;
+}
+
+// Per-component length of the box-column partial area for the key's variant.
+size_t box_part_len(const FusedMass& f, const FusedKey& k) {
+  int BX = 0, BY = 0;
+  require(box_columns(k.elems, &BX, &BY), "fused mass: box-column variant not compiled");
+  const long long ncols = static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY);
+  const int lz = box_lz_of(f, k);
+  const long long nseg = ncols * (f.box_n[2] / lz);
+  return static_cast<size_t>(nseg * box_face_size(BX * (f.P - 1) + 1, BY * (f.P - 1) + 1, lz * (f.P - 1) + 1));
+}
+
+template <class T>
 DevBuf<T>& elem_outputs(FusedMass& f);
 template <>
 DevBuf<double>& elem_outputs<double>(FusedMass& f) {
@@ -446,6 +521,14 @@ FusedMass::StageEvents* next_events(FusedMass& f) {
 // Makes every buffer the key's algorithm needs (so the apply itself allocates nothing).
 template <class T>
 void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
+  if (alg_box(k.alg)) {
+    require(box_fits(f, k.elems), "fused mass: the box-column algorithm needs a structured box its columns tile");
+    (void)elem_qdata<T>(f, s);
+    DevBuf<T>& pb = box_partials<T>(f);
+    const size_t n = std::max<size_t>(box_part_len(f, k), 1) * f.NC;
+    if (pb.size() < n) pb.allocate(n);
+    return;
+  }
   if (alg_blocked(k.alg)) {
     const bool part = alg_partial(k.alg);
     DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), part, s);
@@ -478,7 +561,24 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
   a.basis = f32 ? static_cast<const void*>(f.B32.data()) : static_cast<const void*>(f.B64.data());
   FusedMass::StageEvents* ev = next_events(f);
   if (ev) HIP_CHECK(hipEventRecord(ev->a, s));
-  if (alg_blocked(k.alg)) {
+  if (alg_box(k.alg)) {
+    int BX = 0, BY = 0;
+    box_columns(k.elems, &BX, &BY);
+    a.qdata = elem_qdata<T>(f, s).data();
+    a.box_n[0] = f.box_n[0];
+    a.box_n[1] = f.box_n[1];
+    a.box_n[2] = f.box_n[2];
+    a.box_lz = box_lz_of(f, k);
+    a.v = v;
+    a.partials = box_partials<T>(f).data();
+    a.part_comp_len = std::max<size_t>(box_part_len(f, k), 1);
+    if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
+    a.box_pass = 0;
+    L(a, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
+    a.box_pass = 1;
+    L(a, s);
+  } else if (alg_blocked(k.alg)) {
     DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), alg_partial(k.alg), s);
     a.qdata = plan_qdata<T>(f, d, s).data();
     a.nblk = d.nblk;
@@ -531,6 +631,50 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
 
 }  // namespace
 
+bool detect_box(const int* off, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3) {
+  if (P < 2 || num_elem <= 0 || off[0] != 0) return false;
+  const long long Nx = off[P], Nxy = off[P * P];  // the nodes one step up in y and in z
+  if (Nx < P || (Nx - 1) % (P - 1) != 0 || Nxy <= 0 || Nxy % Nx != 0) return false;
+  const long long Ny = Nxy / Nx;
+  if (Ny < P || (Ny - 1) % (P - 1) != 0) return false;
+  const long long nx = (Nx - 1) / (P - 1), ny = (Ny - 1) / (P - 1);
+  if (num_elem % (nx * ny) != 0) return false;
+  const long long nz = num_elem / (nx * ny), Nz = nz * (P - 1) + 1;
+  // One component: any comp_stride (unused); more: blocked components, one after another.
+  if ((num_comp > 1 && Nx * Ny * Nz != comp_stride) || static_cast<long long>(num_comp) * Nx * Ny * Nz != l_size)
+    return false;
+  long long t = 0;
+  for (long long ez = 0; ez < nz; ++ez)
+    for (long long ey = 0; ey < ny; ++ey)
+      for (long long ex = 0; ex < nx; ++ex)
+        for (int iz = 0; iz < P; ++iz)
+          for (int iy = 0; iy < P; ++iy) {
+            const long long row = ex * (P - 1) + Nx * ((ey * (P - 1) + iy) + Ny * (ez * (P - 1) + iz));
+            for (int ix = 0; ix < P; ++ix, ++t)
+              if (off[t] != row + ix) return false;
+          }
+  n3[0] = static_cast<int>(nx);
+  n3[1] = static_cast<int>(ny);
+  n3[2] = static_cast<int>(nz);
+  return true;
+}
+
+int box_layers(int nz, long long ncols, long long slots, int P) {
+  int best = 0;
+  long long best_cost = 0;
+  slots = std::max<long long>(slots, 1);
+  for (int d = 1; d <= nz; ++d) {
+    if (nz % d != 0 || d * (P - 1) < 2) continue;
+    const long long nseg = ncols * (nz / d);
+    const long long cost = (nseg + slots - 1) / slots * d;
+    if (best == 0 || cost <= best_cost) {
+      best = d;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
 FusedKey fused_resolve(const FusedMass& f, int fp32) {
   FusedKey k{f.P, f.Q, f.NC, fp32, -1, 0, 0};
   const bool det = f.scatter == kScatterDeterministic;
@@ -543,15 +687,20 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
     return k;
   }
   if (f.choice.alg >= 0) {
-    require(valid_for(f, f.choice.alg), "fused mass: a deterministic scatter needs a deterministic algorithm");
+    require(f.scatter != kScatterDeterministic || alg_deterministic(f.choice.alg),
+            "fused mass: a deterministic scatter needs a deterministic algorithm");
+    require(!alg_box(f.choice.alg) || f.box,
+            "fused mass: the box-column algorithm needs a structured box mesh (lexicographic node numbering)");
     k.alg = f.choice.alg;
     k.elems = f.choice.elems;
     if (k.elems <= 0) {
       std::lock_guard<std::mutex> lk(tune_mutex());
       auto it = tune_alg_table().find(TuneAlgKey{f.P, f.Q, f.NC, fp32, k.alg});
       k.elems = it != tune_alg_table().end() ? it->second.elems : default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
+      if (alg_box(k.alg) && !box_fits(f, k.elems)) k.elems = box_default_elems(f, fp32);
     }
     require(k.elems > 0 && find(k), "fused mass: the chosen algorithm is not compiled for this shape");
+    require(key_valid(f, k), "fused mass: the box-column variant does not tile this box");
     return k;
   }
   {
@@ -561,18 +710,20 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
       FusedKey t = k;
       t.alg = it->second.alg;
       t.elems = it->second.elems;
-      if (t.alg >= 0 && valid_for(f, t.alg) && find(t)) return t;
+      if (t.alg >= 0 && key_valid(f, t) && find(t)) return t;
     }
   }
-  // Default order: partial-sum blocks, then (deterministic) transpose map or (atomic) block atomics, then elements.
-  const int order_det[] = {kAlgBlockPart, kAlgElemTmapPipe, kAlgElemTmap};
-  const int order_any[] = {kAlgBlockPartLds, kAlgBlockPartLdsPipe, kAlgBlockPart, kAlgBlockAtomic, kAlgElemAtomic,
-                           kAlgElemTmapPipe, kAlgElemTmap};
+  // Default order: box columns (structured boxes), partial-sum blocks, then
+  // (deterministic) transpose map or (atomic) block atomics, then elements.
+  const int order_det[] = {kAlgBoxColumn, kAlgBlockPart, kAlgElemTmapPipe, kAlgElemTmap};
+  const int order_any[] = {kAlgBoxColumn, kAlgBlockPartLds, kAlgBlockPartLdsPipe, kAlgBlockPart, kAlgBlockAtomic,
+                           kAlgElemAtomic, kAlgElemTmapPipe, kAlgElemTmap};
   const int* order = det ? order_det : order_any;
-  const int n = det ? 3 : 7;
+  const int n = det ? 4 : 8;
   for (int i = 0; i < n; ++i) {
     k.alg = order[i];
-    k.elems = default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
+    if (!valid_for(f, k.alg)) continue;
+    k.elems = alg_box(k.alg) ? box_default_elems(f, fp32) : default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
     if (k.elems > 0 && find(k)) return k;
   }
   throw Unsupported("fused mass: no compiled variant for this shape");
@@ -595,6 +746,16 @@ std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBas
   f->B32.assign(b.interp.begin(), b.interp.end());
   f->Bsq.resize(b.interp.size());
   for (size_t i = 0; i < b.interp.size(); ++i) f->Bsq[i] = b.interp[i] * b.interp[i];
+  f->box = detect_box(r.h_offsets.data(), r.num_elem, b.P, b.num_comp, r.comp_stride, r.l_size, f->box_n);
+  {
+    double bmax = 0.0, asym = 0.0;
+    for (int q = 0; q < b.Q; ++q)
+      for (int p = 0; p < b.P; ++p) {
+        bmax = std::max(bmax, std::fabs(b.interp[q * b.P + p]));
+        asym = std::max(asym, std::fabs(b.interp[q * b.P + p] - b.interp[(b.Q - 1 - q) * b.P + (b.P - 1 - p)]));
+      }
+    f->basis_symmetric = asym <= 1e-14 * bmax;
+  }
   {
     // Single-contributor nodes: the element atomic kernel stores them instead of adding.
     const size_t nt = r.h_offsets.size();
@@ -687,6 +848,7 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
       if (!valid_for(f, alg)) continue;
       for (int ne : fused_elem_variants(f.P, f.Q, f.NC, fp32, alg, 0)) {
         const FusedKey k{f.P, f.Q, f.NC, fp32, alg, 0, ne};
+        if (!key_valid(f, k)) continue;
         apply_key<T>(f, k, u.data(), v.data(), s);  // warm-up, and builds the variant's state
         HIP_CHECK(hipStreamSynchronize(s));
         HIP_CHECK(hipEventRecord(e0, s));

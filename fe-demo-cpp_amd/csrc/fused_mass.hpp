@@ -26,11 +26,14 @@ enum FusedAlg : int {
   kAlgElemTmapPipe = 4,  // kAlgElemTmap with a persistent, software-pipelined element kernel
   kAlgBlockPartLds = 5,  // kAlgBlockPart with LDS atomic sums in place of the colour rounds (any order)
   kAlgBlockPartLdsPipe = 6,  // kAlgBlockPartLds as a persistent, software-pipelined kernel
-  kAlgCount = 7
+  kAlgBoxColumn = 7,  // structured boxes only: element columns marching in z, node planes in LDS,
+                      // shared column faces summed by a second pass (deterministic)
+  kAlgCount = 8
 };
 inline bool alg_deterministic(int alg) {
-  return alg == kAlgElemTmap || alg == kAlgBlockPart || alg == kAlgElemTmapPipe;
+  return alg == kAlgElemTmap || alg == kAlgBlockPart || alg == kAlgElemTmapPipe || alg == kAlgBoxColumn;
 }
+inline bool alg_box(int alg) { return alg == kAlgBoxColumn; }
 inline bool alg_tmap(int alg) { return alg == kAlgElemTmap || alg == kAlgElemTmapPipe; }
 inline bool alg_blocked(int alg) {
   return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds || alg == kAlgBlockPartLdsPipe;
@@ -74,8 +77,13 @@ struct FusedArgs {
   const int* ncolor = nullptr;     // [nblk]
   const int* part_ptr = nullptr;   // [nblk + 1]
   void* v = nullptr;               // block atomic: pre-zeroed v; block part: v (interior nodes stored)
-  void* partials = nullptr;        // block part: [NC][npart]
+  void* partials = nullptr;        // block part: [NC][npart]; box column: [NC][segments * face slots]
   size_t part_comp_len = 0;
+  // box column: elements per direction, layers per segment, pass (0 the column kernel, 1 the shared-node sums)
+  int box_n[3] = {0, 0, 0};
+  int box_lz = 0;
+  int box_pass = 0;        // 2: only report the workgroups that fit the GPU at once in *box_slots
+  int* box_slots = nullptr;
 };
 
 using FusedLauncher = void (*)(const FusedArgs&, hipStream_t);
@@ -126,6 +134,14 @@ struct FusedMass {
   bool tm_built = false, tm_covers_all = false;
   int tm_nrows = 0;
   DevBuf<int> tm_rows, tm_ptr, tm_idx;
+  // Structured box (every offset is the lexicographic box formula with node
+  // stride 1, comp_stride = node count, l_size = NC * node count): elements per direction.
+  bool box = false;
+  bool basis_symmetric = false;  // B(Q-1-q, P-1-p) = B(q, p) to rounding (the box kernel's even-odd form)
+  int box_n[3] = {0, 0, 0};
+  int box_lz = 0;             // element layers per box-column segment (0: box_layers)
+  DevBuf<double> box_part64;  // box-column partials (lazy)
+  DevBuf<float> box_part32;
   int scatter = kScatterAtomic;  // requested: atomic (any order) or deterministic (reproducible)
   int exact = 0;                 // requested: the reference's arithmetic (bit-equal with deterministic)
   FusedChoice choice;            // explicit algorithm, or automatic
@@ -156,6 +172,16 @@ FusedKey fused_resolve(const FusedMass& f, int fp32);
 // Builds lazily made state (block plans, FP32 copies) for the current choice,
 // so later applies allocate nothing (graph capture).
 void fused_prepare(FusedMass& f, int fp32, hipStream_t s);
+// Whether offsets [E][P^3] are a structured nx*ny*nz box of P^3-node hexes
+// with lexicographic node numbering (node stride 1), l_size = NC * node count
+// and (NC > 1) comp_stride = node count; on success n3 = {nx, ny, nz}.
+bool detect_box(const int* offsets, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3);
+// Element layers per segment of the box-column algorithm for nz layers,
+// ncols columns and `slots` workgroups resident at once: the divisor lz of nz
+// (with lz*(P-1) >= 2) that minimises the makespan in layers,
+// ceil(ncols*(nz/lz) / slots) * lz; ties go to the larger lz (fewer shared
+// segment ends). 0 if no divisor qualifies.
+int box_layers(int nz, long long ncols, long long slots, int P);
 // Jacobi diagonal: diag = E^T [ (B o B)^T x3 qdata ] (FP64).
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);
 

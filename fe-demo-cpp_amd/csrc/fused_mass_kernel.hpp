@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "fused_mass.hpp"
 
@@ -69,12 +70,23 @@ struct Shape {
   static constexpr int FBUF = P * ZS;
 };
 
+// Fused multiply-add in T's own precision (__builtin_fma is the double one and
+// would promote float operands).
+template <class T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+  if constexpr (std::is_same_v<T, float>) {
+    return __builtin_fmaf(a, b, c);
+  } else {
+    return __builtin_fma(a, b, c);
+  }
+}
+
 template <bool EXACT, class T>
 __device__ __forceinline__ T mac(T a, T b, T acc) {
   if constexpr (EXACT) {
     return acc + a * b;  // separate rounding (the library is built with -ffp-contract=off)
   } else {
-    return __builtin_fma(a, b, acc);
+    return fma_t(a, b, acc);
   }
 }
 
@@ -299,7 +311,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
       for (int qx = 0; qx < Q; ++qx) {
         T acc = T(0);
 #pragma unroll
-        for (int ix = 0; ix < P; ++ix) acc = __builtin_fma(B(qx, ix), x[ix], acc);
+        for (int ix = 0; ix < P; ++ix) acc = fma_t(B(qx, ix), x[ix], acc);
         b[row + qx] = acc;
       }
     }
@@ -314,7 +326,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
       for (int qy = 0; qy < Q; ++qy) {
         T acc = T(0);
 #pragma unroll
-        for (int iy = 0; iy < P; ++iy) acc = __builtin_fma(B(qy, iy), x[iy], acc);
+        for (int iy = 0; iy < P; ++iy) acc = fma_t(B(qy, iy), x[iy], acc);
         b[col + qy * RY] = acc;
       }
     }
@@ -330,14 +342,14 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
       for (int qz = 0; qz < Q; ++qz) {
         T acc = T(0);
 #pragma unroll
-        for (int iz = 0; iz < P; ++iz) acc = __builtin_fma(B(qz, iz), x[iz], acc);
+        for (int iz = 0; iz < P; ++iz) acc = fma_t(B(qz, iz), x[iz], acc);
         d[qz] = qd[qz] * acc;
       }
 #pragma unroll
       for (int iz = 0; iz < P; ++iz) {
         T acc = T(0);
 #pragma unroll
-        for (int qz = 0; qz < Q; ++qz) acc = __builtin_fma(B(qz, iz), d[qz], acc);
+        for (int qz = 0; qz < Q; ++qz) acc = fma_t(B(qz, iz), d[qz], acc);
         b[iz * ZS + line] = acc;
       }
     }
@@ -352,7 +364,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
       for (int iy = 0; iy < P; ++iy) {
         T acc = T(0);
 #pragma unroll
-        for (int qy = 0; qy < Q; ++qy) acc = __builtin_fma(B(qy, iy), x[qy], acc);
+        for (int qy = 0; qy < Q; ++qy) acc = fma_t(B(qy, iy), x[qy], acc);
         b[col + iy * RY] = acc;
       }
     }
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
       for (int ix = 0; ix < P; ++ix) {
         T acc = T(0);
 #pragma unroll
-        for (int qx = 0; qx < Q; ++qx) acc = __builtin_fma(B(qx, ix), x[qx], acc);
+        for (int qx = 0; qx < Q; ++qx) acc = fma_t(B(qx, ix), x[qx], acc);
         b[row + ix] = acc;
       }
     }
@@ -468,7 +480,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe
         for (int qx = 0; qx < Q; ++qx) {
           T acc = T(0);
 #pragma unroll
-          for (int ix = 0; ix < P; ++ix) acc = __builtin_fma(B(qx, ix), x[ix], acc);
+          for (int ix = 0; ix < P; ++ix) acc = fma_t(B(qx, ix), x[ix], acc);
           b[row + qx] = acc;
         }
       }
@@ -483,7 +495,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe
         for (int qy = 0; qy < Q; ++qy) {
           T acc = T(0);
 #pragma unroll
-          for (int iy = 0; iy < P; ++iy) acc = __builtin_fma(B(qy, iy), x[iy], acc);
+          for (int iy = 0; iy < P; ++iy) acc = fma_t(B(qy, iy), x[iy], acc);
           b[col + qy * RY] = acc;
         }
       }
@@ -504,14 +516,14 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe
         for (int qz = 0; qz < Q; ++qz) {
           T acc = T(0);
 #pragma unroll
-          for (int iz = 0; iz < P; ++iz) acc = __builtin_fma(B(qz, iz), x[iz], acc);
+          for (int iz = 0; iz < P; ++iz) acc = fma_t(B(qz, iz), x[iz], acc);
           d[qz] = qd[qz] * acc;
         }
 #pragma unroll
         for (int iz = 0; iz < P; ++iz) {
           T acc = T(0);
 #pragma unroll
-          for (int qz = 0; qz < Q; ++qz) acc = __builtin_fma(B(qz, iz), d[qz], acc);
+          for (int qz = 0; qz < Q; ++qz) acc = fma_t(B(qz, iz), d[qz], acc);
           b[iz * ZS + line] = acc;
         }
       }
@@ -526,7 +538,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe
         for (int iy = 0; iy < P; ++iy) {
           T acc = T(0);
 #pragma unroll
-          for (int qy = 0; qy < Q; ++qy) acc = __builtin_fma(B(qy, iy), x[qy], acc);
+          for (int qy = 0; qy < Q; ++qy) acc = fma_t(B(qy, iy), x[qy], acc);
           b[col + iy * RY] = acc;
         }
       }
@@ -541,7 +553,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe
         for (int ix = 0; ix < P; ++ix) {
           T acc = T(0);
 #pragma unroll
-          for (int qx = 0; qx < Q; ++qx) acc = __builtin_fma(B(qx, ix), x[qx], acc);
+          for (int qx = 0; qx < Q; ++qx) acc = fma_t(B(qx, ix), x[qx], acc);
           b[row + ix] = acc;
         }
       }
@@ -648,7 +660,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_bloc
       for (int qx = 0; qx < Q; ++qx) {
         T acc = T(0);
 #pragma unroll
-        for (int ix = 0; ix < P; ++ix) acc = __builtin_fma(B(qx, ix), x[ix], acc);
+        for (int ix = 0; ix < P; ++ix) acc = fma_t(B(qx, ix), x[ix], acc);
         b[row + qx] = acc;
       }
     }
@@ -662,7 +674,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_bloc
       for (int qy = 0; qy < Q; ++qy) {
         T acc = T(0);
 #pragma unroll
-        for (int iy = 0; iy < P; ++iy) acc = __builtin_fma(B(qy, iy), x[iy], acc);
+        for (int iy = 0; iy < P; ++iy) acc = fma_t(B(qy, iy), x[iy], acc);
         b[col + qy * RY] = acc;
       }
     }
@@ -677,14 +689,14 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_bloc
       for (int qz = 0; qz < Q; ++qz) {
         T acc = T(0);
 #pragma unroll
-        for (int iz = 0; iz < P; ++iz) acc = __builtin_fma(B(qz, iz), x[iz], acc);
+        for (int iz = 0; iz < P; ++iz) acc = fma_t(B(qz, iz), x[iz], acc);
         d[qz] = qd[qz] * acc;
       }
 #pragma unroll
       for (int iz = 0; iz < P; ++iz) {
         T acc = T(0);
 #pragma unroll
-        for (int qz = 0; qz < Q; ++qz) acc = __builtin_fma(B(qz, iz), d[qz], acc);
+        for (int qz = 0; qz < Q; ++qz) acc = fma_t(B(qz, iz), d[qz], acc);
         b[iz * ZS + line] = acc;
       }
     }
@@ -698,7 +710,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_bloc
       for (int iy = 0; iy < P; ++iy) {
         T acc = T(0);
 #pragma unroll
-        for (int qy = 0; qy < Q; ++qy) acc = __builtin_fma(B(qy, iy), x[qy], acc);
+        for (int qy = 0; qy < Q; ++qy) acc = fma_t(B(qy, iy), x[qy], acc);
         b[col + iy * RY] = acc;
       }
     }
@@ -712,7 +724,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_bloc
       for (int ix = 0; ix < P; ++ix) {
         T acc = T(0);
 #pragma unroll
-        for (int qx = 0; qx < Q; ++qx) acc = __builtin_fma(B(qx, ix), x[qx], acc);
+        for (int qx = 0; qx < Q; ++qx) acc = fma_t(B(qx, ix), x[qx], acc);
         unsafeAtomicAdd(&s_acc[c][lo[ix]], acc);  // LDS add
       }
     }
@@ -816,7 +828,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
       for (int qx = 0; qx < Q; ++qx) {
         T acc = T(0);
 #pragma unroll
-        for (int ix = 0; ix < P; ++ix) acc = __builtin_fma(B(qx, ix), x[ix], acc);
+        for (int ix = 0; ix < P; ++ix) acc = fma_t(B(qx, ix), x[ix], acc);
         b[row + qx] = acc;
       }
     }
@@ -833,7 +845,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
       for (int qy = 0; qy < Q; ++qy) {
         T acc = T(0);
 #pragma unroll
-        for (int iy = 0; iy < P; ++iy) acc = __builtin_fma(B(qy, iy), x[iy], acc);
+        for (int iy = 0; iy < P; ++iy) acc = fma_t(B(qy, iy), x[iy], acc);
         b[col + qy * RY] = acc;
       }
     }
@@ -849,14 +861,14 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
       for (int qz = 0; qz < Q; ++qz) {
         T acc = T(0);
 #pragma unroll
-        for (int iz = 0; iz < P; ++iz) acc = __builtin_fma(B(qz, iz), x[iz], acc);
+        for (int iz = 0; iz < P; ++iz) acc = fma_t(B(qz, iz), x[iz], acc);
         d[qz] = qd[qz] * acc;
       }
 #pragma unroll
       for (int iz = 0; iz < P; ++iz) {
         T acc = T(0);
 #pragma unroll
-        for (int qz = 0; qz < Q; ++qz) acc = __builtin_fma(B(qz, iz), d[qz], acc);
+        for (int qz = 0; qz < Q; ++qz) acc = fma_t(B(qz, iz), d[qz], acc);
         b[iz * ZS + line] = acc;
       }
     }
@@ -871,7 +883,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
       for (int iy = 0; iy < P; ++iy) {
         T acc = T(0);
 #pragma unroll
-        for (int qy = 0; qy < Q; ++qy) acc = __builtin_fma(B(qy, iy), x[qy], acc);
+        for (int qy = 0; qy < Q; ++qy) acc = fma_t(B(qy, iy), x[qy], acc);
         b[col + iy * RY] = acc;
       }
     }
@@ -887,7 +899,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
       for (int ix = 0; ix < P; ++ix) {
         T acc = T(0);
 #pragma unroll
-        for (int qx = 0; qx < Q; ++qx) acc = __builtin_fma(B(qx, ix), x[qx], acc);
+        for (int qx = 0; qx < Q; ++qx) acc = fma_t(B(qx, ix), x[qx], acc);
         out[ix] = acc;
       }
     }
@@ -1043,7 +1055,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
         for (int qx = 0; qx < Q; ++qx) {
           T acc = T(0);
 #pragma unroll
-          for (int ix = 0; ix < P; ++ix) acc = __builtin_fma(B(qx, ix), x[ix], acc);
+          for (int ix = 0; ix < P; ++ix) acc = fma_t(B(qx, ix), x[ix], acc);
           b[row + qx] = acc;
         }
       }
@@ -1058,7 +1070,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
         for (int qy = 0; qy < Q; ++qy) {
           T acc = T(0);
 #pragma unroll
-          for (int iy = 0; iy < P; ++iy) acc = __builtin_fma(B(qy, iy), x[iy], acc);
+          for (int iy = 0; iy < P; ++iy) acc = fma_t(B(qy, iy), x[iy], acc);
           b[col + qy * RY] = acc;
         }
       }
@@ -1082,14 +1094,14 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
         for (int qz = 0; qz < Q; ++qz) {
           T acc = T(0);
 #pragma unroll
-          for (int iz = 0; iz < P; ++iz) acc = __builtin_fma(B(qz, iz), x[iz], acc);
+          for (int iz = 0; iz < P; ++iz) acc = fma_t(B(qz, iz), x[iz], acc);
           d[qz] = qd[qz] * acc;
         }
 #pragma unroll
         for (int iz = 0; iz < P; ++iz) {
           T acc = T(0);
 #pragma unroll
-          for (int qz = 0; qz < Q; ++qz) acc = __builtin_fma(B(qz, iz), d[qz], acc);
+          for (int qz = 0; qz < Q; ++qz) acc = fma_t(B(qz, iz), d[qz], acc);
           b[iz * ZS + line] = acc;
         }
       }
@@ -1103,7 +1115,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
         for (int iy = 0; iy < P; ++iy) {
           T acc = T(0);
 #pragma unroll
-          for (int qy = 0; qy < Q; ++qy) acc = __builtin_fma(B(qy, iy), x[qy], acc);
+          for (int qy = 0; qy < Q; ++qy) acc = fma_t(B(qy, iy), x[qy], acc);
           b[col + iy * RY] = acc;
         }
       }
@@ -1117,7 +1129,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
         for (int ix = 0; ix < P; ++ix) {
           T acc = T(0);
 #pragma unroll
-          for (int qx = 0; qx < Q; ++qx) acc = __builtin_fma(B(qx, ix), x[qx], acc);
+          for (int qx = 0; qx < Q; ++qx) acc = fma_t(B(qx, ix), x[qx], acc);
           unsafeAtomicAdd(&sn[lo[ix]], acc);  // LDS add
         }
       }

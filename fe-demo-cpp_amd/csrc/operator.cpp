@@ -163,6 +163,7 @@ FusedMass& operator_fused_mass(Operator& op, hipStream_t s) {
   f.exact = op.opt_exact;
   f.choice.alg = op.opt_alg == -2 ? (f.scatter == kScatterDeterministic ? kAlgElemTmap : kAlgElemAtomic) : op.opt_alg;
   f.choice.elems = op.opt_elems;
+  f.box_lz = op.opt_box_lz;
   return f;
 }
 

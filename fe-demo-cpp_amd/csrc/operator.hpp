@@ -38,7 +38,7 @@ struct Operator {
   // Fused options: scatter (-1: the restriction's mode), exact arithmetic, and the
   // algorithm (-1 automatic, -2 the element kernels of the scatter mode, else a
   // FusedAlg) with its elements per workgroup/block (0 = default).
-  int opt_scatter = -1, opt_exact = 0, opt_alg = -1, opt_elems = 0;
+  int opt_scatter = -1, opt_exact = 0, opt_alg = -1, opt_elems = 0, opt_box_lz = 0;
 };
 
 Operator* operator_create(const QFunctionDev& qf);

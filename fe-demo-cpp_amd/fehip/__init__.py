@@ -21,9 +21,9 @@ EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 3
 QUAD_GAUSS, QUAD_GAUSS_LOBATTO = 0, 1
 SCATTER_ATOMIC, SCATTER_DETERMINISTIC = 0, 1
 FUSED_AUTO, FUSED_ELEM_ATOMIC, FUSED_ELEM_TMAP, FUSED_BLOCK_ATOMIC, FUSED_BLOCK_PARTIAL = -1, 0, 1, 2, 3
-FUSED_ELEM_TMAP_PIPE, FUSED_BLOCK_PARTIAL_LDS, FUSED_BLOCK_PARTIAL_LDS_PIPE = 4, 5, 6
+FUSED_ELEM_TMAP_PIPE, FUSED_BLOCK_PARTIAL_LDS, FUSED_BLOCK_PARTIAL_LDS_PIPE, FUSED_BOX_COLUMN = 4, 5, 6, 7
 FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_partial", 4: "elem_tmap_pipe",
-               5: "block_partial_lds", 6: "block_partial_lds_pipe"}
+               5: "block_partial_lds", 6: "block_partial_lds_pipe", 7: "box_column"}
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
 QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR, QF_BUILD_DIFFUSION, QF_APPLY_DIFFUSION = 1, 2, 3, 4, 5
 SOLVE_FP64, SOLVE_MIXED = 0, 1
@@ -145,6 +145,7 @@ SIGNATURES = {
     "fem_hip_operator_get_info": [_vp, _llp],
     "fem_hip_operator_set_fused_options": [_vp, _i, _i, _i],
     "fem_hip_operator_set_fused_algorithm": [_vp, _i, _i],
+    "fem_hip_operator_set_box_layers": [_vp, _i],
     "fem_hip_operator_get_fused_choice": [_vp, _i, _ip, _ip],
     "fem_hip_operator_autotune": [_vp, _i, _i, _vp, _ip, _ip, ctypes.POINTER(_d), ctypes.POINTER(_d), _i, _ip],
     "fem_hip_tune_table_get": [_ip, ctypes.POINTER(_d), _llp, _i, _ip],
@@ -169,6 +170,8 @@ SIGNATURES = {
     "fem_hip_box_offsets": [_i, _i, _i, _i, _i, _vp],
     "fem_hip_box_coords": [_i, _d, ctypes.c_uint64, _vp],
     "fem_hip_block_plan_stats": [_i, _i, _i, _vp, _i, _i, _i, _i, _llp],
+    "fem_hip_box_detect": [_i, _i, _vp, _i, _i, _i, _vp, _vp],
+    "fem_hip_box_column_check": [_i, _i, _i, _i, _i, _i, _llp],
     "fem_hip_mass_solve": [_vp, _vp, _vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveStats), _vp],
 }
 _RESTYPES = {"fem_hip_last_error": ctypes.c_char_p, "fem_hip_version": ctypes.c_char_p}
@@ -382,6 +385,26 @@ def block_plan_stats(num_elem, elem_size, l_size, offsets, nb, max_nodes=0, num_
     call("fem_hip_block_plan_stats", num_elem, elem_size, l_size, _ptr(offsets), nb, max_nodes, num_comp, cs, st)
     keys = ["blocks", "max_nodes", "unique_nodes", "interior_nodes", "boundary_nodes", "element_nodes", "max_colors",
             "rows", "covers_all"]
+    return dict(zip(keys, [int(x) for x in st]))
+
+
+def box_detect(num_elem, P, offsets, num_comp=1, comp_stride=None, l_size=None):
+    """(nx, ny, nz) if offsets are a lexicographic structured box of P^3-node hexes, else None."""
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    n3 = (ctypes.c_int * 3)()
+    ok = ctypes.c_int()
+    n = int(offsets.max()) + 1
+    cs = n if comp_stride is None else comp_stride
+    ls = num_comp * n if l_size is None else l_size
+    call("fem_hip_box_detect", num_elem, P, _ptr(offsets), num_comp, cs, ls, n3, ctypes.byref(ok))
+    return tuple(n3) if ok.value else None
+
+
+def box_column_check(nx, ny, nz, P, elems, layers=0):
+    """Host check of the box-column index arithmetic (raises on a violation)."""
+    st = (ctypes.c_longlong * 6)()
+    call("fem_hip_box_column_check", nx, ny, nz, P, elems, layers, st)
+    keys = ["segments", "layers", "shared_nodes", "partial_slots", "direct_nodes", "max_owners"]
     return dict(zip(keys, [int(x) for x in st]))
 
 
@@ -613,6 +636,10 @@ class Operator:
 
     def set_fused_algorithm(self, alg, elems=0):
         call("fem_hip_operator_set_fused_algorithm", self.h, alg, elems)
+
+    def set_box_layers(self, layers):
+        """Element layers per workgroup of FUSED_BOX_COLUMN (0: automatic)."""
+        call("fem_hip_operator_set_box_layers", self.h, layers)
 
     def fused_choice(self, fp32=False):
         a, e = ctypes.c_int(), ctypes.c_int()

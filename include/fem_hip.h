@@ -222,10 +222,17 @@ int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, in
 #define FEM_HIP_FUSED_ELEM_TMAP_PIPE 4 /* ELEM_TMAP with a persistent, software-pipelined element kernel */
 #define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS 5 /* BLOCK_PARTIAL summing in LDS with atomics (any order) */
 #define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS_PIPE 6 /* BLOCK_PARTIAL_LDS as a persistent, software-pipelined kernel */
+#define FEM_HIP_FUSED_BOX_COLUMN 7 /* structured boxes (fem_hip_box_detect): element columns marching in z with node
+                                      planes in LDS, shared column faces summed by a second pass (deterministic;
+                                      no index arrays read) */
 /* Explicit algorithm and its elements per workgroup (element kernels) or per
  * block (block kernels); 0 = that algorithm's default. A deterministic scatter
  * mode accepts only the deterministic algorithms. exact = 1 overrides. */
 int fem_hip_operator_set_fused_algorithm(fem_hip_operator op, int alg, int elems);
+/* Element layers per workgroup of FEM_HIP_FUSED_BOX_COLUMN (a divisor of the
+ * box's nz; 0 = automatic: the most that still gives 2048 workgroups). A value
+ * that does not divide nz makes the box-column algorithm unavailable. */
+int fem_hip_operator_set_box_layers(fem_hip_operator op, int layers);
 /* The algorithm and elems the next apply in this precision (fp32 = 0/1) will run. */
 int fem_hip_operator_get_fused_choice(fem_hip_operator op, int fp32, int* alg, int* elems);
 /* Autotuner: times every compiled algorithm/packing valid for the operator's
@@ -292,6 +299,22 @@ int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords);
  * by the row of its node, in ascending order. */
 int fem_hip_block_plan_stats(int num_elem, int elem_size, int l_size, const int* offsets, int nb, int max_nodes,
                              int num_comp, int comp_stride, long long* stats);
+
+/* Structured boxes (no counterpart in the reference). is_box = 1 when offsets
+ * [num_elem][P^3] are an nx*ny*nz box of P^3-node hexes with lexicographic
+ * node numbering (tests/cuda/test-cuda-elem-restriction.cu:24-42 with node
+ * stride 1), l_size = num_comp * node count and, for num_comp > 1, comp_stride =
+ * node count; n3 = {nx, ny, nz}. Fused mass operators on such restrictions can
+ * run FEM_HIP_FUSED_BOX_COLUMN. */
+int fem_hip_box_detect(int num_elem, int P, const int* offsets, int num_comp, int comp_stride, int l_size, int* n3,
+                       int* is_box);
+/* Host-only check of the box-column algorithm's index arithmetic for an
+ * nx*ny*nz box with `elems` elements per column layer and `layers` element
+ * layers per workgroup (0 = automatic): every node is written
+ * once (to v by its segment, or by the shared-node pass), every partial slot is
+ * written once and read once, owners ascend. stats[6] = {segments, layers per
+ * segment, shared nodes, partial slots, nodes written directly, most owners of a node}. */
+int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layers, long long* stats);
 
 /* ---- mass solve ------------------------------------------------------------ */
 #define FEM_HIP_SOLVE_FP64 0  /* Jacobi-PCG, FP64 throughout */

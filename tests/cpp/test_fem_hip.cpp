@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <functional>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -235,6 +236,166 @@ TEST_CASE(operator_apply_mass_matches_cpu) {
     for (double x : v_cpu) scale = std::fmax(scale, std::fabs(x));
     for (size_t i = 0; i < v.size(); ++i) REQUIRE(std::fabs(v[i] - v_cpu[i]) <= 1e-13 * scale);
   }
+}
+
+namespace {
+// The mass operator of a mesh file's node restriction on straight (unjittered)
+// unit-cube elements: qdata from the device build, then both the host operator
+// (on the host copy of qdata) and the device operator.
+struct MeshMass {
+  fem::ElemRestriction rstr;
+  fem::Operator host;
+  std::vector<double> qdata;
+};
+MeshMass mesh_mass(const fem::hip::Mesh& m, int Q) {
+  const int E = m.info.num_elem, P = m.info.P;
+  MeshMass mm;
+  mm.rstr = fem::hip::mesh_restriction(m);
+  fem::ElemRestriction coords_r, q_rstr;
+  fem::elem_restriction_create(E, 8, 3, m.info.num_vertices, 3 * m.info.num_vertices, m.vertex_offsets, coords_r);
+  fem::elem_restriction_create_strided(E, Q * Q * Q, 1, E * Q * Q * Q, q_rstr);
+  auto mesh_basis = fem::TensorBasis::create_tensor_H1_lagrange(3, 3, 2, Q);
+  auto basis = fem::TensorBasis::create_tensor_H1_lagrange(3, 1, P, Q);
+  fem::QFunction build_qf;
+  fem::qfunction_create(fem::qfunctions::build_mass, build_qf);
+  fem::qfunction_add_input(build_qf, "dx", 9, fem::EvalMode::Grad);
+  fem::qfunction_add_input(build_qf, "weights", 1, fem::EvalMode::Weight);
+  fem::qfunction_add_output(build_qf, "qdata", 1, fem::EvalMode::None);
+  fem::qfunction_set_context(build_qf, fem::qfunctions::BuildMassContext{3});
+  fem::Operator build;
+  fem::operator_create(build_qf, build);
+  fem::operator_set_field(build, "dx", coords_r, mesh_basis, fem::vector_active());
+  fem::operator_set_field(build, "weights", std::nullopt, mesh_basis, fem::vector_none());
+  fem::operator_set_field(build, "qdata", q_rstr, std::nullopt, fem::vector_active());
+  fem::operator_apply(build, m.coords, mm.qdata);
+  fem::QFunction mass_qf;
+  fem::qfunction_create(fem::qfunctions::apply_mass, mass_qf);
+  fem::qfunction_add_input(mass_qf, "u", 1, fem::EvalMode::Interp);
+  fem::qfunction_add_input(mass_qf, "qdata", 1, fem::EvalMode::None);
+  fem::qfunction_add_output(mass_qf, "v", 1, fem::EvalMode::Interp);
+  fem::operator_create(mass_qf, mm.host);
+  fem::operator_set_field(mm.host, "u", mm.rstr, basis, fem::vector_active());
+  fem::operator_set_field(mm.host, "qdata", q_rstr, std::nullopt, fem::vector_passive(mm.qdata));
+  fem::operator_set_field(mm.host, "v", mm.rstr, basis, fem::vector_active());
+  return mm;
+}
+
+// An n^3 box as a mesh file's contents, with node numbers shuffled by a fixed
+// permutation (i -> (i * a) mod N, a coprime to N).
+fem::hip::Mesh shuffled_box(int n, int P) {
+  fem::hip::Mesh m;
+  const int N1 = n * (P - 1) + 1, N = N1 * N1 * N1;
+  m.info = {3, n * n * n, P, N, (n + 1) * (n + 1) * (n + 1)};
+  int a = 7919;
+  while (std::gcd(a, N) != 1) a += 2;
+  m.offsets.resize(static_cast<size_t>(m.info.num_elem) * P * P * P);
+  FEM_HIP_CHECK(fem_hip_box_offsets(n, n, n, P, 1, m.offsets.data()));
+  for (int& o : m.offsets) o = static_cast<int>((static_cast<long long>(o) * a) % N);
+  m.vertex_offsets.resize(static_cast<size_t>(m.info.num_elem) * 8);
+  FEM_HIP_CHECK(fem_hip_box_offsets(n, n, n, 2, 1, m.vertex_offsets.data()));
+  m.coords.resize(3 * static_cast<size_t>(m.info.num_vertices));
+  FEM_HIP_CHECK(fem_hip_box_coords(n, 0.1, 11, m.coords.data()));
+  return m;
+}
+}  // namespace
+
+TEST_CASE(mesh_file_round_trip_and_every_fused_algorithm) {
+  // A mesh written and read back through the library's file format drives the
+  // reference's host operator and the device one; every fused algorithm agrees
+  // with the host to 1e-13, the exact transpose-map one bit for bit.
+  const fem::hip::Mesh written = shuffled_box(4, 4);
+  const std::string path = "/tmp/fem_hip_cpp_test_mesh.femh";
+  fem::hip::mesh_write(path, written);
+  const fem::hip::Mesh m = fem::hip::mesh_read(path);
+  std::remove(path.c_str());
+  REQUIRE(m.offsets == written.offsets && m.vertex_offsets == written.vertex_offsets && m.coords == written.coords);
+
+  MeshMass mm = mesh_mass(m, 5);
+  const std::vector<double> u = wiggly(mm.rstr.l_size, 0.3);
+  std::vector<double> v_cpu;
+  fem::operator_apply(mm.host, u, v_cpu);
+  double scale = 0.0;
+  for (double x : v_cpu) scale = std::fmax(scale, std::fabs(x));
+  DeviceArray<double> d_u(u), d_v;
+
+  using A = fem::hip::FusedAlgorithm;
+  for (A alg : {A::ElemAtomic, A::ElemTmap, A::BlockAtomic, A::BlockPartial, A::ElemTmapPipe, A::BlockPartialLds,
+                A::BlockPartialLdsPipe}) {
+    fem::hip::Operator dmass;
+    fem::hip::operator_create(mm.host, dmass);
+    fem::hip::operator_set_fused_algorithm(dmass, alg);
+    REQUIRE(fem::hip::operator_fused_choice(dmass).alg == alg);
+    fem::hip::operator_apply(dmass, d_u, d_v);
+    const std::vector<double> v = d_v.to_host();
+    for (size_t i = 0; i < v.size(); ++i) REQUIRE(std::fabs(v[i] - v_cpu[i]) <= 1e-13 * scale);
+  }
+  fem::hip::Operator exact;
+  fem::hip::operator_create(mm.host, exact, ScatterMode::Deterministic);
+  FEM_HIP_CHECK(fem_hip_operator_set_fused_options(exact.handle.get(), -1, 1, 0));
+  fem::hip::operator_apply(exact, d_u, d_v);
+  REQUIRE(d_v.to_host() == v_cpu);
+
+  // A deterministic operator refuses an algorithm that sums in any order (when
+  // the choice is resolved, at the next apply or query).
+  fem::hip::Operator det;
+  fem::hip::operator_create(mm.host, det, ScatterMode::Deterministic);
+  fem::hip::operator_set_fused_algorithm(det, A::BlockAtomic);
+  REQUIRE_THROWS_AS(fem::hip::operator_fused_choice(det), std::invalid_argument);
+  REQUIRE_THROWS_AS(fem::hip::operator_apply(det, d_u, d_v), std::invalid_argument);
+}
+
+TEST_CASE(mass_diagonal_autotune_and_solve) {
+  const fem::hip::Mesh m = shuffled_box(3, 3);
+  MeshMass mm = mesh_mass(m, 4);
+  fem::hip::Operator dmass;
+  fem::hip::operator_create(mm.host, dmass, ScatterMode::Deterministic);
+
+  // Diagonal: e_i^T M e_i through the host operator for a few i.
+  DeviceArray<double> d_diag;
+  fem::hip::mass_diagonal(dmass, d_diag);
+  const std::vector<double> diag = d_diag.to_host();
+  for (int i : {0, 7, mm.rstr.l_size / 2, mm.rstr.l_size - 1}) {
+    std::vector<double> e(mm.rstr.l_size, 0.0), Me;
+    e[i] = 1.0;
+    fem::operator_apply(mm.host, e, Me);
+    REQUIRE(std::fabs(diag[i] - Me[i]) <= 1e-14 * std::fabs(Me[i]));
+  }
+
+  const fem::hip::FusedChoice best = fem::hip::operator_autotune(dmass, false, 3);
+  REQUIRE(best.ms > 0.0);
+  REQUIRE(best.alg == fem::hip::FusedAlgorithm::ElemTmap || best.alg == fem::hip::FusedAlgorithm::BlockPartial ||
+          best.alg == fem::hip::FusedAlgorithm::ElemTmapPipe);
+  REQUIRE(fem::hip::operator_fused_choice(dmass).alg == best.alg);
+
+  // Solve M x = M x* and compare with x*.
+  std::vector<double> xs = wiggly(mm.rstr.l_size, 0.7), b;
+  fem::operator_apply(mm.host, xs, b);
+  DeviceArray<double> d_b(b), d_x;
+  for (bool mixed : {false, true}) {
+    const fem_hip_solve_stats st = fem::hip::mass_solve(dmass, d_b, d_x, mixed, 1e-11);
+    REQUIRE(st.rel_residual <= 1e-11);
+    const std::vector<double> x = d_x.to_host();
+    double err = 0.0, nrm = 0.0;
+    for (size_t i = 0; i < x.size(); ++i) {
+      err = std::fmax(err, std::fabs(x[i] - xs[i]));
+      nrm = std::fmax(nrm, std::fabs(xs[i]));
+    }
+    REQUIRE(err <= 1e-8 * nrm);
+  }
+}
+
+TEST_CASE(server_requests_converge_and_repeat) {
+  // Two requests through the C++ server wrapper: both converge, and the same
+  // request twice gives the same solution bit for bit.
+  fem::hip::Server srv(fem::hip::server_options());
+  const fem_hip_serve_request req{6, 3, 4, 2, FEM_HIP_SOLVE_MIXED, 0.1, 1e-10, 5, 6};
+  const long long l_size = 2LL * (6 * 2 + 1) * (6 * 2 + 1) * (6 * 2 + 1);
+  const long long a = srv.submit(req), b = srv.submit(req);
+  std::vector<double> xa, xb;
+  const fem_hip_serve_result ra = srv.wait(a, l_size, xa), rb = srv.wait(b, l_size, xb);
+  REQUIRE(ra.status == 0 && rb.status == 0 && ra.l_size == l_size);
+  REQUIRE(ra.rel_residual <= 1e-10 && rb.rel_residual <= 1e-10);
+  REQUIRE(xa == xb);
 }
 
 TEST_CASE(operator_create_rejects_unknown_qfunction) {

@@ -720,7 +720,7 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
     g.ncx = nx / BX;
     g.ncy = ny / BY;
     require(layers >= 0 && (layers == 0 || nz % layers == 0), "box_column_check: layers must divide nz");
-    g.lz = layers > 0 ? layers : box_layers(nz, static_cast<long long>(g.ncx) * g.ncy, 2048, P);
+    g.lz = layers > 0 ? layers : box_layers(nz, static_cast<long long>(g.ncx) * g.ncy, 32 * 256 / elems, P);
     require(g.lz > 0, "box_column_check: no layer count fits");
     g.nsz = nz / g.lz;
     g.cs = g.Nx * g.Ny * g.Nz;

@@ -213,7 +213,10 @@ __device__ __forceinline__ void eo_transpose(const T (&y)[Q], T (&out)[P], const
   }
 }
 
-template <int P, int Q, int NC, class T, int BX, int BY>
+// UM: how u reaches the x-rows. 0: each row thread loads its row when the
+// step starts; 1: the row of the next step is loaded one step ahead; 2: the
+// node planes go through an LDS ring, the next layer's loaded one layer ahead.
+template <int P, int Q, int NC, class T, int BX, int BY, int UM>
 __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_kernel(
     BoxGeom g, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
     size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em) {
@@ -223,6 +226,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   constexpr int LXn = BX * (P - 1) + 1, LYn = BY * (P - 1) + 1, PLANE = LXn * LYn;
   __shared__ T s_b[NE][S::FBUF];
   __shared__ T s_top[NC][PLANE];  // top node plane of the last layer, completed by the next one
+  __shared__ T s_u[UM == 2 ? NC : 1][UM == 2 ? P : 1][UM == 2 ? PLANE : 1];  // u ring (UM 2), slot gz % P
   static_assert(PLANE <= NE * S::TPE, "one node position per thread");
 
   // Segment of this workgroup; XCD x (which receives workgroups h with
@@ -274,12 +278,20 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   const bool pys = (ply == 0 && cy > 0) || (ply == LYn - 1 && cy < g.ncy - 1);
   T* const pv = v + static_cast<size_t>(gx0 + plx) + static_cast<size_t>(g.Nx) * (gy0 + ply);
 
-  // Software pipeline over the (layer, component) steps: the u row of the
-  // next step and the qdata of the next layer are loaded while the current
-  // step contracts, so HBM latency overlaps the LDS work.
-  // u row of thread (iz,iy) = t for element layer ez, component c:
-  const int riz = t / P, riy = t - (t / P) * P;
+  // Software pipeline over the (layer, component) steps (see UM); qdata of
+  // the next layer is loaded one layer ahead. With the LDS ring, plane gz of
+  // the column's node region sits in slot gz % P and is loaded as whole rows;
+  // the P-1 new planes of the next layer are stored into the slots the
+  // current layer's x-rows have left (the shared plane stays).
+  constexpr int NT = round64(NE * S::TPE);
+  constexpr int NEW = (P - 1) * PLANE;       // new u values per layer and component
+  constexpr int UPT = UM == 2 ? (NEW + NT - 1) / NT : 1;
   const bool row_thread = live && t < P2, line_thread = live && t < Q2;
+  const int riz = t / P, riy = t - (t / P) * P;
+  auto node_addr = [&](int gz, int p) -> size_t {
+    const int ly = p / LXn, lx = p - ly * LXn;
+    return static_cast<size_t>(gx0 + lx) + static_cast<size_t>(g.Nx) * (gy0 + ly) + Nxy * gz;
+  };
   auto row_ptr = [&](int ez, int c) -> const T* {
     return u + static_cast<size_t>(c) * g.cs + static_cast<size_t>(ex * (P - 1)) +
            static_cast<size_t>(g.Nx) * (ey * (P - 1) + riy) + Nxy * (ez * (P - 1) + riz);
@@ -288,17 +300,28 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
     const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ez);
     return qdata + e * Q3 + t;
   };
-  T xc[P] = {}, qc[Q] = {};
-  if (row_thread) {
-    const T* r = row_ptr(sz * g.lz, 0);
-#pragma unroll
-    for (int ix = 0; ix < P; ++ix) xc[ix] = r[ix];
+  if constexpr (UM == 2) {
+    const int gzb = sz * g.lz * (P - 1);  // all P planes of the first layer
+    for (int idx = threadIdx.x; idx < NC * P * PLANE; idx += NT) {
+      const int c = idx / (P * PLANE), r = idx - c * (P * PLANE), j = r / PLANE, p = r - j * PLANE;
+      s_u[c][(gzb + j) % P][p] = u[static_cast<size_t>(c) * g.cs + node_addr(gzb + j, p)];
+    }
   }
+  T xc[UM == 1 ? P : 1] = {};
+  if constexpr (UM == 1) {
+    if (row_thread) {
+      const T* r = row_ptr(sz * g.lz, 0);
+#pragma unroll
+      for (int ix = 0; ix < P; ++ix) xc[ix] = r[ix];
+    }
+  }
+  T qc[Q] = {};
   if (line_thread) {
     const T* r = qd_ptr(sz * g.lz);
 #pragma unroll
     for (int qz = 0; qz < Q; ++qz) qc[qz] = r[qz * Q2];
   }
+  if constexpr (UM == 2) __syncthreads();
 
 #pragma unroll 1
   for (int k = 0; k < g.lz; ++k) {
@@ -314,20 +337,43 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
     for (int qz = 0; qz < Q; ++qz) qd[qz] = qc[qz];
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {
-      // The next step's u row.
-      T xn[P] = {};
-      const int cn = c + 1 < NC ? c + 1 : 0, kn = c + 1 < NC ? k : k + 1;
-      if (row_thread && kn < g.lz) {
-        const T* r = row_ptr(sz * g.lz + kn, cn);
+      const bool more = k + 1 < g.lz;
+      const int gzn = (ez + 1) * (P - 1) + 1;  // first new plane of the next layer
+      T un[UPT] = {};
+      T xn[UM == 1 ? P : 1] = {};
+      if constexpr (UM == 2) {
 #pragma unroll
-        for (int ix = 0; ix < P; ++ix) xn[ix] = r[ix];
+        for (int r = 0; r < UPT; ++r) {
+          const int idx = threadIdx.x + r * NT;
+          if (more && idx < NEW) {
+            const int j = idx / PLANE, p = idx - j * PLANE;
+            un[r] = u[static_cast<size_t>(c) * g.cs + node_addr(gzn + j, p)];
+          }
+        }
+      } else if constexpr (UM == 1) {
+        const int cn = c + 1 < NC ? c + 1 : 0, kn = c + 1 < NC ? k : k + 1;
+        if (row_thread && kn < g.lz) {
+          const T* r = row_ptr(sz * g.lz + kn, cn);
+#pragma unroll
+          for (int ix = 0; ix < P; ++ix) xn[ix] = r[ix];
+        }
       }
       // S1: x-rows (iz,iy): contract x.
       if (row_thread) {
         const int iz = riz, iy = riy;
         T x[P], y[Q];
+        if constexpr (UM == 2) {
+          const T* ur = &s_u[c][(ez * (P - 1) + iz) % P][(ley * (P - 1) + iy) * LXn + lex * (P - 1)];
 #pragma unroll
-        for (int ix = 0; ix < P; ++ix) x[ix] = xc[ix];
+          for (int ix = 0; ix < P; ++ix) x[ix] = ur[ix];
+        } else if constexpr (UM == 1) {
+#pragma unroll
+          for (int ix = 0; ix < P; ++ix) x[ix] = xc[ix];
+        } else {
+          const T* r = row_ptr(ez, c);
+#pragma unroll
+          for (int ix = 0; ix < P; ++ix) x[ix] = r[ix];
+        }
         eo_forward<P, Q, T>(x, y, Em);
         const int row = iz * ZS + iy * RY;
 #pragma unroll
@@ -415,9 +461,23 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
           }
         }
       }
-      __syncthreads();  // the next step's S1 overwrites the tiles
+      if constexpr (UM == 2) {
+        // The next layer's new planes into the slots this layer's S1 has left.
 #pragma unroll
-      for (int ix = 0; ix < P; ++ix) xc[ix] = xn[ix];
+        for (int r = 0; r < UPT; ++r) {
+          const int idx = threadIdx.x + r * NT;
+          if (more && idx < NEW) {
+            const int j = idx / PLANE, p = idx - j * PLANE;
+            s_u[c][(gzn + j) % P][p] = un[r];
+          }
+        }
+      } else if constexpr (UM == 1) {
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) xc[ix] = xn[ix];
+      }
+      (void)more;
+      (void)gzn;
+      __syncthreads();  // the next step's S1 overwrites the tiles (and reads the ring)
     }
 #pragma unroll
     for (int qz = 0; qz < Q; ++qz) qc[qz] = qn[qz];
@@ -461,13 +521,9 @@ inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
   return g;
 }
 
-template <int P, int Q, int NC, class T, int BX, int BY>
+template <int P, int Q, int NC, class T, int BX, int BY, int UM>
 void launch_box(const FusedArgs& a, hipStream_t s) {
   using S = Shape<P, Q>;
-  if (a.box_pass == 2) {  // report how many workgroups fit the GPU at once
-    *a.box_slots = persistent_grid(fused_box_kernel<P, Q, NC, T, BX, BY>, round64(BX * BY * S::TPE), 0);
-    return;
-  }
   const BoxGeom g = box_geom(a, P, BX, BY);
   require(g.nx % BX == 0 && g.ny % BY == 0 && g.lz > 0 && g.nz % g.lz == 0 && BX * (P - 1) >= 2 &&
               BY * (P - 1) >= 2 && g.lz * (P - 1) >= 2,
@@ -479,7 +535,7 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
           "fused mass: box partial buffer too small");
   if (a.box_pass == 0) {
     constexpr int threads = round64(BX * BY * S::TPE);
-    fused_box_kernel<P, Q, NC, T, BX, BY><<<static_cast<unsigned>(nseg), threads, 0, s>>>(
+    fused_box_kernel<P, Q, NC, T, BX, BY, UM><<<static_cast<unsigned>(nseg), threads, 0, s>>>(
         g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
         static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
   } else {
@@ -491,12 +547,14 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
   HIP_CHECK_LAUNCH();
 }
 
-// Key elems of a box-column variant: BX * BY elements per layer.
+// Key elems of a box-column variant: BX * BY elements per layer. FP64 loads
+// its x-rows straight from u, FP32 goes through the LDS ring (the faster of
+// the two for each precision on MI355X at 64^3, P=5).
 template <int P, int Q, int NC, int BX, int BY>
 struct FusedBoxReg {
   FusedBoxReg() {
-    register_fused(FusedKey{P, Q, NC, 0, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, double, BX, BY>, 0);
-    register_fused(FusedKey{P, Q, NC, 1, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, float, BX, BY>, 0);
+    register_fused(FusedKey{P, Q, NC, 0, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, double, BX, BY, 0>, 0);
+    register_fused(FusedKey{P, Q, NC, 1, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, float, BX, BY, 2>, 0);
   }
 };
 

@@ -281,32 +281,24 @@ bool valid_for(const FusedMass& f, int alg) {
   return (f.scatter != kScatterDeterministic || alg_deterministic(alg)) && (!alg_box(alg) || f.box);
 }
 
-// Workgroups of a box-column kernel resident on the GPU at once (asked of the
-// kernel's launcher once per key).
-long long box_slots(const FusedKey& k) {
-  static std::mutex m;
-  static std::map<FusedKey, int> cache;
-  std::lock_guard<std::mutex> lk(m);
-  auto it = cache.find(k);
-  if (it != cache.end()) return it->second;
-  FusedLauncher L = find(k);
-  require(L != nullptr, "fused mass: box-column variant not compiled");
-  int slots = 0;
-  FusedArgs a;
-  a.box_pass = 2;
-  a.box_slots = &slots;
-  L(a, nullptr);
-  cache[k] = slots;
-  return slots;
+// Segments the box-column apply aims at: 32 elements per column layer per CU.
+long long box_target_segments(int elems) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  return std::max<long long>(1, 32LL * cus / std::max(elems, 1));
 }
 
-// Element layers per segment: the operator's setting, else box_layers for the
-// kernel's residency.
+// Element layers per segment: the operator's setting, else box_layers.
 int box_lz_of(const FusedMass& f, const FusedKey& k) {
   if (f.box_lz > 0) return f.box_lz;
   int BX = 0, BY = 0;
   box_columns(k.elems, &BX, &BY);
-  return box_layers(f.box_n[2], static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY), box_slots(k), f.P);
+  return box_layers(f.box_n[2], static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY),
+                    box_target_segments(k.elems), f.P);
 }
 
 // Whether the box-column variant with `elems` elements per layer tiles f's box
@@ -659,20 +651,14 @@ bool detect_box(const int* off, int num_elem, int P, int num_comp, int comp_stri
   return true;
 }
 
-int box_layers(int nz, long long ncols, long long slots, int P) {
-  int best = 0;
-  long long best_cost = 0;
-  slots = std::max<long long>(slots, 1);
+int box_layers(int nz, long long ncols, long long target, int P) {
+  int best = 0, smallest = 0;
   for (int d = 1; d <= nz; ++d) {
     if (nz % d != 0 || d * (P - 1) < 2) continue;
-    const long long nseg = ncols * (nz / d);
-    const long long cost = (nseg + slots - 1) / slots * d;
-    if (best == 0 || cost <= best_cost) {
-      best = d;
-      best_cost = cost;
-    }
+    if (!smallest) smallest = d;
+    if (ncols * (nz / d) >= target) best = d;
   }
-  return best;
+  return best ? best : smallest;
 }
 
 FusedKey fused_resolve(const FusedMass& f, int fp32) {

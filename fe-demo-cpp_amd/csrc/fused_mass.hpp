@@ -82,8 +82,7 @@ struct FusedArgs {
   // box column: elements per direction, layers per segment, pass (0 the column kernel, 1 the shared-node sums)
   int box_n[3] = {0, 0, 0};
   int box_lz = 0;
-  int box_pass = 0;        // 2: only report the workgroups that fit the GPU at once in *box_slots
-  int* box_slots = nullptr;
+  int box_pass = 0;
 };
 
 using FusedLauncher = void (*)(const FusedArgs&, hipStream_t);
@@ -176,12 +175,14 @@ void fused_prepare(FusedMass& f, int fp32, hipStream_t s);
 // with lexicographic node numbering (node stride 1), l_size = NC * node count
 // and (NC > 1) comp_stride = node count; on success n3 = {nx, ny, nz}.
 bool detect_box(const int* offsets, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3);
-// Element layers per segment of the box-column algorithm for nz layers,
-// ncols columns and `slots` workgroups resident at once: the divisor lz of nz
-// (with lz*(P-1) >= 2) that minimises the makespan in layers,
-// ceil(ncols*(nz/lz) / slots) * lz; ties go to the larger lz (fewer shared
-// segment ends). 0 if no divisor qualifies.
-int box_layers(int nz, long long ncols, long long slots, int P);
+// Element layers per segment of the box-column algorithm for nz layers and
+// ncols columns: the largest divisor lz of nz (with lz*(P-1) >= 2) that still
+// leaves at least `target` segments, else the smallest such divisor; 0 if
+// none qualifies. The apply aims at 32 elements per column layer per CU
+// (target = 32 * CUs / elements per layer): measured on MI355X, more
+// segments per CU only share the CU, fewer leave it idle, and longer segments
+// share fewer segment ends.
+int box_layers(int nz, long long ncols, long long target, int P);
 // Jacobi diagonal: diag = E^T [ (B o B)^T x3 qdata ] (FP64).
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);
 

@@ -230,8 +230,9 @@ int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, in
  * mode accepts only the deterministic algorithms. exact = 1 overrides. */
 int fem_hip_operator_set_fused_algorithm(fem_hip_operator op, int alg, int elems);
 /* Element layers per workgroup of FEM_HIP_FUSED_BOX_COLUMN (a divisor of the
- * box's nz; 0 = automatic: the most that still gives 2048 workgroups). A value
- * that does not divide nz makes the box-column algorithm unavailable. */
+ * box's nz; 0 = automatic: the most that still gives 32 * CUs / (elements per
+ * column layer) workgroups). A value that does not divide nz makes the
+ * box-column algorithm unavailable. */
 int fem_hip_operator_set_box_layers(fem_hip_operator op, int layers);
 /* The algorithm and elems the next apply in this precision (fp32 = 0/1) will run. */
 int fem_hip_operator_get_fused_choice(fem_hip_operator op, int fp32, int* alg, int* elems);

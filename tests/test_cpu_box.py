@@ -60,10 +60,16 @@ def test_box_column_index_arithmetic(n3, P, elems, layers):
 
 
 def test_box_column_layers_automatic():
-    # 64^3 with 4x4 columns: 256 columns; 8 layers keep 2048 workgroups.
-    assert fehip.box_column_check(64, 64, 64, 5, 16)["layers"] == 8
-    assert fehip.box_column_check(64, 64, 64, 5, 8)["layers"] == 16
+    # The check assumes 256 CUs and aims at 32 elements per column layer per
+    # CU: 64^3 with 4x4 columns (256 columns) -> 512 segments of 32 layers;
+    # 4x2 columns (512) -> 1024 segments, also of 32 layers.
+    assert fehip.box_column_check(64, 64, 64, 5, 16)["layers"] == 32
+    assert fehip.box_column_check(64, 64, 64, 5, 8)["layers"] == 32
+    assert fehip.box_column_check(64, 64, 64, 5, 4)["layers"] == 32
+    # Too few columns for the target: one layer per segment.
     assert fehip.box_column_check(8, 8, 8, 5, 16)["layers"] == 1
+    # P = 2 needs two layers per segment (a segment end must hold 2 node planes).
+    assert fehip.box_column_check(8, 8, 6, 2, 16)["layers"] == 2
 
 
 def test_box_column_check_rejects_bad_sizes():

@@ -563,17 +563,17 @@ int fem_hip_operator_autotune(fem_hip_operator h, int fp32, int reps, void* s, i
     if (n_results) *n_results = static_cast<int>(all.size());
   });
 }
-int fem_hip_tune_table_get(int* rows7, double* ms, long long* num_elem, int max_rows, int* n_rows) {
+int fem_hip_tune_table_get(int* rows8, double* ms, long long* num_elem, int max_rows, int* n_rows) {
   return guard([&] {
     need(n_rows, "n_rows");
-    require(max_rows >= 0 && (max_rows == 0 || rows7), "tune_table_get: bad rows buffer");
+    require(max_rows >= 0 && (max_rows == 0 || rows8), "tune_table_get: bad rows buffer");
     int i = 0;
     std::lock_guard<std::mutex> lk(tune_mutex());
     for (const auto& kv : tune_table()) {
       if (i < max_rows) {
         const TuneKey& k = kv.first;
-        const int r[7] = {k.P, k.Q, k.NC, k.fp32, k.det, kv.second.alg, kv.second.elems};
-        std::memcpy(rows7 + 7 * i, r, sizeof(r));
+        const int r[8] = {k.P, k.Q, k.NC, k.fp32, k.det, k.box, kv.second.alg, kv.second.elems};
+        std::memcpy(rows8 + 8 * i, r, sizeof(r));
         if (ms) ms[i] = kv.second.ms;
         if (num_elem) num_elem[i] = kv.second.num_elem;
       }
@@ -587,18 +587,19 @@ int fem_hip_tune_table_set(const int* r, double ms, long long num_elem) {
     need(r, "row");
     require(r[3] == 0 || r[3] == 1, "tune_table_set: fp32 must be 0 or 1");
     require(r[4] == 0 || r[4] == 1, "tune_table_set: deterministic must be 0 or 1");
-    require(r[5] >= 0 && r[5] < kAlgCount, "tune_table_set: unknown algorithm");
-    require(!r[4] || alg_deterministic(r[5]), "tune_table_set: a deterministic row needs a deterministic algorithm");
-    require(!fused_elem_variants(r[0], r[1], r[2], r[3], r[5], 0).empty() &&
-                fused_registry().count(FusedKey{r[0], r[1], r[2], r[3], r[5], 0, r[6]}),
+    require(r[5] == 0 || r[5] == 1, "tune_table_set: box must be 0 or 1");
+    require(r[6] >= 0 && r[6] < kAlgCount, "tune_table_set: unknown algorithm");
+    require(!r[4] || alg_deterministic(r[6]), "tune_table_set: a deterministic row needs a deterministic algorithm");
+    require(r[5] || !alg_box(r[6]), "tune_table_set: the box-column algorithm needs a box row");
+    require(fused_registry().count(FusedKey{r[0], r[1], r[2], r[3], r[6], 0, r[7]}) != 0,
             "tune_table_set: algorithm/elems not compiled for this shape");
     TuneEntry e;
-    e.alg = r[5];
-    e.elems = r[6];
+    e.alg = r[6];
+    e.elems = r[7];
     e.ms = ms;
     e.num_elem = num_elem;
     std::lock_guard<std::mutex> lk(tune_mutex());
-    tune_table()[TuneKey{r[0], r[1], r[2], r[3], r[4]}] = e;
+    tune_table()[TuneKey{r[0], r[1], r[2], r[3], r[4], r[5]}] = e;
   });
 }
 int fem_hip_tune_table_clear(void) {
@@ -760,30 +761,40 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
             }
           }
     }
-    // What the shared-node pass reads: each shared node once, each written slot once.
+    // What the shared-node pass reads, over its whole grid: each shared node
+    // once, each written slot once.
     const long long count = box_shared_count(g, P, BX, BY);
+    const dim3 grid = box_shared_grid(g);
     std::vector<char> shared(N, 0);
     int max_owners = 0;
-    long long slots_read = 0;
-    for (long long id = 0; id < count; ++id) {
-      int gx, gy, gz, seg[8], sl[8];
-      box_shared_node(g, P, BX, BY, id, &gx, &gy, &gz);
-      if (gx < 0 || gx >= g.Nx || gy < 0 || gy >= g.Ny || gz < 0 || gz >= g.Nz) fail("shared node out of the box");
-      const long long node = gx + static_cast<long long>(g.Nx) * (gy + static_cast<long long>(g.Ny) * gz);
-      if (shared[node]) fail("a shared node is enumerated twice");
-      shared[node] = 1;
-      const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, sl);
-      if (n < 2) fail("a shared node has fewer than 2 segments");
-      max_owners = std::max(max_owners, n);
-      for (int i = 0; i < n; ++i) {
-        if (i > 0 && seg[i] <= seg[i - 1]) fail("segments not ascending");
-        if (seg[i] < 0 || seg[i] >= nseg || sl[i] < 0 || sl[i] >= face) fail("owner out of range");
-        char& c = slot[static_cast<size_t>(static_cast<long long>(seg[i]) * face + sl[i])];
-        if (c != 1) fail("a shared node reads a slot no segment wrote for it, or reads it twice");
-        c = 2;
-        ++slots_read;
+    long long slots_read = 0, visited = 0;
+    for (unsigned pl = 0; pl < grid.z; ++pl) {
+      for (unsigned ln = 0; ln < grid.y; ++ln) {
+        for (unsigned pt = 0; pt < grid.x * 64; ++pt) {
+          int gx, gy, gz, seg[8], sl[8];
+          if (!box_shared_point(g, P, BX, BY, static_cast<int>(pl), static_cast<int>(ln), static_cast<int>(pt), &gx, &gy,
+                                &gz))
+            continue;
+          ++visited;
+          if (gx < 0 || gx >= g.Nx || gy < 0 || gy >= g.Ny || gz < 0 || gz >= g.Nz) fail("shared node out of the box");
+          const long long node = gx + static_cast<long long>(g.Nx) * (gy + static_cast<long long>(g.Ny) * gz);
+          if (shared[node]) fail("a shared node is enumerated twice");
+          shared[node] = 1;
+          const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, sl);
+          if (n < 2) fail("a shared node has fewer than 2 segments");
+          max_owners = std::max(max_owners, n);
+          for (int i = 0; i < n; ++i) {
+            if (i > 0 && seg[i] <= seg[i - 1]) fail("segments not ascending");
+            if (seg[i] < 0 || seg[i] >= nseg || sl[i] < 0 || sl[i] >= face) fail("owner out of range");
+            char& c = slot[static_cast<size_t>(static_cast<long long>(seg[i]) * face + sl[i])];
+            if (c != 1) fail("a shared node reads a slot no segment wrote for it, or reads it twice");
+            c = 2;
+            ++slots_read;
+          }
+        }
       }
     }
+    if (visited != count) fail("the shared-node grid misses or repeats nodes");
     for (long long node = 0; node < N; ++node)
       if (direct[node] + shared[node] != 1) fail("a node is not written exactly once");
     if (slots_read != slots_written) fail("a written slot is never read");

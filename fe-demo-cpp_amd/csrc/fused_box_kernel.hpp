@@ -72,34 +72,40 @@ __host__ __device__ inline long long box_shared_count(const BoxGeom& g, int P, i
   return nbx * g.Ny * g.Nz + nby * (g.Nx - nbx) * g.Nz + nbz * (g.Nx - nbx) * (g.Ny - nby);
 }
 
-// Coordinates of shared node `id` (0 <= id < box_shared_count).
-__host__ __device__ inline void box_shared_node(const BoxGeom& g, int P, int BX, int BY, long long id, int* gx,
-                                                int* gy, int* gz) {
+// The shared-node pass runs on a grid of (face planes) x (lines) x (points
+// along a line): plane f < ncx-1 is the x-face gx = (f+1)*SX with lines gz and
+// points gy; the next ncy-1 planes are y-faces gy = (f'+1)*SY, lines gz,
+// points gx; the last nsz-1 are segment ends gz = (f''+1)*SZ, lines gy, points
+// gx. A point on a cut of an earlier kind (x before y before z) belongs to
+// that plane and is skipped here. Returns whether (plane, line, point) is a
+// shared node of this plane, and its coordinates.
+__host__ __device__ inline bool box_shared_point(const BoxGeom& g, int P, int BX, int BY, int plane, int line,
+                                                 int point, int* gx, int* gy, int* gz) {
   const int SX = BX * (P - 1), SY = BY * (P - 1), SZ = g.lz * (P - 1);
   const int nbx = g.ncx - 1, nby = g.ncy - 1;
-  const long long cx = static_cast<long long>(nbx) * g.Ny * g.Nz;
-  if (id < cx) {
-    *gy = static_cast<int>(id % g.Ny);
-    const long long r = id / g.Ny;
-    *gz = static_cast<int>(r % g.Nz);
-    *gx = static_cast<int>(r / g.Nz + 1) * SX;
-    return;
+  auto xcut = [&](int x) { return x % SX == 0 && x > 0 && x < g.Nx - 1; };
+  auto ycut = [&](int y) { return y % SY == 0 && y > 0 && y < g.Ny - 1; };
+  if (plane < nbx) {
+    if (line >= g.Nz || point >= g.Ny) return false;
+    *gx = (plane + 1) * SX;
+    *gy = point;
+    *gz = line;
+    return true;
   }
-  id -= cx;
-  const int Nxu = g.Nx - nbx, Nyu = g.Ny - nby;
-  const long long cy = static_cast<long long>(nby) * Nxu * g.Nz;
-  if (id < cy) {
-    *gx = box_uncut(static_cast<int>(id % Nxu), SX, nbx);
-    const long long r = id / Nxu;
-    *gz = static_cast<int>(r % g.Nz);
-    *gy = static_cast<int>(r / g.Nz + 1) * SY;
-    return;
+  plane -= nbx;
+  if (plane < nby) {
+    if (line >= g.Nz || point >= g.Nx || xcut(point)) return false;
+    *gx = point;
+    *gy = (plane + 1) * SY;
+    *gz = line;
+    return true;
   }
-  id -= cy;
-  *gx = box_uncut(static_cast<int>(id % Nxu), SX, nbx);
-  const long long r = id / Nxu;
-  *gy = box_uncut(static_cast<int>(r % Nyu), SY, nby);
-  *gz = static_cast<int>(r / Nyu + 1) * SZ;
+  plane -= nby;
+  if (line >= g.Ny || point >= g.Nx || ycut(line) || xcut(point)) return false;
+  *gx = point;
+  *gy = line;
+  *gz = (plane + 1) * SZ;
+  return true;
 }
 
 // The segments (ascending) whose node regions hold node (gx, gy, gz), with the
@@ -484,15 +490,13 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   }
 }
 
-// v at every shared node = its partials summed in ascending segment order.
+// v at every shared node = its partials summed in ascending segment order
+// (grid: see box_shared_point; blockDim.x = 64 points of a line).
 template <class T, int P, int BX, int BY>
-__global__ __launch_bounds__(256) void box_shared_sum_kernel(BoxGeom g, int nc, long long count,
-                                                             const T* __restrict__ partials, size_t part_comp_len,
-                                                             T* __restrict__ v) {
-  const long long id = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id >= count) return;
+__global__ __launch_bounds__(64) void box_shared_sum_kernel(BoxGeom g, int nc, const T* __restrict__ partials,
+                                                            size_t part_comp_len, T* __restrict__ v) {
   int gx, gy, gz, seg[8], slot[8];
-  box_shared_node(g, P, BX, BY, id, &gx, &gy, &gz);
+  if (!box_shared_point(g, P, BX, BY, blockIdx.z, blockIdx.y, blockIdx.x * 64 + threadIdx.x, &gx, &gy, &gz)) return;
   const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, slot);
   const int LZn = g.lz * (P - 1) + 1;
   const size_t face = static_cast<size_t>(box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1, LZn));
@@ -503,6 +507,14 @@ __global__ __launch_bounds__(256) void box_shared_sum_kernel(BoxGeom g, int nc, 
     for (int i = 1; i < n; ++i) s += pc[seg[i] * face + slot[i]];
     v[static_cast<size_t>(c) * g.cs + node] = s;
   }
+}
+
+// Grid of the shared-node pass.
+inline dim3 box_shared_grid(const BoxGeom& g) {
+  const int planes = (g.ncx - 1) + (g.ncy - 1) + (g.nsz - 1);
+  const int lines = g.Nz > g.Ny ? g.Nz : g.Ny;
+  const int points = g.Nx > g.Ny ? g.Nx : g.Ny;
+  return dim3(static_cast<unsigned>((points + 63) / 64), static_cast<unsigned>(lines), static_cast<unsigned>(planes));
 }
 
 inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
@@ -539,10 +551,11 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
         g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
         static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
   } else {
-    const long long count = box_shared_count(g, P, BX, BY);
-    if (count == 0) return;
-    box_shared_sum_kernel<T, P, BX, BY><<<static_cast<unsigned>((count + 255) / 256), 256, 0, s>>>(
-        g, NC, count, static_cast<const T*>(a.partials), a.part_comp_len, static_cast<T*>(a.v));
+    const dim3 grid = box_shared_grid(g);
+    if (grid.z == 0) return;
+    require(grid.y < 65536 && grid.z < 65536, "fused mass: box too large for the shared-node pass");
+    box_shared_sum_kernel<T, P, BX, BY><<<grid, 64, 0, s>>>(g, NC, static_cast<const T*>(a.partials), a.part_comp_len,
+                                                           static_cast<T*>(a.v));
   }
   HIP_CHECK_LAUNCH();
 }

@@ -207,8 +207,8 @@ void tune_table_save(const std::string& path) {
   for (const auto& kv : tune_table()) {
     const TuneKey& k = kv.first;
     const TuneEntry& e = kv.second;
-    out << "best " << k.P << ' ' << k.Q << ' ' << k.NC << ' ' << k.fp32 << ' ' << k.det << ' ' << e.alg << ' '
-        << e.elems << ' ' << e.ms << ' ' << e.num_elem << '\n';
+    out << "best " << k.P << ' ' << k.Q << ' ' << k.NC << ' ' << k.fp32 << ' ' << k.det << ' ' << k.box << ' '
+        << e.alg << ' ' << e.elems << ' ' << e.ms << ' ' << e.num_elem << '\n';
   }
   for (const auto& kv : tune_alg_table()) {
     const TuneAlgKey& k = kv.first;
@@ -233,11 +233,30 @@ int tune_table_load(const std::string& path) {
     ls >> kind;
     TuneEntry e;
     if (kind == "best") {
+      std::vector<std::string> tok;
+      for (std::string w; ls >> w;) tok.push_back(w);
+      const bool has_box = tok.size() == 10;
+      require(has_box || tok.size() == 9, "tune_table_load: bad row at line " + std::to_string(lineno));
       TuneKey k;
-      ls >> k.P >> k.Q >> k.NC >> k.fp32 >> k.det >> e.alg >> e.elems >> e.ms >> e.num_elem;
-      require(!ls.fail(), "tune_table_load: bad row at line " + std::to_string(lineno));
+      try {
+        size_t i = 0;
+        auto next = [&]() { return std::stoi(tok[i++]); };
+        k.P = next();
+        k.Q = next();
+        k.NC = next();
+        k.fp32 = next();
+        k.det = next();
+        k.box = has_box ? next() : 0;
+        e.alg = next();
+        e.elems = next();
+        e.ms = std::stod(tok[i++]);
+        e.num_elem = std::stoll(tok[i++]);
+      } catch (const std::exception&) {
+        throw std::invalid_argument("tune_table_load: bad row at line " + std::to_string(lineno));
+      }
       if (!find(FusedKey{k.P, k.Q, k.NC, k.fp32, e.alg, 0, e.elems})) continue;
       if (k.det && !alg_deterministic(e.alg)) continue;
+      if (!k.box && alg_box(e.alg)) continue;
       tune_table()[k] = e;
       ++taken;
     } else if (kind == "alg") {
@@ -623,6 +642,10 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
 
 }  // namespace
 
+bool fused_box_operator(const FusedMass& f) {
+  return f.box && f.basis_symmetric && !fused_elem_variants(f.P, f.Q, f.NC, 0, kAlgBoxColumn, 0).empty();
+}
+
 bool detect_box(const int* off, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3) {
   if (P < 2 || num_elem <= 0 || off[0] != 0) return false;
   const long long Nx = off[P], Nxy = off[P * P];  // the nodes one step up in y and in z
@@ -691,8 +714,9 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
   }
   {
     std::lock_guard<std::mutex> lk(tune_mutex());
-    auto it = tune_table().find(TuneKey{f.P, f.Q, f.NC, fp32, det ? 1 : 0});
-    if (it != tune_table().end()) {
+    for (int box = fused_box_operator(f) ? 1 : 0; box >= 0; --box) {
+      auto it = tune_table().find(TuneKey{f.P, f.Q, f.NC, fp32, det ? 1 : 0, box});
+      if (it == tune_table().end()) continue;
       FusedKey t = k;
       t.alg = it->second.alg;
       t.elems = it->second.elems;
@@ -863,7 +887,8 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
   require(best.alg >= 0, "autotune: no candidate");
   {
     std::lock_guard<std::mutex> lk(tune_mutex());
-    tune_table()[TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0}] = best;
+    tune_table()[TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0, fused_box_operator(f) ? 1 : 0}] =
+        best;
     for (const TuneEntry& t : local) {
       TuneEntry& slot = tune_alg_table()[TuneAlgKey{f.P, f.Q, f.NC, fp32, t.alg}];
       if (slot.alg < 0 || t.ms < slot.ms || slot.num_elem != t.num_elem) slot = t;

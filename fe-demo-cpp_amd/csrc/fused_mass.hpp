@@ -175,6 +175,9 @@ void fused_prepare(FusedMass& f, int fp32, hipStream_t s);
 // with lexicographic node numbering (node stride 1), l_size = NC * node count
 // and (NC > 1) comp_stride = node count; on success n3 = {nx, ny, nz}.
 bool detect_box(const int* offsets, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3);
+// Whether the operator is a structured box with box-column kernels compiled
+// for its shape (its tuning rows are the box = 1 ones).
+bool fused_box_operator(const FusedMass& f);
 // Element layers per segment of the box-column algorithm for nz layers and
 // ncols columns: the largest divisor lz of nz (with lz*(P-1) >= 2) that still
 // leaves at least `target` segments, else the smallest such divisor; 0 if
@@ -186,12 +189,15 @@ int box_layers(int nz, long long ncols, long long target, int P);
 // Jacobi diagonal: diag = E^T [ (B o B)^T x3 qdata ] (FP64).
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);
 
-// Autotuning table: the fastest (alg, elems) per (P, Q, NC, fp32, deterministic),
-// measured by fused_autotune and consulted by automatic choice.
+// Autotuning table: the fastest (alg, elems) per (P, Q, NC, fp32,
+// deterministic, box), measured by fused_autotune and consulted by automatic
+// choice; box = 1 rows are for structured boxes (which can also run the
+// box-column algorithm), box = 0 rows for any node numbering. A box operator
+// without a box row uses the box = 0 row.
 struct TuneKey {
-  int P, Q, NC, fp32, det;
+  int P, Q, NC, fp32, det, box;
   bool operator<(const TuneKey& o) const {
-    return std::tie(P, Q, NC, fp32, det) < std::tie(o.P, o.Q, o.NC, o.fp32, o.det);
+    return std::tie(P, Q, NC, fp32, det, box) < std::tie(o.P, o.Q, o.NC, o.fp32, o.det, o.box);
   }
 };
 struct TuneEntry {
@@ -211,8 +217,9 @@ struct TuneAlgKey {
 };
 std::map<TuneAlgKey, TuneEntry>& tune_alg_table();  // guarded by tune_mutex()
 // Text form of both tables: one row per line,
-//   best P Q NC fp32 det alg elems ms num_elem
+//   best P Q NC fp32 det box alg elems ms num_elem
 //   alg  P Q NC fp32 alg elems ms num_elem
+// (a best row without the box field reads as box = 0)
 // load replaces rows with the same key, skips rows not compiled in this
 // library, and returns the rows taken.
 void tune_table_save(const std::string& path);

@@ -427,7 +427,7 @@ class MassServer {
     std::lock_guard<std::mutex> lk(tune_mutex());
     for (int fp32 : {0, 1}) {
       if (fp32 && !mixed) continue;
-      if (!tune_table().count(TuneKey{f.P, f.Q, f.NC, fp32, 1})) return false;
+      if (!tune_table().count(TuneKey{f.P, f.Q, f.NC, fp32, 1, fused_box_operator(f) ? 1 : 0})) return false;
     }
     return true;
   }
@@ -436,7 +436,7 @@ class MassServer {
   void maybe_tune(FusedMass& f, bool mixed, hipStream_t s) {
     for (int fp32 : {0, 1}) {
       if (fp32 && !mixed) continue;
-      const TuneKey k{f.P, f.Q, f.NC, fp32, 1};
+      const TuneKey k{f.P, f.Q, f.NC, fp32, 1, fused_box_operator(f) ? 1 : 0};
       {
         std::unique_lock<std::mutex> lk(mu_);
         if (tuned_.count(k)) continue;

@@ -409,19 +409,20 @@ def box_column_check(nx, ny, nz, P, elems, layers=0):
 
 
 def tune_table():
-    """Rows (P, Q, num_comp, fp32, deterministic, alg, elems, ms, num_elem) of the process-wide tuning table."""
+    """Rows (P, Q, num_comp, fp32, deterministic, box, alg, elems, ms, num_elem) of the
+    process-wide tuning table (box = 1: rows for structured boxes)."""
     n = ctypes.c_int()
     call("fem_hip_tune_table_get", None, None, None, 0, ctypes.byref(n))
     k = n.value
-    rows = (ctypes.c_int * (7 * max(k, 1)))()
+    rows = (ctypes.c_int * (8 * max(k, 1)))()
     ms = (_d * max(k, 1))()
     ne = (ctypes.c_longlong * max(k, 1))()
     call("fem_hip_tune_table_get", rows, ms, ne, k, ctypes.byref(n))
-    return [tuple(rows[7 * i:7 * i + 7]) + (float(ms[i]), int(ne[i])) for i in range(k)]
+    return [tuple(rows[8 * i:8 * i + 8]) + (float(ms[i]), int(ne[i])) for i in range(k)]
 
 
-def tune_table_set(P, Q, num_comp, fp32, deterministic, alg, elems, ms=0.0, num_elem=0):
-    row = (ctypes.c_int * 7)(P, Q, num_comp, int(fp32), int(deterministic), alg, elems)
+def tune_table_set(P, Q, num_comp, fp32, deterministic, alg, elems, ms=0.0, num_elem=0, box=0):
+    row = (ctypes.c_int * 8)(P, Q, num_comp, int(fp32), int(deterministic), int(box), alg, elems)
     call("fem_hip_tune_table_set", row, float(ms), int(num_elem))
 
 

@@ -4,12 +4,15 @@
 For each (P, Q, num_comp) with a compiled fused kernel, a jittered structured
 hex mesh of about 24 million element nodes (at most 96^3 hexes) is built, so
 the vectors are far larger than the caches, and the autotuner times every
-compiled algorithm and packing: FP64 with the atomic scatter allowed (all four
-algorithms; the fastest deterministic one of them is recorded as the FP64
+compiled algorithm and packing: FP64 with the atomic scatter allowed (every
+algorithm; the fastest deterministic one is recorded as the FP64
 deterministic choice) and FP32 with the deterministic scatter (the mixed
-solve's inner apply). The fastest choices go to the tuning table,
-written as text (default: mi355x_gfx950.txt beside this script), which
-fehip.load_default_tuning() and fem_hip_tune_table_load() read.
+solve's inner apply). Each choice is recorded twice: for structured boxes
+(box = 1, the box-column algorithm included where it is compiled) and for any
+node numbering (box = 0, the best algorithm other than the box columns, as
+timed on the box). The table is written as text (default: mi355x_gfx950.txt
+beside this script), which fehip.load_default_tuning() and
+fem_hip_tune_table_load() read.
 
 Usage: python fe-demo-cpp_amd/tuning/tune_all.py [--out PATH] [--reps N]
 """
@@ -26,12 +29,15 @@ from fehip.problems import MassProblem  # noqa: E402
 
 SHAPES = [(2, 3, (1, 2, 3)), (3, 4, (1, 2, 3)), (4, 5, (1, 2, 3)), (5, 6, (1, 2, 3)), (6, 7, (1, 2, 3)),
           (8, 10, (1, 3))]
+DET = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BOX_COLUMN)
 
 
 def mesh_size(P, target=24_000_000, n_max=96):
-    n = 2
-    while n < n_max and (n + 1) ** 3 * P ** 3 <= target:
-        n += 1
+    """Hexes per side: about `target` element nodes, a multiple of 4 (so the
+    box-column cross-sections tile the box)."""
+    n = 4
+    while n < n_max and (n + 4) ** 3 * P ** 3 <= target:
+        n += 4
     return n
 
 
@@ -49,14 +55,22 @@ def main():
             p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
             p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
             (alg, ne, ms), rows = p.op.autotune(fp32=False, reps=args.reps)
-            det = min((r for r in rows if r[0] in (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL)),
-                      key=lambda r: r[2])
-            fehip.tune_table_set(P, Q, nc, 0, 1, det[0], det[1], det[2], n ** 3)
             p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC, exact=0)
-            (alg32, ne32, ms32), _ = p.op.autotune(fp32=True, reps=args.reps)
-            print("P=%d Q=%d nc=%d n=%d  fp64: %s/%d %.4f ms (det %s/%d %.4f ms)  fp32 det: %s/%d %.4f ms" % (
-                P, Q, nc, n, fehip.FUSED_NAMES[alg], ne, ms, fehip.FUSED_NAMES[det[0]], det[1], det[2],
-                fehip.FUSED_NAMES[alg32], ne32, ms32), flush=True)
+            _, rows32 = p.op.autotune(fp32=True, reps=args.reps)
+            picks = {}
+            for fp32, det, cands in ((0, 0, rows), (0, 1, rows), (1, 1, rows32)):
+                if det:
+                    cands = [r for r in cands if r[0] in DET]
+                for box in (1, 0):
+                    pool = cands if box else [r for r in cands if r[0] != fehip.FUSED_BOX_COLUMN]
+                    if not pool or (box and not any(r[0] == fehip.FUSED_BOX_COLUMN for r in cands)):
+                        continue
+                    a, e, t = min(pool, key=lambda r: r[2])
+                    fehip.tune_table_set(P, Q, nc, fp32, det, a, e, t, n ** 3, box=box)
+                    picks[(fp32, det, box)] = "%s/%d %.4f ms" % (fehip.FUSED_NAMES[a], e, t)
+            print("P=%d Q=%d nc=%d n=%d  %s" % (P, Q, nc, n, "  ".join(
+                "%s%s%s: %s" % ("fp32" if k[0] else "fp64", " det" if k[1] else "", " box" if k[2] else "", v)
+                for k, v in sorted(picks.items()))), flush=True)
             del p
             print("  (%.1f s)" % (time.perf_counter() - t0), flush=True)
     fehip.tune_table_save(args.out)

@@ -239,18 +239,20 @@ int fem_hip_operator_get_fused_choice(fem_hip_operator op, int fp32, int* alg, i
 /* Autotuner: times every compiled algorithm/packing valid for the operator's
  * scatter mode in one precision on the operator's own data (reps applies each,
  * after a warm-up; the operator's output is not touched), records the fastest
- * for (P, Q, num_comp, fp32, deterministic) in the process-wide tuning table,
+ * for (P, Q, num_comp, fp32, deterministic, box) in the process-wide tuning table,
  * which automatic choice then follows, and returns it. results (may be NULL)
  * receives up to max_results rows {alg, elems, ms_per_apply} of every candidate;
  * *n_results is set to the number of candidates. Synchronous. */
 int fem_hip_operator_autotune(fem_hip_operator op, int fp32, int reps, void* stream, int* best_alg, int* best_elems,
                               double* best_ms, double* results, int max_results, int* n_results);
-/* Tuning table rows {P, Q, num_comp, fp32, deterministic, alg, elems} with the
- * time per apply and the element count it was measured on. get: copies up to
- * max_rows rows and sets *n_rows to the table size; set: adds or replaces one
- * row (alg must be compiled for the shape); clear: empties the table. */
-int fem_hip_tune_table_get(int* rows7, double* ms, long long* num_elem, int max_rows, int* n_rows);
-int fem_hip_tune_table_set(const int* row7, double ms, long long num_elem);
+/* Tuning table rows {P, Q, num_comp, fp32, deterministic, box, alg, elems} with
+ * the time per apply and the element count it was measured on; box = 1 rows
+ * serve operators on structured boxes (fem_hip_box_detect), box = 0 rows any
+ * operator (a box operator without a box row uses the box = 0 row). get:
+ * copies up to max_rows rows and sets *n_rows to the table size; set: adds or
+ * replaces one row (alg must be compiled for the shape); clear: empties it. */
+int fem_hip_tune_table_get(int* rows8, double* ms, long long* num_elem, int max_rows, int* n_rows);
+int fem_hip_tune_table_set(const int* row8, double ms, long long num_elem);
 int fem_hip_tune_table_clear(void);
 /* Save / load the tuning tables as text (the best algorithm per shape and
  * determinism, and the fastest packing per algorithm). Load replaces rows with

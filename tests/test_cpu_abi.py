@@ -177,18 +177,24 @@ def test_tune_table_save_load_roundtrip(tmp_path):
     fehip.tune_table_clear()
     try:
         fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BLOCK_PARTIAL, 14, 0.39, 262144)
+        fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BOX_COLUMN, 16, 0.26, 262144, box=1)
         fehip.tune_table_set(3, 4, 2, 1, 0, fehip.FUSED_ELEM_ATOMIC, 16, 0.05, 4096)
+        with pytest.raises(fehip.InvalidArgument):  # box columns only in a box row
+            fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BOX_COLUMN, 16, 0.26, 262144, box=0)
         path = str(tmp_path / "t.txt")
         fehip.tune_table_save(path)
         rows = sorted(fehip.tune_table())
         fehip.tune_table_clear()
         assert fehip.tune_table() == []
         with open(path, "a") as f:
-            f.write("# comment\nbest 7 7 1 0 0 0 8 0.1 10\n")  # no (7, 7) kernel: skipped
-        assert fehip.tune_table_load(path) == 2
+            f.write("# comment\nbest 7 7 1 0 0 0 0 8 0.1 10\n")  # no (7, 7) kernel: skipped
+            f.write("best 5 6 1 1 1 1 7 0.2 512\n")  # a row without the box field: box = 0
+        assert fehip.tune_table_load(path) == 4
         got = sorted(fehip.tune_table())
-        assert [r[:7] for r in got] == [r[:7] for r in rows]
-        np.testing.assert_allclose([r[7] for r in got], [r[7] for r in rows], rtol=1e-5)
+        assert (5, 6, 1, 1, 1, 0, fehip.FUSED_ELEM_TMAP, 7) in [r[:8] for r in got]
+        got = [r for r in got if r[:4] != (5, 6, 1, 1)]
+        assert [r[:8] for r in got] == [r[:8] for r in rows]
+        np.testing.assert_allclose([r[8] for r in got], [r[8] for r in rows], rtol=1e-5)
         with open(path, "a") as f:
             f.write("bogus 1 2 3\n")
         with pytest.raises(fehip.InvalidArgument):

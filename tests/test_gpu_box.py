@@ -80,7 +80,7 @@ def test_box_column_is_the_default_on_boxes(gpu):
     a layer count that does not divide nz makes it fall back."""
     mesh = rect_mesh(8, 8, 8, 5)
     p = MeshProblem(mesh, 6, 1, scatter=fehip.SCATTER_DETERMINISTIC)
-    if any(r[:5] == (5, 6, 1, 0, 1) for r in fehip.tune_table()):
+    if any(r[:5] == (5, 6, 1, 0, 1) for r in fehip.tune_table()):  # any (P, Q, nc, FP64, det) row
         pytest.skip("a tuning row for this shape is loaded")
     assert p.op.fused_choice()[0] == BOX
     p.op.set_box_layers(3)

@@ -225,7 +225,8 @@ def main():
             kernel_names[label] = "fused_mass_pipe_kernel<%d, %d, 1, %s, %d>" % (P, Q, tname, ne)
         elif alg == fehip.FUSED_BOX_COLUMN:
             bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne]
-            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d>" % (P, Q, tname, bx, by)
+            # the last template argument is how u reaches the x-rows: rows (FP64), LDS ring (FP32)
+            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d, %d>" % (P, Q, tname, bx, by, 2 if fp32 else 0)
         else:
             kernel_names[label] = {
                 fehip.FUSED_BLOCK_PARTIAL: "fused_mass_part_kernel<%d, %d, 1, %s, %d, false>",

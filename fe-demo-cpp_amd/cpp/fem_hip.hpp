@@ -329,14 +329,22 @@ inline void operator_apply(Operator& op, const DeviceArray<float>& in, DeviceArr
   FEM_HIP_CHECK(fem_hip_operator_apply_f32(op.handle.get(), in.data(), in.size(), out.data(), out.size(), stream));
 }
 
-// Mass solve M x = b on a fused mass operator (FP64 PCG or FP32-inner / FP64-refined).
-inline fem_hip_solve_stats mass_solve(Operator& op, const DeviceArray<double>& b, DeviceArray<double>& x,
-                                      bool mixed, double rtol = 1e-10, void* stream = nullptr) {
+// Mass solve M x = b on a fused mass operator: mode FEM_HIP_SOLVE_FP64 (PCG),
+// _MIXED (FP32 inner PCG, FP64 refinement) or _MIXED_RELIABLE (FP32 PCG with
+// FP64 residual replacement).
+// inner_rtol <= 0: 1e-4 per inner solve (MIXED), 1e-2 between replacements (MIXED_RELIABLE).
+inline fem_hip_solve_stats mass_solve(Operator& op, const DeviceArray<double>& b, DeviceArray<double>& x, int mode,
+                                      double rtol = 1e-10, void* stream = nullptr, double inner_rtol = 0.0) {
+  if (inner_rtol <= 0.0) inner_rtol = mode == FEM_HIP_SOLVE_MIXED_RELIABLE ? 1e-2 : 1e-4;
   x.resize(b.size());
-  fem_hip_solve_options o{mixed ? FEM_HIP_SOLVE_MIXED : FEM_HIP_SOLVE_FP64, 2000, 30, rtol, 1e-4};
+  fem_hip_solve_options o{mode, 2000, 30, rtol, inner_rtol};
   fem_hip_solve_stats st{};
   FEM_HIP_CHECK(fem_hip_mass_solve(op.handle.get(), b.data(), x.data(), &o, &st, stream));
   return st;
+}
+inline fem_hip_solve_stats mass_solve(Operator& op, const DeviceArray<double>& b, DeviceArray<double>& x,
+                                      bool mixed, double rtol = 1e-10, void* stream = nullptr) {
+  return mass_solve(op, b, x, mixed ? FEM_HIP_SOLVE_MIXED : FEM_HIP_SOLVE_FP64, rtol, stream);
 }
 
 // Jacobi diagonal of a fused mass operator (deterministic).

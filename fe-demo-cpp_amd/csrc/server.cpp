@@ -92,7 +92,7 @@ void validate(const ServeRequest& r) {
   require(r.n >= 1 && r.n <= 512, "serve: n must be in [1, 512]");
   require(r.P >= 2 && r.Q >= 1 && r.num_comp >= 1 && r.num_comp <= 3, "serve: bad P, Q or num_comp");
   require(fused_mass_supported(r.P, r.Q, r.num_comp), "serve: (P, Q, num_comp) has no compiled fused kernel");
-  require(r.mode == 0 || r.mode == 1, "serve: mode must be 0 (FP64) or 1 (mixed)");
+  require(r.mode >= 0 && r.mode <= 2, "serve: mode must be 0 (FP64), 1 (mixed) or 2 (mixed, reliable updates)");
   require(r.rtol > 0 && r.jitter >= 0 && r.jitter < 0.5, "serve: bad rtol or jitter");
   require(ipow3(static_cast<long long>(r.n) * (r.P - 1) + 1) * r.num_comp < (1ll << 31), "serve: L-vector too large");
 }
@@ -353,8 +353,8 @@ class MassServer {
         j->b.upload(rhs);
         j->x.allocate(L);
         FusedMass& f = operator_fused_mass(*j->op, j->stream);
-        if (opt_.autotune) maybe_tune(f, j->req.mode == 1, j->stream);
-        if (opt_.cheap_setup || !tuned_shape(f, j->req.mode == 1)) {
+        if (opt_.autotune) maybe_tune(f, j->req.mode != 0, j->stream);
+        if (opt_.cheap_setup || !tuned_shape(f, j->req.mode != 0)) {
           // The transpose-map algorithm (at its tuned packing, if any): its setup is
           // a counting sort, where the block algorithms cluster elements on the host.
           j->op->opt_alg = kAlgElemTmap;

@@ -5,6 +5,15 @@
 //          fused apply, rounds r to FP32, runs Jacobi-PCG with the FP32 fused
 //          apply (FP32 vectors, FP64 dot-product accumulation) to a relative
 //          reduction inner_rtol, and adds the correction to x in FP64.
+//   MIXED_RELIABLE: one FP32 Jacobi-PCG recursion with reliable updates
+//          (residual replacement, van der Vorst & Ye, SIAM J. Sci. Comput. 22(3),
+//          2000; the mixed-precision form of Clark et al., Comput. Phys. Commun.
+//          181, 2010): the FP32 iterate is an increment d; whenever the
+//          recursive residual has dropped by inner_rtol since the last update,
+//          x += d in FP64, r = b - M x with the FP64 apply replaces the
+//          recursive residual, d = 0, and the recursion goes on with the same
+//          search direction p. Without restarts the Krylov space is kept, so
+//          the iteration count stays near the FP64 one.
 //
 // All CG scalars live on the device (no host round trip per iteration); a
 // batch of iterations is captured once into a hipGraph and replayed, and the
@@ -42,7 +51,10 @@ int red_blocks(size_t n) {
 }
 
 // Scalar slots.
-enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kNSCAL };
+// kPFROZEN: set by the iteration after the one that met the residual test, so
+// that iteration still forms the next search direction (a residual replacement
+// continues from it) and later ones leave p alone.
+enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kPFROZEN, kNSCAL };
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -112,6 +124,48 @@ __global__ __launch_bounds__(kThreads) void cg_init_scalars_kernel(const double*
   sc[kTHRESH] = rtol2 * rr;
   sc[kDONE] = (rr <= sc[kTHRESH]) ? 1.0 : 0.0;
   sc[kITERS] = 0.0;
+  sc[kPFROZEN] = sc[kDONE];
+}
+
+// Residual replacement: r = rnew, z = dinv * r, x = 0 (the increment restarts);
+// p is kept. Partials of r.z and r.r.
+template <class T>
+__global__ __launch_bounds__(kThreads) void cg_replace_kernel(size_t n, const T* __restrict__ rnew,
+                                                              const T* __restrict__ dinv, T* __restrict__ x,
+                                                              T* __restrict__ r, T* __restrict__ z,
+                                                              double* __restrict__ part) {
+  __shared__ double sh[2][kThreads / 64];
+  double srz = 0.0, srr = 0.0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    const T ri = rnew[i];
+    const T zi = dinv[i] * ri;
+    x[i] = T(0);
+    r[i] = ri;
+    z[i] = zi;
+    srz += static_cast<double>(ri) * static_cast<double>(zi);
+    srr += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  const double a = block_sum(srz, sh[0]);
+  const double c = block_sum(srr, sh[1]);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kMaxRed + blockIdx.x] = c;
+  }
+}
+
+// After a replacement: new r.z and r.r, the next segment's threshold; the
+// iteration count runs on.
+__global__ __launch_bounds__(kThreads) void cg_replace_scalars_kernel(const double* __restrict__ part,
+                                                                      double* __restrict__ sc, double rtol2, int nb) {
+  __shared__ double sh[2][kThreads / 64];
+  const double rz = sum_partials(part, nb, sh[0]), rr = sum_partials(part + kMaxRed, nb, sh[1]);
+  if (threadIdx.x != 0) return;
+  sc[kRZ] = rz;
+  sc[kRR] = rr;
+  sc[kTHRESH] = rtol2 * rr;
+  sc[kDONE] = (rr <= sc[kTHRESH]) ? 1.0 : 0.0;
+  sc[kPFROZEN] = sc[kDONE];
 }
 
 __global__ __launch_bounds__(kThreads) void cg_alpha_kernel(const double* __restrict__ part, double* __restrict__ sc,
@@ -121,6 +175,7 @@ __global__ __launch_bounds__(kThreads) void cg_alpha_kernel(const double* __rest
   if (threadIdx.x != 0) return;
   sc[kPAP] = pap;
   sc[kALPHA] = (sc[kDONE] != 0.0 || pap == 0.0) ? 0.0 : sc[kRZ] / pap;
+  if (sc[kDONE] != 0.0) sc[kPFROZEN] = 1.0;
 }
 
 template <class T>
@@ -173,7 +228,7 @@ __global__ __launch_bounds__(kThreads) void cg_beta_kernel(const double* __restr
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, const double* __restrict__ sc,
                                                         const T* __restrict__ z, T* __restrict__ p) {
-  if (sc[kDONE] != 0.0) return;
+  if (sc[kPFROZEN] != 0.0) return;
   const T beta = static_cast<T>(sc[kBETA]);
   for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * kThreads)
@@ -255,6 +310,14 @@ struct Pcg {
     HIP_CHECK_LAUNCH();
   }
 
+  // Residual replacement: r = rnew, x = 0, p kept; the next segment ends when
+  // ||r|| <= rtol ||rnew||.
+  void replace(const T* rnew, T* x, double rtol, hipStream_t s) {
+    cg_replace_kernel<T><<<nb, kThreads, 0, s>>>(n, rnew, dinv.data(), x, r.data(), z.data(), part.data());
+    cg_replace_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb);
+    HIP_CHECK_LAUNCH();
+  }
+
   // Records the graph of `batch_iters` iterations for this b and x (if it is not
   // recorded yet). No other thread does device setup or teardown meanwhile.
   template <class Apply>
@@ -311,7 +374,7 @@ enum { kHostB = kNSCAL, kHostRes, kHostN };
 }  // namespace
 
 struct SolveJob {
-  enum Phase { kStart, kIter, kFinal, kCheck, kInner, kDone };
+  enum Phase { kStart, kIter, kFinal, kCheck, kInner, kRelCheck, kRelIter, kDone };
   Operator* op = nullptr;
   FusedMass* f = nullptr;
   MassSolverState* st = nullptr;
@@ -324,6 +387,7 @@ struct SolveJob {
   Phase phase = kStart;
   bool in_flight = false;
   int batch = 8, outer = 0, inner_total = 0;
+  bool started = false;  // MIXED_RELIABLE: the FP32 recursion has begun
   double bnorm = 0.0;
   SolveStats stats;
   std::chrono::steady_clock::time_point t0;
@@ -381,7 +445,39 @@ void advance(SolveJob& j) {
       } else {
         zero_fill(j.x, st.n * sizeof(double), j.s);
         enqueue_residual(j, st.r32.data());
-        j.phase = SolveJob::kCheck;
+        j.phase = j.opt.mode == 1 ? SolveJob::kCheck : SolveJob::kRelCheck;
+      }
+      break;
+    case SolveJob::kRelCheck: {  // reliable updates: the FP64 residual of x is in hs, its FP32 copy in r32
+      j.bnorm = std::sqrt(j.hs[kHostB]);
+      const double rel = rel_of(j);
+      const int iters = j.started ? static_cast<int>(j.hs[kITERS]) : 0;
+      if (rel <= j.opt.rtol || iters >= j.opt.max_iter || j.outer >= j.opt.max_outer) {
+        j.stats.rel_residual = rel;
+        j.stats.iterations = iters;
+        j.stats.outer = j.outer;
+        j.phase = SolveJob::kDone;
+      } else {
+        if (!j.started) {
+          st.pcg32->init(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
+          j.started = true;
+        } else {
+          st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
+        }
+        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
+        j.phase = SolveJob::kRelIter;
+      }
+      break;
+    }
+    case SolveJob::kRelIter:
+      if (j.hs[kDONE] != 0.0) {
+        add_correction_kernel<<<grid_for(st.n, kThreads), kThreads, 0, j.s>>>(st.n, st.d32.data(), j.x);
+        HIP_CHECK_LAUNCH();
+        ++j.outer;
+        enqueue_residual(j, st.r32.data());
+        j.phase = SolveJob::kRelCheck;
+      } else {
+        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
       }
       break;
     case SolveJob::kIter:  // FP64 PCG batches
@@ -441,7 +537,7 @@ void advance(SolveJob& j) {
 SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* b, double* x, const SolveOptions& opt,
                            hipStream_t s) {
   require(s != nullptr, "solve job: needs a non-default stream (graph capture)");
-  require(opt.mode == 0 || opt.mode == 1, "mass_solve: unknown mode");
+  require(opt.mode >= 0 && opt.mode <= 2, "mass_solve: unknown mode");
   require(opt.rtol > 0 && opt.max_iter > 0, "mass_solve: rtol and max_iter must be positive");
   require(b != nullptr && x != nullptr, "mass_solve: b and x must be device arrays");
   FusedMass& f = operator_fused_mass(op, s);
@@ -467,7 +563,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
     recip_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg64->dinv.data());
     HIP_CHECK_LAUNCH();
   }
-  if (opt.mode == 1 && !st.pcg32) {
+  if (opt.mode != 0 && !st.pcg32) {
     st.pcg32 = std::make_unique<Pcg<float>>(n);
     st.r32.allocate(n);
     st.d32.allocate(n);
@@ -475,7 +571,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
     HIP_CHECK_LAUNCH();
   }
   fused_mass_apply64(f, x, st.Mx.data(), s);
-  if (opt.mode == 1) fused_mass_apply32(f, st.d32.data(), st.r32.data(), s);
+  if (opt.mode != 0) fused_mass_apply32(f, st.d32.data(), st.r32.data(), s);
   auto j = std::make_unique<SolveJob>();
   j->op = &op;
   j->f = &f;

@@ -5,7 +5,7 @@
 namespace femhip {
 
 struct SolveOptions {
-  int mode = 0;  // 0 FP64 PCG, 1 mixed (FP32 inner PCG + FP64 refinement)
+  int mode = 0;  // 0 FP64 PCG, 1 mixed (FP32 inner PCG + FP64 refinement), 2 mixed with reliable updates
   int max_iter = 1000;
   int max_outer = 20;
   double rtol = 1e-10;

@@ -26,7 +26,7 @@ FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_pa
                5: "block_partial_lds", 6: "block_partial_lds_pipe", 7: "box_column"}
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
 QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR, QF_BUILD_DIFFUSION, QF_APPLY_DIFFUSION = 1, 2, 3, 4, 5
-SOLVE_FP64, SOLVE_MIXED = 0, 1
+SOLVE_FP64, SOLVE_MIXED, SOLVE_MIXED_RELIABLE = 0, 1, 2
 
 
 ERR_NOT_READY = 6
@@ -675,8 +675,12 @@ class Operator:
     def mass_diagonal(self, d_diag, stream=None):
         call("fem_hip_operator_mass_diagonal", self.h, d_diag.ptr, d_diag.size, _s(stream))
 
-    def solve(self, d_b, d_x, mode=SOLVE_FP64, rtol=1e-10, max_iter=2000, max_outer=30, inner_rtol=1e-4,
+    def solve(self, d_b, d_x, mode=SOLVE_FP64, rtol=1e-10, max_iter=2000, max_outer=30, inner_rtol=None,
               stream=None):
+        """M x = b. inner_rtol: reduction of each inner FP32 solve (SOLVE_MIXED, default 1e-4)
+        or between residual replacements (SOLVE_MIXED_RELIABLE, default 1e-2)."""
+        if inner_rtol is None:
+            inner_rtol = 1e-2 if mode == SOLVE_MIXED_RELIABLE else 1e-4
         opt = SolveOptions(mode, max_iter, max_outer, rtol, inner_rtol)
         st = SolveStats()
         call("fem_hip_mass_solve", self.h, d_b.ptr, d_x.ptr, ctypes.byref(opt), ctypes.byref(st), _s(stream))

@@ -322,12 +322,15 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
 /* ---- mass solve ------------------------------------------------------------ */
 #define FEM_HIP_SOLVE_FP64 0  /* Jacobi-PCG, FP64 throughout */
 #define FEM_HIP_SOLVE_MIXED 1 /* FP32 inner Jacobi-PCG, FP64 iterative refinement */
+#define FEM_HIP_SOLVE_MIXED_RELIABLE 2 /* one FP32 Jacobi-PCG recursion whose residual is replaced by the FP64
+                                          b - M x whenever it has dropped by inner_rtol (x accumulated in FP64) */
 typedef struct {
   int mode;          /* FEM_HIP_SOLVE_* */
   int max_iter;      /* CG iterations (FP64) / per inner solve (mixed) */
-  int max_outer;     /* refinement steps (mixed) */
+  int max_outer;     /* refinement steps (mixed) / residual replacements (mixed reliable) */
   double rtol;       /* target ||b - M x|| / ||b|| (FP64 residual) */
-  double inner_rtol; /* inner FP32 solve reduction per refinement step */
+  double inner_rtol; /* inner FP32 solve reduction per refinement step (1e-4 works well) / residual drop
+                        between replacements (1e-2 keeps the FP64 iteration count) */
 } fem_hip_solve_options;
 typedef struct {
   int iterations;    /* total CG iterations (inner ones for mixed) */
@@ -376,7 +379,7 @@ typedef struct {
 } fem_hip_server_options;
 typedef struct {
   int n, P, Q, num_comp; /* n^3 hexes of the unit cube, order P, Q Gauss points */
-  int mode;              /* FEM_HIP_SOLVE_FP64 or FEM_HIP_SOLVE_MIXED */
+  int mode;              /* FEM_HIP_SOLVE_FP64, _MIXED or _MIXED_RELIABLE */
   double jitter, rtol;   /* vertex jitter (fraction of h), relative residual target */
   uint64_t mesh_seed, rhs_seed; /* b ~ U[-1,1] from rhs_seed (splitmix64) */
 } fem_hip_serve_request;

@@ -371,8 +371,8 @@ TEST_CASE(mass_diagonal_autotune_and_solve) {
   std::vector<double> xs = wiggly(mm.rstr.l_size, 0.7), b;
   fem::operator_apply(mm.host, xs, b);
   DeviceArray<double> d_b(b), d_x;
-  for (bool mixed : {false, true}) {
-    const fem_hip_solve_stats st = fem::hip::mass_solve(dmass, d_b, d_x, mixed, 1e-11);
+  for (int mode : {FEM_HIP_SOLVE_FP64, FEM_HIP_SOLVE_MIXED, FEM_HIP_SOLVE_MIXED_RELIABLE}) {
+    const fem_hip_solve_stats st = fem::hip::mass_solve(dmass, d_b, d_x, mode, 1e-11);
     REQUIRE(st.rel_residual <= 1e-11);
     const std::vector<double> x = d_x.to_host();
     double err = 0.0, nrm = 0.0;

@@ -1,6 +1,7 @@
-"""Mass solves: FP64 Jacobi-PCG and FP32-inner / FP64-refined PCG on the fused
-operator, checked against the oracle's FP64 CG on the same operator (the
-reference has no solver; the oracle CG runs on the restated operator_apply)."""
+"""Mass solves: FP64 Jacobi-PCG, FP32-inner / FP64-refined PCG and FP32 PCG with
+FP64 reliable updates on the fused operator, checked against the oracle's FP64
+CG on the same operator (the reference has no solver; the oracle CG runs on the
+restated operator_apply)."""
 import numpy as np
 import pytest
 
@@ -30,10 +31,20 @@ def test_c1_solves(gpu, scatter):
     s_mx = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED, rtol=1e-10, inner_rtol=1e-4)
     xm = d_x.to_host()
     assert s_mx["rel_residual"] <= 1e-10, s_mx
-    # Both agree with the oracle's solution and the manufactured x* to solver accuracy.
+    # Every mode agrees with the oracle's solution and the manufactured x* to solver accuracy.
     assert rel_err(x64, x_cpu) < 1e-8
     assert rel_err(xm, x_cpu) < 1e-8
     assert rel_err(xm, x_star) < 1e-8
+    for inner in (1e-2, 1e-3, 1e-4):
+        s_rel = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED_RELIABLE, rtol=1e-10, inner_rtol=inner)
+        xr = d_x.to_host()
+        assert s_rel["rel_residual"] <= 1e-10, (inner, s_rel)
+        assert rel_err(xr, x_cpu) < 1e-8 and rel_err(xr, x_star) < 1e-8, inner
+        # No restarts: with a replacement every 100x drop (the default) the
+        # iteration count stays close to the FP64 one.
+        if inner == 1e-2:
+            assert s_rel["iterations"] <= s64["iterations"] + 4, (inner, s_rel, s64)
+        print("reliable", inner, s_rel, flush=True)
     print("fp64", s64, "mixed", s_mx, "cpu iters", it_cpu, flush=True)
 
 
@@ -42,7 +53,8 @@ def test_solve_repeatable(gpu):
     b = fehip.rng_uniform(5, p.l_size)
     d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
     p.op.set_fused_options(fehip.SCATTER_DETERMINISTIC)
-    p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED)
-    x1 = d_x.to_host()
-    p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED)
-    assert d_x.to_host().tobytes() == x1.tobytes()
+    for mode in (fehip.SOLVE_MIXED, fehip.SOLVE_MIXED_RELIABLE):
+        p.op.solve(d_b, d_x, mode=mode)
+        x1 = d_x.to_host()
+        p.op.solve(d_b, d_x, mode=mode)
+        assert d_x.to_host().tobytes() == x1.tobytes(), mode

@@ -29,8 +29,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <chrono>
 #include <cmath>
+#include <map>
+#include <vector>
 
 #include "solver.hpp"
 
@@ -85,6 +88,44 @@ __device__ double sum_partials(const double* part, int nb, double* sh) {
   double s = 0.0;
   for (int i = threadIdx.x; i < nb; i += kThreads) s += part[i];
   return block_sum(s, sh);
+}
+
+// 16-byte packs of T (double2 / float4 loads and stores) for the per-iteration
+// kernels; the last n % W entries are handled by block 0.
+template <class T>
+struct alignas(16) Pack {
+  static constexpr int W = 16 / sizeof(T);
+  T v[W];
+};
+template <class T>
+__device__ __forceinline__ Pack<T> ldp(const T* a, size_t iv) {
+  return reinterpret_cast<const Pack<T>*>(a)[iv];
+}
+template <class T>
+__device__ __forceinline__ void stp(T* a, size_t iv, const Pack<T>& x) {
+  reinterpret_cast<Pack<T>*>(a)[iv] = x;
+}
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void dot_vec_kernel(size_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                           double* __restrict__ part) {
+  constexpr int W = Pack<T>::W;
+  __shared__ double sh[kThreads / 64];
+  double s = 0.0;
+  const size_t nv = n / W;
+  for (size_t iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; iv < nv;
+       iv += static_cast<size_t>(gridDim.x) * kThreads) {
+    const Pack<T> x = ldp(a, iv), y = ldp(b, iv);
+#pragma unroll
+    for (int k = 0; k < W; ++k) s += static_cast<double>(x.v[k]) * static_cast<double>(y.v[k]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+    const size_t i = nv * W + threadIdx.x;
+    s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // r = b, z = dinv * r, p = z; partials of r.z and r.r
@@ -225,6 +266,83 @@ __global__ __launch_bounds__(kThreads) void cg_beta_kernel(const double* __restr
   if (rr <= sc[kTHRESH] || sc[kITERS] >= max_iter) sc[kDONE] = 1.0;
 }
 
+// The update kernel on 16-byte packs (same arithmetic per entry).
+template <class T>
+__global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, const double* __restrict__ sc,
+                                                                 const T* __restrict__ p, const T* __restrict__ Ap,
+                                                                 const T* __restrict__ dinv, T* __restrict__ x,
+                                                                 T* __restrict__ r, T* __restrict__ z,
+                                                                 double* __restrict__ part) {
+  constexpr int W = Pack<T>::W;
+  __shared__ double sh[2][kThreads / 64];
+  const bool done = sc[kDONE] != 0.0;
+  const T alpha = static_cast<T>(sc[kALPHA]);
+  double srz = 0.0, srr = 0.0;
+  auto one = [&](T& xi, T& ri, T pi, T api, T di, T& zi) {
+    if (!done) {
+      xi = xi + alpha * pi;
+      ri = ri - alpha * api;
+      zi = di * ri;
+      srz += static_cast<double>(ri) * static_cast<double>(zi);
+    }
+    srr += static_cast<double>(ri) * static_cast<double>(ri);
+  };
+  const size_t nv = n / W;
+  for (size_t iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; iv < nv;
+       iv += static_cast<size_t>(gridDim.x) * kThreads) {
+    Pack<T> rv = ldp(r, iv);
+    if (!done) {
+      Pack<T> xv = ldp(x, iv), zv;
+      const Pack<T> pv = ldp(p, iv), av = ldp(Ap, iv), dv = ldp(dinv, iv);
+#pragma unroll
+      for (int k = 0; k < W; ++k) one(xv.v[k], rv.v[k], pv.v[k], av.v[k], dv.v[k], zv.v[k]);
+      stp(x, iv, xv);
+      stp(r, iv, rv);
+      stp(z, iv, zv);
+    } else {
+#pragma unroll
+      for (int k = 0; k < W; ++k) srr += static_cast<double>(rv.v[k]) * static_cast<double>(rv.v[k]);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+    const size_t i = nv * W + threadIdx.x;
+    T xi = x[i], ri = r[i], zi = T(0);
+    one(xi, ri, p[i], Ap[i], dinv[i], zi);
+    if (!done) {
+      x[i] = xi;
+      r[i] = ri;
+      z[i] = zi;
+    }
+  }
+  const double a = block_sum(srz, sh[0]);
+  const double c = block_sum(srr, sh[1]);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kMaxRed + blockIdx.x] = c;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, const double* __restrict__ sc,
+                                                            const T* __restrict__ z, T* __restrict__ p) {
+  constexpr int W = Pack<T>::W;
+  if (sc[kPFROZEN] != 0.0) return;
+  const T beta = static_cast<T>(sc[kBETA]);
+  const size_t nv = n / W;
+  for (size_t iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; iv < nv;
+       iv += static_cast<size_t>(gridDim.x) * kThreads) {
+    const Pack<T> zv = ldp(z, iv);
+    Pack<T> pv = ldp(p, iv);
+#pragma unroll
+    for (int k = 0; k < W; ++k) pv.v[k] = zv.v[k] + beta * pv.v[k];
+    stp(p, iv, pv);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+    const size_t i = nv * W + threadIdx.x;
+    p[i] = z[i] + beta * p[i];
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, const double* __restrict__ sc,
                                                         const T* __restrict__ z, T* __restrict__ p) {
@@ -277,30 +395,48 @@ struct Pcg {
   size_t n;
   DevBuf<T> r, z, p, Ap, dinv;
   DevBuf<double> part, sc;
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
+  // Captured batches of iterations: the configured batch size and every power
+  // of two below it, so any count of iterations is a few graph launches.
+  struct Exec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<int, Exec> execs;   // by iterations per launch
   int batch = 0;
-  const T* b_bound = nullptr;  // the b and x the graph was captured with
+  const T* b_bound = nullptr;  // the b and x the graphs were captured with
   T* x_bound = nullptr;
 
   int nb;  // reduction blocks
 
   explicit Pcg(size_t n_)
       : n(n_), r(n_), z(n_), p(n_), Ap(n_), dinv(n_), part(2 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)) {}
-  ~Pcg() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+  ~Pcg() { drop_graphs(); }
+  void drop_graphs() {
+    for (auto& kv : execs) {
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+      if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    execs.clear();
   }
 
   template <class Apply>
   void iteration(Apply&& apply, T* x, int max_iter, hipStream_t s) {
     apply(p.data(), Ap.data(), s);
-    dot_partial_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), part.data());
-    cg_alpha_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), nb);
-    cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), p.data(), Ap.data(), dinv.data(), x, r.data(), z.data(),
-                                                part.data());
-    cg_beta_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), max_iter, nb);
-    cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
+    if (aligned16(x)) {  // every other vector is the solver's own allocation
+      dot_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), part.data());
+      cg_alpha_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), nb);
+      cg_update_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), p.data(), Ap.data(), dinv.data(), x, r.data(),
+                                                      z.data(), part.data());
+      cg_beta_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), max_iter, nb);
+      cg_p_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
+    } else {
+      dot_partial_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), part.data());
+      cg_alpha_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), nb);
+      cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), p.data(), Ap.data(), dinv.data(), x, r.data(),
+                                                  z.data(), part.data());
+      cg_beta_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), max_iter, nb);
+      cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
+    }
   }
 
   // x = 0 and the first residual, for A x = b with ||r|| <= rtol ||b|| as the goal.
@@ -322,34 +458,43 @@ struct Pcg {
   // recorded yet). No other thread does device setup or teardown meanwhile.
   template <class Apply>
   void prepare_graph(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, hipStream_t s) {
-    if (!exec || batch != batch_iters || b_bound != b || x_bound != x) {
-      if (exec) HIP_CHECK(hipGraphExecDestroy(exec));
-      if (graph) HIP_CHECK(hipGraphDestroy(graph));
-      exec = nullptr;
-      graph = nullptr;
-      ExclusiveCapture lk;
+    if (!execs.empty() && batch == batch_iters && b_bound == b && x_bound == x) return;
+    drop_graphs();
+    std::vector<int> sizes{batch_iters};
+    for (int k = 1; k < batch_iters; k *= 2) sizes.push_back(k);
+    ExclusiveCapture lk;
+    for (int size : sizes) {
+      Exec& e = execs[size];
       HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        for (int k = 0; k < batch_iters; ++k) iteration(apply, x, max_iter, s);
+        for (int k = 0; k < size; ++k) iteration(apply, x, max_iter, s);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
       }
-      HIP_CHECK(hipStreamEndCapture(s, &graph));
-      HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      batch = batch_iters;
-      b_bound = b;
-      x_bound = x;
+      HIP_CHECK(hipStreamEndCapture(s, &e.graph));
+      HIP_CHECK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
     }
+    batch = batch_iters;
+    b_bound = b;
+    x_bound = x;
   }
 
-  // Enqueues `batch_iters` iterations (one graph launch) and the copy of the scalars into host_sc.
+  // Enqueues `count` iterations (at most batch_iters; the fewest graph
+  // launches) and the copy of the scalars into host_sc.
   template <class Apply>
-  void launch_batch(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, double* host_sc, hipStream_t s) {
+  void launch_iters(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, int count, double* host_sc,
+                    hipStream_t s) {
     prepare_graph(apply, b, x, max_iter, batch_iters, s);
-    HIP_CHECK(hipGraphLaunch(exec, s));
+    count = std::max(1, std::min(count, batch_iters));
+    for (auto it = execs.rbegin(); it != execs.rend() && count > 0; ++it) {
+      while (count >= it->first) {
+        HIP_CHECK(hipGraphLaunch(it->second.exec, s));
+        count -= it->first;
+      }
+    }
     HIP_CHECK(hipMemcpyAsync(host_sc, sc.data(), kNSCAL * sizeof(double), hipMemcpyDeviceToHost, s));
   }
 };
@@ -388,6 +533,11 @@ struct SolveJob {
   bool in_flight = false;
   int batch = 8, outer = 0, inner_total = 0;
   bool started = false;  // MIXED_RELIABLE: the FP32 recursion has begun
+  // Convergence rate of the CG recursion (factor on r.r per iteration) and the
+  // last observation it was measured from; used to launch just the
+  // iterations a run still needs instead of whole batches.
+  double rate = 0.0, rr_seen = -1.0;
+  int it_seen = 0;
   double bnorm = 0.0;
   SolveStats stats;
   std::chrono::steady_clock::time_point t0;
@@ -423,6 +573,30 @@ void enqueue_bnorm(SolveJob& j) {
   HIP_CHECK(hipMemcpyAsync(j.hs + kHostB, st.scal.data(), sizeof(double), hipMemcpyDeviceToHost, j.s));
 }
 
+// Iterations to launch after a batch that did not finish: what the observed
+// rate needs to bring r.r from its current value to the threshold.
+int next_count(SolveJob& j) {
+  const double rr = j.hs[kRR], thresh = j.hs[kTHRESH];
+  const int it = static_cast<int>(j.hs[kITERS]);
+  if (j.rr_seen > 0.0 && rr > 0.0 && it > j.it_seen) j.rate = std::pow(rr / j.rr_seen, 1.0 / (it - j.it_seen));
+  j.rr_seen = rr;
+  j.it_seen = it;
+  if (!(j.rate > 0.0 && j.rate < 1.0) || !(thresh > 0.0) || rr <= thresh) return j.batch;
+  const double need = std::ceil(std::log(thresh / rr) / std::log(j.rate));
+  return static_cast<int>(std::max(1.0, std::min(need, static_cast<double>(j.batch))));
+}
+
+// Iterations to launch at the start of a run that must reduce r.r by
+// `reduction2` (the rate of the previous run, else a whole batch). The next
+// observation starts a new baseline.
+int start_count(SolveJob& j, double reduction2, int iters_now) {
+  j.rr_seen = -1.0;
+  j.it_seen = iters_now;
+  if (!(j.rate > 0.0 && j.rate < 1.0) || !(reduction2 > 0.0 && reduction2 < 1.0)) return j.batch;
+  const double need = std::ceil(std::log(reduction2) / std::log(j.rate));
+  return static_cast<int>(std::max(1.0, std::min(need, static_cast<double>(j.batch))));
+}
+
 double rel_of(const SolveJob& j) {
   const double rr = std::sqrt(std::max(j.hs[kHostRes], 0.0));
   return j.bnorm > 0 ? rr / j.bnorm : rr;
@@ -440,7 +614,7 @@ void advance(SolveJob& j) {
       enqueue_bnorm(j);
       if (j.opt.mode == 0) {
         st.pcg64->init(j.b, j.x, j.opt.rtol, j.s);
-        st.pcg64->launch_batch(apply64, j.b, j.x, j.opt.max_iter, j.batch, j.hs, j.s);
+        st.pcg64->launch_iters(apply64, j.b, j.x, j.opt.max_iter, j.batch, start_count(j, 0.0, 0), j.hs, j.s);
         j.phase = SolveJob::kIter;
       } else {
         zero_fill(j.x, st.n * sizeof(double), j.s);
@@ -464,7 +638,8 @@ void advance(SolveJob& j) {
         } else {
           st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
         }
-        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
+        const int count = start_count(j, j.opt.inner_rtol * j.opt.inner_rtol, iters);
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, count, j.hs, j.s);
         j.phase = SolveJob::kRelIter;
       }
       break;
@@ -477,7 +652,8 @@ void advance(SolveJob& j) {
         enqueue_residual(j, st.r32.data());
         j.phase = SolveJob::kRelCheck;
       } else {
-        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, next_count(j), j.hs,
+                               j.s);
       }
       break;
     case SolveJob::kIter:  // FP64 PCG batches
@@ -487,7 +663,7 @@ void advance(SolveJob& j) {
         enqueue_residual(j, nullptr);
         j.phase = SolveJob::kFinal;
       } else {
-        st.pcg64->launch_batch(apply64, j.b, j.x, j.opt.max_iter, j.batch, j.hs, j.s);
+        st.pcg64->launch_iters(apply64, j.b, j.x, j.opt.max_iter, j.batch, next_count(j), j.hs, j.s);
       }
       break;
     case SolveJob::kFinal:
@@ -504,7 +680,8 @@ void advance(SolveJob& j) {
         j.phase = SolveJob::kDone;
       } else {
         st.pcg32->init(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
-        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
+        const int count = start_count(j, j.opt.inner_rtol * j.opt.inner_rtol, 0);
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, count, j.hs, j.s);
         j.phase = SolveJob::kInner;
       }
       break;
@@ -518,7 +695,8 @@ void advance(SolveJob& j) {
         enqueue_residual(j, st.r32.data());
         j.phase = SolveJob::kCheck;
       } else {
-        st.pcg32->launch_batch(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.hs, j.s);
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, next_count(j), j.hs,
+                               j.s);
       }
       break;
     case SolveJob::kDone:

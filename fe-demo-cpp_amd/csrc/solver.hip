@@ -519,7 +519,7 @@ enum { kHostB = kNSCAL, kHostRes, kHostN };
 }  // namespace
 
 struct SolveJob {
-  enum Phase { kStart, kIter, kFinal, kCheck, kInner, kRelCheck, kRelIter, kDone };
+  enum Phase { kStart, kIter, kFinal, kCheck, kInner, kRelCheck, kRelIter, kRelSpec, kDone };
   Operator* op = nullptr;
   FusedMass* f = nullptr;
   MassSolverState* st = nullptr;
@@ -533,6 +533,7 @@ struct SolveJob {
   bool in_flight = false;
   int batch = 8, outer = 0, inner_total = 0;
   bool started = false;  // MIXED_RELIABLE: the FP32 recursion has begun
+  int spec_iters = 0;    // MIXED_RELIABLE: iterations when the speculative segment began
   // Convergence rate of the CG recursion (factor on r.r per iteration) and the
   // last observation it was measured from; used to launch just the
   // iterations a run still needs instead of whole batches.
@@ -650,12 +651,40 @@ void advance(SolveJob& j) {
         HIP_CHECK_LAUNCH();
         ++j.outer;
         enqueue_residual(j, st.r32.data());
-        j.phase = SolveJob::kRelCheck;
+        // The recursive residual tracks the true one closely: when it is well
+        // above the target, the replacement and the next segment go in behind
+        // the FP64 residual at once (one host round trip per segment). If the
+        // true residual turns out to have met the target, the speculative
+        // segment is dropped: it only changes the FP32 state, never x.
+        const int iters = static_cast<int>(j.hs[kITERS]);
+        const double rel_rec = j.bnorm > 0 ? std::sqrt(std::max(j.hs[kRR], 0.0)) / j.bnorm : 0.0;
+        if (rel_rec > 4.0 * j.opt.rtol && iters < j.opt.max_iter && j.outer < j.opt.max_outer) {
+          j.spec_iters = iters;
+          st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
+          const int count = start_count(j, j.opt.inner_rtol * j.opt.inner_rtol, iters);
+          st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, count, j.hs, j.s);
+          j.phase = SolveJob::kRelSpec;
+        } else {
+          j.phase = SolveJob::kRelCheck;
+        }
       } else {
         st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, next_count(j), j.hs,
                                j.s);
       }
       break;
+    case SolveJob::kRelSpec: {  // the FP64 residual and a speculative segment have finished
+      const double rel = rel_of(j);
+      if (rel <= j.opt.rtol || j.spec_iters >= j.opt.max_iter || j.outer >= j.opt.max_outer) {
+        j.stats.rel_residual = rel;
+        j.stats.iterations = j.spec_iters;
+        j.stats.outer = j.outer;
+        j.phase = SolveJob::kDone;
+        break;
+      }
+      j.phase = SolveJob::kRelIter;  // the segment is valid: carry on with its scalars
+      advance(j);
+      return;
+    }
     case SolveJob::kIter:  // FP64 PCG batches
       j.bnorm = std::sqrt(j.hs[kHostB]);
       if (j.hs[kDONE] != 0.0) {

@@ -107,16 +107,22 @@ def cpu_baseline(problem, slab_layers=8, reps=3):
                       (slab_layers, l_s, reps, t, platform.processor() or platform.machine(), os.cpu_count())}
 
 
-def solve_latency(fehip, reps):
-    """Config 3: M x = b on the 16^3, P=3, Q=4 mesh, b = M x*, x* ~ U[-1,1] seed 2."""
+def solve_latency(fehip, reps, n=16, P=3, Q=4):
+    """M x = b on the n^3 mesh (P, Q), b = M x*, x* ~ U[-1,1] seed 2, to a 1e-10
+    relative residual: FP64 PCG, FP32-inner / FP64-refined PCG, and FP32 PCG with
+    FP64 reliable updates. Latency percentiles over `reps` solves, iteration
+    counts, and each solution's error against x* and against the FP64 solution."""
     from fehip.problems import MassProblem
 
-    p = MassProblem(16, 3, 4, 1)
+    p = MassProblem(n, P, Q, 1)
     x_star = fehip.DeviceArray.from_host(fehip.rng_uniform(2, p.l_size))
     d_b, d_x = fehip.DeviceArray(p.l_size), fehip.DeviceArray(p.l_size)
     p.op.apply(x_star, d_b)
-    out = {}
-    for name, mode in (("fp64", fehip.SOLVE_FP64), ("mixed", fehip.SOLVE_MIXED)):
+    xs = x_star.to_host()
+    out = {"workload": "%d^3 hexes, P=%d, Q=%d, %d DoF, rtol 1e-10" % (n, P, Q, p.l_size)}
+    x64 = None
+    for name, mode in (("fp64", fehip.SOLVE_FP64), ("mixed", fehip.SOLVE_MIXED),
+                       ("mixed_reliable", fehip.SOLVE_MIXED_RELIABLE)):
         p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
         lat, st = [], None
         for _ in range(reps):
@@ -124,11 +130,15 @@ def solve_latency(fehip, reps):
             st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
             lat.append((time.perf_counter() - t0) * 1e3)
         x = d_x.to_host()
-        err = float(np.max(np.abs(x - x_star.to_host())) / np.max(np.abs(x_star.to_host())))
+        if x64 is None:
+            x64 = x
+        scale = float(np.max(np.abs(x64)))
         out[name] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                      "iterations": st["iterations"], "outer": st["outer"], "rel_residual": st["rel_residual"],
-                     "err_vs_xstar": err}
+                     "err_vs_xstar": float(np.max(np.abs(x - xs)) / np.max(np.abs(xs))),
+                     "err_vs_fp64": float(np.max(np.abs(x - x64)) / scale)}
     out["speedup_mixed_vs_fp64_p50"] = out["fp64"]["p50_ms"] / out["mixed"]["p50_ms"]
+    out["speedup_reliable_vs_fp64_p50"] = out["fp64"]["p50_ms"] / out["mixed_reliable"]["p50_ms"]
     return out
 
 
@@ -288,10 +298,12 @@ def main():
             log("config5", label, json.dumps(big[label]))
         del p5
 
-    solve = None
+    solve = solve64 = None
     if not args.no_solve:
-        solve = solve_latency(fehip, 20)
+        solve = solve_latency(fehip, 20)  # config 3: the 16^3, P=3, Q=4 mesh of config 1
         log("solve", solve)
+        solve64 = solve_latency(fehip, 5, 64, 5, 6)  # the headline mesh
+        log("solve 64^3", solve64)
     serve = None
     if not args.no_serve:
         # Config 4: mass-solve requests (n ~ U{4..40}, P ~ U{2..6}, Q = P+1,
@@ -346,6 +358,7 @@ def main():
         "tuning": tuning,
         "config5": big,
         "solve_c3": solve,
+        "solve_64cube": solve64,
         "serving_c4": serve,
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "cu": info.cu_count},
     }

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   constexpr int NE = BX * BY;
   constexpr int P2 = S::P2, Q2 = S::Q2, Q3 = S::Q3, TPE = S::TPE, RY = S::RY, ZS = S::ZS;
   constexpr int LXn = BX * (P - 1) + 1, LYn = BY * (P - 1) + 1, PLANE = LXn * LYn;
-  __shared__ T s_b[NE][S::FBUF];
+  __shared__ T s_b[NE + 1][S::FBUF];  // element tiles, then one tile of zeros (absent contributions)
   __shared__ T s_top[NC][PLANE];  // top node plane of the last layer, completed by the next one
   __shared__ T s_u[UM == 2 ? NC : 1][UM == 2 ? P : 1][UM == 2 ? PLANE : 1];  // u ring (UM 2), slot gz % P
   static_assert(PLANE <= NE * S::TPE, "one node position per thread");
@@ -252,6 +252,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   const int lex = live ? el % BX : 0, ley = live ? el / BX : 0;
   const int ex = cx * BX + lex, ey = cy * BY + ley;
   T* b = live ? s_b[el] : nullptr;
+  for (int i = threadIdx.x; i < S::FBUF; i += blockDim.x) s_b[NE][i] = T(0);  // read after the first barriers
 
   // Thread `pos` owns node position (plx, ply) of the column cross-section in
   // every plane: the layer's elements holding it (1, 2 or 4, ascending) and
@@ -260,7 +261,8 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   const int pos = threadIdx.x;
   const bool pos_thread = pos < PLANE;
   const int ply = pos_thread ? pos / LXn : 0, plx = pos_thread ? pos - (pos / LXn) * LXn : 0;
-  int toff[4] = {0, 0, 0, 0};
+  // Absent contributions (a node held by fewer than 4 elements) read the zero tile.
+  int toff[4] = {NE * S::FBUF, NE * S::FBUF, NE * S::FBUF, NE * S::FBUF};
   int ncon = 0;
   if (pos_thread) {
     int xl[2], xi[2], yl[2], yi[2], nxl = 0, nyl = 0;
@@ -443,17 +445,11 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
         T* pc = my_part + static_cast<size_t>(c) * part_comp_len;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          T sum;
-          int m0 = 1;
-          if (j == 0 && k > 0) {
-            sum = s_top[c][pos];
-            m0 = 0;
-          } else {
-            sum = tb[toff[0] + j * ZS];
-          }
-#pragma unroll
-          for (int m = 0; m < 4; ++m)
-            if (m >= m0 && m < ncon) sum += tb[toff[m] + j * ZS];
+          // All four reads issue together; absent ones add the zero tile.
+          const T t0 = tb[toff[0] + j * ZS], t1 = tb[toff[1] + j * ZS], t2 = tb[toff[2] + j * ZS],
+                  t3 = tb[toff[3] + j * ZS];
+          T sum = (j == 0 && k > 0) ? s_top[c][pos] + t0 : t0;
+          sum = ((sum + t1) + t2) + t3;
           if (j == P - 1 && k + 1 < g.lz) {
             s_top[c][pos] = sum;
           } else {

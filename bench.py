@@ -283,7 +283,9 @@ def main():
         log("config 5 setup %.1f s" % (time.perf_counter() - t1))
         L5 = p5.l_size
         big = {"workload": "48^3 hexes, P=8, Q=10, num_comp=3, %d DoF" % L5}
-        for label, dt, scatter, alg in (("fp64_block_partial", np.float64, D, fehip.FUSED_BLOCK_PARTIAL),
+        for label, dt, scatter, alg in (("fp64_box_column", np.float64, D, fehip.FUSED_BOX_COLUMN),
+                                        ("fp32_box_column", np.float32, D, fehip.FUSED_BOX_COLUMN),
+                                        ("fp64_block_partial", np.float64, D, fehip.FUSED_BLOCK_PARTIAL),
                                         ("fp64_elem_tmap", np.float64, D, fehip.FUSED_ELEM_TMAP),
                                         ("fp64_block_atomic", np.float64, A, fehip.FUSED_BLOCK_ATOMIC),
                                         ("fp32_block_partial", np.float32, D, fehip.FUSED_BLOCK_PARTIAL)):

@@ -1,4 +1,5 @@
 // Compiled fused-mass shapes with P = 8 (registered at library load).
+#include "fused_box_kernel.hpp"
 #include "fused_mass_kernel.hpp"
 
 namespace femhip {
@@ -11,6 +12,9 @@ const FusedShapeReg<8, 10, 3, 2, 2> reg_8_10_3_2;
 
 const FusedBlockShapeReg<8, 10, 1, 3, 3> blk_8_10_1;
 const FusedBlockShapeReg<8, 10, 3, 2, 2> blk_8_10_3;
+
+const FusedBoxReg<8, 10, 1, 2, 2> box_8_10_1_4;
+const FusedBoxReg<8, 10, 3, 2, 2> box_8_10_3_4;
 
 }  // namespace
 }  // namespace femhip

@@ -241,7 +241,8 @@ def test_autotune_records_fastest_valid_choice(gpu):
         (alg, ne, ms), rows = p.op.autotune(reps=3)
         assert {r[0] for r in rows} == {a for a in ALGS if usable(p.op, a)}
         assert ms == min(r[2] for r in rows) and (alg, ne, ms) in rows
-        assert (4, 5, 1, 0, 0, 0, alg, ne) in [t[:8] for t in fehip.tune_table()]  # not a box variant: box = 0
+        box = int(usable(p.op, fehip.FUSED_BOX_COLUMN))  # a box row where box-column kernels exist
+        assert (4, 5, 1, 0, 0, box, alg, ne) in [t[:8] for t in fehip.tune_table()]
         assert p.op.fused_choice() == (alg, ne)
         p.op.apply(d_u, d_v)
         assert rel_err(d_v.to_host(), want) < FP64_RTOL

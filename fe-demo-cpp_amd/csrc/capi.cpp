@@ -570,14 +570,17 @@ int fem_hip_tune_table_get(int* rows8, double* ms, long long* num_elem, int max_
     int i = 0;
     std::lock_guard<std::mutex> lk(tune_mutex());
     for (const auto& kv : tune_table()) {
-      if (i < max_rows) {
-        const TuneKey& k = kv.first;
-        const int r[8] = {k.P, k.Q, k.NC, k.fp32, k.det, k.box, kv.second.alg, kv.second.elems};
-        std::memcpy(rows8 + 8 * i, r, sizeof(r));
-        if (ms) ms[i] = kv.second.ms;
-        if (num_elem) num_elem[i] = kv.second.num_elem;
+      for (const auto& row : kv.second) {
+        if (i < max_rows) {
+          const TuneKey& k = kv.first;
+          const TuneEntry& e = row.second;
+          const int r[8] = {k.P, k.Q, k.NC, k.fp32, k.det, k.box, e.alg, e.elems};
+          std::memcpy(rows8 + 8 * i, r, sizeof(r));
+          if (ms) ms[i] = e.ms;
+          if (num_elem) num_elem[i] = e.num_elem;
+        }
+        ++i;
       }
-      ++i;
     }
     *n_rows = i;
   });
@@ -598,8 +601,9 @@ int fem_hip_tune_table_set(const int* r, double ms, long long num_elem) {
     e.elems = r[7];
     e.ms = ms;
     e.num_elem = num_elem;
+    require(num_elem > 0, "tune_table_set: num_elem (the mesh size the row was measured on) must be positive");
     std::lock_guard<std::mutex> lk(tune_mutex());
-    tune_table()[TuneKey{r[0], r[1], r[2], r[3], r[4], r[5]}] = e;
+    tune_record(TuneKey{r[0], r[1], r[2], r[3], r[4], r[5]}, e);
   });
 }
 int fem_hip_tune_table_clear(void) {

@@ -184,10 +184,28 @@ bool fused_mass_supported(int P, int Q, int NC) {
   return !fused_elem_variants(P, Q, NC, 0, kAlgElemAtomic, 0).empty();
 }
 
-std::map<TuneKey, TuneEntry>& tune_table() {
-  static std::map<TuneKey, TuneEntry> t;
+std::map<TuneKey, TuneRows>& tune_table() {
+  static std::map<TuneKey, TuneRows> t;
   return t;
 }
+
+const TuneEntry* tune_lookup(const TuneKey& k, long long num_elem, double max_ratio) {
+  auto it = tune_table().find(k);
+  if (it == tune_table().end() || it->second.empty()) return nullptr;
+  const TuneRows& rows = it->second;
+  const double n = static_cast<double>(std::max(num_elem, 1LL));
+  auto dist = [n](const TuneEntry& e) { return std::fabs(std::log(static_cast<double>(std::max(e.num_elem, 1LL)) / n)); };
+  auto hi = rows.lower_bound(num_elem);  // first row measured on at least num_elem elements
+  const TuneEntry* best = hi != rows.end() ? &hi->second : nullptr;
+  if (hi != rows.begin()) {
+    const TuneEntry& lo = std::prev(hi)->second;
+    if (!best || dist(lo) < dist(*best)) best = &lo;
+  }
+  if (max_ratio > 0.0 && dist(*best) > std::log(max_ratio)) return nullptr;
+  return best;
+}
+
+void tune_record(const TuneKey& k, const TuneEntry& e) { tune_table()[k][e.num_elem] = e; }
 
 std::mutex& tune_mutex() {
   static std::mutex m;
@@ -206,9 +224,11 @@ void tune_table_save(const std::string& path) {
   std::lock_guard<std::mutex> lk(tune_mutex());
   for (const auto& kv : tune_table()) {
     const TuneKey& k = kv.first;
-    const TuneEntry& e = kv.second;
-    out << "best " << k.P << ' ' << k.Q << ' ' << k.NC << ' ' << k.fp32 << ' ' << k.det << ' ' << k.box << ' '
-        << e.alg << ' ' << e.elems << ' ' << e.ms << ' ' << e.num_elem << '\n';
+    for (const auto& row : kv.second) {
+      const TuneEntry& e = row.second;
+      out << "best " << k.P << ' ' << k.Q << ' ' << k.NC << ' ' << k.fp32 << ' ' << k.det << ' ' << k.box << ' '
+          << e.alg << ' ' << e.elems << ' ' << e.ms << ' ' << e.num_elem << '\n';
+    }
   }
   for (const auto& kv : tune_alg_table()) {
     const TuneAlgKey& k = kv.first;
@@ -257,7 +277,7 @@ int tune_table_load(const std::string& path) {
       if (!find(FusedKey{k.P, k.Q, k.NC, k.fp32, e.alg, 0, e.elems})) continue;
       if (k.det && !alg_deterministic(e.alg)) continue;
       if (!k.box && alg_box(e.alg)) continue;
-      tune_table()[k] = e;
+      tune_record(k, e);
       ++taken;
     } else if (kind == "alg") {
       TuneAlgKey k;
@@ -715,24 +735,26 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
   {
     std::lock_guard<std::mutex> lk(tune_mutex());
     for (int box = fused_box_operator(f) ? 1 : 0; box >= 0; --box) {
-      auto it = tune_table().find(TuneKey{f.P, f.Q, f.NC, fp32, det ? 1 : 0, box});
-      if (it == tune_table().end()) continue;
+      const TuneEntry* e = tune_lookup(TuneKey{f.P, f.Q, f.NC, fp32, det ? 1 : 0, box}, f.num_elem);
+      if (!e) continue;
       FusedKey t = k;
-      t.alg = it->second.alg;
-      t.elems = it->second.elems;
+      t.alg = e->alg;
+      t.elems = e->elems;
       if (t.alg >= 0 && key_valid(f, t) && find(t)) return t;
     }
   }
-  // Default order: box columns (structured boxes), partial-sum blocks, then
-  // (deterministic) transpose map or (atomic) block atomics, then elements.
+  // Default order: box columns (structured boxes of at least kBoxMinNodes
+  // element nodes), partial-sum blocks, then (deterministic) transpose map or
+  // (atomic) block atomics, then elements.
   const int order_det[] = {kAlgBoxColumn, kAlgBlockPart, kAlgElemTmapPipe, kAlgElemTmap};
   const int order_any[] = {kAlgBoxColumn, kAlgBlockPartLds, kAlgBlockPartLdsPipe, kAlgBlockPart, kAlgBlockAtomic,
                            kAlgElemAtomic, kAlgElemTmapPipe, kAlgElemTmap};
   const int* order = det ? order_det : order_any;
   const int n = det ? 4 : 8;
+  const long long elem_nodes = static_cast<long long>(f.num_elem) * f.P * f.P * f.P * f.NC;
   for (int i = 0; i < n; ++i) {
     k.alg = order[i];
-    if (!valid_for(f, k.alg)) continue;
+    if (!valid_for(f, k.alg) || (alg_box(k.alg) && elem_nodes < kBoxMinNodes)) continue;
     k.elems = alg_box(k.alg) ? box_default_elems(f, fp32) : default_elems(f.P, f.Q, f.NC, fp32, k.alg, 0);
     if (k.elems > 0 && find(k)) return k;
   }
@@ -887,8 +909,8 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
   require(best.alg >= 0, "autotune: no candidate");
   {
     std::lock_guard<std::mutex> lk(tune_mutex());
-    tune_table()[TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0, fused_box_operator(f) ? 1 : 0}] =
-        best;
+    tune_record(TuneKey{f.P, f.Q, f.NC, fp32, f.scatter == kScatterDeterministic ? 1 : 0, fused_box_operator(f) ? 1 : 0},
+                best);
     for (const TuneEntry& t : local) {
       TuneEntry& slot = tune_alg_table()[TuneAlgKey{f.P, f.Q, f.NC, fp32, t.alg}];
       if (slot.alg < 0 || t.ms < slot.ms || slot.num_elem != t.num_elem) slot = t;

@@ -34,6 +34,11 @@ inline bool alg_deterministic(int alg) {
   return alg == kAlgElemTmap || alg == kAlgBlockPart || alg == kAlgElemTmapPipe || alg == kAlgBoxColumn;
 }
 inline bool alg_box(int alg) { return alg == kAlgBoxColumn; }
+// Element nodes (num_elem * P^3 * NC) from which an untuned box operator
+// defaults to the box columns. Below it there are too few column segments of
+// enough layers to fill the GPU: on MI355X the box columns lost to the element
+// and block algorithms at 4-7M element nodes and won at 12-33M.
+constexpr long long kBoxMinNodes = 8LL << 20;
 inline bool alg_tmap(int alg) { return alg == kAlgElemTmap || alg == kAlgElemTmapPipe; }
 inline bool alg_blocked(int alg) {
   return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds || alg == kAlgBlockPartLdsPipe;
@@ -190,10 +195,13 @@ int box_layers(int nz, long long ncols, long long target, int P);
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);
 
 // Autotuning table: the fastest (alg, elems) per (P, Q, NC, fp32,
-// deterministic, box), measured by fused_autotune and consulted by automatic
-// choice; box = 1 rows are for structured boxes (which can also run the
-// box-column algorithm), box = 0 rows for any node numbering. A box operator
-// without a box row uses the box = 0 row.
+// deterministic, box) and per mesh size (elements), measured by
+// fused_autotune and consulted by automatic choice; box = 1 rows are for
+// structured boxes (which can also run the box-column algorithm), box = 0 rows
+// for any node numbering. A box operator without a box row uses the box = 0
+// row. A key may hold rows measured on several mesh sizes (the winner changes
+// with size: the box columns need many elements per column segment); an
+// operator uses the row whose size is nearest its own (by ratio).
 struct TuneKey {
   int P, Q, NC, fp32, det, box;
   bool operator<(const TuneKey& o) const {
@@ -205,8 +213,16 @@ struct TuneEntry {
   double ms = 0.0;  // measured time per apply
   long long num_elem = 0;  // size it was measured on
 };
-std::map<TuneKey, TuneEntry>& tune_table();  // guard every access with tune_mutex()
+using TuneRows = std::map<long long, TuneEntry>;  // by TuneEntry::num_elem
+std::map<TuneKey, TuneRows>& tune_table();  // guard every access with tune_mutex()
 std::mutex& tune_mutex();
+// The row of `k` measured on the mesh size nearest num_elem (ratio of sizes;
+// ties go to the larger mesh) within a factor max_ratio (0: any), or null.
+// The caller holds tune_mutex().
+const TuneEntry* tune_lookup(const TuneKey& k, long long num_elem, double max_ratio = 0.0);
+// Adds e as k's row for e.num_elem, replacing a row of that size. The caller
+// holds tune_mutex().
+void tune_record(const TuneKey& k, const TuneEntry& e);
 // The fastest packing of each algorithm: (P, Q, NC, fp32, alg) -> (elems, ms).
 // An explicit algorithm with elems 0 uses it (else the default packing).
 struct TuneAlgKey {
@@ -220,8 +236,8 @@ std::map<TuneAlgKey, TuneEntry>& tune_alg_table();  // guarded by tune_mutex()
 //   best P Q NC fp32 det box alg elems ms num_elem
 //   alg  P Q NC fp32 alg elems ms num_elem
 // (a best row without the box field reads as box = 0)
-// load replaces rows with the same key, skips rows not compiled in this
-// library, and returns the rows taken.
+// load replaces best rows with the same key and num_elem (alg rows: the same
+// key), skips rows not compiled in this library, and returns the rows taken.
 void tune_table_save(const std::string& path);
 int tune_table_load(const std::string& path);
 // Times every compiled candidate valid for the operator's requested scatter and

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -422,25 +423,33 @@ class MassServer {
     cv_.notify_all();
   }
 
-  // Whether the tuning table holds every precision this solve applies in.
+  // Mesh sizes within this factor share a tuning row.
+  static constexpr double kTuneSizeRatio = 4.0;
+
+  // Whether the tuning table holds, for every precision this solve applies
+  // in, a row measured on a mesh within kTuneSizeRatio of this one's size.
   static bool tuned_shape(const FusedMass& f, bool mixed) {
     std::lock_guard<std::mutex> lk(tune_mutex());
     for (int fp32 : {0, 1}) {
       if (fp32 && !mixed) continue;
-      if (!tune_table().count(TuneKey{f.P, f.Q, f.NC, fp32, 1, fused_box_operator(f) ? 1 : 0})) return false;
+      if (!tune_lookup(TuneKey{f.P, f.Q, f.NC, fp32, 1, fused_box_operator(f) ? 1 : 0}, f.num_elem, kTuneSizeRatio))
+        return false;
     }
     return true;
   }
 
-  // Autotunes (P, Q, num_comp, precision, deterministic) once per server.
+  // Autotunes (P, Q, num_comp, precision, deterministic) once per server and
+  // size class (factors of kTuneSizeRatio in elements).
   void maybe_tune(FusedMass& f, bool mixed, hipStream_t s) {
+    const int size_class = static_cast<int>(std::floor(std::log(std::max(f.num_elem, 1) * 1.0) /
+                                                       std::log(kTuneSizeRatio)));
     for (int fp32 : {0, 1}) {
       if (fp32 && !mixed) continue;
       const TuneKey k{f.P, f.Q, f.NC, fp32, 1, fused_box_operator(f) ? 1 : 0};
       {
         std::unique_lock<std::mutex> lk(mu_);
-        if (tuned_.count(k)) continue;
-        tuned_.insert(k);
+        if (tuned_.count({k, size_class})) continue;
+        tuned_.insert({k, size_class});
       }
       fused_autotune(f, fp32, 5, s);
     }
@@ -458,7 +467,7 @@ class MassServer {
   std::vector<std::unique_ptr<Job>> ready_q_;
   std::set<Job*> running_;
   std::map<long long, ServeResult> done_;
-  std::set<TuneKey> tuned_;
+  std::set<std::pair<TuneKey, int>> tuned_;  // (shape, size class) autotuned by this server
   ServerStats stats_;
   std::thread scheduler_;
   std::vector<std::thread> workers_;

@@ -410,7 +410,9 @@ def box_column_check(nx, ny, nz, P, elems, layers=0):
 
 def tune_table():
     """Rows (P, Q, num_comp, fp32, deterministic, box, alg, elems, ms, num_elem) of the
-    process-wide tuning table (box = 1: rows for structured boxes)."""
+    process-wide tuning table (box = 1: rows for structured boxes). A shape may
+    hold rows measured on several mesh sizes (num_elem elements); an operator
+    follows the row measured nearest its own size."""
     n = ctypes.c_int()
     call("fem_hip_tune_table_get", None, None, None, 0, ctypes.byref(n))
     k = n.value
@@ -421,7 +423,9 @@ def tune_table():
     return [tuple(rows[8 * i:8 * i + 8]) + (float(ms[i]), int(ne[i])) for i in range(k)]
 
 
-def tune_table_set(P, Q, num_comp, fp32, deterministic, alg, elems, ms=0.0, num_elem=0, box=0):
+def tune_table_set(P, Q, num_comp, fp32, deterministic, alg, elems, ms, num_elem, box=0):
+    """Adds the row for a shape measured on a mesh of num_elem elements (> 0),
+    replacing the shape's row of that size."""
     row = (ctypes.c_int * 8)(P, Q, num_comp, int(fp32), int(deterministic), int(box), alg, elems)
     call("fem_hip_tune_table_set", row, float(ms), int(num_elem))
 

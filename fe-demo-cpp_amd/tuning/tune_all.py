@@ -1,16 +1,18 @@
 #!/usr/bin/env python3
 """Autotunes every compiled fused-mass shape on this GPU and writes the table.
 
-For each (P, Q, num_comp) with a compiled fused kernel, a jittered structured
-hex mesh of about 24 million element nodes (at most 96^3 hexes) is built, so
-the vectors are far larger than the caches, and the autotuner times every
-compiled algorithm and packing: FP64 with the atomic scatter allowed (every
+For each (P, Q, num_comp) with a compiled fused kernel and each of four mesh
+sizes (jittered structured hex meshes of about 24M, 6M, 1.5M and 0.375M element
+nodes, at most 96^3 hexes: the largest are far larger than the caches, the
+smallest launch-latency-bound), the autotuner times every compiled algorithm
+and packing: FP64 with the atomic scatter allowed (every
 algorithm; the fastest deterministic one is recorded as the FP64
 deterministic choice) and FP32 with the deterministic scatter (the mixed
 solve's inner apply). Each choice is recorded twice: for structured boxes
 (box = 1, the box-column algorithm included where it is compiled) and for any
 node numbering (box = 0, the best algorithm other than the box columns, as
-timed on the box). The table is written as text (default: mi355x_gfx950.txt
+timed on the box). An operator follows the rows measured on the size
+nearest its own. The table is written as text (default: mi355x_gfx950.txt
 beside this script), which fehip.load_default_tuning() and
 fem_hip_tune_table_load() read.
 
@@ -32,11 +34,14 @@ SHAPES = [(2, 3, (1, 2, 3)), (3, 4, (1, 2, 3)), (4, 5, (1, 2, 3)), (5, 6, (1, 2,
 DET = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BOX_COLUMN)
 
 
-def mesh_size(P, target=24_000_000, n_max=96):
-    """Hexes per side: about `target` element nodes, a multiple of 4 (so the
-    box-column cross-sections tile the box)."""
+TARGETS = (375_000, 1_500_000, 6_000_000, 24_000_000)  # element nodes; the largest last (it sets the alg rows)
+
+
+def mesh_size(P, nc, target, n_max=96):
+    """Hexes per side: about `target` element nodes (all components), a
+    multiple of 4 (so the box-column cross-sections tile the box)."""
     n = 4
-    while n < n_max and (n + 4) ** 3 * P ** 3 <= target:
+    while n < n_max and (n + 4) ** 3 * P ** 3 * nc <= target:
         n += 4
     return n
 
@@ -49,8 +54,7 @@ def main():
     info = fehip.device_info(0)
     fehip.tune_table_clear()
     for P, Q, ncs in SHAPES:
-        for nc in ncs:
-            n = mesh_size(P)
+        for nc, n in sorted({(nc, mesh_size(P, nc, t)) for nc in ncs for t in TARGETS}):
             t0 = time.perf_counter()
             p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
             p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)

@@ -248,16 +248,19 @@ int fem_hip_operator_autotune(fem_hip_operator op, int fp32, int reps, void* str
 /* Tuning table rows {P, Q, num_comp, fp32, deterministic, box, alg, elems} with
  * the time per apply and the element count it was measured on; box = 1 rows
  * serve operators on structured boxes (fem_hip_box_detect), box = 0 rows any
- * operator (a box operator without a box row uses the box = 0 row). get:
- * copies up to max_rows rows and sets *n_rows to the table size; set: adds or
- * replaces one row (alg must be compiled for the shape); clear: empties it. */
+ * operator (a box operator without a box row uses the box = 0 row). A key may
+ * hold rows for several mesh sizes; an operator follows the row measured on
+ * the size nearest its own element count (by ratio). get: copies up to
+ * max_rows rows and sets *n_rows to the table size; set: adds the key's row for
+ * num_elem (> 0), replacing one of that size (alg must be compiled for the
+ * shape); clear: empties it. */
 int fem_hip_tune_table_get(int* rows8, double* ms, long long* num_elem, int max_rows, int* n_rows);
 int fem_hip_tune_table_set(const int* row8, double ms, long long num_elem);
 int fem_hip_tune_table_clear(void);
-/* Save / load the tuning tables as text (the best algorithm per shape and
- * determinism, and the fastest packing per algorithm). Load replaces rows with
- * the same key, skips rows for kernels this library lacks, and reports the rows
- * taken. */
+/* Save / load the tuning tables as text (the best algorithm per shape,
+ * determinism and mesh size, and the fastest packing per algorithm). Load
+ * replaces rows with the same key and size, skips rows for kernels this library
+ * lacks, and reports the rows taken. */
 int fem_hip_tune_table_save(const char* path);
 int fem_hip_tune_table_load(const char* path, int* rows_taken);
 /* FP32 fused mass apply on FP32 L-vectors (qdata and basis rounded to FP32 once). */

@@ -179,8 +179,15 @@ def test_tune_table_save_load_roundtrip(tmp_path):
         fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BLOCK_PARTIAL, 14, 0.39, 262144)
         fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BOX_COLUMN, 16, 0.26, 262144, box=1)
         fehip.tune_table_set(3, 4, 2, 1, 0, fehip.FUSED_ELEM_ATOMIC, 16, 0.05, 4096)
+        # One key, two mesh sizes: two rows; the same size again replaces its row.
+        fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_ELEM_TMAP, 7, 0.02, 4096, box=1)
+        fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BLOCK_PARTIAL, 7, 0.03, 4096, box=1)
+        assert sorted(r[6:8] + (r[9],) for r in fehip.tune_table() if r[:6] == (5, 6, 1, 0, 1, 1)) == [
+            (fehip.FUSED_BLOCK_PARTIAL, 7, 4096), (fehip.FUSED_BOX_COLUMN, 16, 262144)]
         with pytest.raises(fehip.InvalidArgument):  # box columns only in a box row
             fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_BOX_COLUMN, 16, 0.26, 262144, box=0)
+        with pytest.raises(fehip.InvalidArgument):  # the size a row was measured on is required
+            fehip.tune_table_set(5, 6, 1, 0, 1, fehip.FUSED_ELEM_TMAP, 7, 0.02, 0)
         path = str(tmp_path / "t.txt")
         fehip.tune_table_save(path)
         rows = sorted(fehip.tune_table())
@@ -189,7 +196,7 @@ def test_tune_table_save_load_roundtrip(tmp_path):
         with open(path, "a") as f:
             f.write("# comment\nbest 7 7 1 0 0 0 0 8 0.1 10\n")  # no (7, 7) kernel: skipped
             f.write("best 5 6 1 1 1 1 7 0.2 512\n")  # a row without the box field: box = 0
-        assert fehip.tune_table_load(path) == 4
+        assert fehip.tune_table_load(path) == 5
         got = sorted(fehip.tune_table())
         assert (5, 6, 1, 1, 1, 0, fehip.FUSED_ELEM_TMAP, 7) in [r[:8] for r in got]
         got = [r for r in got if r[:4] != (5, 6, 1, 1)]

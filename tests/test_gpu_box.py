@@ -75,13 +75,20 @@ def test_box_column_matches_oracle(gpu, n3, P, Q, nc, elems, layers):
         assert rel_err(d_v32.to_host().astype(np.float64), want) < 2e-6, lz
 
 
-def test_box_column_is_the_default_on_boxes(gpu):
-    """Without a tuning row, a deterministic box operator runs the box columns;
-    a layer count that does not divide nz makes it fall back."""
-    mesh = rect_mesh(8, 8, 8, 5)
-    p = MeshProblem(mesh, 6, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+def test_box_column_is_the_default_on_large_boxes(gpu):
+    """Without a tuning row, a deterministic box operator of at least 8 Mi
+    element nodes runs the box columns, a smaller one does not (too few column
+    segments to fill the GPU); a layer count that does not divide nz makes it
+    fall back."""
     if any(r[:5] == (5, 6, 1, 0, 1) for r in fehip.tune_table()):  # any (P, Q, nc, FP64, det) row
         pytest.skip("a tuning row for this shape is loaded")
+    small = MeshProblem(rect_mesh(8, 8, 8, 5), 6, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+    assert small.op.fused_choice()[0] != BOX
+    small.op.set_fused_algorithm(BOX)
+    assert small.op.fused_choice()[0] == BOX
+    del small
+    mesh = rect_mesh(44, 44, 44, 5)  # 85,184 elements, 10.6M element nodes
+    p = MeshProblem(mesh, 6, 1, scatter=fehip.SCATTER_DETERMINISTIC)
     assert p.op.fused_choice()[0] == BOX
     p.op.set_box_layers(3)
     assert p.op.fused_choice()[0] != BOX

@@ -252,10 +252,19 @@ def test_autotune_records_fastest_valid_choice(gpu):
         assert {r[0] for r in rows2} == {a for a in DET_ALGS if usable(p.op, a)}
         assert p.op.fused_choice(fp32=True) == (alg2, ne2)
         # A table row set by hand steers automatic choice.
-        fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_ELEM_TMAP, 23)
+        fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_ELEM_TMAP, 23, 0.01, 512)
         assert p.op.fused_choice() == (fehip.FUSED_ELEM_TMAP, 23)
         with pytest.raises(fehip.InvalidArgument):
-            fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_BLOCK_ATOMIC, 16)
+            fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_BLOCK_ATOMIC, 16, 0.01, 512)
+        # Rows measured on other mesh sizes: the one nearest this mesh (512
+        # elements) by ratio steers the choice.
+        ne_bp = min(r[1] for r in rows if r[0] == fehip.FUSED_BLOCK_PARTIAL)
+        fehip.tune_table_clear()
+        fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_ELEM_TMAP, 23, 0.01, 100_000, box=box)
+        fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_BLOCK_PARTIAL, ne_bp, 0.01, 100, box=box)
+        assert p.op.fused_choice() == (fehip.FUSED_BLOCK_PARTIAL, ne_bp)  # 5x below, against 195x above
+        fehip.tune_table_set(4, 5, 1, 0, 1, fehip.FUSED_ELEM_TMAP, 23, 0.01, 2048, box=box)
+        assert p.op.fused_choice() == (fehip.FUSED_ELEM_TMAP, 23)  # 4x above
     finally:
         fehip.tune_table_clear()
 

@@ -42,19 +42,25 @@ def log(*a):
 
 
 def time_applies(fehip, op, d_u, d_v, steps, warmup, stream):
+    """(ms per apply, ms in the main kernel, ms in the other stages). The apply
+    time is the product path's: `steps` back-to-back applies between two
+    events. The split comes from a second run of `steps` applies with the
+    operator's stage events on (four events per apply, which add their own
+    gaps, so that run is not the apply time)."""
     for _ in range(warmup):
         op.apply(d_u, d_v, stream)
     stream.synchronize()
     e0, e1 = fehip.Event(), fehip.Event()
-    op.timing_begin()
-    fehip.synchronize()
     e0.record(stream)
     for _ in range(steps):
         op.apply(d_u, d_v, stream)
     e1.record(stream)
     e1.synchronize()
-    fehip.synchronize()
     total_ms = e0.elapsed_ms(e1)
+    op.timing_begin()
+    for _ in range(steps):
+        op.apply(d_u, d_v, stream)
+    stream.synchronize()
     k_ms, aux_ms, n = op.timing_end()
     return total_ms / steps, k_ms / max(n, 1), aux_ms / max(n, 1)
 

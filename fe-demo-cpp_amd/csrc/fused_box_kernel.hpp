@@ -10,14 +10,14 @@
 //   * each of the BX*BY elements runs the fast kernel's five in-place stages
 //     in its own LDS tile (u gathered straight from its x-rows), each 1D
 //     product in the even-odd form of the symmetric basis (eo_forward);
-//   * the element outputs are added into a ring of P node planes in LDS in 4
-//     colour rounds ((ex&1) | (ey&1)<<1): elements of one colour share no node,
-//     so every node sums its contributions in a fixed order;
-//   * the layer's lower P-1 planes are then complete (the top one waits for the
-//     next layer, whose bottom plane it is): their nodes go straight to v,
-//     except nodes on a face the segment shares with another segment (a
-//     column face inside the box, or a segment end inside the box), which go
-//     to the segment's partial area.
+//   * one thread per node position of the column cross-section then sums,
+//     plane by plane, the (up to 4) element tiles holding the node, in
+//     ascending element order;
+//   * the layer's lower P-1 planes are then complete (the top one is kept in
+//     LDS for the next layer, whose bottom plane it is): their nodes go
+//     straight to v, except nodes on a face the segment shares with another
+//     segment (a column face inside the box, or a segment end inside the box),
+//     which go to the segment's partial area.
 // A second pass gives each shared node the sum of its 2, 4 or 8 partials in
 // ascending segment order. Every node of v is written exactly once, the
 // result is the same bit for bit on every run, and no atomics are used.
@@ -261,26 +261,25 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   const int pos = threadIdx.x;
   const bool pos_thread = pos < PLANE;
   const int ply = pos_thread ? pos / LXn : 0, plx = pos_thread ? pos - (pos / LXn) * LXn : 0;
-  // Absent contributions (a node held by fewer than 4 elements) read the zero tile.
-  int toff[4] = {NE * S::FBUF, NE * S::FBUF, NE * S::FBUF, NE * S::FBUF};
-  int ncon = 0;
+  // The four candidates, in ascending element order: (lower y, lower x),
+  // (lower y, upper x), (upper y, lower x), (upper y, upper x), where "lower"
+  // is the element below a cut holding the node at index P-1 and "upper" the
+  // element holding it at index r (0 on a cut). Absent contributions (a node
+  // held by fewer than 4 elements) read the zero tile, whose +0 leaves every
+  // sum unchanged. Plain selects, no per-thread arrays: a dynamically indexed
+  // array would be placed in LDS or scratch memory by the compiler.
+  constexpr int ZT = NE * S::FBUF;
+  int toff0 = ZT, toff1 = ZT, toff2 = ZT, toff3 = ZT;
   if (pos_thread) {
-    int xl[2], xi[2], yl[2], yi[2], nxl = 0, nyl = 0;
     const int qx = plx / (P - 1), rx = plx - qx * (P - 1), qy = ply / (P - 1), ry = ply - qy * (P - 1);
-    if (rx == 0) {
-      if (qx > 0) { xl[nxl] = qx - 1; xi[nxl++] = P - 1; }
-      if (qx < BX) { xl[nxl] = qx; xi[nxl++] = 0; }
-    } else {
-      xl[nxl] = qx; xi[nxl++] = rx;
-    }
-    if (ry == 0) {
-      if (qy > 0) { yl[nyl] = qy - 1; yi[nyl++] = P - 1; }
-      if (qy < BY) { yl[nyl] = qy; yi[nyl++] = 0; }
-    } else {
-      yl[nyl] = qy; yi[nyl++] = ry;
-    }
-    for (int a = 0; a < nyl; ++a)
-      for (int c2 = 0; c2 < nxl; ++c2) toff[ncon++] = (yl[a] * BX + xl[c2]) * S::FBUF + yi[a] * RY + xi[c2];
+    const bool xlo = rx == 0 && qx > 0, xup = rx != 0 || qx < BX;
+    const bool ylo = ry == 0 && qy > 0, yup = ry != 0 || qy < BY;
+    const int oxl = (qx - 1) * S::FBUF + (P - 1), oxu = qx * S::FBUF + rx;
+    const int oyl = (qy - 1) * BX * S::FBUF + (P - 1) * RY, oyu = qy * BX * S::FBUF + ry * RY;
+    if (ylo && xlo) toff0 = oyl + oxl;
+    if (ylo && xup) toff1 = oyl + oxu;
+    if (yup && xlo) toff2 = oyu + oxl;
+    if (yup && xup) toff3 = oyu + oxu;
   }
   const bool pxs = (plx == 0 && cx > 0) || (plx == LXn - 1 && cx < g.ncx - 1);
   const bool pys = (ply == 0 && cy > 0) || (ply == LYn - 1 && cy < g.ncy - 1);
@@ -446,8 +445,8 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
 #pragma unroll
         for (int j = 0; j < P; ++j) {
           // All four reads issue together; absent ones add the zero tile.
-          const T t0 = tb[toff[0] + j * ZS], t1 = tb[toff[1] + j * ZS], t2 = tb[toff[2] + j * ZS],
-                  t3 = tb[toff[3] + j * ZS];
+          const T t0 = tb[toff0 + j * ZS], t1 = tb[toff1 + j * ZS], t2 = tb[toff2 + j * ZS],
+                  t3 = tb[toff3 + j * ZS];
           T sum = (j == 0 && k > 0) ? s_top[c][pos] + t0 : t0;
           sum = ((sum + t1) + t2) + t3;
           if (j == P - 1 && k + 1 < g.lz) {

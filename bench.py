@@ -151,8 +151,11 @@ def solve_latency(fehip, reps, n=16, P=3, Q=4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 200 applies of 0.25 ms after 100 untimed ones: on MI355X the first
+    # milliseconds of load after an idle spell run up to 20% slower while the
+    # clocks settle, which 5 warm-up applies do not cover.
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")

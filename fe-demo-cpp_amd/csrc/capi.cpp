@@ -992,6 +992,8 @@ int fem_hip_server_wait(fem_hip_server s, long long id, double timeout_ms, fem_h
     res->latency_ms = r.latency_ms;
     res->l_size = r.l_size;
     res->est_bytes = r.est_bytes;
+    res->build_ms = r.build_ms;
+    res->graph_ms = r.graph_ms;
     if (r.status != 0) {
       rc = r.status;
       throw std::runtime_error(r.error);

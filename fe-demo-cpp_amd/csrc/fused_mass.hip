@@ -419,9 +419,48 @@ DevPlan& ensure_plan(FusedMass& f, int NB, int budget, bool part, hipStream_t s)
 
 // Transpose map of the referenced nodes (host build, once): the element kernel
 // writes every element output to Y, and this pass sums each node's outputs.
+// Whether the scatter writes every L entry: every component of every
+// referenced node, and those cover [0, l_size).
+bool tmap_covers_all(const FusedMass& f, const std::vector<char>& referenced) {
+  std::vector<char> hit(f.l_size, 0);
+  for (int l = 0; l < f.l_size; ++l) {
+    if (!referenced[l]) continue;
+    for (int c = 0; c < f.NC; ++c) {
+      const long long g = static_cast<long long>(l) + static_cast<long long>(c) * f.comp_stride;
+      if (g >= f.l_size) return false;
+      hit[g] = 1;
+    }
+  }
+  for (int l = 0; l < f.l_size; ++l)
+    if (!hit[l]) return false;
+  return true;
+}
+
 void ensure_tmap(FusedMass& f) {
   if (f.tm_built) return;
-  const std::vector<int>& off = f.rstr->h_offsets;
+  const Restriction& r = *f.rstr;
+  if (r.type == 0 && r.scatter_mode == kScatterDeterministic && !r.t_offsets.empty()) {
+    // The restriction's map is this map (rows: used nodes ascending; entries: t ascending).
+    f.tm_nrows = r.num_nodes;
+    // Rows are the distinct used offsets ascending: num_nodes of them in
+    // [0, max_offset] are exactly 0..max_offset, and then row m is node m.
+    f.tm_rows_p = r.num_nodes == r.max_offset + 1 ? nullptr : r.l_vec_indices.data();
+    f.tm_ptr_p = r.t_offsets.data();
+    f.tm_idx_p = r.t_indices.data();
+    if (f.NC == 1) {
+      f.tm_covers_all = r.identity_nodes;
+    } else if (r.num_nodes == r.comp_stride && r.max_offset == r.comp_stride - 1 &&
+               static_cast<long long>(f.NC) * r.comp_stride == f.l_size) {
+      f.tm_covers_all = true;  // the nodes are exactly 0..comp_stride-1: the components tile [0, l_size)
+    } else {
+      std::vector<char> referenced(f.l_size, 0);
+      for (int o : r.h_offsets) referenced[o] = 1;
+      f.tm_covers_all = tmap_covers_all(f, referenced);
+    }
+    f.tm_built = true;
+    return;
+  }
+  const std::vector<int>& off = r.h_offsets;
   const size_t nt = off.size();
   std::vector<int> mult(static_cast<size_t>(f.l_size) + 1, 0);
   for (size_t t = 0; t < nt; ++t) ++mult[off[t]];
@@ -436,18 +475,9 @@ void ensure_tmap(FusedMass& f) {
   std::vector<int> idx(static_cast<size_t>(ptr.back())), fill(ptr.begin(), ptr.end() - 1);
   for (size_t t = 0; t < nt; ++t)  // ascending t within each row
     if (row_of[off[t]] >= 0) idx[fill[row_of[off[t]]]++] = static_cast<int>(t);
-  // Written entries: every referenced node, for every component.
-  std::vector<char> hit(f.l_size, 0);
-  bool ok = true;
-  for (int l = 0; l < f.l_size && ok; ++l) {
-    if (!mult[l]) continue;
-    for (int c = 0; c < f.NC && ok; ++c) {
-      const long long g = static_cast<long long>(l) + static_cast<long long>(c) * f.comp_stride;
-      ok = g < f.l_size;
-      if (ok) hit[g] = 1;
-    }
-  }
-  for (int l = 0; l < f.l_size && ok; ++l) ok = hit[l] != 0;
+  std::vector<char> referenced(f.l_size, 0);
+  for (int l = 0; l < f.l_size; ++l) referenced[l] = mult[l] > 0;
+  const bool ok = tmap_covers_all(f, referenced);
   f.tm_nrows = static_cast<int>(rows.size());
   bool identity = true;
   for (size_t m = 0; m < rows.size() && identity; ++m) identity = rows[m] == static_cast<int>(m);
@@ -456,6 +486,9 @@ void ensure_tmap(FusedMass& f) {
   f.tm_ptr.upload(ptr);
   f.tm_idx.upload(idx);
   f.tm_covers_all = ok;
+  f.tm_rows_p = f.tm_rows.empty() ? nullptr : f.tm_rows.data();
+  f.tm_ptr_p = f.tm_ptr.data();
+  f.tm_idx_p = f.tm_idx.data();
   f.tm_built = true;
 }
 
@@ -647,7 +680,7 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
       if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
       L(a, s);
       if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-      launch_row_sum<T>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows.data(), f.tm_ptr.data(), f.tm_idx.data(),
+      launch_row_sum<T>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows_p, f.tm_ptr_p, f.tm_idx_p,
                         elem_outputs<T>(f).data(), a.y_comp_len, v, s);
     } else {
       zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
@@ -947,7 +980,7 @@ void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s) {
                                                                Yd.data());
   HIP_CHECK_LAUNCH();
   // Every component gets the same diagonal (value stride 0 across components).
-  launch_row_sum<double>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows.data(), f.tm_ptr.data(), f.tm_idx.data(),
+  launch_row_sum<double>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows_p, f.tm_ptr_p, f.tm_idx_p,
                          Yd.data(), 0, diag, s);
   // The host copy of Bsq must outlive the async copy.
   HIP_CHECK(hipStreamSynchronize(s));

@@ -134,10 +134,13 @@ struct FusedMass {
   DevBuf<double> Y64;                 // element outputs of the transpose-map algorithm (lazy)
   DevBuf<float> Y32;
   // Transpose map over the referenced nodes: rows ascending, entries
-  // t = i + e*P^3 ascending (the CPU scatter's order).
+  // t = i + e*P^3 ascending (the CPU scatter's order). The restriction's own
+  // map when it has one (deterministic restrictions), else built here; the
+  // pointers below are the ones in use (rows null: row m is node m).
   bool tm_built = false, tm_covers_all = false;
   int tm_nrows = 0;
   DevBuf<int> tm_rows, tm_ptr, tm_idx;
+  const int *tm_rows_p = nullptr, *tm_ptr_p = nullptr, *tm_idx_p = nullptr;
   // Structured box (every offset is the lexicographic box formula with node
   // stride 1, comp_stride = node count, l_size = NC * node count): elements per direction.
   bool box = false;

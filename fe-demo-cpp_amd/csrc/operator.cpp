@@ -4,10 +4,16 @@
 // scatter, else basis^T + scatter). A mass operator (apply_mass or the vector
 // mass with u Interp active, qdata None passive on a backend-strided
 // restriction, v Interp active on the same restriction and basis, dim 3, a
-// compiled (P, Q, num_comp)) runs the fused per-element kernel instead.
+// compiled (P, Q, num_comp)) runs the fused per-element kernel instead. A
+// build_mass operator on trilinear hex geometry (dx Grad active on 8-node
+// elements with 3 components and a P=2 basis, weights Weight on the same basis,
+// qdata None on a backend-strided restriction, dim 3) runs the fused geometry
+// kernel, which computes the same bits without the gradient Q-vector.
 #include "operator.hpp"
 
 #include <cstring>
+
+#include "fused_build.hpp"
 
 namespace femhip {
 
@@ -22,9 +28,36 @@ bool same_basis(const Basis& a, const Basis& b) {
          a.h.interp == b.h.interp && a.h.grad == b.h.grad && a.h.q_weight == b.h.q_weight;
 }
 
+// build_mass (include/qfunctions/mass.hpp:18-40) with dim 3 on trilinear hexes.
+bool fused_build_ok(const Operator& op) {
+  if (op.qf.kind != kQfBuildMass || op.inputs.size() != 2 || op.outputs.size() != 1) return false;
+  int dim = 0;
+  if (op.qf.ctx.size() < sizeof(int)) return false;
+  std::memcpy(&dim, op.qf.ctx.data(), sizeof(int));
+  const OpField& dx = op.inputs[0];
+  const OpField& w = op.inputs[1];
+  const OpField& q = op.outputs[0];
+  if (dim != 3 || dx.eval_mode != kEvalGrad || dx.vkind != kVectorActive || !dx.rstr || !dx.basis || dx.size != 9)
+    return false;
+  if (!fused_build_mass_supported(*dx.rstr, *dx.basis)) return false;
+  if (w.eval_mode != kEvalWeight || !w.basis || w.size != 1) return false;
+  if (!(w.basis == dx.basis || (w.basis->h.dim == 3 && w.basis->h.Q == dx.basis->h.Q &&
+                                 w.basis->h.q_weight == dx.basis->h.q_weight)))
+    return false;
+  if (q.eval_mode != kEvalNone || q.vkind != kVectorActive || !q.rstr || q.size != 1) return false;
+  const Restriction& qr = *q.rstr;
+  const int Q = dx.basis->h.Q;
+  return qr.type == 1 && qr.backend_strides && qr.num_comp == 1 && qr.num_elem == dx.rstr->num_elem &&
+         qr.elem_size == Q * Q * Q;
+}
+
 void decide_fused(Operator& op) {
   op.fused_checked = true;
   op.fused_kind = 0;
+  if (fused_build_ok(op)) {
+    op.fused_kind = 2;
+    return;
+  }
   const bool mass_qf = op.qf.kind == kQfApplyMass || op.qf.kind == kQfApplyMassVector;
   if (!mass_qf || op.inputs.size() != 2 || op.outputs.size() != 1) return;
   const OpField& u = op.inputs[0];
@@ -178,6 +211,10 @@ void operator_apply(Operator& op, const double* in, size_t in_len, double* out, 
   if (op.fused_kind == 1) {
     FusedMass& f = operator_fused_mass(op, s);
     fused_mass_apply64(f, in, out, s);
+    return;
+  }
+  if (op.fused_kind == 2) {
+    fused_build_mass(*op.inputs[0].rstr, *op.inputs[0].basis, in, out, s);
     return;
   }
 

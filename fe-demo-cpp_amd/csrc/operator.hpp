@@ -33,7 +33,7 @@ struct Operator {
 
   // Fused path (decided at the first apply after the fields are complete).
   bool fused_checked = false;
-  int fused_kind = 0;  // 0 none, 1 mass
+  int fused_kind = 0;  // 0 none, 1 mass, 2 build_mass on trilinear hexes
   std::unique_ptr<FusedMass> fused;
   // Fused options: scatter (-1: the restriction's mode), exact arithmetic, and the
   // algorithm (-1 automatic, -2 the element kernels of the scatter mode, else a

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <hipcub/hipcub.hpp>
 #include <numeric>
 
 #include "restriction.hpp"
@@ -150,6 +151,101 @@ __global__ __launch_bounds__(kThreads) void strided_scatter_kernel(size_t e_size
   L[i * s0 + k * s1 + e * s2] += E[n];
 }
 
+// Transpose map on the device, entry for entry the host build_transpose_map's:
+// per-node counts (atomics), the used nodes and the row starts (exclusive
+// scans), an unordered fill by atomic cursors, then every row sorted ascending
+// (a row holds the elements' local nodes that touch one L-node: a handful).
+__global__ __launch_bounds__(kThreads) void tmap_count_kernel(size_t nt, const int* __restrict__ off,
+                                                             int* __restrict__ count) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t < nt) atomicAdd(&count[off[t]], 1);
+}
+
+__global__ __launch_bounds__(kThreads) void tmap_used_kernel(int l_size, const int* __restrict__ count,
+                                                            int* __restrict__ used) {
+  const int l = blockIdx.x * kThreads + threadIdx.x;
+  if (l < l_size) used[l] = count[l] > 0 ? 1 : 0;
+}
+
+// Row m = row_of[l] of every used node l: its node id and its start.
+__global__ __launch_bounds__(kThreads) void tmap_rows_kernel(int l_size, const int* __restrict__ count,
+                                                            const int* __restrict__ row_of,
+                                                            const int* __restrict__ start, int* __restrict__ l_vec,
+                                                            int* __restrict__ t_offsets) {
+  const int l = blockIdx.x * kThreads + threadIdx.x;
+  if (l >= l_size || count[l] == 0) return;
+  const int m = row_of[l];
+  if (l_vec) l_vec[m] = l;
+  t_offsets[m] = start[l];
+}
+
+__global__ __launch_bounds__(kThreads) void tmap_fill_kernel(size_t nt, const int* __restrict__ off,
+                                                            const int* __restrict__ start, int* __restrict__ cursor,
+                                                            int* __restrict__ t_indices) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= nt) return;
+  const int l = off[t];
+  t_indices[start[l] + atomicAdd(&cursor[l], 1)] = static_cast<int>(t);
+}
+
+__global__ __launch_bounds__(kThreads) void tmap_sort_kernel(int rows, const int* __restrict__ t_offsets,
+                                                            int* __restrict__ t_indices) {
+  const int m = blockIdx.x * kThreads + threadIdx.x;
+  if (m >= rows) return;
+  const int b = t_offsets[m], e = t_offsets[m + 1];
+  for (int i = b + 1; i < e; ++i) {  // insertion sort: the t's are distinct
+    const int x = t_indices[i];
+    int j = i - 1;
+    while (j >= b && t_indices[j] > x) {
+      t_indices[j + 1] = t_indices[j];
+      --j;
+    }
+    t_indices[j + 1] = x;
+  }
+}
+
+// Exclusive prefix sum of in[0..n) into out[0..n] (out[n] = the total).
+void exclusive_scan(const int* in, int* out, int n, hipStream_t s) {
+  size_t bytes = 0;
+  HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out + 1, n, s));
+  StreamTmp<unsigned char> tmp(bytes, s);
+  zero_fill(out, sizeof(int), s);
+  HIP_CHECK(hipcub::DeviceScan::InclusiveSum(tmp.data(), bytes, in, out + 1, n, s));
+}
+
+// Builds r's transpose map from r.offsets (on the device) on stream s.
+void build_transpose_map_device(Restriction& r, hipStream_t s) {
+  const size_t nt = static_cast<size_t>(r.num_elem) * r.elem_size;
+  const int L = r.l_size;
+  StreamTmp<int> count(static_cast<size_t>(L), s), used(static_cast<size_t>(L), s),
+      row_of(static_cast<size_t>(L) + 1, s), start(static_cast<size_t>(L) + 1, s);
+  zero_fill(count.data(), static_cast<size_t>(L) * sizeof(int), s);
+  tmap_count_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, r.offsets.data(), count.data());
+  tmap_used_kernel<<<grid_for(L, kThreads), kThreads, 0, s>>>(L, count.data(), used.data());
+  HIP_CHECK_LAUNCH();
+  exclusive_scan(used.data(), row_of.data(), L, s);
+  exclusive_scan(count.data(), start.data(), L, s);
+  int rows = 0;
+  HIP_CHECK(hipMemcpyAsync(&rows, row_of.data() + L, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  r.num_nodes = rows;
+  r.identity_nodes = (rows == L);
+  if (!r.identity_nodes) r.l_vec_indices.allocate(static_cast<size_t>(rows));
+  r.t_offsets.allocate(static_cast<size_t>(rows) + 1);
+  r.t_indices.allocate(nt);
+  tmap_rows_kernel<<<grid_for(L, kThreads), kThreads, 0, s>>>(
+      L, count.data(), row_of.data(), start.data(), r.identity_nodes ? nullptr : r.l_vec_indices.data(),
+      r.t_offsets.data());
+  HIP_CHECK_LAUNCH();
+  HIP_CHECK(hipMemcpyAsync(r.t_offsets.data() + rows, start.data() + L, sizeof(int), hipMemcpyDeviceToDevice, s));
+  zero_fill(count.data(), static_cast<size_t>(L) * sizeof(int), s);  // now the fill cursors
+  tmap_fill_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, r.offsets.data(), start.data(), count.data(),
+                                                              r.t_indices.data());
+  tmap_sort_kernel<<<grid_for(rows, kThreads), kThreads, 0, s>>>(rows, r.t_offsets.data(), r.t_indices.data());
+  HIP_CHECK_LAUNCH();
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
 }  // namespace
 
 bool Restriction::same_layout(const Restriction& o) const {
@@ -177,7 +273,7 @@ void build_transpose_map(int num_elem, int elem_size, int l_size, const int* off
 }
 
 Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp, int comp_stride, int l_size,
-                                       const int* offsets, int scatter_mode) {
+                                       const int* offsets, int scatter_mode, hipStream_t s) {
   // Validation of src/elem-restriction.cpp:102-138.
   require(num_elem > 0, "elem_restriction_create: num_elem must be positive");
   require(elem_size > 0, "elem_restriction_create: elem_size must be at least 1");
@@ -191,11 +287,14 @@ Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp
   const size_t nt = static_cast<size_t>(num_elem) * elem_size;
   require(nt < (1ull << 31), "elem_restriction_create: num_elem * elem_size must fit in int32");
   const long long top_comp = static_cast<long long>(num_comp - 1) * comp_stride;
+  int lo = 0, hi = 0;
+  if (nt) lo = hi = offsets[0];
   for (size_t t = 0; t < nt; ++t) {
-    const int o = offsets[t];
-    require(o >= 0 && o + top_comp < l_size,
-            "elem_restriction_create: offsets (including component offsets) must be in the range [0, l_size - 1]");
+    lo = std::min(lo, offsets[t]);
+    hi = std::max(hi, offsets[t]);
   }
+  require(lo >= 0 && hi + top_comp < l_size,
+          "elem_restriction_create: offsets (including component offsets) must be in the range [0, l_size - 1]");
   auto* r = new Restriction();
   try {
     r->type = 0;
@@ -206,17 +305,10 @@ Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp
     r->l_size = l_size;
     r->e_size = static_cast<size_t>(num_comp) * nt;
     r->scatter_mode = scatter_mode;
+    r->max_offset = hi;
     r->h_offsets.assign(offsets, offsets + nt);
     r->offsets.upload(r->h_offsets);
-    if (scatter_mode == kScatterDeterministic) {
-      TransposeMapHost map;
-      build_transpose_map(num_elem, elem_size, l_size, offsets, map);
-      r->num_nodes = static_cast<int>(map.l_vec_indices.size());
-      r->identity_nodes = (r->num_nodes == l_size);
-      if (!r->identity_nodes) r->l_vec_indices.upload(map.l_vec_indices);
-      r->t_offsets.upload(map.t_offsets);
-      r->t_indices.upload(map.t_indices);
-    }
+    if (scatter_mode == kScatterDeterministic) build_transpose_map_device(*r, s);
   } catch (...) {
     delete r;
     throw;
@@ -271,6 +363,7 @@ Restriction* restriction_clone(const Restriction& src) {
     r->h_offsets = src.h_offsets;
     r->num_nodes = src.num_nodes;
     r->identity_nodes = src.identity_nodes;
+    r->max_offset = src.max_offset;
     auto dup = [](const DevBuf<int>& from, DevBuf<int>& to) {
       to.allocate(from.size());
       SharedCapture lk;

@@ -34,7 +34,10 @@ struct Restriction {
   std::vector<int> h_offsets;  // host copy, for operators built on this restriction
   DevBuf<int> offsets;         // [num_elem][elem_size]
 
-  // Deterministic offset restrictions only.
+  int max_offset = 0;  // largest offset (offset restrictions)
+
+  // Deterministic offset restrictions only (built on the device: the same map
+  // as build_transpose_map).
   int num_nodes = 0;
   bool identity_nodes = false;  // every L-node 0..l_size-1 used: l_vec_indices not read
   DevBuf<int> l_vec_indices, t_offsets, t_indices;
@@ -42,8 +45,10 @@ struct Restriction {
   bool same_layout(const Restriction& o) const;
 };
 
+// The deterministic map is built on stream s (nullptr: the null stream), which
+// is synchronized before the call returns.
 Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp, int comp_stride, int l_size,
-                                       const int* offsets, int scatter_mode);
+                                       const int* offsets, int scatter_mode, hipStream_t s = nullptr);
 Restriction* restriction_create_strided(int num_elem, int elem_size, int num_comp, int l_size, const int* strides);
 // Deep copy (device arrays duplicated), for operators that store restrictions by value.
 Restriction* restriction_clone(const Restriction& r);

@@ -54,8 +54,7 @@ size_t serve_request_bytes(const ServeRequest& r) {
   b += E * P3 * (4 + 4 + 4 + 4);       // offsets, flagged copy, transpose map, multi-row map
   b += E * P3 * (2 + 12 * nc);         // block-local indices, element outputs (FP64 + FP32)
   b += E * Q3 * (8 + 8 + 8 + 8 + 4 + 4);  // qdata: built, operator copy, private, block order, FP32 copies
-  b += E * Q3 * 8 * 13;                // build_mass temporaries (dx gradients, weights, output)
-  b += E * (8 * 3 * 8 + 500 * nc);     // mesh E-vector, block node lists and partials
+  b += E * (500 * nc);                 // block node lists and partials (build_mass is fused: no temporaries)
   b += N * 16 + L * (8 * 10 + 4 * 7);  // node maps; solver vectors (FP64 and FP32)
   b += nv * 3 * 8 * 2 + E * 8 * 4 * 2;  // coordinates, mesh restriction
   b += 16ll << 20;                      // small buffers, allocator granularity
@@ -111,8 +110,9 @@ std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s) {
   box_coords(n, r.jitter, r.mesh_seed, coords.data());
 
   std::shared_ptr<const Restriction> rstr(
-      restriction_create_offset(E, P3, nc, N, nc * N, off.data(), kScatterDeterministic));
-  std::shared_ptr<const Restriction> mrstr(restriction_create_offset(E, 8, 3, nv, 3 * nv, moff.data(), kScatterAtomic));
+      restriction_create_offset(E, P3, nc, N, nc * N, off.data(), kScatterDeterministic, s));
+  std::shared_ptr<const Restriction> mrstr(
+      restriction_create_offset(E, 8, 3, nv, 3 * nv, moff.data(), kScatterAtomic, s));
   std::shared_ptr<const Restriction> qrstr(restriction_create_strided(E, Q3, 1, E * Q3, nullptr));
   std::shared_ptr<const Basis> basis(basis_create(make_h1_lagrange(3, nc, P, Q, 0)));
   std::shared_ptr<const Basis> mbasis(basis_create(make_h1_lagrange(3, 3, 2, Q, 0)));
@@ -370,7 +370,10 @@ class MassServer {
         j->solve = solve_job_create(*j->op, j->state, j->b.data(), j->x.data(), so, j->stream);
         HIP_CHECK(hipStreamSynchronize(j->stream));
       }
+      const Clock::time_point t_built = Clock::now();
+      j->res.build_ms = ms_between(j->t_admit, t_built);
       solve_job_prepare_graphs(*j->solve);  // exclusive: other threads' allocations and copies wait
+      j->res.graph_ms = ms_between(t_built, Clock::now());
     } catch (const std::exception& e) {
       ok = false;
       j->res.status = status_of(e);

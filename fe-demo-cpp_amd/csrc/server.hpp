@@ -36,6 +36,8 @@ struct ServeResult {
   double rel_residual = 0.0;
   // Milliseconds: submit -> admitted, admitted -> set up, solve, submit -> done.
   double queue_ms = 0.0, setup_ms = 0.0, solve_ms = 0.0, latency_ms = 0.0;
+  // Parts of setup_ms: building the problem and solver state; recording the graphs.
+  double build_ms = 0.0, graph_ms = 0.0;
   long long l_size = 0;
   size_t est_bytes = 0;
   std::vector<double> x;

@@ -745,7 +745,7 @@ class ServeResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("outer", ctypes.c_int),
                 ("rel_residual", ctypes.c_double), ("queue_ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("l_size", ctypes.c_longlong),
-                ("est_bytes", ctypes.c_size_t)]
+                ("est_bytes", ctypes.c_size_t), ("build_ms", ctypes.c_double), ("graph_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -87,6 +87,8 @@ def run_workload(reqs, rate_per_s, seed=1, server=None, keep_x=False, **server_o
                        "mean": float(lat.mean()), "max": float(lat.max())},
         "queue_ms_p50": float(np.median([res[0]["queue_ms"] for res in ok])) if ok else 0.0,
         "setup_ms_p50": float(np.median([res[0]["setup_ms"] for res in ok])) if ok else 0.0,
+        "setup_build_ms_p50": float(np.median([res[0]["build_ms"] for res in ok])) if ok else 0.0,
+        "setup_graph_ms_p50": float(np.median([res[0]["graph_ms"] for res in ok])) if ok else 0.0,
         "solve_ms_p50": float(np.median([res[0]["solve_ms"] for res in ok])) if ok else 0.0,
         "max_rel_residual": float(max(res[0]["rel_residual"] for res in ok)) if ok else None,
         "server": stats,

@@ -393,6 +393,9 @@ typedef struct {
   double queue_ms, setup_ms, solve_ms, latency_ms; /* submit->admit, admit->ready, solve, submit->done */
   long long l_size;
   size_t est_bytes;
+  /* Parts of setup_ms: building the problem (mesh, maps, qdata, solver state)
+   * and recording the CG graphs (with the wait for other threads' recordings). */
+  double build_ms, graph_ms;
 } fem_hip_serve_result;
 int fem_hip_server_create(const fem_hip_server_options* opt, fem_hip_server* out);
 int fem_hip_server_submit(fem_hip_server s, const fem_hip_serve_request* req, long long* id);

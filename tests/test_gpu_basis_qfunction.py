@@ -147,3 +147,56 @@ def test_qfunction_field_validation(gpu):
     qf.add_input("u", 1, fehip.EVAL_INTERP)
     with pytest.raises(fehip.InvalidArgument):
         qf.add_output("u", 1, fehip.EVAL_INTERP)
+
+
+def build_op(mesh_rstr, mesh_basis, q_rstr):
+    qf = fehip.QFunction(fehip.QF_BUILD_MASS)
+    qf.add_input("dx", 9, fehip.EVAL_GRAD)
+    qf.add_input("weights", 1, fehip.EVAL_WEIGHT)
+    qf.add_output("qdata", 1, fehip.EVAL_NONE)
+    qf.set_context_int(3)
+    op = fehip.Operator(qf)
+    op.set_field("dx", mesh_rstr, mesh_basis, fehip.VECTOR_ACTIVE)
+    op.set_field("weights", None, mesh_basis, fehip.VECTOR_NONE)
+    op.set_field("qdata", q_rstr, None, fehip.VECTOR_ACTIVE)
+    return op
+
+
+@pytest.mark.parametrize("interleaved,Q,mesh_P", [(False, 2, 2), (True, 4, 2), (True, 7, 2), (False, 3, 3)])
+def test_build_mass_operator_matches_oracle_bitwise(gpu, interleaved, Q, mesh_P):
+    """qdata through operator_apply of the build_mass operator
+    (tests/test-operator.cpp:103-112) equals the oracle's composition bit for
+    bit: trilinear geometry (mesh_P = 2) runs the fused geometry kernel, with
+    blocked or interleaved coordinates, on a mesh whose elements and vertices
+    are permuted; mesh_P = 3 runs the unfused path."""
+    from fehip.meshes import l_shape, permute
+    print("build_mass operator", interleaved, Q, mesh_P, flush=True)
+    m = permute(l_shape(4, 2), seed=9)
+    E, nv = m["num_elem"], m["num_vertices"]
+    moff, coords = m["vertex_offsets"], m["coords"]  # coords blocked [3][nv]
+    if mesh_P == 3:  # quadratic geometry: the midpoints of a 2x refined vertex grid
+        n = 3
+        moff = fehip.box_offsets(n, n, n, 3, 1)
+        nv = (2 * n + 1) ** 3
+        g = np.linspace(0.0, 1.0, 2 * n + 1)
+        z, y, x = np.meshgrid(g, g, g, indexing="ij")
+        coords = np.stack([x.ravel() + 0.01 * np.sin(7 * y.ravel()), y.ravel(), z.ravel() * (1 + 0.1 * x.ravel())])
+        coords = coords.ravel()
+        E = n ** 3
+    es = mesh_P ** 3
+    if interleaved:
+        off_dev, cs = (np.asarray(moff) * 3).astype(np.int32), 1
+        host_coords = coords.reshape(3, nv).T.ravel().copy()
+    else:
+        off_dev, cs, host_coords = np.asarray(moff, np.int32), nv, coords
+    mesh_rstr = fehip.elem_restriction_create(E, es, 3, cs, 3 * nv, off_dev)
+    mesh_basis = fehip.TensorBasis.create_tensor_H1_lagrange(3, 3, mesh_P, Q)
+    q_rstr = fehip.elem_restriction_create_strided(E, Q ** 3, 1, E * Q ** 3)
+    op = build_op(mesh_rstr, mesh_basis, q_rstr)
+    d_x = fehip.DeviceArray.from_host(host_coords)
+    d_q = fehip.DeviceArray.from_host(np.full(E * Q ** 3, np.nan))
+    op.apply(d_x, d_q)
+    want = oracle().build_qdata(3, E, mesh_P, Q, 3 * nv, cs, off_dev, host_coords)
+    assert d_q.to_host().tobytes() == want.tobytes()
+    if mesh_P == 2:
+        assert abs(want.sum() - 0.75) < 1e-12  # the L-shape's volume

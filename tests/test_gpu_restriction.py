@@ -73,12 +73,32 @@ def test_deterministic_bitwise_reproducible(gpu):
     assert first.tobytes() == cpu.tobytes()
 
 
-def test_transpose_map_on_device_matches_oracle(gpu):
-    ne, es, nc, cs, l_size, off = box_restriction_args(2, 2, 1, 2, 3, True)
+def transpose_map_cases():
+    """(name, ne, es, nc, cs, l_size, offsets): a small interleaved box, a
+    permuted L-shape (every node used, irregular valence), the same with every
+    other L-node unused, and a 0.9M-entry box with blocked components."""
+    from fehip.meshes import l_shape, permute
+    yield ("box interleaved",) + tuple(box_restriction_args(2, 2, 1, 2, 3, True))
+    m = permute(l_shape(6, 3), seed=4)
+    E, N = m["num_elem"], m["num_nodes"]
+    yield "permuted l-shape", E, 27, 1, N, N, np.asarray(m["offsets"], np.int32)
+    yield "unused nodes", E, 27, 2, 1, 2 * N + 1, (2 * np.asarray(m["offsets"])).astype(np.int32)
+    n, P = 24, 4
+    N = (n * (P - 1) + 1) ** 3
+    yield "box 24^3 P=4", n ** 3, P ** 3, 3, N, 3 * N, fehip.box_offsets(n, n, n, P, 1)
+
+
+@pytest.mark.parametrize("case", list(range(4)))
+def test_transpose_map_on_device_matches_oracle(gpu, case):
+    """The deterministic restriction builds its map on the device (atomic
+    counts, scans, atomic fill, per-row sort); it equals the CPU counting sort
+    entry for entry."""
+    name, ne, es, nc, cs, l_size, off = list(transpose_map_cases())[case]
+    print("device transpose map:", name, flush=True)
     r = fehip.elem_restriction_create(ne, es, nc, cs, l_size, off, fehip.SCATTER_DETERMINISTIC)
     lv, to, ti = r.transpose_map()
     lv2, to2, ti2 = oracle().transpose_map(ne, es, l_size, off)
-    assert (lv == lv2).all() and (to == to2).all() and (ti == ti2).all()
+    assert len(lv) == len(lv2) and (lv == lv2).all() and (to == to2).all() and (ti == ti2).all()
 
 
 def test_strided(gpu):

@@ -604,6 +604,7 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
     }
   } else {
     (void)elem_qdata<T>(f, s);
+    if (k.alg == kAlgElemAtomic && !k.exact) ensure_flagged_offsets(f, s);
     if (alg_tmap(k.alg)) {
       ensure_tmap(f);
       DevBuf<T>& Y = elem_outputs<T>(f);
@@ -671,7 +672,9 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
     a.qdata = elem_qdata<T>(f, s).data();
     a.num_elem = f.num_elem;
     a.offsets = f.rstr->offsets.data();
-    a.flagged_offsets = f.flagged_offsets.data();
+    // Only the atomic element kernel reads the single-contributor flags (the
+    // others read plain offsets through the same argument).
+    a.flagged_offsets = k.alg == kAlgElemAtomic && !k.exact ? f.flagged_offsets.data() : f.rstr->offsets.data();
     a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
     a.v = v;
     if (alg_tmap(k.alg)) {
@@ -699,7 +702,11 @@ bool fused_box_operator(const FusedMass& f) {
   return f.box && f.basis_symmetric && !fused_elem_variants(f.P, f.Q, f.NC, 0, kAlgBoxColumn, 0).empty();
 }
 
-bool detect_box(const int* off, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3) {
+namespace {
+
+// The only box the numbering can be, from offsets 0, P and P*P (element 0's
+// nodes (0,0,0), (0,1,0) and (0,0,1)) and the sizes; false if none fits.
+bool box_shape(const int* off, int num_elem, int P, int num_comp, int comp_stride, int l_size, long long* n3l) {
   if (P < 2 || num_elem <= 0 || off[0] != 0) return false;
   const long long Nx = off[P], Nxy = off[P * P];  // the nodes one step up in y and in z
   if (Nx < P || (Nx - 1) % (P - 1) != 0 || Nxy <= 0 || Nxy % Nx != 0) return false;
@@ -711,6 +718,83 @@ bool detect_box(const int* off, int num_elem, int P, int num_comp, int comp_stri
   // One component: any comp_stride (unused); more: blocked components, one after another.
   if ((num_comp > 1 && Nx * Ny * Nz != comp_stride) || static_cast<long long>(num_comp) * Nx * Ny * Nz != l_size)
     return false;
+  n3l[0] = nx;
+  n3l[1] = ny;
+  n3l[2] = nz;
+  return true;
+}
+
+// mismatch[0] = 1 where an offset differs from the box formula (every writer
+// stores the same value).
+__global__ __launch_bounds__(kThreads) void box_check_kernel(size_t nt, int P, int nx, int ny, int Nx, long long Nxy,
+                                                            const int* __restrict__ off, int* __restrict__ mismatch) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= nt) return;
+  const int P3 = P * P * P;
+  const long long e = static_cast<long long>(t / P3);
+  const int i = static_cast<int>(t % P3), ix = i % P, iy = (i / P) % P, iz = i / (P * P);
+  const long long ex = e % nx, ey = (e / nx) % ny, ez = e / (static_cast<long long>(nx) * ny);
+  const long long want = ex * (P - 1) + ix + Nx * (ey * (P - 1) + iy) + Nxy * (ez * (P - 1) + iz);
+  if (off[t] != want) mismatch[0] = 1;
+}
+
+// detect_box on the restriction's device copy of the offsets.
+bool detect_box_device(const Restriction& r, int P, int num_comp, int* n3, hipStream_t s) {
+  long long n3l[3];
+  if (!box_shape(r.h_offsets.data(), r.num_elem, P, num_comp, r.comp_stride, r.l_size, n3l)) return false;
+  const size_t nt = static_cast<size_t>(r.num_elem) * P * P * P;
+  const int Nx = static_cast<int>(n3l[0] * (P - 1) + 1);
+  const long long Nxy = static_cast<long long>(Nx) * (n3l[1] * (P - 1) + 1);
+  StreamTmp<int> mismatch(1, s);
+  zero_fill(mismatch.data(), sizeof(int), s);
+  box_check_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, P, static_cast<int>(n3l[0]),
+                                                              static_cast<int>(n3l[1]), Nx, Nxy, r.offsets.data(),
+                                                              mismatch.data());
+  HIP_CHECK_LAUNCH();
+  int bad = 1;
+  HIP_CHECK(hipMemcpyAsync(&bad, mismatch.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (bad) return false;
+  for (int d = 0; d < 3; ++d) n3[d] = static_cast<int>(n3l[d]);
+  return true;
+}
+
+// Multiplicity of every node (atomics), then the offsets with bit 31 set on
+// single-contributor nodes (the atomic element kernel stores those instead of adding).
+__global__ __launch_bounds__(kThreads) void node_count_kernel(size_t nt, const int* __restrict__ off,
+                                                             int* __restrict__ count) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t < nt) atomicAdd(&count[off[t]], 1);
+}
+
+__global__ __launch_bounds__(kThreads) void flag_single_kernel(size_t nt, const int* __restrict__ off,
+                                                              const int* __restrict__ count,
+                                                              int* __restrict__ flagged) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= nt) return;
+  const int o = off[t];
+  flagged[t] = count[o] == 1 ? static_cast<int>(static_cast<unsigned>(o) | 0x80000000u) : o;
+}
+
+}  // namespace
+
+void ensure_flagged_offsets(FusedMass& f, hipStream_t s) {
+  const size_t nt = f.rstr->offsets.size();
+  if (f.flagged_offsets.size() == nt) return;
+  StreamTmp<int> count(static_cast<size_t>(f.l_size), s);
+  zero_fill(count.data(), static_cast<size_t>(f.l_size) * sizeof(int), s);
+  f.flagged_offsets.allocate(nt);
+  node_count_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, f.rstr->offsets.data(), count.data());
+  flag_single_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, f.rstr->offsets.data(), count.data(),
+                                                                f.flagged_offsets.data());
+  HIP_CHECK_LAUNCH();
+}
+
+bool detect_box(const int* off, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3) {
+  long long n3l[3];
+  if (!box_shape(off, num_elem, P, num_comp, comp_stride, l_size, n3l)) return false;
+  const long long nx = n3l[0], ny = n3l[1], nz = n3l[2];
+  const long long Nx = nx * (P - 1) + 1, Ny = ny * (P - 1) + 1;
   long long t = 0;
   for (long long ez = 0; ez < nz; ++ez)
     for (long long ey = 0; ey < ny; ++ey)
@@ -811,7 +895,7 @@ std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBas
   f->B32.assign(b.interp.begin(), b.interp.end());
   f->Bsq.resize(b.interp.size());
   for (size_t i = 0; i < b.interp.size(); ++i) f->Bsq[i] = b.interp[i] * b.interp[i];
-  f->box = detect_box(r.h_offsets.data(), r.num_elem, b.P, b.num_comp, r.comp_stride, r.l_size, f->box_n);
+  f->box = detect_box_device(r, b.P, b.num_comp, f->box_n, s);
   {
     double bmax = 0.0, asym = 0.0;
     for (int q = 0; q < b.Q; ++q)
@@ -820,16 +904,6 @@ std::unique_ptr<FusedMass> fused_mass_create(const Restriction& r, const HostBas
         asym = std::max(asym, std::fabs(b.interp[q * b.P + p] - b.interp[(b.Q - 1 - q) * b.P + (b.P - 1 - p)]));
       }
     f->basis_symmetric = asym <= 1e-14 * bmax;
-  }
-  {
-    // Single-contributor nodes: the element atomic kernel stores them instead of adding.
-    const size_t nt = r.h_offsets.size();
-    std::vector<int> mult(static_cast<size_t>(r.l_size), 0);
-    for (size_t t = 0; t < nt; ++t) ++mult[r.h_offsets[t]];
-    std::vector<int> flagged(r.h_offsets);
-    for (size_t t = 0; t < nt; ++t)
-      if (mult[flagged[t]] == 1) flagged[t] = static_cast<int>(static_cast<unsigned>(flagged[t]) | 0x80000000u);
-    f->flagged_offsets.upload(flagged);
   }
   const size_t nq = static_cast<size_t>(b.Q) * b.Q * b.Q * r.num_elem;
   f->qdata64.allocate(nq);

@@ -124,7 +124,7 @@ struct FusedChoice {
 struct FusedMass {
   int P = 0, Q = 0, NC = 0, num_elem = 0, comp_stride = 0, l_size = 0;
   const Restriction* rstr = nullptr;  // owned by the operator (node restriction)
-  DevBuf<int> flagged_offsets;        // offsets with bit 31 set where the node has one contributor
+  DevBuf<int> flagged_offsets;        // offsets with bit 31 set where the node has one contributor (lazy)
   DevBuf<double> qdata64;             // private layout [E][Q^3]
   DevBuf<float> qdata32;              // lazily made from qdata64
   std::vector<double> B64;            // interp_1d [Q][P]
@@ -183,6 +183,8 @@ void fused_prepare(FusedMass& f, int fp32, hipStream_t s);
 // with lexicographic node numbering (node stride 1), l_size = NC * node count
 // and (NC > 1) comp_stride = node count; on success n3 = {nx, ny, nz}.
 bool detect_box(const int* offsets, int num_elem, int P, int num_comp, int comp_stride, int l_size, int* n3);
+// Makes f.flagged_offsets (on the device; the atomic element kernel's input).
+void ensure_flagged_offsets(FusedMass& f, hipStream_t s);
 // Whether the operator is a structured box with box-column kernels compiled
 // for its shape (its tuning rows are the box = 1 ones).
 bool fused_box_operator(const FusedMass& f);

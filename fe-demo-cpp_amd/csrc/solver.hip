@@ -53,11 +53,20 @@ int red_blocks(size_t n) {
   return static_cast<int>(std::min<size_t>(kMaxRed, std::max<size_t>(1, (n + per - 1) / per)));
 }
 
-// Scalar slots.
-// kPFROZEN: set by the iteration after the one that met the residual test, so
-// that iteration still forms the next search direction (a residual replacement
-// continues from it) and later ones leave p alone.
-enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kPFROZEN, kNSCAL };
+// Scalar slots. The state a CG iteration starts from (r.z, r.r, the iteration
+// count, the done flag) has two copies: an iteration of parity k reads copy k
+// and its last kernel writes copy 1-k, so no kernel writes a slot that its own
+// blocks read. Graphs hold even numbers of iterations, so between launches the
+// state is in copy 0 (kRZ, kRR, kITERS, kDONE), where the host reads it.
+// An iteration that starts done leaves x, r, z and p alone; the iteration that
+// meets the residual test still forms the next search direction (a residual
+// replacement continues from it).
+enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kRZ1, kRR1, kDONE1, kITERS1, kNSCAL };
+__host__ __device__ constexpr int slot(int s, int par) {
+  return par == 0 ? s : (s == kRZ ? kRZ1 : s == kRR ? kRR1 : s == kDONE ? kDONE1 : kITERS1);
+}
+// Partial-sum regions of `part`: r.z and r.r (update and init), p.Ap (dot).
+constexpr int kPartRZ = 0, kPartRR = kMaxRed, kPartPAP = 2 * kMaxRed;
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -165,7 +174,6 @@ __global__ __launch_bounds__(kThreads) void cg_init_scalars_kernel(const double*
   sc[kTHRESH] = rtol2 * rr;
   sc[kDONE] = (rr <= sc[kTHRESH]) ? 1.0 : 0.0;
   sc[kITERS] = 0.0;
-  sc[kPFROZEN] = sc[kDONE];
 }
 
 // Residual replacement: r = rnew, z = dinv * r, x = 0 (the increment restarts);
@@ -206,28 +214,37 @@ __global__ __launch_bounds__(kThreads) void cg_replace_scalars_kernel(const doub
   sc[kRR] = rr;
   sc[kTHRESH] = rtol2 * rr;
   sc[kDONE] = (rr <= sc[kTHRESH]) ? 1.0 : 0.0;
-  sc[kPFROZEN] = sc[kDONE];
 }
 
-__global__ __launch_bounds__(kThreads) void cg_alpha_kernel(const double* __restrict__ part, double* __restrict__ sc,
-                                                            int nb) {
-  __shared__ double sh[kThreads / 64];
-  const double pap = sum_partials(part, nb, sh);
-  if (threadIdx.x != 0) return;
-  sc[kPAP] = pap;
-  sc[kALPHA] = (sc[kDONE] != 0.0 || pap == 0.0) ? 0.0 : sc[kRZ] / pap;
-  if (sc[kDONE] != 0.0) sc[kPFROZEN] = 1.0;
+// The sum of the nb partials at part, the same in every thread and every block
+// (sum_partials' fixed order, then a broadcast through LDS).
+__device__ double all_partials(const double* part, int nb, double* sh, double* out) {
+  const double v = sum_partials(part, nb, sh);
+  if (threadIdx.x == 0) *out = v;
+  __syncthreads();
+  return *out;
 }
 
+// One CG step's update, with alpha = r.z / p.Ap formed by every block from the
+// p.Ap partials: x += alpha p, r -= alpha Ap, z = dinv r, and the partials of
+// the new r.z and r.r. An iteration that starts done changes nothing (alpha 0)
+// and only sums r.r.
 template <class T>
-__global__ __launch_bounds__(kThreads) void cg_update_kernel(size_t n, const double* __restrict__ sc,
+__global__ __launch_bounds__(kThreads) void cg_update_kernel(size_t n, int par, int nb, double* __restrict__ sc,
                                                              const T* __restrict__ p, const T* __restrict__ Ap,
                                                              const T* __restrict__ dinv, T* __restrict__ x,
                                                              T* __restrict__ r, T* __restrict__ z,
                                                              double* __restrict__ part) {
-  __shared__ double sh[2][kThreads / 64];
-  const bool done = sc[kDONE] != 0.0;
-  const T alpha = static_cast<T>(sc[kALPHA]);
+  __shared__ double sh[3][kThreads / 64];
+  __shared__ double s_pap;
+  const bool done = sc[slot(kDONE, par)] != 0.0;
+  const double pap = all_partials(part + kPartPAP, nb, sh[2], &s_pap);
+  const double alpha_d = (done || pap == 0.0) ? 0.0 : sc[slot(kRZ, par)] / pap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[kPAP] = pap;
+    sc[kALPHA] = alpha_d;
+  }
+  const T alpha = static_cast<T>(alpha_d);
   double srz = 0.0, srr = 0.0;
   for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * kThreads) {
@@ -245,38 +262,29 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(size_t n, const dou
   const double a = block_sum(srz, sh[0]);
   const double c = block_sum(srr, sh[1]);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = a;
-    part[kMaxRed + blockIdx.x] = c;
+    part[kPartRZ + blockIdx.x] = a;
+    part[kPartRR + blockIdx.x] = c;
   }
-}
-
-__global__ __launch_bounds__(kThreads) void cg_beta_kernel(const double* __restrict__ part, double* __restrict__ sc,
-                                                           int max_iter, int nb) {
-  __shared__ double sh[2][kThreads / 64];
-  const double rz_new = sum_partials(part, nb, sh[0]), rr = sum_partials(part + kMaxRed, nb, sh[1]);
-  if (threadIdx.x != 0) return;
-  if (sc[kDONE] != 0.0) {
-    sc[kBETA] = 0.0;
-    return;
-  }
-  sc[kITERS] += 1.0;
-  sc[kRR] = rr;
-  sc[kBETA] = (sc[kRZ] == 0.0) ? 0.0 : rz_new / sc[kRZ];
-  sc[kRZ] = rz_new;
-  if (rr <= sc[kTHRESH] || sc[kITERS] >= max_iter) sc[kDONE] = 1.0;
 }
 
 // The update kernel on 16-byte packs (same arithmetic per entry).
 template <class T>
-__global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, const double* __restrict__ sc,
+__global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int par, int nb, double* __restrict__ sc,
                                                                  const T* __restrict__ p, const T* __restrict__ Ap,
                                                                  const T* __restrict__ dinv, T* __restrict__ x,
                                                                  T* __restrict__ r, T* __restrict__ z,
                                                                  double* __restrict__ part) {
   constexpr int W = Pack<T>::W;
-  __shared__ double sh[2][kThreads / 64];
-  const bool done = sc[kDONE] != 0.0;
-  const T alpha = static_cast<T>(sc[kALPHA]);
+  __shared__ double sh[3][kThreads / 64];
+  __shared__ double s_pap;
+  const bool done = sc[slot(kDONE, par)] != 0.0;
+  const double pap = all_partials(part + kPartPAP, nb, sh[2], &s_pap);
+  const double alpha_d = (done || pap == 0.0) ? 0.0 : sc[slot(kRZ, par)] / pap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[kPAP] = pap;
+    sc[kALPHA] = alpha_d;
+  }
+  const T alpha = static_cast<T>(alpha_d);
   double srz = 0.0, srr = 0.0;
   auto one = [&](T& xi, T& ri, T pi, T api, T di, T& zi) {
     if (!done) {
@@ -317,17 +325,53 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, const
   const double a = block_sum(srz, sh[0]);
   const double c = block_sum(srr, sh[1]);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = a;
-    part[kMaxRed + blockIdx.x] = c;
+    part[kPartRZ + blockIdx.x] = a;
+    part[kPartRR + blockIdx.x] = c;
   }
 }
 
+// The end of a CG step, with beta = r.z(new) / r.z(old) formed by every block
+// from the update's partials: p = z + beta p, and (block 0) the state copy
+// 1-par: the iteration count, r.r, r.z and the done test r.r <= threshold or
+// the iteration limit. An iteration that started done copies the state on and
+// leaves p alone (*frozen). Returns beta.
+__device__ double cg_step_end(int par, int max_iter, int nb, double* sc, const double* part, double (*sh)[kThreads / 64],
+                              double* s_val, bool* frozen) {
+  const bool done0 = sc[slot(kDONE, par)] != 0.0;
+  const double rz_new = all_partials(part + kPartRZ, nb, sh[0], &s_val[0]);
+  const double rr = all_partials(part + kPartRR, nb, sh[1], &s_val[1]);
+  const double rz_old = sc[slot(kRZ, par)];
+  const double beta = done0 ? 0.0 : (rz_old == 0.0 ? 0.0 : rz_new / rz_old);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int q = 1 - par;
+    if (done0) {
+      sc[slot(kRZ, q)] = rz_old;
+      sc[slot(kRR, q)] = sc[slot(kRR, par)];
+      sc[slot(kITERS, q)] = sc[slot(kITERS, par)];
+      sc[slot(kDONE, q)] = 1.0;
+    } else {
+      const double iters = sc[slot(kITERS, par)] + 1.0;
+      sc[slot(kITERS, q)] = iters;
+      sc[slot(kRR, q)] = rr;
+      sc[slot(kRZ, q)] = rz_new;
+      sc[slot(kDONE, q)] = (rr <= sc[kTHRESH] || iters >= max_iter) ? 1.0 : 0.0;
+    }
+    sc[kBETA] = beta;
+  }
+  *frozen = done0;
+  return beta;
+}
+
 template <class T>
-__global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, const double* __restrict__ sc,
+__global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, int par, int max_iter, int nb,
+                                                            double* __restrict__ sc, const double* __restrict__ part,
                                                             const T* __restrict__ z, T* __restrict__ p) {
   constexpr int W = Pack<T>::W;
-  if (sc[kPFROZEN] != 0.0) return;
-  const T beta = static_cast<T>(sc[kBETA]);
+  __shared__ double sh[2][kThreads / 64];
+  __shared__ double s_val[2];
+  bool frozen = false;
+  const T beta = static_cast<T>(cg_step_end(par, max_iter, nb, sc, part, sh, s_val, &frozen));
+  if (frozen) return;  // p stays
   const size_t nv = n / W;
   for (size_t iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; iv < nv;
        iv += static_cast<size_t>(gridDim.x) * kThreads) {
@@ -344,10 +388,14 @@ __global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, const doub
 }
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, const double* __restrict__ sc,
+__global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, int par, int max_iter, int nb,
+                                                        double* __restrict__ sc, const double* __restrict__ part,
                                                         const T* __restrict__ z, T* __restrict__ p) {
-  if (sc[kPFROZEN] != 0.0) return;
-  const T beta = static_cast<T>(sc[kBETA]);
+  __shared__ double sh[2][kThreads / 64];
+  __shared__ double s_val[2];
+  bool frozen = false;
+  const T beta = static_cast<T>(cg_step_end(par, max_iter, nb, sc, part, sh, s_val, &frozen));
+  if (frozen) return;  // p stays
   for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * kThreads)
     p[i] = z[i] + beta * p[i];
@@ -408,8 +456,11 @@ struct Pcg {
 
   int nb;  // reduction blocks
 
+  // Iterations per batch graph: even, at least 2.
+  static int even_batch(int b) { return std::max(2, b + (b & 1)); }
+
   explicit Pcg(size_t n_)
-      : n(n_), r(n_), z(n_), p(n_), Ap(n_), dinv(n_), part(2 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)) {}
+      : n(n_), r(n_), z(n_), p(n_), Ap(n_), dinv(n_), part(3 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)) {}
   ~Pcg() { drop_graphs(); }
   void drop_graphs() {
     for (auto& kv : execs) {
@@ -419,23 +470,23 @@ struct Pcg {
     execs.clear();
   }
 
+  // One iteration of parity par (0 or 1; see the scalar slots): the apply,
+  // p.Ap partials, the update (alpha formed in every block) and the step end
+  // (beta and the state copy formed in every block).
   template <class Apply>
-  void iteration(Apply&& apply, T* x, int max_iter, hipStream_t s) {
+  void iteration(Apply&& apply, T* x, int max_iter, int par, hipStream_t s) {
     apply(p.data(), Ap.data(), s);
+    double* pap = part.data() + kPartPAP;
     if (aligned16(x)) {  // every other vector is the solver's own allocation
-      dot_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), part.data());
-      cg_alpha_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), nb);
-      cg_update_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), p.data(), Ap.data(), dinv.data(), x, r.data(),
-                                                      z.data(), part.data());
-      cg_beta_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), max_iter, nb);
-      cg_p_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
+      dot_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), pap);
+      cg_update_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), p.data(), Ap.data(), dinv.data(), x,
+                                                      r.data(), z.data(), part.data());
+      cg_p_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, max_iter, nb, sc.data(), part.data(), z.data(), p.data());
     } else {
-      dot_partial_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), part.data());
-      cg_alpha_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), nb);
-      cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), p.data(), Ap.data(), dinv.data(), x, r.data(),
-                                                  z.data(), part.data());
-      cg_beta_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), max_iter, nb);
-      cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, sc.data(), z.data(), p.data());
+      dot_partial_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), pap);
+      cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), p.data(), Ap.data(), dinv.data(), x,
+                                                  r.data(), z.data(), part.data());
+      cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, par, max_iter, nb, sc.data(), part.data(), z.data(), p.data());
     }
   }
 
@@ -458,16 +509,17 @@ struct Pcg {
   // recorded yet). No other thread does device setup or teardown meanwhile.
   template <class Apply>
   void prepare_graph(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, hipStream_t s) {
+    batch_iters = even_batch(batch_iters);
     if (!execs.empty() && batch == batch_iters && b_bound == b && x_bound == x) return;
     drop_graphs();
-    std::vector<int> sizes{batch_iters};
-    for (int k = 1; k < batch_iters; k *= 2) sizes.push_back(k);
+    std::vector<int> sizes{batch_iters};  // even sizes only: the state ends in copy 0
+    for (int k = 2; k < batch_iters; k *= 2) sizes.push_back(k);
     ExclusiveCapture lk;
     for (int size : sizes) {
       Exec& e = execs[size];
       HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        for (int k = 0; k < size; ++k) iteration(apply, x, max_iter, s);
+        for (int k = 0; k < size; ++k) iteration(apply, x, max_iter, k & 1, s);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s, &g);
@@ -488,7 +540,10 @@ struct Pcg {
   void launch_iters(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, int count, double* host_sc,
                     hipStream_t s) {
     prepare_graph(apply, b, x, max_iter, batch_iters, s);
-    count = std::max(1, std::min(count, batch_iters));
+    batch_iters = even_batch(batch_iters);
+    // Whole pairs of iterations: one more than asked costs at most one
+    // iteration (and nothing once the run is done).
+    count = std::max(2, std::min(count + (count & 1), batch_iters));
     for (auto it = execs.rbegin(); it != execs.rend() && count > 0; ++it) {
       while (count >= it->first) {
         HIP_CHECK(hipGraphLaunch(it->second.exec, s));

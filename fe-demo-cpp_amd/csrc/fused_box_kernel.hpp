@@ -329,16 +329,16 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
     for (int qz = 0; qz < Q; ++qz) qc[qz] = r[qz * Q2];
   }
   if constexpr (UM == 2) __syncthreads();
+  // Nothing loaded above is still in flight when the layer loop starts. (Those
+  // loads target registers that later stages reuse; left in flight, the
+  // compiler's wait analysis would carry them around the loop and wait for
+  // every outstanding load before those stages, the qdata prefetch included.)
+  __builtin_amdgcn_s_waitcnt(0);
 
 #pragma unroll 1
   for (int k = 0; k < g.lz; ++k) {
     const int ez = sz * g.lz + k;
     T qn[Q] = {};
-    if (line_thread && k + 1 < g.lz) {
-      const T* r = qd_ptr(ez + 1);
-#pragma unroll
-      for (int qz = 0; qz < Q; ++qz) qn[qz] = r[qz * Q2];
-    }
     T qd[Q];
 #pragma unroll
     for (int qz = 0; qz < Q; ++qz) qd[qz] = qc[qz];
@@ -366,9 +366,9 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
         }
       }
       // S1: x-rows (iz,iy): contract x.
+      T x[P];
       if (row_thread) {
         const int iz = riz, iy = riy;
-        T x[P], y[Q];
         if constexpr (UM == 2) {
           const T* ur = &s_u[c][(ez * (P - 1) + iz) % P][(ley * (P - 1) + iy) * LXn + lex * (P - 1)];
 #pragma unroll
@@ -381,6 +381,22 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
 #pragma unroll
           for (int ix = 0; ix < P; ++ix) x[ix] = r[ix];
         }
+      }
+      // The next layer's qdata, issued after this step's u loads: outstanding
+      // loads complete in issue order, so S1's wait for u leaves it in flight.
+      // Every lane loads (lanes past the lines and the last layer read a valid
+      // row they do not use), so the number of loads in flight is known where
+      // S1 waits.
+      if (NC == 1 || c == 0) {
+        const int tq = t < Q2 ? t : Q2 - 1, ezq = k + 1 < g.lz ? ez + 1 : ez;
+        const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ezq);
+        const T* r = qdata + e * Q3 + tq;
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) qn[qz] = r[qz * Q2];
+      }
+      if (row_thread) {
+        const int iz = riz, iy = riy;
+        T y[Q];
         eo_forward<P, Q, T>(x, y, Em);
         const int row = iz * ZS + iy * RY;
 #pragma unroll

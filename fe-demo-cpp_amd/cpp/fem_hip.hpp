@@ -332,10 +332,10 @@ inline void operator_apply(Operator& op, const DeviceArray<float>& in, DeviceArr
 // Mass solve M x = b on a fused mass operator: mode FEM_HIP_SOLVE_FP64 (PCG),
 // _MIXED (FP32 inner PCG, FP64 refinement) or _MIXED_RELIABLE (FP32 PCG with
 // FP64 residual replacement).
-// inner_rtol <= 0: 1e-4 per inner solve (MIXED), 1e-2 between replacements (MIXED_RELIABLE).
+// inner_rtol <= 0: 1e-5 per inner solve (MIXED), 1e-2 between replacements (MIXED_RELIABLE).
 inline fem_hip_solve_stats mass_solve(Operator& op, const DeviceArray<double>& b, DeviceArray<double>& x, int mode,
                                       double rtol = 1e-10, void* stream = nullptr, double inner_rtol = 0.0) {
-  if (inner_rtol <= 0.0) inner_rtol = mode == FEM_HIP_SOLVE_MIXED_RELIABLE ? 1e-2 : 1e-4;
+  if (inner_rtol <= 0.0) inner_rtol = 0.0;  // the library's default for the mode
   x.resize(b.size());
   fem_hip_solve_options o{mode, 2000, 30, rtol, inner_rtol};
   fem_hip_solve_stats st{};

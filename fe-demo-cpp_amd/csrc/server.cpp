@@ -367,6 +367,7 @@ class MassServer {
         so.max_iter = 2000;
         so.max_outer = 30;
         so.batch = opt_.batch_iters;
+        so.inner_rtol = default_inner_rtol(so.mode);
         j->solve = solve_job_create(*j->op, j->state, j->b.data(), j->x.data(), so, j->stream);
         HIP_CHECK(hipStreamSynchronize(j->stream));
       }

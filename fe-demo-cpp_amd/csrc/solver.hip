@@ -763,8 +763,10 @@ void advance(SolveJob& j) {
         j.stats.outer = j.outer;
         j.phase = SolveJob::kDone;
       } else {
-        st.pcg32->init(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
-        const int count = start_count(j, j.opt.inner_rtol * j.opt.inner_rtol, 0);
+        // No deeper than the target needs: the last step stops at about half of it.
+        const double inner = std::max(j.opt.inner_rtol, std::min(0.5 * j.opt.rtol / rel, 0.5));
+        st.pcg32->init(st.r32.data(), st.d32.data(), inner, j.s);
+        const int count = start_count(j, inner * inner, 0);
         st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, count, j.hs, j.s);
         j.phase = SolveJob::kInner;
       }
@@ -801,6 +803,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   require(s != nullptr, "solve job: needs a non-default stream (graph capture)");
   require(opt.mode >= 0 && opt.mode <= 2, "mass_solve: unknown mode");
   require(opt.rtol > 0 && opt.max_iter > 0, "mass_solve: rtol and max_iter must be positive");
+  require(opt.inner_rtol < 1.0, "mass_solve: inner_rtol must be below 1");
   require(b != nullptr && x != nullptr, "mass_solve: b and x must be device arrays");
   FusedMass& f = operator_fused_mass(op, s);
   const size_t n = static_cast<size_t>(f.l_size);
@@ -841,6 +844,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   j->b = b;
   j->x = x;
   j->opt = opt;
+  if (j->opt.inner_rtol <= 0.0) j->opt.inner_rtol = default_inner_rtol(opt.mode);
   j->s = s;
   j->batch = opt.batch > 0 ? opt.batch : 8;
   HIP_CHECK(hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));

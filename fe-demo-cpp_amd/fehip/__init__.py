@@ -681,10 +681,10 @@ class Operator:
 
     def solve(self, d_b, d_x, mode=SOLVE_FP64, rtol=1e-10, max_iter=2000, max_outer=30, inner_rtol=None,
               stream=None):
-        """M x = b. inner_rtol: reduction of each inner FP32 solve (SOLVE_MIXED, default 1e-4)
+        """M x = b. inner_rtol: reduction of each inner FP32 solve (SOLVE_MIXED, default 1e-5)
         or between residual replacements (SOLVE_MIXED_RELIABLE, default 1e-2)."""
         if inner_rtol is None:
-            inner_rtol = 1e-2 if mode == SOLVE_MIXED_RELIABLE else 1e-4
+            inner_rtol = 0.0  # the library's default for the mode
         opt = SolveOptions(mode, max_iter, max_outer, rtol, inner_rtol)
         st = SolveStats()
         call("fem_hip_mass_solve", self.h, d_b.ptr, d_x.ptr, ctypes.byref(opt), ctypes.byref(st), _s(stream))

@@ -332,8 +332,10 @@ typedef struct {
   int max_iter;      /* CG iterations (FP64) / per inner solve (mixed) */
   int max_outer;     /* refinement steps (mixed) / residual replacements (mixed reliable) */
   double rtol;       /* target ||b - M x|| / ||b|| (FP64 residual) */
-  double inner_rtol; /* inner FP32 solve reduction per refinement step (1e-4 works well) / residual drop
-                        between replacements (1e-2 keeps the FP64 iteration count) */
+  double inner_rtol; /* inner FP32 solve reduction per refinement step / residual drop between
+                        replacements; <= 0: 1e-5 (MIXED: two steps to 1e-10) / 1e-2 (MIXED_RELIABLE:
+                        keeps the FP64 iteration count). A refinement step never goes below half
+                        of what the target still needs. */
 } fem_hip_solve_options;
 typedef struct {
   int iterations;    /* total CG iterations (inner ones for mixed) */

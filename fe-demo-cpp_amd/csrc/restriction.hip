@@ -153,8 +153,8 @@ __global__ __launch_bounds__(kThreads) void strided_scatter_kernel(size_t e_size
 
 // Transpose map on the device, entry for entry the host build_transpose_map's:
 // per-node counts (atomics), the used nodes and the row starts (exclusive
-// scans), an unordered fill by atomic cursors, then every row sorted ascending
-// (a row holds the elements' local nodes that touch one L-node: a handful).
+// scans), and the entries from a stable radix sort of the (offset, t) pairs,
+// which keeps t ascending within each row for any row length.
 __global__ __launch_bounds__(kThreads) void tmap_count_kernel(size_t nt, const int* __restrict__ off,
                                                              int* __restrict__ count) {
   const size_t t = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
@@ -179,29 +179,9 @@ __global__ __launch_bounds__(kThreads) void tmap_rows_kernel(int l_size, const i
   t_offsets[m] = start[l];
 }
 
-__global__ __launch_bounds__(kThreads) void tmap_fill_kernel(size_t nt, const int* __restrict__ off,
-                                                            const int* __restrict__ start, int* __restrict__ cursor,
-                                                            int* __restrict__ t_indices) {
-  const size_t t = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (t >= nt) return;
-  const int l = off[t];
-  t_indices[start[l] + atomicAdd(&cursor[l], 1)] = static_cast<int>(t);
-}
-
-__global__ __launch_bounds__(kThreads) void tmap_sort_kernel(int rows, const int* __restrict__ t_offsets,
-                                                            int* __restrict__ t_indices) {
-  const int m = blockIdx.x * kThreads + threadIdx.x;
-  if (m >= rows) return;
-  const int b = t_offsets[m], e = t_offsets[m + 1];
-  for (int i = b + 1; i < e; ++i) {  // insertion sort: the t's are distinct
-    const int x = t_indices[i];
-    int j = i - 1;
-    while (j >= b && t_indices[j] > x) {
-      t_indices[j + 1] = t_indices[j];
-      --j;
-    }
-    t_indices[j + 1] = x;
-  }
+__global__ __launch_bounds__(kThreads) void iota_kernel(size_t n, int* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) out[i] = static_cast<int>(i);
 }
 
 // Exclusive prefix sum of in[0..n) into out[0..n] (out[n] = the total).
@@ -238,11 +218,20 @@ void build_transpose_map_device(Restriction& r, hipStream_t s) {
       r.t_offsets.data());
   HIP_CHECK_LAUNCH();
   HIP_CHECK(hipMemcpyAsync(r.t_offsets.data() + rows, start.data() + L, sizeof(int), hipMemcpyDeviceToDevice, s));
-  zero_fill(count.data(), static_cast<size_t>(L) * sizeof(int), s);  // now the fill cursors
-  tmap_fill_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, r.offsets.data(), start.data(), count.data(),
-                                                              r.t_indices.data());
-  tmap_sort_kernel<<<grid_for(rows, kThreads), kThreads, 0, s>>>(rows, r.t_offsets.data(), r.t_indices.data());
-  HIP_CHECK_LAUNCH();
+  {
+    // Entries: t sorted by offset, stably (t ascending within a row).
+    StreamTmp<int> t_in(nt, s), keys_out(nt, s);
+    iota_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, t_in.data());
+    HIP_CHECK_LAUNCH();
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) <= static_cast<long long>(r.max_offset)) ++bits;
+    size_t bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, r.offsets.data(), keys_out.data(), t_in.data(),
+                                                 r.t_indices.data(), static_cast<int>(nt), 0, bits, s));
+    StreamTmp<unsigned char> tmp(bytes, s);
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.data(), bytes, r.offsets.data(), keys_out.data(), t_in.data(),
+                                                 r.t_indices.data(), static_cast<int>(nt), 0, bits, s));
+  }
   HIP_CHECK(hipStreamSynchronize(s));
 }
 

@@ -76,7 +76,8 @@ def test_deterministic_bitwise_reproducible(gpu):
 def transpose_map_cases():
     """(name, ne, es, nc, cs, l_size, offsets): a small interleaved box, a
     permuted L-shape (every node used, irregular valence), the same with every
-    other L-node unused, and a 0.9M-entry box with blocked components."""
+    other L-node unused, a 0.9M-entry box with blocked components, and a hub
+    node that 20,000 entries map to (one very long row)."""
     from fehip.meshes import l_shape, permute
     yield ("box interleaved",) + tuple(box_restriction_args(2, 2, 1, 2, 3, True))
     m = permute(l_shape(6, 3), seed=4)
@@ -86,9 +87,14 @@ def transpose_map_cases():
     n, P = 24, 4
     N = (n * (P - 1) + 1) ** 3
     yield "box 24^3 P=4", n ** 3, P ** 3, 3, N, 3 * N, fehip.box_offsets(n, n, n, P, 1)
+    rng = np.random.default_rng(11)
+    ne, es, l_size = 20000, 8, 5000
+    off = rng.integers(1, l_size, ne * es).astype(np.int32)
+    off[::es] = 0
+    yield "hub node", ne, es, 1, l_size, l_size, off
 
 
-@pytest.mark.parametrize("case", list(range(4)))
+@pytest.mark.parametrize("case", list(range(5)))
 def test_transpose_map_on_device_matches_oracle(gpu, case):
     """The deterministic restriction builds its map on the device (atomic
     counts, scans, atomic fill, per-row sort); it equals the CPU counting sort

@@ -37,8 +37,9 @@ inline bool alg_box(int alg) { return alg == kAlgBoxColumn; }
 // Element nodes (num_elem * P^3 * NC) from which an untuned box operator
 // defaults to the box columns. Below it there are too few column segments of
 // enough layers to fill the GPU: on MI355X the box columns lost to the element
-// and block algorithms at 4-7M element nodes and won at 12-33M.
-constexpr long long kBoxMinNodes = 8LL << 20;
+// and block algorithms at 0.2-0.4M element nodes and won from about 1M
+// (P = 3..6, one component).
+constexpr long long kBoxMinNodes = 1LL << 20;
 inline bool alg_tmap(int alg) { return alg == kAlgElemTmap || alg == kAlgElemTmapPipe; }
 inline bool alg_blocked(int alg) {
   return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds || alg == kAlgBlockPartLdsPipe;

@@ -76,7 +76,7 @@ def test_box_column_matches_oracle(gpu, n3, P, Q, nc, elems, layers):
 
 
 def test_box_column_is_the_default_on_large_boxes(gpu):
-    """Without a tuning row, a deterministic box operator of at least 8 Mi
+    """Without a tuning row, a deterministic box operator of at least 1 Mi
     element nodes runs the box columns, a smaller one does not (too few column
     segments to fill the GPU); a layer count that does not divide nz makes it
     fall back."""
@@ -87,10 +87,10 @@ def test_box_column_is_the_default_on_large_boxes(gpu):
     small.op.set_fused_algorithm(BOX)
     assert small.op.fused_choice()[0] == BOX
     del small
-    mesh = rect_mesh(44, 44, 44, 5)  # 85,184 elements, 10.6M element nodes
+    mesh = rect_mesh(24, 24, 24, 5)  # 13,824 elements, 1.7M element nodes
     p = MeshProblem(mesh, 6, 1, scatter=fehip.SCATTER_DETERMINISTIC)
     assert p.op.fused_choice()[0] == BOX
-    p.op.set_box_layers(3)
+    p.op.set_box_layers(5)  # does not divide nz = 24
     assert p.op.fused_choice()[0] != BOX
     p.op.set_fused_algorithm(BOX)
     with pytest.raises(ValueError):

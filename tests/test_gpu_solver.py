@@ -64,3 +64,29 @@ def test_solve_repeatable(gpu):
         x1 = d_x.to_host()
         p.op.solve(d_b, d_x, mode=mode)
         assert d_x.to_host().tobytes() == x1.tobytes(), mode
+
+
+@pytest.mark.parametrize("n,P,Q,nc", [(16, 3, 4, 1), (8, 4, 5, 2)])
+def test_solves_with_box_columns(gpu, n, P, Q, nc):
+    """Solves on the box-column algorithm (the solver's apply on structured boxes
+    of 1M element nodes and up): FP64 takes the oracle CG's iteration count
+    (within 2), every mode reaches 1e-10 and the oracle's solution, and a second
+    solve repeats the first bit for bit."""
+    print("box-column solves", n, P, Q, nc, flush=True)
+    p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
+    p.op.set_fused_algorithm(fehip.FUSED_BOX_COLUMN)
+    assert p.op.fused_choice()[0] == fehip.FUSED_BOX_COLUMN
+    assert p.op.fused_choice(fp32=True)[0] == fehip.FUSED_BOX_COLUMN
+    orc = oracle().mass(3, p.num_elem, P, Q, nc, p.l_size, p.num_nodes, p.offsets, p.qdata())
+    b = orc.apply(fehip.rng_uniform(2, p.l_size))
+    x_cpu, it_cpu, rel_cpu = orc.cg(b, orc.diagonal(), rtol=1e-10)
+    d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
+    for mode in (fehip.SOLVE_FP64, fehip.SOLVE_MIXED, fehip.SOLVE_MIXED_RELIABLE):
+        st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+        x1 = d_x.to_host()
+        assert st["rel_residual"] <= 1e-10, (mode, st)
+        assert rel_err(x1, x_cpu) < 1e-8, mode
+        if mode == fehip.SOLVE_FP64:
+            assert abs(st["iterations"] - it_cpu) <= 2, (st, it_cpu)
+        p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+        assert d_x.to_host().tobytes() == x1.tobytes(), mode

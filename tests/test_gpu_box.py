@@ -39,11 +39,20 @@ def rect_mesh(nx, ny, nz, P, jitter=0.1, seed=3):
             "vertex_offsets": fehip.box_offsets(nx, ny, nz, 2, 1), "coords": coords.ravel()}
 
 
-# (box, P, Q, num_comp, elems per column layer, layers per workgroup to try)
+# (box, P, Q, num_comp, elems per column layer, layers per workgroup to try):
+# every cross-section at P=5, and one compiled variant of each other shape
+# (P = 2, 3, 4, 6, 8; 1 to 3 components).
 CASES = [((8, 8, 8), 5, 6, 1, 16, (0, 1, 2, 8)),
          ((8, 8, 6), 5, 6, 1, 8, (0, 3)),
          ((4, 6, 6), 5, 6, 1, 4, (0, 2, 6)),
-         ((4, 6, 4), 5, 6, 3, 8, (0, 2))]
+         ((4, 6, 4), 5, 6, 3, 8, (0, 2)),
+         ((4, 4, 4), 2, 3, 3, 8, (0, 2)),
+         ((8, 4, 4), 2, 3, 1, 16, (0, 2)),
+         ((4, 4, 4), 3, 4, 2, 8, (0, 1)),
+         ((4, 4, 4), 4, 5, 3, 4, (0, 2)),
+         ((4, 4, 2), 6, 7, 1, 8, (0,)),
+         ((2, 2, 4), 8, 10, 3, 4, (0, 2)),
+         ((4, 2, 2), 8, 10, 1, 4, (0,))]
 
 
 @pytest.mark.parametrize("n3,P,Q,nc,elems,layers", CASES)

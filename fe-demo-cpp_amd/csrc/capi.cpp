@@ -685,6 +685,12 @@ int fem_hip_rng_uniform(uint64_t seed, size_t n, double lo, double hi, double* o
     rng_uniform(seed, n, lo, hi, out);
   });
 }
+int fem_hip_rng_uniform_device(uint64_t seed, size_t n, double lo, double hi, double* out_device, void* stream) {
+  return guard([&] {
+    if (n) need(out_device, "out_device");
+    rng_uniform_device(seed, n, lo, hi, out_device, static_cast<hipStream_t>(stream));
+  });
+}
 int fem_hip_box_offsets(int nx, int ny, int nz, int P, int node_stride, int* offsets) {
   return guard([&] {
     need(offsets, "offsets");

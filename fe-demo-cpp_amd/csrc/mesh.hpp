@@ -12,6 +12,8 @@ struct SplitMix64 {
 };
 
 void rng_uniform(uint64_t seed, size_t n, double lo, double hi, double* out);
+// The same stream into device memory out[0..n), enqueued on s.
+void rng_uniform_device(uint64_t seed, size_t n, double lo, double hi, double* out, hipStream_t s);
 void box_offsets(int nx, int ny, int nz, int P, int node_stride, int* offsets);
 void box_coords(int n, double jitter, uint64_t seed, double* coords);
 

@@ -349,9 +349,8 @@ class MassServer {
         j->op = build_problem(j->req, j->stream);
         const size_t L = static_cast<size_t>(j->op->active_in_l_size);
         j->res.l_size = static_cast<long long>(L);
-        std::vector<double> rhs(L);
-        rng_uniform(j->req.rhs_seed, L, -1.0, 1.0, rhs.data());
-        j->b.upload(rhs);
+        j->b.allocate(L);  // the rhs stream generated in place, no host copy
+        rng_uniform_device(j->req.rhs_seed, L, -1.0, 1.0, j->b.data(), j->stream);
         j->x.allocate(L);
         FusedMass& f = operator_fused_mass(*j->op, j->stream);
         if (opt_.autotune) maybe_tune(f, j->req.mode != 0, j->stream);

@@ -167,6 +167,7 @@ SIGNATURES = {
     "fem_hip_operator_destroy": [_vp],
     "fem_hip_operator_create_mass": [_vp, _vp, _vp, ctypes.POINTER(_vp)],
     "fem_hip_rng_uniform": [ctypes.c_uint64, _sz, _d, _d, _vp],
+    "fem_hip_rng_uniform_device": [ctypes.c_uint64, _sz, _d, _d, _vp, _vp],
     "fem_hip_box_offsets": [_i, _i, _i, _i, _i, _vp],
     "fem_hip_box_coords": [_i, _d, ctypes.c_uint64, _vp],
     "fem_hip_block_plan_stats": [_i, _i, _i, _vp, _i, _i, _i, _i, _llp],
@@ -363,6 +364,13 @@ def lagrange_basis_matrix(nodes, pts):
 def rng_uniform(seed, n, lo=-1.0, hi=1.0):
     out = np.empty(n)
     call("fem_hip_rng_uniform", seed, n, lo, hi, _ptr(out))
+    return out
+
+
+def rng_uniform_device(seed, n, lo=-1.0, hi=1.0, stream=None):
+    """rng_uniform's stream, generated on the device into a new DeviceArray."""
+    out = DeviceArray(n)
+    call("fem_hip_rng_uniform_device", seed, n, lo, hi, out.ptr, stream)
     return out
 
 

@@ -285,6 +285,10 @@ int fem_hip_operator_create_mass(fem_hip_restriction r, fem_hip_basis b, const d
 /* ---- synthetic structured meshes (host only) ---------------------------------- */
 /* splitmix64 stream: out[i] = lo + (hi-lo) * ((x_i >> 11) * 2^-53). */
 int fem_hip_rng_uniform(uint64_t seed, size_t n, double lo, double hi, double* out);
+/* The same stream, bit for bit, written to device memory out_device[0..n) by a
+ * kernel enqueued on stream (element i depends only on seed and i). No
+ * counterpart in the reference, which reads its vectors from the host. */
+int fem_hip_rng_uniform_device(uint64_t seed, size_t n, double lo, double hi, double* out_device, void* stream);
 /* Lexicographic offsets of an nx*ny*nz box of P^3-node hexes on the (nx(P-1)+1)x.. grid,
  * offsets (gx + Nx*(gy + Ny*gz)) * node_stride (tests/cuda/test-cuda-elem-restriction.cu:24-42). */
 int fem_hip_box_offsets(int nx, int ny, int nz, int P, int node_stride, int* offsets);

@@ -49,3 +49,12 @@ def test_errors_forced_from_host(gpu):
     with pytest.raises(fehip.FemHipError):
         fehip.call("fem_hip_probe_launch", 1, 4096)
     fehip.call("fem_hip_probe_launch", 64, 256)
+
+
+@pytest.mark.parametrize("seed,n,lo,hi", [(1, 0, -1.0, 1.0), (0, 1, 0.0, 1.0), (2**64 - 1, 257, -1.0, 1.0),
+                                          (12345, 100003, 2.5, -3.0), (7, 5_000_000, -1.0, 1.0)])
+def test_device_rng_equals_host_stream(gpu, seed, n, lo, hi):
+    """The device splitmix64 stream is the host one bit for bit (5M exceeds one grid pass)."""
+    d = fehip.rng_uniform_device(seed, n, lo, hi)
+    fehip.synchronize()
+    assert d.to_host().tobytes() == fehip.rng_uniform(seed, n, lo, hi).tobytes()

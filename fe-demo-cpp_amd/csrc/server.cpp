@@ -63,12 +63,48 @@ size_t serve_request_bytes(const ServeRequest& r) {
 
 namespace {
 
+// Streams reused across jobs: creating one takes milliseconds while other
+// jobs' work runs on the device.
+class StreamPool {
+ public:
+  ~StreamPool() {
+    for (hipStream_t s : free_) {
+      (void)hipStreamSynchronize(s);
+      SharedCapture lk;
+      (void)hipStreamDestroy(s);
+    }
+  }
+  hipStream_t acquire() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      if (!free_.empty()) {
+        hipStream_t s = free_.back();
+        free_.pop_back();
+        return s;
+      }
+    }
+    hipStream_t s = nullptr;
+    SharedCapture lk;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+  }
+  void release(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(m_);
+    free_.push_back(s);
+  }
+
+ private:
+  std::mutex m_;
+  std::vector<hipStream_t> free_;
+};
+
 struct Job {
   long long id = 0;
   ServeRequest req;
   size_t bytes = 0;
   Clock::time_point t_submit, t_admit, t_ready, t_done;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // from `streams`
+  StreamPool* streams = nullptr;
   std::unique_ptr<Operator> op;
   DevBuf<double> b, x;
   MassSolverState* state = nullptr;
@@ -82,8 +118,7 @@ struct Job {
     x.release();
     if (stream) {
       (void)hipStreamSynchronize(stream);
-      SharedCapture lk;
-      (void)hipStreamDestroy(stream);
+      streams->release(stream);
     }
   }
 };
@@ -342,10 +377,8 @@ class MassServer {
     bool ok = true;
     try {
       {
-        {
-          SharedCapture lk;
-          HIP_CHECK(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
-        }
+        j->stream = streams_.acquire();
+        j->streams = &streams_;
         j->op = build_problem(j->req, j->stream);
         const size_t L = static_cast<size_t>(j->op->active_in_l_size);
         j->res.l_size = static_cast<long long>(L);
@@ -459,6 +492,7 @@ class MassServer {
   }
 
   ServerOptions opt_;
+  StreamPool streams_;  // destroyed after every job (members go in reverse order)
   int device_ = 0;
   size_t budget_ = 0, used_ = 0;
   std::mutex mu_;

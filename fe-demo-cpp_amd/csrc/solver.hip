@@ -33,6 +33,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "solver.hpp"
@@ -571,6 +572,43 @@ namespace {
 // Pinned host words the device reports into: the CG scalars, ||b||^2, ||b - Mx||^2.
 enum { kHostB = kNSCAL, kHostRes, kHostN };
 
+// Pinned host slots for those words, kept for the life of the process and
+// reused across solves: hipHostFree waits for the whole device, which would
+// stall every other stream's solve each time one finished.
+class PinnedSlots {
+ public:
+  double* acquire() {
+    std::lock_guard<std::mutex> lk(m_);
+    if (free_.empty()) {
+      char* chunk = nullptr;
+      {
+        SharedCapture cl;
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&chunk), kChunk * kSlotBytes, hipHostMallocDefault));
+      }
+      for (int i = kChunk - 1; i >= 0; --i) free_.push_back(reinterpret_cast<double*>(chunk + i * kSlotBytes));
+    }
+    double* p = free_.back();
+    free_.pop_back();
+    return p;
+  }
+  void release(double* p) {
+    std::lock_guard<std::mutex> lk(m_);
+    free_.push_back(p);
+  }
+
+ private:
+  static constexpr int kChunk = 256;
+  static constexpr size_t kSlotBytes = 128;
+  static_assert(kHostN * sizeof(double) <= kSlotBytes, "a slot holds the reported words");
+  std::mutex m_;
+  std::vector<double*> free_;
+};
+
+PinnedSlots& pinned_slots() {
+  static PinnedSlots* slots = new PinnedSlots();  // never destroyed: the runtime may be gone at exit
+  return *slots;
+}
+
 }  // namespace
 
 struct SolveJob {
@@ -600,8 +638,8 @@ struct SolveJob {
   ~SolveJob() {
     if (ev) (void)hipEventDestroy(ev);
     if (hs) {
-      SharedCapture lk;
-      (void)hipHostFree(hs);
+      if (s) (void)hipStreamSynchronize(s);  // no copy into the slot is still pending
+      pinned_slots().release(hs);
     }
   }
 };
@@ -848,10 +886,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   j->s = s;
   j->batch = opt.batch > 0 ? opt.batch : 8;
   HIP_CHECK(hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));
-  {
-    SharedCapture lk;
-    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&j->hs), kHostN * sizeof(double), hipHostMallocDefault));
-  }
+  j->hs = pinned_slots().acquire();
   for (int i = 0; i < kHostN; ++i) j->hs[i] = 0.0;
   return j.release();
 }

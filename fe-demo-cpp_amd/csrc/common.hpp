@@ -10,6 +10,7 @@
 #include <string>
 #include <utility>
 #include <condition_variable>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -76,6 +77,56 @@ struct ExclusiveCapture {
   ExclusiveCapture& operator=(const ExclusiveCapture&) = delete;
 };
 
+// Device blocks kept for reuse by a long-lived owner (the solve server):
+// hipFree waits for every stream of the device, so a server that freed each
+// job's buffers would stall all other jobs' solves. take() hands out a cached
+// block at most 1/8 (+64 KiB) larger than asked (best fit), else hipMalloc's a
+// new one; on an out-of-memory failure it frees the cache and tries once more.
+// give() caches a block; past max_cached bytes it frees the largest ones.
+// A block must be idle when given back (DevBuf synchronizes its stream first).
+class BlockCache {
+ public:
+  explicit BlockCache(size_t max_cached) : max_(max_cached) {}
+  ~BlockCache() { trim(0); }
+  BlockCache(const BlockCache&) = delete;
+  BlockCache& operator=(const BlockCache&) = delete;
+  void* take(size_t bytes, size_t* got);
+  void give(void* p, size_t bytes);
+  void trim(size_t keep);  // frees cached blocks, largest first, down to `keep` bytes
+  size_t cached_bytes();
+
+ private:
+  std::mutex m_;
+  std::multimap<size_t, void*> free_;
+  size_t cached_ = 0, max_;
+};
+
+// While a CachedAllocScope is alive on a thread, DevBuf allocations on that
+// thread come from `cache` and belong to `stream`: all work on them must be
+// ordered on that stream, which must outlive them; release synchronizes it and
+// gives the block back to the cache.
+struct CacheScope {
+  BlockCache* cache;
+  hipStream_t stream;
+};
+inline const CacheScope*& cache_scope() {
+  static thread_local const CacheScope* scope = nullptr;
+  return scope;
+}
+class CachedAllocScope {
+ public:
+  CachedAllocScope(BlockCache* cache, hipStream_t s) : scope_{cache, s}, prev_(cache_scope()) {
+    cache_scope() = &scope_;
+  }
+  ~CachedAllocScope() { cache_scope() = prev_; }
+  CachedAllocScope(const CachedAllocScope&) = delete;
+  CachedAllocScope& operator=(const CachedAllocScope&) = delete;
+
+ private:
+  CacheScope scope_;
+  const CacheScope* prev_;
+};
+
 // Owner of one device allocation (move-only), the counterpart of the reference's
 // DeviceArray (include/cuda/device-array.hpp:18-95) without the synchronous copies.
 template <class T>
@@ -86,31 +137,49 @@ class DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) {
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), cache_(o.cache_), s_(o.s_), bytes_(o.bytes_) {
     o.p_ = nullptr;
     o.n_ = 0;
+    o.cache_ = nullptr;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       p_ = o.p_;
       n_ = o.n_;
+      cache_ = o.cache_;
+      s_ = o.s_;
+      bytes_ = o.bytes_;
       o.p_ = nullptr;
       o.n_ = 0;
+      o.cache_ = nullptr;
     }
     return *this;
   }
   void allocate(size_t n) {
     release();
     if (n == 0) return;
+    if (const CacheScope* cs = cache_scope()) {
+      p_ = static_cast<T*>(cs->cache->take(n * sizeof(T), &bytes_));
+      cache_ = cs->cache;
+      s_ = cs->stream;
+      n_ = n;
+      return;
+    }
     SharedCapture lk;
     HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)));
     n_ = n;
   }
   void upload(const T* host, size_t n) {
     allocate(n);
+    if (n == 0) return;
+    if (cache_) {  // in the order of the block's stream, complete on return
+      HIP_CHECK(hipMemcpyAsync(p_, host, n * sizeof(T), hipMemcpyHostToDevice, s_));
+      HIP_CHECK(hipStreamSynchronize(s_));
+      return;
+    }
     SharedCapture lk;
-    if (n) HIP_CHECK(hipMemcpy(p_, host, n * sizeof(T), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(p_, host, n * sizeof(T), hipMemcpyHostToDevice));
   }
   void upload(const std::vector<T>& host) { upload(host.data(), host.size()); }
   std::vector<T> download() const {
@@ -124,17 +193,24 @@ class DevBuf {
   size_t size() const { return n_; }
   bool empty() const { return n_ == 0; }
   void release() noexcept {
-    if (p_) {
+    if (p_ && cache_) {
+      (void)hipStreamSynchronize(s_);  // idle before another job may take it
+      cache_->give(p_, bytes_);
+    } else if (p_) {
       SharedCapture lk;
       (void)hipFree(p_);
     }
     p_ = nullptr;
     n_ = 0;
+    cache_ = nullptr;
   }
 
  private:
   T* p_ = nullptr;
   size_t n_ = 0;
+  BlockCache* cache_ = nullptr;  // set when the block came from a BlockCache
+  hipStream_t s_ = nullptr;      // the stream a cached block belongs to
+  size_t bytes_ = 0;             // the cached block's size
 };
 
 // Stream-ordered temporary (hipMallocAsync/hipFreeAsync on the caller's stream),

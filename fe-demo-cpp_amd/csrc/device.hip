@@ -58,6 +58,67 @@ CaptureLock& capture_lock() {
   return l;
 }
 
+void* BlockCache::take(size_t bytes, size_t* got) {
+  const size_t want = (bytes + 255) & ~static_cast<size_t>(255);
+  {
+    std::lock_guard<std::mutex> lk(m_);
+    auto it = free_.lower_bound(want);
+    if (it != free_.end() && it->first <= want + want / 8 + (static_cast<size_t>(64) << 10)) {
+      void* p = it->second;
+      *got = it->first;
+      cached_ -= it->first;
+      free_.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e;
+  {
+    SharedCapture lk;
+    e = hipMalloc(&p, want);
+  }
+  if (e == hipErrorOutOfMemory) {  // cached blocks may be what is missing
+    (void)hipGetLastError();
+    trim(0);
+    SharedCapture lk;
+    e = hipMalloc(&p, want);
+  }
+  HIP_CHECK(e);
+  *got = want;
+  return p;
+}
+
+void BlockCache::give(void* p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(m_);
+    free_.emplace(bytes, p);
+    cached_ += bytes;
+    if (cached_ <= max_) return;
+  }
+  trim(max_);
+}
+
+void BlockCache::trim(size_t keep) {
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> lk(m_);
+    while (cached_ > keep && !free_.empty()) {
+      auto it = std::prev(free_.end());
+      cached_ -= it->first;
+      drop.push_back(it->second);
+      free_.erase(it);
+    }
+  }
+  if (drop.empty()) return;
+  SharedCapture lk;
+  for (void* p : drop) (void)hipFree(p);
+}
+
+size_t BlockCache::cached_bytes() {
+  std::lock_guard<std::mutex> lk(m_);
+  return cached_;
+}
+
 void zero_fill(void* p, size_t bytes, hipStream_t s) {
   if (bytes == 0) return;
   // Allocations are at least 256-byte aligned; callers pass whole arrays.

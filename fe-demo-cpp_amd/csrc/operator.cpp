@@ -163,6 +163,9 @@ void operator_set_field(Operator& op, const std::string& name, std::shared_ptr<c
     SharedCapture lk;
     HIP_CHECK(hipMemcpy(f.passive.data(), passive, passive_len * sizeof(double),
                         passive_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    // A device-to-device hipMemcpy may return before the copy is done: finish it,
+    // so the caller may reuse the source and other streams may read the copy.
+    if (passive_on_device) HIP_CHECK(hipStreamSynchronize(nullptr));
   }
   if (vkind == kVectorActive) (is_input ? op.active_in_l_size : op.active_out_l_size) = f.rstr->l_size;
   op.num_qpts = nq;

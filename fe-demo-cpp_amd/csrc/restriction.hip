@@ -357,6 +357,7 @@ Restriction* restriction_clone(const Restriction& src) {
       to.allocate(from.size());
       SharedCapture lk;
       if (from.size()) HIP_CHECK(hipMemcpy(to.data(), from.data(), from.size() * sizeof(int), hipMemcpyDeviceToDevice));
+      if (from.size()) HIP_CHECK(hipStreamSynchronize(nullptr));  // the copy is done, not only enqueued
     };
     dup(src.offsets, r->offsets);
     dup(src.l_vec_indices, r->l_vec_indices);

@@ -197,6 +197,9 @@ class MassServer {
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     budget_ = static_cast<size_t>(static_cast<double>(free_b) * o.hbm_fraction);
     HIP_CHECK(hipGetDevice(&device_));
+    // The jobs' device buffers come from a block cache: at most half the budget
+    // stays cached between jobs.
+    cache_ = std::make_unique<BlockCache>(budget_ / 2);
     scheduler_ = std::thread([this] { scheduler_loop(); });
     for (int i = 0; i < o.workers; ++i) workers_.emplace_back([this] { worker_loop(); });
   }
@@ -379,6 +382,7 @@ class MassServer {
       {
         j->stream = streams_.acquire();
         j->streams = &streams_;
+        CachedAllocScope cached(cache_.get(), j->stream);
         j->op = build_problem(j->req, j->stream);
         const size_t L = static_cast<size_t>(j->op->active_in_l_size);
         j->res.l_size = static_cast<long long>(L);
@@ -492,7 +496,8 @@ class MassServer {
   }
 
   ServerOptions opt_;
-  StreamPool streams_;  // destroyed after every job (members go in reverse order)
+  StreamPool streams_;  // these two outlive every job (members go in reverse order)
+  std::unique_ptr<BlockCache> cache_;
   int device_ = 0;
   size_t budget_ = 0, used_ = 0;
   std::mutex mu_;

@@ -133,6 +133,19 @@ __host__ __device__ inline int box_owners(const BoxGeom& g, int P, int BX, int B
   return n;
 }
 
+// qdata is read exactly once per apply. FP64 loads it as streaming
+// (nontemporal) data, which leaves the caches to the u rows that neighbouring
+// elements and columns read again: 64^3, P=5 FP64 box kernel 0.192 -> 0.177 ms
+// on MI355X. FP32 measured the same either way and keeps plain loads.
+template <class T>
+__device__ __forceinline__ T load_qdata(const T* p) {
+  if constexpr (sizeof(T) == 8) {
+    return __builtin_nontemporal_load(p);
+  } else {
+    return *p;
+  }
+}
+
 // Even-odd form of a symmetric 1D interpolation matrix B [Q][P]
 // (B(Q-1-q, P-1-p) = B(q, p), true for Lagrange bases on symmetric nodes at
 // symmetric points): with x_e = x_j + x_(P-1-j), x_o = x_j - x_(P-1-j),
@@ -326,7 +339,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   if (line_thread) {
     const T* r = qd_ptr(sz * g.lz);
 #pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qc[qz] = r[qz * Q2];
+    for (int qz = 0; qz < Q; ++qz) qc[qz] = load_qdata(r + qz * Q2);
   }
   if constexpr (UM == 2) __syncthreads();
   // Nothing loaded above is still in flight when the layer loop starts. (Those
@@ -392,7 +405,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
         const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ezq);
         const T* r = qdata + e * Q3 + tq;
 #pragma unroll
-        for (int qz = 0; qz < Q; ++qz) qn[qz] = r[qz * Q2];
+        for (int qz = 0; qz < Q; ++qz) qn[qz] = load_qdata(r + qz * Q2);
       }
       if (row_thread) {
         const int iz = riz, iy = riy;

@@ -65,6 +65,28 @@ def time_applies(fehip, op, d_u, d_v, steps, warmup, stream):
     return total_ms / steps, k_ms / max(n, 1), aux_ms / max(n, 1)
 
 
+def copy_ceiling(fehip, stream, n=1 << 27, reps=20):
+    """The device's streaming bandwidth: an n-double (1 GiB) array copied by the
+    streaming copy kernel (one 16-byte nontemporal load and store per thread),
+    each copy timed alone with HIP events on `stream`; the median's read plus
+    written bytes per second."""
+    a, b = fehip.DeviceArray(n), fehip.DeviceArray(n)
+    a.zero(stream.h)
+    b.zero(stream.h)
+    for _ in range(5):
+        fehip.stream_copy(b, a, stream)
+    e0, e1 = fehip.Event(), fehip.Event()
+    times = []
+    for _ in range(reps):
+        e0.record(stream)
+        fehip.stream_copy(b, a, stream)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    ms = float(np.median(times))
+    return {"GBs": 16.0 * n / (ms * 1e-3) / 1e9, "bytes_moved": 16 * n, "ms": ms}
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if it holds
@@ -342,6 +364,8 @@ def main():
 
     k_ms = best["kernel_ms"]
     achieved = best["kernel_bytes"] / (k_ms * 1e-3) / 1e9
+    copy = copy_ceiling(fehip, stream)
+    log("streaming copy ceiling:", json.dumps(copy))
     line = {
         "metric": "mass-apply GDoF/s & %HBM roofline, FP64 vs mixed; p50/p99 solve latency",
         "value": L / ms / 1e6,
@@ -363,7 +387,8 @@ def main():
                      "kernel": best["kernel"],
                      "kernel_ms": k_ms, "algorithmic_bytes_per_launch": best["kernel_bytes"],
                      "apply_algorithmic_bytes": bytes64, "apply_frac": best["apply_frac_hbm"],
-                     "other_stages_ms": best["other_stages_ms"]},
+                     "other_stages_ms": best["other_stages_ms"],
+                     "copy_GBs": copy["GBs"], "frac_of_copy": achieved / copy["GBs"]},
         "cpu_baseline": cpu,
         "variants": variants,
         "tuning": tuning,

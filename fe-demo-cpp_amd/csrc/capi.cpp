@@ -169,6 +169,9 @@ int fem_hip_memcpy_dtod(void* dst, const void* src, size_t bytes, void* s) {
     if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st(s)));
   });
 }
+int fem_hip_stream_copy(void* dst, const void* src, size_t bytes, void* s) {
+  return guard([&] { stream_copy(dst, src, bytes, st(s)); });
+}
 int fem_hip_memset_zero(void* dst, size_t bytes, void* s) {
   return guard([&] { zero_fill(dst, bytes, st(s)); });
 }

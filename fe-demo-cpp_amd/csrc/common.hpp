@@ -253,6 +253,7 @@ inline unsigned grid_for(size_t items, int threads) {
 // Fill n elements of p with zero on `s` (a kernel, not hipMemsetAsync, so it is
 // safe inside a captured hipGraph).
 void zero_fill(void* p, size_t bytes, hipStream_t s);
+void stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Restriction / basis / qfunction vocabulary shared by the translation units.
 enum : int { kNoTranspose = 0, kTranspose = 1 };

@@ -16,6 +16,17 @@ __global__ __launch_bounds__(256) void zero_kernel(uint4* __restrict__ p, size_t
   if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
 
+// dst[i] = src[i] over n16 16-byte words, one word per thread, nontemporal
+// both ways (the data is touched once).
+__global__ __launch_bounds__(256) void stream_copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                          size_t n16) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n16) return;
+  const d2 x = __builtin_nontemporal_load(reinterpret_cast<const d2*>(src) + i);
+  __builtin_nontemporal_store(x, reinterpret_cast<d2*>(dst) + i);
+}
+
 __global__ void probe_kernel(int* sink) {
   if (sink) sink[threadIdx.x] = 0;
 }
@@ -128,6 +139,18 @@ void zero_fill(void* p, size_t bytes, hipStream_t s) {
   const size_t want = (n16 + 255) / 256;
   const unsigned grid = static_cast<unsigned>(std::min<size_t>(std::max<size_t>(want, 1), 8192));
   zero_kernel<<<grid, 256, 0, s>>>(static_cast<uint4*>(p), n16, static_cast<unsigned char*>(p) + n16 * 16, ntail);
+  HIP_CHECK_LAUNCH();
+}
+
+void stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  require(dst != nullptr && src != nullptr, "stream_copy: pointers must not be null");
+  require(reinterpret_cast<uintptr_t>(dst) % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0 && bytes % 16 == 0,
+          "stream_copy: pointers and size must be multiples of 16 bytes");
+  const size_t n16 = bytes / 16, blocks = (n16 + 255) / 256;
+  require(blocks < (1ull << 31), "stream_copy: size too large");
+  stream_copy_kernel<<<static_cast<unsigned>(blocks), 256, 0, s>>>(static_cast<const double2*>(src),
+                                                                  static_cast<double2*>(dst), n16);
   HIP_CHECK_LAUNCH();
 }
 

@@ -168,6 +168,7 @@ SIGNATURES = {
     "fem_hip_operator_create_mass": [_vp, _vp, _vp, ctypes.POINTER(_vp)],
     "fem_hip_rng_uniform": [ctypes.c_uint64, _sz, _d, _d, _vp],
     "fem_hip_rng_uniform_device": [ctypes.c_uint64, _sz, _d, _d, _vp, _vp],
+    "fem_hip_stream_copy": [_vp, _vp, _sz, _vp],
     "fem_hip_box_offsets": [_i, _i, _i, _i, _i, _vp],
     "fem_hip_box_coords": [_i, _d, ctypes.c_uint64, _vp],
     "fem_hip_block_plan_stats": [_i, _i, _i, _vp, _i, _i, _i, _i, _llp],
@@ -365,6 +366,14 @@ def rng_uniform(seed, n, lo=-1.0, hi=1.0):
     out = np.empty(n)
     call("fem_hip_rng_uniform", seed, n, lo, hi, _ptr(out))
     return out
+
+
+def stream_copy(dst, src, stream=None):
+    """dst = src (device arrays of equal byte size, a multiple of 16) by the
+    streaming copy kernel."""
+    if dst.nbytes != src.nbytes:
+        raise InvalidArgument("stream_copy: arrays differ in size")
+    call("fem_hip_stream_copy", dst.ptr, src.ptr, src.nbytes, getattr(stream, "h", stream))
 
 
 def rng_uniform_device(seed, n, lo=-1.0, hi=1.0, stream=None):

@@ -99,6 +99,11 @@ int fem_hip_memcpy_htod(void* dst, const void* src, size_t bytes);
 int fem_hip_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int fem_hip_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 int fem_hip_memset_zero(void* dst, size_t bytes, void* stream); /* a fill kernel, graph-capture safe */
+/* dst = src by a streaming copy kernel (one 16-byte nontemporal load and store
+ * per thread); pointers and bytes must be multiples of 16. No counterpart in
+ * the reference: it measures the device's streaming bandwidth, the ceiling the
+ * fused kernels are compared with. */
+int fem_hip_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 int fem_hip_device_synchronize(void);
 int fem_hip_stream_create(void** stream);
 int fem_hip_stream_destroy(void* stream);

@@ -58,3 +58,19 @@ def test_device_rng_equals_host_stream(gpu, seed, n, lo, hi):
     d = fehip.rng_uniform_device(seed, n, lo, hi)
     fehip.synchronize()
     assert d.to_host().tobytes() == fehip.rng_uniform(seed, n, lo, hi).tobytes()
+
+
+def test_stream_copy(gpu):
+    # bit-equal copy, a size that is not a whole grid, and the 16-byte rule
+    host = fehip.rng_uniform(7, 1_000_001 * 2 + 6)
+    src = fehip.DeviceArray.from_host(host)
+    dst = fehip.DeviceArray(host.size)
+    s = fehip.Stream()
+    fehip.stream_copy(dst, src, s)
+    s.synchronize()
+    assert np.array_equal(dst.to_host().view(np.uint64), host.view(np.uint64))
+    odd_src, odd_dst = fehip.DeviceArray(3), fehip.DeviceArray(3)
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.stream_copy(odd_dst, odd_src, s)
+    with pytest.raises(fehip.InvalidArgument):
+        fehip.stream_copy(dst, odd_src, s)

@@ -41,15 +41,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_applies(fehip, op, d_u, d_v, steps, warmup, stream):
-    """(ms per apply, ms in the main kernel, ms in the other stages). The apply
-    time is the product path's: `steps` back-to-back applies between two
-    events. The split comes from a second run of `steps` applies with the
-    operator's stage events on (four events per apply, which add their own
-    gaps, so that run is not the apply time)."""
+# The GPU's clocks settle only after some load: on MI355X the first
+# milliseconds after an idle spell ran up to 20% slower. Every timed region is
+# therefore preceded by at least this much back-to-back work, whatever --warmup
+# asks for.
+WARM_S = 0.25
+
+
+def warm_up(op, d_u, d_v, warmup, stream, seconds=WARM_S):
+    """`warmup` applies, then more until `seconds` of wall time have passed."""
     for _ in range(warmup):
         op.apply(d_u, d_v, stream)
     stream.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(20):
+            op.apply(d_u, d_v, stream)
+        stream.synchronize()
+        n += 20
+    return n
+
+
+def time_applies(fehip, op, d_u, d_v, steps, warmup, stream):
+    """(ms per apply, ms in the main kernel, ms in the other stages). The apply
+    time is the product path's: `steps` back-to-back applies between two
+    events, after warm_up. The split comes from a second run of `steps`
+    applies with the operator's stage events on (four events per apply, which
+    add their own gaps, so that run is not the apply time)."""
+    warm_up(op, d_u, d_v, warmup, stream)
     e0, e1 = fehip.Event(), fehip.Event()
     e0.record(stream)
     for _ in range(steps):
@@ -98,28 +118,22 @@ def traffic_from_profile(kernel):
         return json.load(f).get("traffic_bytes_per_launch", {}).get(kernel)
 
 
-def cpu_baseline(problem, slab_layers=8, reps=3):
+def cpu_baseline(problem, v_gpu=None, slab_layers=8, reps=3):
     """The oracle's operator_apply (CPU restatement of the reference, one core) on a
-    z-slab of the same mesh: the first `slab_layers` element layers of 64^3."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import oracle
-
-    p = problem
-    n, P, Q = p.n, p.P, p.Q
-    E_s = n * n * slab_layers
-    N1 = n * (P - 1) + 1
-    l_s = N1 * N1 * (slab_layers * (P - 1) + 1)
-    off = p.offsets[: E_s * P ** 3]
-    qd = p.qdata().reshape(Q ** 3, p.num_elem)[:, :E_s].copy().ravel()
-    op = oracle().mass(3, E_s, P, Q, 1, l_s, l_s, off, qd)
+    z-slab of the same mesh: the first `slab_layers` element layers of 64^3, on
+    the slab's part of the same u (splitmix64 seed 1). With v_gpu (the device
+    apply of the whole mesh) also the device's error against it on the slab's
+    interior rows."""
     import fehip
 
+    p = problem
+    op, take, l_s, rows = slab_oracle(p, slab_layers)
     u = fehip.rng_uniform(1, l_s)
     # One core, as the reference runs (single-threaded): pin to the first allowed CPU.
     allowed = os.sched_getaffinity(0)
     os.sched_setaffinity(0, {min(allowed)})
     try:
-        op.apply(u)
+        v = op.apply(u)
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -128,11 +142,126 @@ def cpu_baseline(problem, slab_layers=8, reps=3):
     finally:
         os.sched_setaffinity(0, allowed)
     t = float(np.median(times))
-    return {"value": l_s / t / 1e9, "unit": "GDoF/s", "cores": 1, "kind": "port",
-            "sample": "oracle operator_apply (single-threaded CPU restatement of the reference, -O2 "
-                      "-ffp-contract=off) on a 64x64x%d-element z-slab of the 64^3 P=5 Q=6 mesh (%d DoF), "
-                      "median of %d applies pinned to one core: %.3f s/apply; host %s, nproc %d" %
-                      (slab_layers, l_s, reps, t, platform.processor() or platform.machine(), os.cpu_count())}
+    out = {"value": l_s / t / 1e9, "unit": "GDoF/s", "cores": 1, "kind": "port",
+           "sample": "oracle operator_apply (single-threaded CPU restatement of the reference, -O2 "
+                     "-ffp-contract=off) on a 64x64x%d-element z-slab of the 64^3 P=5 Q=6 mesh (%d DoF), "
+                     "median of %d applies pinned to one core: %.3f s/apply; host %s, nproc %d" %
+                     (slab_layers, l_s, reps, t, platform.processor() or platform.machine(), os.cpu_count())}
+    if v_gpu is not None:
+        out["gpu_rel_err_on_slab"] = slab_rel_err(v_gpu, v, l_s, rows, 1, p.num_nodes)
+    return out
+
+
+def slab_oracle(p, layers):
+    """The oracle's mass operator on the first `layers` element layers (a z-slab)
+    of problem p, with the qdata p built on the device. Returns (op, take, l_s,
+    rows): take(vec) cuts a full L-vector to the slab's l_s nodes per component
+    (every component), rows is the number of leading nodes per component whose
+    rows the slab's elements alone make (every node below the slab's top node
+    plane)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import oracle
+
+    n, P, Q, nc = p.n, p.P, p.Q, p.nc
+    E_s = n * n * layers
+    N1 = n * (P - 1) + 1
+    l_s = N1 * N1 * (layers * (P - 1) + 1)
+    off = p.offsets[: E_s * P ** 3]
+    qd = p.qdata().reshape(Q ** 3, p.num_elem)[:, :E_s].copy().ravel()
+    op = oracle().mass(3, E_s, P, Q, nc, nc * l_s, l_s, off, qd)
+    N = p.num_nodes
+
+    def take(vec):
+        return np.concatenate([vec[c * N: c * N + l_s] for c in range(nc)])
+
+    return op, take, l_s, N1 * N1 * layers * (P - 1)
+
+
+def slab_rel_err(gpu_vec, cpu_slab, l_s, rows, nc, N):
+    """max |gpu - cpu| / max |cpu| over the slab-interior rows of every component."""
+    g = np.concatenate([gpu_vec[c * N: c * N + rows] for c in range(nc)])
+    c_ = np.concatenate([cpu_slab[c * l_s: c * l_s + rows] for c in range(nc)])
+    return float(np.max(np.abs(g - c_)) / np.max(np.abs(c_)))
+
+
+def config5(fehip, args, stream, layers=2):
+    """48^3 hexes, P=8, Q=10, 3 components (114.8M DoF): the FP64 and FP32 applies
+    of each algorithm (GDoF/s, %HBM), the FP64 apply's error against the oracle's
+    operator_apply on a 2-layer z-slab (its interior rows), and the FP64 and
+    FP32-apply / FP64-refined solves of M x = b, b = M x* (x* ~ U[-1,1] seed 2)
+    to 1e-10, with the solution's residual recomputed against the oracle's
+    operator_apply on the same slab."""
+    from fehip.problems import MassProblem
+
+    D, A = fehip.SCATTER_DETERMINISTIC, fehip.SCATTER_ATOMIC
+    t1 = time.perf_counter()
+    p5 = MassProblem(48, 8, 10, 3, scatter=D)
+    fehip.synchronize()
+    log("config 5 setup %.1f s" % (time.perf_counter() - t1))
+    L5 = p5.l_size
+    big = {"workload": "48^3 hexes, P=8, Q=10, num_comp=3, %d DoF" % L5}
+    d_in = {np.float64: fehip.rng_uniform_device(1, L5, stream=stream)}
+    u_host = d_in[np.float64].to_host()
+    d_in[np.float32] = fehip.DeviceArray.from_host(u_host.astype(np.float32))
+    d_out = {np.float64: fehip.DeviceArray(L5, np.float64), np.float32: fehip.DeviceArray(L5, np.float32)}
+    steps = max(args.steps // 2, 10)
+    for label, dt, scatter, alg in (("fp64_box_column", np.float64, D, fehip.FUSED_BOX_COLUMN),
+                                    ("fp32_box_column", np.float32, D, fehip.FUSED_BOX_COLUMN),
+                                    ("fp64_block_partial", np.float64, D, fehip.FUSED_BLOCK_PARTIAL),
+                                    ("fp64_elem_tmap", np.float64, D, fehip.FUSED_ELEM_TMAP),
+                                    ("fp64_block_atomic", np.float64, A, fehip.FUSED_BLOCK_ATOMIC),
+                                    ("fp32_block_partial", np.float32, D, fehip.FUSED_BLOCK_PARTIAL)):
+        p5.op.set_fused_options(scatter, exact=0)
+        p5.op.set_fused_algorithm(alg)
+        m, k, a = time_applies(fehip, p5.op, d_in[dt], d_out[dt], steps, args.warmup, stream)
+        sz = 4 if dt == np.float32 else 8
+        big[label] = {"ms_per_apply": m, "gdofs": L5 / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
+                      "apply_frac_hbm": p5.apply_bytes(sz, sz, sz) / (m * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log("config5", label, json.dumps(big[label]))
+    best = min((v["ms_per_apply"], k2) for k2, v in big.items() if k2.startswith("fp64"))[1]
+    p5.op.set_fused_options(D, exact=0)
+    p5.op.set_fused_algorithm({"fp64_box_column": fehip.FUSED_BOX_COLUMN, "fp64_block_partial":
+                               fehip.FUSED_BLOCK_PARTIAL, "fp64_elem_tmap": fehip.FUSED_ELEM_TMAP}.get(
+                                   best, fehip.FUSED_AUTO))
+    # The FP64 apply against the oracle on the slab.
+    t1 = time.perf_counter()
+    orc, take, l_s, rows = slab_oracle(p5, layers)
+    p5.op.apply(d_in[np.float64], d_out[np.float64], stream)
+    stream.synchronize()
+    v_gpu = d_out[np.float64].to_host()
+    v_cpu = orc.apply(take(u_host))
+    big["apply_err_vs_cpu_slab"] = slab_rel_err(v_gpu, v_cpu, l_s, rows, 3, p5.num_nodes)
+    big["cpu_slab"] = "oracle operator_apply on the first %d element layers (48x48x%d elements), rows below " \
+                      "the slab's top node plane (%d nodes per component); %.1f s" % (
+                          layers, layers, rows, time.perf_counter() - t1)
+    # Solves.
+    x_star = fehip.rng_uniform_device(2, L5, stream=stream)
+    d_b, d_x = fehip.DeviceArray(L5), fehip.DeviceArray(L5)
+    p5.op.apply(x_star, d_b, stream)
+    stream.synchronize()
+    xs, bh = x_star.to_host(), d_b.to_host()
+    solves = {}
+    for name, mode in (("fp64", fehip.SOLVE_FP64), ("mixed", fehip.SOLVE_MIXED)):
+        p5.op.solve(d_b, d_x, mode=mode, rtol=1e-10)  # records the graphs
+        lat = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            st = p5.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+            lat.append((time.perf_counter() - t0) * 1e3)
+        x = d_x.to_host()
+        Mx_cpu = orc.apply(take(x))
+        b_s = take(bh)
+        r = np.concatenate([b_s[c * l_s: c * l_s + rows] - Mx_cpu[c * l_s: c * l_s + rows] for c in range(3)])
+        bn = np.concatenate([b_s[c * l_s: c * l_s + rows] for c in range(3)])
+        solves[name] = {"p50_ms": float(np.median(lat)), "iterations": st["iterations"], "outer": st["outer"],
+                        "rel_residual": st["rel_residual"],
+                        "rel_residual_cpu_slab": float(np.linalg.norm(r) / np.linalg.norm(bn)),
+                        "err_vs_xstar": float(np.max(np.abs(x - xs)) / np.max(np.abs(xs)))}
+        log("config5 solve", name, json.dumps(solves[name]))
+    solves["speedup_mixed_vs_fp64_p50"] = solves["fp64"]["p50_ms"] / solves["mixed"]["p50_ms"]
+    big["solve"] = solves
+    p5.op.set_fused_algorithm(fehip.FUSED_AUTO)
+    return big
 
 
 def solve_latency(fehip, reps, n=16, P=3, Q=4):
@@ -243,16 +372,23 @@ def main():
             ("fp32_elem_tmap", np.float32, D, 0, fehip.FUSED_ELEM_TMAP),
             ("fp32_elem_tmap_pipe", np.float32, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
             ("fp32_block_atomic", np.float32, A, 0, fehip.FUSED_BLOCK_ATOMIC))
+    VARIANT_ALG = {r[0]: r[4] for r in runs}
+    DET_VARIANTS = {r[0] for r in runs if r[2] == D}
     wanted = set(filter(None, args.variants.split(",")))
     unknown = wanted - {r[0] for r in runs}
     if unknown:
         ap.error("unknown variants: %s" % ", ".join(sorted(unknown)))
     kernel_names = {}
+    # Inputs made once, before any timing: u ~ U[-1,1] (splitmix64 seed 1)
+    # generated on the device in FP64, its FP32 copy uploaded once; outputs
+    # allocated once. Nothing runs on the host between the variants.
+    d_in = {np.float64: fehip.rng_uniform_device(1, L, stream=stream), np.float32: None}
+    d_in[np.float32] = fehip.DeviceArray.from_host(d_in[np.float64].to_host().astype(np.float32))
+    d_out = {np.float64: fehip.DeviceArray(L, np.float64), np.float32: fehip.DeviceArray(L, np.float32)}
     for label, dt, scatter, exact, alg in runs:
         if wanted and label not in wanted:
             continue
-        d_u = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L).astype(dt))
-        d_v = fehip.DeviceArray(L, dt)
+        d_u, d_v = d_in[dt], d_out[dt]
         fp32 = dt == np.float32
         p.op.set_fused_options(scatter, exact=exact)
         if not exact:
@@ -307,29 +443,8 @@ def main():
 
     big = None
     if not args.no_big:
-        # Config 5: 48^3 hexes, P=8, Q=10, 3 components (114.8M DoF).
-        t1 = time.perf_counter()
-        p5 = MassProblem(48, 8, 10, 3, scatter=fehip.SCATTER_DETERMINISTIC)
-        fehip.synchronize()
-        log("config 5 setup %.1f s" % (time.perf_counter() - t1))
-        L5 = p5.l_size
-        big = {"workload": "48^3 hexes, P=8, Q=10, num_comp=3, %d DoF" % L5}
-        for label, dt, scatter, alg in (("fp64_box_column", np.float64, D, fehip.FUSED_BOX_COLUMN),
-                                        ("fp32_box_column", np.float32, D, fehip.FUSED_BOX_COLUMN),
-                                        ("fp64_block_partial", np.float64, D, fehip.FUSED_BLOCK_PARTIAL),
-                                        ("fp64_elem_tmap", np.float64, D, fehip.FUSED_ELEM_TMAP),
-                                        ("fp64_block_atomic", np.float64, A, fehip.FUSED_BLOCK_ATOMIC),
-                                        ("fp32_block_partial", np.float32, D, fehip.FUSED_BLOCK_PARTIAL)):
-            d_u = fehip.DeviceArray.from_host(fehip.rng_uniform(1, L5).astype(dt))
-            d_v = fehip.DeviceArray(L5, dt)
-            p5.op.set_fused_options(scatter, exact=0)
-            p5.op.set_fused_algorithm(alg)
-            m, k, a = time_applies(fehip, p5.op, d_u, d_v, 10, 2, stream)
-            s = 4 if dt == np.float32 else 8
-            big[label] = {"ms_per_apply": m, "gdofs": L5 / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
-                          "apply_frac_hbm": p5.apply_bytes(s, s, s) / (m * 1e-3) / 1e9 / HBM_PEAK_GBS}
-            log("config5", label, json.dumps(big[label]))
-        del p5
+        big = config5(fehip, args, stream)
+        log("config5", json.dumps(big))
 
     solve = solve64 = None
     if not args.no_solve:
@@ -359,7 +474,12 @@ def main():
         log("serve (%.1f s)" % (time.perf_counter() - t1), json.dumps(serve))
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(p)
+        # The headline variant's output on the same u, for its error against the oracle.
+        p.op.set_fused_options(D if best_name in DET_VARIANTS else A, exact=0)
+        p.op.set_fused_algorithm(VARIANT_ALG[best_name], tuned_elems("fp64", VARIANT_ALG[best_name]))
+        p.op.apply(d_in[np.float64], d_out[np.float64], stream)
+        stream.synchronize()
+        cpu = cpu_baseline(p, d_out[np.float64].to_host())
         log("cpu", cpu)
 
     k_ms = best["kernel_ms"]

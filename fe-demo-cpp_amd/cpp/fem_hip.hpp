@@ -10,6 +10,7 @@
 // qfunctions/mass.hpp) and link libfemhip.so.
 #pragma once
 
+#include <array>
 #include <cstddef>
 #include <cstring>
 #include <memory>
@@ -169,19 +170,23 @@ inline void build_transpose_map(const fem::ElemRestriction& r, TransposeMap& map
   map.t_indices = std::move(ti);
 }
 
-// include/cuda/cuda-elem-restriction.hpp:46-58: sizes plus a handle owning the device copies.
+// include/cuda/cuda-elem-restriction.hpp:46-58: the sizes, and the index arrays
+// in device memory owned by the struct itself (move-only through DeviceArray).
 struct ElemRestriction {
   RestrictionType type = RestrictionType::Offset;
-  int num_elem = 0, elem_size = 0, num_comp = 0, comp_stride = 0, l_size = 0, e_size = 0;
+  int num_elem = 0, elem_size = 0, num_comp = 0, comp_stride = 0;
+  int l_size = 0, e_size = 0;
+  std::array<int, 3> strides{};  // Strided only
   ScatterMode scatter_mode = ScatterMode::Atomic;
-  int num_nodes = 0;
-  std::shared_ptr<fem_hip_restriction_s> handle;
+
+  DeviceArray<int> offsets;  // Offset only: copy of the host offsets, [num_elem][elem_size]
+
+  // Offset + Deterministic only (empty otherwise): the TransposeMap on the device
+  int num_nodes = 0;  // l_vec_indices.size()
+  DeviceArray<int> l_vec_indices, t_offsets, t_indices;
 };
 
 namespace detail {
-inline std::shared_ptr<fem_hip_restriction_s> own(fem_hip_restriction h) {
-  return std::shared_ptr<fem_hip_restriction_s>(h, [](fem_hip_restriction p) { (void)fem_hip_restriction_destroy(p); });
-}
 inline int cmode(ContractMode m) { return m == ContractMode::Transpose ? FEM_HIP_TRANSPOSE : FEM_HIP_NO_TRANSPOSE; }
 inline int emode(EvalMode m) {
   switch (m) {
@@ -191,38 +196,61 @@ inline int emode(EvalMode m) {
     default: return FEM_HIP_EVAL_WEIGHT;
   }
 }
-}  // namespace detail
-
-// include/cuda/cuda-elem-restriction.hpp:60-66: copies the host restriction's index
-// arrays to the device once; `restriction` is replaced only on success.
-inline void elem_restriction_create(const fem::ElemRestriction& host, ScatterMode scatter_mode,
-                                    ElemRestriction& restriction) {
+inline int smode(ScatterMode m) {
+  return m == ScatterMode::Deterministic ? FEM_HIP_SCATTER_DETERMINISTIC : FEM_HIP_SCATTER_ATOMIC;
+}
+// A library restriction handle (its own device copies) for the operator path.
+inline std::shared_ptr<fem_hip_restriction_s> restriction_handle(const fem::ElemRestriction& host,
+                                                                 ScatterMode scatter_mode) {
   fem_hip_restriction h = nullptr;
   if (host.type == RestrictionType::Offset) {
     if (host.offsets.size() != static_cast<std::size_t>(host.num_elem) * host.elem_size)
       throw std::invalid_argument("hip::elem_restriction_create: offsets size != num_elem * elem_size");
-    FEM_HIP_CHECK(fem_hip_restriction_create(host.num_elem, host.elem_size, host.num_comp, host.comp_stride, host.l_size,
-                                             host.offsets.data(),
-                                             scatter_mode == ScatterMode::Deterministic ? FEM_HIP_SCATTER_DETERMINISTIC
-                                                                                        : FEM_HIP_SCATTER_ATOMIC,
-                                             &h));
+    FEM_HIP_CHECK(fem_hip_restriction_create(host.num_elem, host.elem_size, host.num_comp, host.comp_stride,
+                                             host.l_size, host.offsets.data(), smode(scatter_mode), &h));
   } else {
     FEM_HIP_CHECK(fem_hip_restriction_create_strided(host.num_elem, host.elem_size, host.num_comp, host.l_size,
                                                      host.has_backend_strides ? nullptr : host.strides.data(), &h));
   }
+  return std::shared_ptr<fem_hip_restriction_s>(h, [](fem_hip_restriction p) { (void)fem_hip_restriction_destroy(p); });
+}
+}  // namespace detail
+
+// include/cuda/cuda-elem-restriction.hpp:60-66 (src/cuda/cuda-elem-restriction.cu:188-214):
+// copies the host restriction's sizes and index arrays to the device once and,
+// for Deterministic, builds the transpose map on the host and copies it;
+// `restriction` is replaced only on success. The offsets are also checked to
+// lie in [0, l_size) with every component, so a malformed host struct cannot
+// make a kernel read out of bounds.
+inline void elem_restriction_create(const fem::ElemRestriction& host, ScatterMode scatter_mode,
+                                    ElemRestriction& restriction) {
   ElemRestriction r;
-  r.handle = detail::own(h);
-  long long s[8];
-  FEM_HIP_CHECK(fem_hip_restriction_get_sizes(h, s));
   r.type = host.type;
-  r.num_elem = static_cast<int>(s[1]);
-  r.elem_size = static_cast<int>(s[2]);
-  r.num_comp = static_cast<int>(s[3]);
-  r.comp_stride = static_cast<int>(s[4]);
-  r.l_size = static_cast<int>(s[5]);
-  r.e_size = static_cast<int>(s[6]);
-  r.num_nodes = static_cast<int>(s[7]);
+  r.num_elem = host.num_elem;
+  r.elem_size = host.elem_size;
+  r.num_comp = host.num_comp;
+  r.comp_stride = host.comp_stride;
+  r.l_size = host.l_size;
+  r.e_size = host.e_size;
+  r.strides = host.strides;
   r.scatter_mode = scatter_mode;
+  if (host.type == RestrictionType::Offset) {
+    if (host.offsets.size() != static_cast<std::size_t>(host.num_elem) * host.elem_size)
+      throw std::invalid_argument("hip::elem_restriction_create: offsets size != num_elem * elem_size");
+    const long long reach = static_cast<long long>(host.num_comp - 1) * host.comp_stride;
+    for (int o : host.offsets)
+      if (o < 0 || o + reach >= host.l_size)
+        throw std::invalid_argument("hip::elem_restriction_create: an offset lies outside [0, l_size)");
+    r.offsets = DeviceArray<int>(host.offsets);
+    if (scatter_mode == ScatterMode::Deterministic) {
+      TransposeMap map;
+      build_transpose_map(host, map);
+      r.num_nodes = static_cast<int>(map.l_vec_indices.size());
+      r.l_vec_indices = DeviceArray<int>(map.l_vec_indices);
+      r.t_offsets = DeviceArray<int>(map.t_offsets);
+      r.t_indices = DeviceArray<int>(map.t_indices);
+    }
+  }
   restriction = std::move(r);
 }
 
@@ -235,8 +263,23 @@ inline void elem_restriction_apply(const ElemRestriction& r, ContractMode t_mode
       throw std::invalid_argument("hip::elem_restriction_apply: in has the wrong size");
     out.resize(r.e_size);
   }
-  FEM_HIP_CHECK(fem_hip_restriction_apply(r.handle.get(), detail::cmode(t_mode), in.data(), in.size(), out.data(),
-                                          out.size(), stream));
+  fem_hip_restriction_arrays a{};
+  a.type = r.type == RestrictionType::Offset ? 0 : 1;
+  a.num_elem = r.num_elem;
+  a.elem_size = r.elem_size;
+  a.num_comp = r.num_comp;
+  a.comp_stride = r.comp_stride;
+  a.l_size = r.l_size;
+  a.e_size = r.e_size;
+  for (int i = 0; i < 3; ++i) a.strides[i] = r.strides[i];
+  a.scatter_mode = detail::smode(r.scatter_mode);
+  a.offsets = r.offsets.data();
+  a.num_nodes = r.num_nodes;
+  a.l_vec_indices = r.l_vec_indices.data();
+  a.t_offsets = r.t_offsets.data();
+  a.t_indices = r.t_indices.data();
+  FEM_HIP_CHECK(fem_hip_restriction_apply_arrays(&a, detail::cmode(t_mode), in.data(), in.size(), out.data(),
+                                                 out.size(), stream));
 }
 
 // Device tensor basis built from the reference's host TensorBasis.
@@ -293,14 +336,14 @@ inline void operator_create(const fem::Operator& host, Operator& op, ScatterMode
   auto set = [&](const std::optional<OperatorField>& fo) {
     if (!fo) return;
     const OperatorField& f = *fo;
-    ElemRestriction r;
+    std::shared_ptr<fem_hip_restriction_s> r;
     TensorBasis b;
-    if (f.restriction) elem_restriction_create(*f.restriction, scatter_mode, r);
+    if (f.restriction) r = detail::restriction_handle(*f.restriction, scatter_mode);
     if (f.basis) basis_create(*f.basis, b);
     const int vk = f.vector.kind == VectorKind::Active    ? FEM_HIP_VECTOR_ACTIVE
                    : f.vector.kind == VectorKind::Passive ? FEM_HIP_VECTOR_PASSIVE
                                                           : FEM_HIP_VECTOR_NONE;
-    FEM_HIP_CHECK(fem_hip_operator_set_field(h, f.name.c_str(), f.restriction ? r.handle.get() : nullptr,
+    FEM_HIP_CHECK(fem_hip_operator_set_field(h, f.name.c_str(), f.restriction ? r.get() : nullptr,
                                              f.basis ? b.handle.get() : nullptr, vk,
                                              f.vector.data.empty() ? nullptr : f.vector.data.data(),
                                              f.vector.data.size(), 0));

@@ -312,6 +312,38 @@ int fem_hip_restriction_get_transpose_map(fem_hip_restriction h, int* lvec, int*
 int fem_hip_restriction_destroy(fem_hip_restriction r) {
   return guard([&] { delete r; });
 }
+int fem_hip_restriction_apply_arrays(const fem_hip_restriction_arrays* a, int t_mode, const double* in,
+                                     size_t in_len, double* out, size_t out_len, void* s) {
+  return guard([&] {
+    need(a, "restriction");
+    require(a->type == 0 || a->type == 1, "elem_restriction_apply: unknown restriction type");
+    require(a->num_elem >= 0 && a->elem_size >= 0 && a->num_comp >= 0 && a->l_size >= 0 && a->e_size >= 0,
+            "elem_restriction_apply: negative size");
+    require(a->e_size == static_cast<long long>(a->num_comp) * a->elem_size * a->num_elem,
+            "elem_restriction_apply: e_size != num_comp * elem_size * num_elem");
+    require(a->scatter_mode == kScatterAtomic || a->scatter_mode == kScatterDeterministic,
+            "elem_restriction_apply: unknown scatter mode");
+    RestrictionView v;
+    v.type = a->type;
+    v.num_elem = a->num_elem;
+    v.elem_size = a->elem_size;
+    v.num_comp = a->num_comp;
+    v.comp_stride = a->comp_stride;
+    v.l_size = a->l_size;
+    v.e_size = static_cast<size_t>(a->e_size);
+    v.strides = {a->strides[0], a->strides[1], a->strides[2]};
+    v.scatter_mode = a->scatter_mode;
+    v.offsets = a->offsets;
+    v.num_nodes = a->num_nodes;
+    v.l_vec_indices = a->l_vec_indices;
+    v.t_offsets = a->t_offsets;
+    v.t_indices = a->t_indices;
+    if (v.type == 0 && v.scatter_mode == kScatterDeterministic && t_mode == kTranspose)
+      require(v.num_nodes >= 0 && v.l_vec_indices != nullptr,
+              "elem_restriction_apply: the deterministic scatter needs l_vec_indices on the device");
+    restriction_apply_view<double>(v, t_mode, in, in_len, out, out_len, st(s));
+  });
+}
 int fem_hip_transpose_map_build(int ne, int es, int l_size, const int* offsets, int* rows, int* lvec, int* toff,
                                 int* tind) {
   return guard([&] {
@@ -508,6 +540,14 @@ int fem_hip_operator_get_info(fem_hip_operator h, long long* info) {
     std::memcpy(info, v, sizeof(v));
   });
 }
+// The solver's captured CG graphs hold the fused apply's kernels and buffers;
+// a change of options or an autotune (which may free and remake buffers)
+// drops them with the rest of the cached solver state (mass_solve also
+// re-records graphs whose apply signature changed).
+static void drop_solver(fem_hip_operator h) {
+  femhip::mass_solver_state_destroy(h->solver);
+  h->solver = nullptr;
+}
 int fem_hip_operator_set_fused_options(fem_hip_operator h, int scatter, int exact, int elems) {
   return guard([&] {
     need(h, "operator");
@@ -518,6 +558,7 @@ int fem_hip_operator_set_fused_options(fem_hip_operator h, int scatter, int exac
     h->op->opt_exact = exact ? 1 : 0;
     h->op->opt_alg = elems > 0 ? -2 : -1;
     h->op->opt_elems = elems;
+    drop_solver(h);
   });
 }
 int fem_hip_operator_set_fused_algorithm(fem_hip_operator h, int alg, int elems) {
@@ -527,6 +568,7 @@ int fem_hip_operator_set_fused_algorithm(fem_hip_operator h, int alg, int elems)
     require(elems >= 0, "set_fused_algorithm: elems must be >= 0");
     h->op->opt_alg = alg;
     h->op->opt_elems = alg < 0 ? 0 : elems;
+    drop_solver(h);
   });
 }
 int fem_hip_operator_set_box_layers(fem_hip_operator h, int layers) {
@@ -534,6 +576,7 @@ int fem_hip_operator_set_box_layers(fem_hip_operator h, int layers) {
     need(h, "operator");
     require(layers >= 0, "set_box_layers: layers must be >= 0");
     h->op->opt_box_lz = layers;
+    drop_solver(h);
   });
 }
 int fem_hip_operator_get_fused_choice(fem_hip_operator h, int fp32, int* alg, int* elems) {
@@ -553,6 +596,7 @@ int fem_hip_operator_autotune(fem_hip_operator h, int fp32, int reps, void* s, i
     require(reps > 0, "autotune: reps must be positive");
     require(max_results >= 0 && (results || max_results == 0), "autotune: bad results buffer");
     FusedMass& f = operator_fused_mass(*h->op, st(s));
+    drop_solver(h);
     std::vector<TuneEntry> all;
     const TuneEntry best = fused_autotune(f, fp32 ? 1 : 0, reps, st(s), &all);
     if (best_alg) *best_alg = best.alg;

@@ -2,6 +2,8 @@
 // launch probe for the host-forced error tests.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace femhip {
@@ -147,11 +149,17 @@ void stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
   require(dst != nullptr && src != nullptr, "stream_copy: pointers must not be null");
   require(reinterpret_cast<uintptr_t>(dst) % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0 && bytes % 16 == 0,
           "stream_copy: pointers and size must be multiples of 16 bytes");
-  const size_t n16 = bytes / 16, blocks = (n16 + 255) / 256;
-  require(blocks < (1ull << 31), "stream_copy: size too large");
-  stream_copy_kernel<<<static_cast<unsigned>(blocks), 256, 0, s>>>(static_cast<const double2*>(src),
-                                                                  static_cast<double2*>(dst), n16);
-  HIP_CHECK_LAUNCH();
+  // One 16-byte word per thread (the fastest form measured); a launch holds at
+  // most 2^24 workgroups of 256 (HIP allows fewer than 2^32 work-items per
+  // dimension), so larger copies take several launches.
+  const size_t n16 = bytes / 16;
+  constexpr size_t kChunk = (size_t{1} << 24) * 256;
+  for (size_t off = 0; off < n16; off += kChunk) {
+    const size_t n = std::min(kChunk, n16 - off), blocks = (n + 255) / 256;
+    stream_copy_kernel<<<static_cast<unsigned>(blocks), 256, 0, s>>>(static_cast<const double2*>(src) + off,
+                                                                    static_cast<double2*>(dst) + off, n);
+    HIP_CHECK_LAUNCH();
+  }
 }
 
 int probe_launch(int blocks, int threads) {

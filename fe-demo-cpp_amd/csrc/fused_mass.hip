@@ -367,6 +367,7 @@ int box_default_elems(const FusedMass& f, int fp32) {
 DevPlan& ensure_plan(FusedMass& f, int NB, int budget, bool part, hipStream_t s) {
   auto& slot = f.plans[{NB, budget}];
   if (!slot) {
+    ++f.gen;
     const int P3 = f.P * f.P * f.P, Q3 = f.Q * f.Q * f.Q;
     BlockPlan plan = build_block_plan(f.rstr->h_offsets.data(), f.num_elem, P3, f.l_size, NB,
                                       budget > 0 ? std::min(budget, NB * P3) : 0);
@@ -490,6 +491,7 @@ void ensure_tmap(FusedMass& f) {
   f.tm_ptr_p = f.tm_ptr.data();
   f.tm_idx_p = f.tm_idx.data();
   f.tm_built = true;
+  ++f.gen;
 }
 
 template <class T>
@@ -499,9 +501,10 @@ DevBuf<double>& plan_qdata<double>(FusedMass&, DevPlan& d, hipStream_t) {
   return d.q64;
 }
 template <>
-DevBuf<float>& plan_qdata<float>(FusedMass&, DevPlan& d, hipStream_t s) {
+DevBuf<float>& plan_qdata<float>(FusedMass& f, DevPlan& d, hipStream_t s) {
   if (d.q32.size() != d.q64.size()) {
     d.q32.allocate(d.q64.size());
+    ++f.gen;
     to_f32_kernel<float><<<grid_for(d.q64.size(), kThreads), kThreads, 0, s>>>(d.q64.size(), d.q64.data(),
                                                                               d.q32.data());
     HIP_CHECK_LAUNCH();
@@ -530,6 +533,7 @@ template <>
 DevBuf<float>& elem_qdata<float>(FusedMass& f, hipStream_t s) {
   if (f.qdata32.size() != f.qdata64.size()) {
     f.qdata32.allocate(f.qdata64.size());
+    ++f.gen;
     to_f32_kernel<float><<<grid_for(f.qdata64.size(), kThreads), kThreads, 0, s>>>(f.qdata64.size(), f.qdata64.data(),
                                                                                   f.qdata32.data());
     HIP_CHECK_LAUNCH();
@@ -590,7 +594,10 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
     (void)elem_qdata<T>(f, s);
     DevBuf<T>& pb = box_partials<T>(f);
     const size_t n = std::max<size_t>(box_part_len(f, k), 1) * f.NC;
-    if (pb.size() < n) pb.allocate(n);
+    if (pb.size() < n) {
+      pb.allocate(n);
+      ++f.gen;
+    }
     return;
   }
   if (alg_blocked(k.alg)) {
@@ -600,7 +607,10 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
     if (part) {
       DevBuf<T>& pb = plan_partials<T>(d);
       const size_t n = std::max<size_t>(d.npart, 1) * f.NC;
-      if (pb.size() != n) pb.allocate(n);
+      if (pb.size() != n) {
+        pb.allocate(n);
+        ++f.gen;
+      }
     }
   } else {
     (void)elem_qdata<T>(f, s);
@@ -609,7 +619,10 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
       ensure_tmap(f);
       DevBuf<T>& Y = elem_outputs<T>(f);
       const size_t n = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P * f.NC;
-      if (Y.size() != n) Y.allocate(n);
+      if (Y.size() != n) {
+        Y.allocate(n);
+        ++f.gen;
+      }
     }
   }
 }
@@ -784,6 +797,7 @@ void ensure_flagged_offsets(FusedMass& f, hipStream_t s) {
   StreamTmp<int> count(static_cast<size_t>(f.l_size), s);
   zero_fill(count.data(), static_cast<size_t>(f.l_size) * sizeof(int), s);
   f.flagged_offsets.allocate(nt);
+  ++f.gen;
   node_count_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, f.rstr->offsets.data(), count.data());
   flag_single_kernel<<<grid_for(nt, kThreads), kThreads, 0, s>>>(nt, f.rstr->offsets.data(), count.data(),
                                                                 f.flagged_offsets.data());
@@ -946,6 +960,20 @@ void fused_timing_end(FusedMass& f, double* kernel_ms, double* aux_ms, int* coun
   f.events_used = 0;
 }
 
+uint64_t fused_signature(const FusedMass& f, int fp32) {
+  const FusedKey k = fused_resolve(f, fp32);
+  const long long parts[] = {k.P, k.Q, k.NC, k.fp32, k.alg, k.exact, k.elems, alg_box(k.alg) ? box_lz_of(f, k) : 0,
+                             f.scatter, f.exact, static_cast<long long>(f.gen)};
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the parts
+  for (long long v : parts) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= static_cast<uint64_t>(v >> (8 * i)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  }
+  return h;
+}
+
 void fused_prepare(FusedMass& f, int fp32, hipStream_t s) {
   const FusedKey k = fused_resolve(f, fp32);
   if (fp32) {
@@ -1033,6 +1061,7 @@ TuneEntry autotune_impl(FusedMass& f, int reps, hipStream_t s, std::vector<TuneE
     f.Y64.release();
     f.Y32.release();
   }
+  ++f.gen;
   return best;
 }
 

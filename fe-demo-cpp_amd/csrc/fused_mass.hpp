@@ -153,6 +153,11 @@ struct FusedMass {
   int scatter = kScatterAtomic;  // requested: atomic (any order) or deterministic (reproducible)
   int exact = 0;                 // requested: the reference's arithmetic (bit-equal with deterministic)
   FusedChoice choice;            // explicit algorithm, or automatic
+  // Bumped whenever a device buffer an apply passes to its kernels is
+  // allocated, reallocated or released (plans, FP32 copies, partials, element
+  // outputs, maps, flags): work captured into a graph stays valid only while
+  // it and the resolved kernel are unchanged (fused_signature).
+  uint64_t gen = 0;
 
   // Optional per-apply stage timing with HIP events on the apply's stream:
   // a..b = pre-pass (output zeroing), b..c = the fused kernel, c..d = post-pass
@@ -177,6 +182,11 @@ void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s)
 void fused_mass_apply32(FusedMass& f, const float* u, float* v, hipStream_t s);
 // The kernel an apply in this precision would launch now (after automatic choice).
 FusedKey fused_resolve(const FusedMass& f, int fp32);
+// Identifies what an apply in this precision would launch now: the resolved
+// kernel, its box layer count, the scatter and exact settings and f.gen. Two
+// equal signatures mean a graph captured at the first still launches the same
+// kernels on live buffers at the second.
+uint64_t fused_signature(const FusedMass& f, int fp32);
 // Builds lazily made state (block plans, FP32 copies) for the current choice,
 // so later applies allocate nothing (graph capture).
 void fused_prepare(FusedMass& f, int fp32, hipStream_t s);

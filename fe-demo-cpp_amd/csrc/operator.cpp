@@ -47,8 +47,11 @@ bool fused_build_ok(const Operator& op) {
   if (q.eval_mode != kEvalNone || q.vkind != kVectorActive || !q.rstr || q.size != 1) return false;
   const Restriction& qr = *q.rstr;
   const int Q = dx.basis->h.Q;
+  // The fused kernel writes exactly num_elem*Q^3 entries; a strided qdata
+  // restriction with a larger l_size (legal) keeps the generic path, which
+  // zeroes the whole output as src/operator.cpp:172 does.
   return qr.type == 1 && qr.backend_strides && qr.num_comp == 1 && qr.num_elem == dx.rstr->num_elem &&
-         qr.elem_size == Q * Q * Q;
+         qr.elem_size == Q * Q * Q && static_cast<long long>(qr.l_size) == static_cast<long long>(qr.num_elem) * Q * Q * Q;
 }
 
 void decide_fused(Operator& op) {

@@ -370,9 +370,34 @@ Restriction* restriction_clone(const Restriction& src) {
   return r;
 }
 
+RestrictionView restriction_view(const Restriction& r) {
+  RestrictionView v;
+  v.type = r.type;
+  v.num_elem = r.num_elem;
+  v.elem_size = r.elem_size;
+  v.num_comp = r.num_comp;
+  v.comp_stride = r.comp_stride;
+  v.l_size = r.l_size;
+  v.e_size = r.e_size;
+  v.strides = r.strides;
+  v.scatter_mode = r.scatter_mode;
+  v.offsets = r.offsets.data();
+  v.num_nodes = r.num_nodes;
+  v.l_vec_indices = r.identity_nodes ? nullptr : r.l_vec_indices.data();
+  v.t_offsets = r.t_offsets.data();
+  v.t_indices = r.t_indices.data();
+  return v;
+}
+
 template <class T>
 void restriction_apply(const Restriction& r, int t_mode, const T* in, size_t in_len, T* out, size_t out_len,
                        hipStream_t s) {
+  restriction_apply_view<T>(restriction_view(r), t_mode, in, in_len, out, out_len, s);
+}
+
+template <class T>
+void restriction_apply_view(const RestrictionView& r, int t_mode, const T* in, size_t in_len, T* out, size_t out_len,
+                            hipStream_t s) {
   require(t_mode == kNoTranspose || t_mode == kTranspose, "elem_restriction_apply: unknown transpose mode");
   const bool tr = (t_mode == kTranspose);
   require(in_len == (tr ? r.e_size : static_cast<size_t>(r.l_size)), "elem_restriction_apply: in has the wrong size");
@@ -395,16 +420,21 @@ void restriction_apply(const Restriction& r, int t_mode, const T* in, size_t in_
     }
   } else if (!tr) {
     dim3 grid((r.num_elem + kTileE - 1) / kTileE, (r.elem_size + kTileI - 1) / kTileI);
-    offset_gather_kernel<T><<<grid, kThreads, 0, s>>>(r.num_elem, r.elem_size, r.num_comp, r.comp_stride,
-                                                      r.offsets.data(), in, out);
+    require(r.offsets != nullptr, "elem_restriction_apply: offsets are not on the device");
+    offset_gather_kernel<T><<<grid, kThreads, 0, s>>>(r.num_elem, r.elem_size, r.num_comp, r.comp_stride, r.offsets,
+                                                      in, out);
   } else if (r.scatter_mode == kScatterAtomic) {
     dim3 grid((r.num_elem + kTileE - 1) / kTileE, (r.elem_size + kTileI - 1) / kTileI);
+    require(r.offsets != nullptr, "elem_restriction_apply: offsets are not on the device");
     offset_scatter_atomic_kernel<T><<<grid, kThreads, 0, s>>>(r.num_elem, r.elem_size, r.num_comp, r.comp_stride,
-                                                              r.offsets.data(), in, out);
+                                                              r.offsets, in, out);
   } else {
+    require(r.t_offsets != nullptr && r.t_indices != nullptr,
+            "elem_restriction_apply: the deterministic scatter needs the transpose map on the device");
+    if (r.num_nodes == 0) return;
     offset_scatter_det_kernel<T><<<grid_for(r.num_nodes, kThreads), kThreads, 0, s>>>(
-        r.num_nodes, r.num_elem, r.elem_size, r.num_comp, r.comp_stride,
-        r.identity_nodes ? nullptr : r.l_vec_indices.data(), r.t_offsets.data(), r.t_indices.data(), in, out);
+        r.num_nodes, r.num_elem, r.elem_size, r.num_comp, r.comp_stride, r.l_vec_indices, r.t_offsets, r.t_indices,
+        in, out);
   }
   HIP_CHECK_LAUNCH();
 }
@@ -422,6 +452,8 @@ void csr_scatter(const Restriction& r, const T* Y, T* L, bool zero_start, hipStr
 
 template void restriction_apply<double>(const Restriction&, int, const double*, size_t, double*, size_t, hipStream_t);
 template void restriction_apply<float>(const Restriction&, int, const float*, size_t, float*, size_t, hipStream_t);
+template void restriction_apply_view<double>(const RestrictionView&, int, const double*, size_t, double*, size_t,
+                                             hipStream_t);
 template void csr_scatter<double>(const Restriction&, const double*, double*, bool, hipStream_t);
 template void csr_scatter<float>(const Restriction&, const float*, float*, bool, hipStream_t);
 

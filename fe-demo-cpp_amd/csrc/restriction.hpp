@@ -57,6 +57,26 @@ template <class T>
 void restriction_apply(const Restriction& r, int t_mode, const T* in, size_t in_len, T* out, size_t out_len,
                        hipStream_t s);
 
+// What one apply reads: the sizes and the device index arrays, owned by
+// whoever made the view (a Restriction, or a caller holding its own device
+// arrays as include/cuda/cuda-elem-restriction.hpp:46-58 does).
+struct RestrictionView {
+  int type = 0;  // 0 offset, 1 strided
+  int num_elem = 0, elem_size = 0, num_comp = 0, comp_stride = 0, l_size = 0;
+  size_t e_size = 0;
+  std::array<int, 3> strides{};
+  int scatter_mode = kScatterAtomic;
+  const int* offsets = nullptr;        // offset: [num_elem][elem_size]
+  int num_nodes = 0;                   // deterministic offset: rows of the transpose map
+  const int* l_vec_indices = nullptr;  // null: row m is L-node m
+  const int* t_offsets = nullptr;
+  const int* t_indices = nullptr;
+};
+RestrictionView restriction_view(const Restriction& r);
+template <class T>
+void restriction_apply_view(const RestrictionView& r, int t_mode, const T* in, size_t in_len, T* out, size_t out_len,
+                            hipStream_t s);
+
 // Deterministic transpose-map sum of an element-output array Y laid out
 // [comp][num_elem*elem_size] (index t = i + e*elem_size): for every used L-node,
 // acc = (zero_start ? 0 : L[l]); acc += Y[t] over its row ascending; L[l] = acc.

@@ -62,7 +62,9 @@ int red_blocks(size_t n) {
 // An iteration that starts done leaves x, r, z and p alone; the iteration that
 // meets the residual test still forms the next search direction (a residual
 // replacement continues from it).
-enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kRZ1, kRR1, kDONE1, kITERS1, kNSCAL };
+// kMAXIT (the iteration limit) and kTHRESH are written by init / replace, so
+// a captured graph follows the limits of the run it is replayed in.
+enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kRZ1, kRR1, kDONE1, kITERS1, kMAXIT, kNSCAL };
 __host__ __device__ constexpr int slot(int s, int par) {
   return par == 0 ? s : (s == kRZ ? kRZ1 : s == kRR ? kRR1 : s == kDONE ? kDONE1 : kITERS1);
 }
@@ -166,7 +168,8 @@ __global__ __launch_bounds__(kThreads) void cg_init_kernel(size_t n, const T* __
 }
 
 __global__ __launch_bounds__(kThreads) void cg_init_scalars_kernel(const double* __restrict__ part,
-                                                                   double* __restrict__ sc, double rtol2, int nb) {
+                                                                   double* __restrict__ sc, double rtol2, int nb,
+                                                                   int max_iter) {
   __shared__ double sh[2][kThreads / 64];
   const double rz = sum_partials(part, nb, sh[0]), rr = sum_partials(part + kMaxRed, nb, sh[1]);
   if (threadIdx.x != 0) return;
@@ -175,6 +178,7 @@ __global__ __launch_bounds__(kThreads) void cg_init_scalars_kernel(const double*
   sc[kTHRESH] = rtol2 * rr;
   sc[kDONE] = (rr <= sc[kTHRESH]) ? 1.0 : 0.0;
   sc[kITERS] = 0.0;
+  sc[kMAXIT] = static_cast<double>(max_iter);
 }
 
 // Residual replacement: r = rnew, z = dinv * r, x = 0 (the increment restarts);
@@ -207,7 +211,8 @@ __global__ __launch_bounds__(kThreads) void cg_replace_kernel(size_t n, const T*
 // After a replacement: new r.z and r.r, the next segment's threshold; the
 // iteration count runs on.
 __global__ __launch_bounds__(kThreads) void cg_replace_scalars_kernel(const double* __restrict__ part,
-                                                                      double* __restrict__ sc, double rtol2, int nb) {
+                                                                      double* __restrict__ sc, double rtol2, int nb,
+                                                                      int max_iter) {
   __shared__ double sh[2][kThreads / 64];
   const double rz = sum_partials(part, nb, sh[0]), rr = sum_partials(part + kMaxRed, nb, sh[1]);
   if (threadIdx.x != 0) return;
@@ -215,6 +220,7 @@ __global__ __launch_bounds__(kThreads) void cg_replace_scalars_kernel(const doub
   sc[kRR] = rr;
   sc[kTHRESH] = rtol2 * rr;
   sc[kDONE] = (rr <= sc[kTHRESH]) ? 1.0 : 0.0;
+  sc[kMAXIT] = static_cast<double>(max_iter);
 }
 
 // The sum of the nb partials at part, the same in every thread and every block
@@ -334,9 +340,9 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int p
 // The end of a CG step, with beta = r.z(new) / r.z(old) formed by every block
 // from the update's partials: p = z + beta p, and (block 0) the state copy
 // 1-par: the iteration count, r.r, r.z and the done test r.r <= threshold or
-// the iteration limit. An iteration that started done copies the state on and
+// the iteration limit (both read from the scalar slots). An iteration that started done copies the state on and
 // leaves p alone (*frozen). Returns beta.
-__device__ double cg_step_end(int par, int max_iter, int nb, double* sc, const double* part, double (*sh)[kThreads / 64],
+__device__ double cg_step_end(int par, int nb, double* sc, const double* part, double (*sh)[kThreads / 64],
                               double* s_val, bool* frozen) {
   const bool done0 = sc[slot(kDONE, par)] != 0.0;
   const double rz_new = all_partials(part + kPartRZ, nb, sh[0], &s_val[0]);
@@ -355,7 +361,7 @@ __device__ double cg_step_end(int par, int max_iter, int nb, double* sc, const d
       sc[slot(kITERS, q)] = iters;
       sc[slot(kRR, q)] = rr;
       sc[slot(kRZ, q)] = rz_new;
-      sc[slot(kDONE, q)] = (rr <= sc[kTHRESH] || iters >= max_iter) ? 1.0 : 0.0;
+      sc[slot(kDONE, q)] = (rr <= sc[kTHRESH] || iters >= sc[kMAXIT]) ? 1.0 : 0.0;
     }
     sc[kBETA] = beta;
   }
@@ -364,14 +370,14 @@ __device__ double cg_step_end(int par, int max_iter, int nb, double* sc, const d
 }
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, int par, int max_iter, int nb,
+__global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, int par, int nb,
                                                             double* __restrict__ sc, const double* __restrict__ part,
                                                             const T* __restrict__ z, T* __restrict__ p) {
   constexpr int W = Pack<T>::W;
   __shared__ double sh[2][kThreads / 64];
   __shared__ double s_val[2];
   bool frozen = false;
-  const T beta = static_cast<T>(cg_step_end(par, max_iter, nb, sc, part, sh, s_val, &frozen));
+  const T beta = static_cast<T>(cg_step_end(par, nb, sc, part, sh, s_val, &frozen));
   if (frozen) return;  // p stays
   const size_t nv = n / W;
   for (size_t iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; iv < nv;
@@ -389,13 +395,13 @@ __global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, int par, i
 }
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, int par, int max_iter, int nb,
+__global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, int par, int nb,
                                                         double* __restrict__ sc, const double* __restrict__ part,
                                                         const T* __restrict__ z, T* __restrict__ p) {
   __shared__ double sh[2][kThreads / 64];
   __shared__ double s_val[2];
   bool frozen = false;
-  const T beta = static_cast<T>(cg_step_end(par, max_iter, nb, sc, part, sh, s_val, &frozen));
+  const T beta = static_cast<T>(cg_step_end(par, nb, sc, part, sh, s_val, &frozen));
   if (frozen) return;  // p stays
   for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * kThreads)
@@ -454,6 +460,7 @@ struct Pcg {
   int batch = 0;
   const T* b_bound = nullptr;  // the b and x the graphs were captured with
   T* x_bound = nullptr;
+  uint64_t sig_bound = 0;      // and the apply's fused_signature
 
   int nb;  // reduction blocks
 
@@ -475,43 +482,46 @@ struct Pcg {
   // p.Ap partials, the update (alpha formed in every block) and the step end
   // (beta and the state copy formed in every block).
   template <class Apply>
-  void iteration(Apply&& apply, T* x, int max_iter, int par, hipStream_t s) {
+  void iteration(Apply&& apply, T* x, int par, hipStream_t s) {
     apply(p.data(), Ap.data(), s);
     double* pap = part.data() + kPartPAP;
     if (aligned16(x)) {  // every other vector is the solver's own allocation
       dot_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), pap);
       cg_update_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), p.data(), Ap.data(), dinv.data(), x,
                                                       r.data(), z.data(), part.data());
-      cg_p_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, max_iter, nb, sc.data(), part.data(), z.data(), p.data());
+      cg_p_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), z.data(), p.data());
     } else {
       dot_partial_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), pap);
       cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), p.data(), Ap.data(), dinv.data(), x,
                                                   r.data(), z.data(), part.data());
-      cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, par, max_iter, nb, sc.data(), part.data(), z.data(), p.data());
+      cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), z.data(), p.data());
     }
   }
 
-  // x = 0 and the first residual, for A x = b with ||r|| <= rtol ||b|| as the goal.
-  void init(const T* b, T* x, double rtol, hipStream_t s) {
+  // x = 0 and the first residual, for A x = b with ||r|| <= rtol ||b|| as the
+  // goal within max_iter iterations.
+  void init(const T* b, T* x, double rtol, int max_iter, hipStream_t s) {
     cg_init_kernel<T><<<nb, kThreads, 0, s>>>(n, b, dinv.data(), x, r.data(), z.data(), p.data(), part.data());
-    cg_init_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb);
+    cg_init_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb, max_iter);
     HIP_CHECK_LAUNCH();
   }
 
   // Residual replacement: r = rnew, x = 0, p kept; the next segment ends when
   // ||r|| <= rtol ||rnew||.
-  void replace(const T* rnew, T* x, double rtol, hipStream_t s) {
+  void replace(const T* rnew, T* x, double rtol, int max_iter, hipStream_t s) {
     cg_replace_kernel<T><<<nb, kThreads, 0, s>>>(n, rnew, dinv.data(), x, r.data(), z.data(), part.data());
-    cg_replace_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb);
+    cg_replace_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb, max_iter);
     HIP_CHECK_LAUNCH();
   }
 
-  // Records the graph of `batch_iters` iterations for this b and x (if it is not
-  // recorded yet). No other thread does device setup or teardown meanwhile.
+  // Records the graph of `batch_iters` iterations for this b and x and the
+  // apply's signature `sig` (if it is not recorded yet: a changed kernel choice
+  // or reallocated operator buffer records it again). No other thread does
+  // device setup or teardown meanwhile.
   template <class Apply>
-  void prepare_graph(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, hipStream_t s) {
+  void prepare_graph(Apply&& apply, const T* b, T* x, uint64_t sig, int batch_iters, hipStream_t s) {
     batch_iters = even_batch(batch_iters);
-    if (!execs.empty() && batch == batch_iters && b_bound == b && x_bound == x) return;
+    if (!execs.empty() && batch == batch_iters && b_bound == b && x_bound == x && sig_bound == sig) return;
     drop_graphs();
     std::vector<int> sizes{batch_iters};  // even sizes only: the state ends in copy 0
     for (int k = 2; k < batch_iters; k *= 2) sizes.push_back(k);
@@ -520,7 +530,7 @@ struct Pcg {
       Exec& e = execs[size];
       HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        for (int k = 0; k < size; ++k) iteration(apply, x, max_iter, k & 1, s);
+        for (int k = 0; k < size; ++k) iteration(apply, x, k & 1, s);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s, &g);
@@ -533,14 +543,15 @@ struct Pcg {
     batch = batch_iters;
     b_bound = b;
     x_bound = x;
+    sig_bound = sig;
   }
 
   // Enqueues `count` iterations (at most batch_iters; the fewest graph
   // launches) and the copy of the scalars into host_sc.
   template <class Apply>
-  void launch_iters(Apply&& apply, const T* b, T* x, int max_iter, int batch_iters, int count, double* host_sc,
+  void launch_iters(Apply&& apply, const T* b, T* x, uint64_t sig, int batch_iters, int count, double* host_sc,
                     hipStream_t s) {
-    prepare_graph(apply, b, x, max_iter, batch_iters, s);
+    prepare_graph(apply, b, x, sig, batch_iters, s);
     batch_iters = even_batch(batch_iters);
     // Whole pairs of iterations: one more than asked costs at most one
     // iteration (and nothing once the run is done).
@@ -707,8 +718,8 @@ void advance(SolveJob& j) {
       j.t0 = std::chrono::steady_clock::now();
       enqueue_bnorm(j);
       if (j.opt.mode == 0) {
-        st.pcg64->init(j.b, j.x, j.opt.rtol, j.s);
-        st.pcg64->launch_iters(apply64, j.b, j.x, j.opt.max_iter, j.batch, start_count(j, 0.0, 0), j.hs, j.s);
+        st.pcg64->init(j.b, j.x, j.opt.rtol, j.opt.max_iter, j.s);
+        st.pcg64->launch_iters(apply64, j.b, j.x, fused_signature(f, 0), j.batch, start_count(j, 0.0, 0), j.hs, j.s);
         j.phase = SolveJob::kIter;
       } else {
         zero_fill(j.x, st.n * sizeof(double), j.s);
@@ -727,13 +738,13 @@ void advance(SolveJob& j) {
         j.phase = SolveJob::kDone;
       } else {
         if (!j.started) {
-          st.pcg32->init(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
+          st.pcg32->init(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.opt.max_iter, j.s);
           j.started = true;
         } else {
-          st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
+          st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.opt.max_iter, j.s);
         }
         const int count = start_count(j, j.opt.inner_rtol * j.opt.inner_rtol, iters);
-        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, count, j.hs, j.s);
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), fused_signature(f, 1), j.batch, count, j.hs, j.s);
         j.phase = SolveJob::kRelIter;
       }
       break;
@@ -753,15 +764,15 @@ void advance(SolveJob& j) {
         const double rel_rec = j.bnorm > 0 ? std::sqrt(std::max(j.hs[kRR], 0.0)) / j.bnorm : 0.0;
         if (rel_rec > 4.0 * j.opt.rtol && iters < j.opt.max_iter && j.outer < j.opt.max_outer) {
           j.spec_iters = iters;
-          st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.s);
+          st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.opt.max_iter, j.s);
           const int count = start_count(j, j.opt.inner_rtol * j.opt.inner_rtol, iters);
-          st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, count, j.hs, j.s);
+          st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), fused_signature(f, 1), j.batch, count, j.hs, j.s);
           j.phase = SolveJob::kRelSpec;
         } else {
           j.phase = SolveJob::kRelCheck;
         }
       } else {
-        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, next_count(j), j.hs,
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), fused_signature(f, 1), j.batch, next_count(j), j.hs,
                                j.s);
       }
       break;
@@ -785,7 +796,7 @@ void advance(SolveJob& j) {
         enqueue_residual(j, nullptr);
         j.phase = SolveJob::kFinal;
       } else {
-        st.pcg64->launch_iters(apply64, j.b, j.x, j.opt.max_iter, j.batch, next_count(j), j.hs, j.s);
+        st.pcg64->launch_iters(apply64, j.b, j.x, fused_signature(f, 0), j.batch, next_count(j), j.hs, j.s);
       }
       break;
     case SolveJob::kFinal:
@@ -803,9 +814,9 @@ void advance(SolveJob& j) {
       } else {
         // No deeper than the target needs: the last step stops at about half of it.
         const double inner = std::max(j.opt.inner_rtol, std::min(0.5 * j.opt.rtol / rel, 0.5));
-        st.pcg32->init(st.r32.data(), st.d32.data(), inner, j.s);
+        st.pcg32->init(st.r32.data(), st.d32.data(), inner, j.opt.max_iter, j.s);
         const int count = start_count(j, inner * inner, 0);
-        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, count, j.hs, j.s);
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), fused_signature(f, 1), j.batch, count, j.hs, j.s);
         j.phase = SolveJob::kInner;
       }
       break;
@@ -819,7 +830,7 @@ void advance(SolveJob& j) {
         enqueue_residual(j, st.r32.data());
         j.phase = SolveJob::kCheck;
       } else {
-        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, next_count(j), j.hs,
+        st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), fused_signature(f, 1), j.batch, next_count(j), j.hs,
                                j.s);
       }
       break;
@@ -896,10 +907,10 @@ void solve_job_prepare_graphs(SolveJob& j) {
   FusedMass& f = *j.f;
   if (j.opt.mode == 0) {
     auto apply64 = [&f](const double* in, double* out, hipStream_t ss) { fused_mass_apply64(f, in, out, ss); };
-    st.pcg64->prepare_graph(apply64, j.b, j.x, j.opt.max_iter, j.batch, j.s);
+    st.pcg64->prepare_graph(apply64, j.b, j.x, fused_signature(f, 0), j.batch, j.s);
   } else {
     auto apply32 = [&f](const float* in, float* out, hipStream_t ss) { fused_mass_apply32(f, in, out, ss); };
-    st.pcg32->prepare_graph(apply32, st.r32.data(), st.d32.data(), j.opt.max_iter, j.batch, j.s);
+    st.pcg32->prepare_graph(apply32, st.r32.data(), st.d32.data(), fused_signature(f, 1), j.batch, j.s);
   }
 }
 

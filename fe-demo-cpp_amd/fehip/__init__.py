@@ -121,6 +121,7 @@ SIGNATURES = {
     "fem_hip_restriction_create_strided": [_i, _i, _i, _i, _vp, ctypes.POINTER(_vp)],
     "fem_hip_restriction_apply": [_vp, _i, _vp, _sz, _vp, _sz, _vp],
     "fem_hip_restriction_apply_f32": [_vp, _i, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_restriction_apply_arrays": [_vp, _i, _vp, _sz, _vp, _sz, _vp],
     "fem_hip_restriction_get_sizes": [_vp, _llp],
     "fem_hip_restriction_get_transpose_map": [_vp, _vp, _vp, _vp],
     "fem_hip_restriction_destroy": [_vp],
@@ -379,7 +380,7 @@ def stream_copy(dst, src, stream=None):
 def rng_uniform_device(seed, n, lo=-1.0, hi=1.0, stream=None):
     """rng_uniform's stream, generated on the device into a new DeviceArray."""
     out = DeviceArray(n)
-    call("fem_hip_rng_uniform_device", seed, n, lo, hi, out.ptr, stream)
+    call("fem_hip_rng_uniform_device", seed, n, lo, hi, out.ptr, _s(stream))
     return out
 
 

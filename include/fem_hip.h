@@ -148,6 +148,30 @@ int fem_hip_restriction_get_sizes(fem_hip_restriction r, long long* sizes);
  * l_vec_indices[num_nodes], t_offsets[num_nodes+1], t_indices[num_elem*elem_size]. */
 int fem_hip_restriction_get_transpose_map(fem_hip_restriction r, int* l_vec_indices, int* t_offsets, int* t_indices);
 int fem_hip_restriction_destroy(fem_hip_restriction r);
+/* A restriction whose device index arrays the caller owns: the shape of the
+ * reference's fem::cuda::ElemRestriction (include/cuda/cuda-elem-restriction.hpp:46-58),
+ * which holds its sizes and DeviceArray<int> offsets, l_vec_indices, t_offsets,
+ * t_indices itself. Every pointer is device memory the caller keeps alive
+ * during the apply; offsets (offset restrictions) hold num_elem*elem_size
+ * entries in [0, l_size) less the component offsets, and the transpose map
+ * (deterministic offset restrictions) holds num_nodes rows as
+ * fem_hip_transpose_map_build makes them. */
+typedef struct {
+  int type; /* 0 offset, 1 strided (include/elem-restriction.hpp:8 RestrictionType) */
+  int num_elem, elem_size, num_comp, comp_stride, l_size;
+  long long e_size;
+  int strides[3];   /* strided: (node, component, element) strides */
+  int scatter_mode; /* offset: FEM_HIP_SCATTER_* */
+  const int* offsets;
+  int num_nodes;
+  const int* l_vec_indices;
+  const int* t_offsets;
+  const int* t_indices;
+} fem_hip_restriction_arrays;
+/* elem_restriction_apply on such a restriction (src/cuda/cuda-elem-restriction.cu:228-273):
+ * the sizes and semantics of fem_hip_restriction_apply, nothing allocated, asynchronous. */
+int fem_hip_restriction_apply_arrays(const fem_hip_restriction_arrays* r, int t_mode, const double* in,
+                                     size_t in_len, double* out, size_t out_len, void* stream);
 /* Host-only transpose map (src/cuda/cuda-elem-restriction.cu:149-177). Arrays must
  * hold l_size, l_size + 1 and num_elem*elem_size ints; *num_nodes receives the row count. */
 int fem_hip_transpose_map_build(int num_elem, int elem_size, int l_size, const int* offsets, int* num_nodes,

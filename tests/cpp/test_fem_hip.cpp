@@ -9,6 +9,8 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "fem_hip.hpp"
@@ -155,6 +157,52 @@ TEST_CASE(restriction_matches_cpu_small_and_3d) {
       check_against_cpu(box_restriction(3, 3, 3, 3, 3, interleaved), mode);
     }
   }
+}
+
+// tests/cuda/test-cuda-elem-restriction.cu:136-157, with fem::cuda:: spelled fem::hip::.
+TEST_CASE(create_copies_sizes_and_index_arrays) {
+  const auto host = box_restriction(2, 2, 1, 2, 3, true);
+  fem::hip::ElemRestriction dev;
+  fem::hip::elem_restriction_create(host, ScatterMode::Deterministic, dev);
+  REQUIRE(dev.num_elem == host.num_elem);
+  REQUIRE(dev.elem_size == host.elem_size);
+  REQUIRE(dev.num_comp == 3);
+  REQUIRE(dev.comp_stride == 1);
+  REQUIRE(dev.l_size == host.l_size);
+  REQUIRE(dev.e_size == host.e_size);
+  REQUIRE(dev.offsets.to_host() == host.offsets);
+
+  fem::hip::TransposeMap map;
+  fem::hip::build_transpose_map(host, map);
+  REQUIRE(dev.num_nodes == static_cast<int>(map.l_vec_indices.size()));
+  REQUIRE(dev.l_vec_indices.to_host() == map.l_vec_indices);
+  REQUIRE(dev.t_offsets.to_host() == map.t_offsets);
+  REQUIRE(dev.t_indices.to_host() == map.t_indices);
+
+  fem::hip::ElemRestriction atomic;  // no transpose map for Atomic
+  fem::hip::elem_restriction_create(host, ScatterMode::Atomic, atomic);
+  REQUIRE(atomic.t_indices.empty());
+}
+
+// The struct is move-only and owns its arrays: a moved-to restriction applies
+// as the original did, and the moved-from one holds no device memory.
+TEST_CASE(restriction_moves_with_its_arrays) {
+  static_assert(!std::is_copy_constructible_v<fem::hip::ElemRestriction>, "move-only, like fem::cuda");
+  const auto host = box_restriction(3, 2, 2, 3, 2, false);
+  fem::hip::ElemRestriction a;
+  fem::hip::elem_restriction_create(host, ScatterMode::Deterministic, a);
+  fem::hip::ElemRestriction b(std::move(a));
+  REQUIRE(a.offsets.empty() && a.t_indices.empty());  // NOLINT(bugprone-use-after-move)
+  const std::vector<double> e_in = wiggly(host.e_size, 0.3);
+  std::vector<double> l_cpu(host.l_size, 1.0);
+  fem::elem_restriction_apply(host, ContractMode::Transpose, e_in, l_cpu);
+  DeviceArray<double> d_e(e_in), d_l(std::vector<double>(host.l_size, 1.0));
+  fem::hip::elem_restriction_apply(b, ContractMode::Transpose, d_e, d_l);
+  REQUIRE(d_l.to_host() == l_cpu);  // deterministic: the CPU's sum order from the current value
+  fem::ElemRestriction bad = host;
+  bad.offsets[5] = host.l_size;  // out of range
+  fem::hip::ElemRestriction c;
+  REQUIRE_THROWS_AS(fem::hip::elem_restriction_create(bad, ScatterMode::Atomic, c), std::invalid_argument);
 }
 
 TEST_CASE(apply_checks_sizes) {

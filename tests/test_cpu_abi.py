@@ -208,3 +208,25 @@ def test_tune_table_save_load_roundtrip(tmp_path):
             fehip.tune_table_load(path)
     finally:
         fehip.tune_table_clear()
+
+
+REF_INCLUDE = "/root/reference/include"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_INCLUDE), reason="the reference is mounted only in the development container")
+def test_dropin_header_compiles_against_reference_headers(tmp_path):
+    """fem_hip.hpp and the drop-in test source compile and link with the
+    reference's own include/ and src/ in place of the oracle's (so the fem::hip
+    types sit on the reference's fem:: types, not on a restatement of them)."""
+    import subprocess
+
+    out = tmp_path / "test_fem_hip_ref"
+    srcs = [os.path.join("/root/reference/src", f) for f in
+            ("basis.cpp", "elem-restriction.cpp", "tensor-contract.cpp", "qfunction.cpp", "operator.cpp")]
+    cmd = ["g++", "-std=c++17", "-O0", "-ffp-contract=off", "-Wall", "-Werror", "-Wno-sign-compare",
+           "-include", "limits", "-I" + REF_INCLUDE, "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(ROOT, "fe-demo-cpp_amd", "cpp"), "-o", str(out),
+           os.path.join(ROOT, "tests", "cpp", "test_fem_hip.cpp")] + srcs + [
+        "-L" + os.path.join(ROOT, "fe-demo-cpp_amd", "lib"), "-lfemhip"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]

@@ -15,3 +15,17 @@ def test_cpp_dropin(gpu):
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+REF_BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                       "test_fem_hip_ref")
+
+
+def test_cpp_dropin_on_reference_headers(gpu):
+    """The same drop-in tests compiled against the reference's own headers and
+    sources (built where the reference is mounted; the binary travels with the tree)."""
+    if not os.path.exists(REF_BIN):
+        pytest.skip("oracle/_ref/test_fem_hip_ref is built only where the reference is mounted")
+    r = subprocess.run([REF_BIN], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr

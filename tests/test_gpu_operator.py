@@ -325,3 +325,34 @@ def test_mass_diagonal_matches_oracle(gpu):
     p.op.mass_diagonal(d)
     want = oracle_problem(p).diagonal()
     assert rel_err(d.to_host(), want) < 1e-13
+
+
+def test_build_mass_with_padded_qdata_l_size(gpu):
+    """A backend-strided qdata restriction may have a larger l_size than
+    num_elem*Q^3 (src/elem-restriction.cpp:155-181); operator_apply then zeroes
+    the whole output first (src/operator.cpp:172), so the tail past the
+    element entries is 0, and the element entries equal the oracle's qdata."""
+    n, Q = 3, 4
+    E, nv = n ** 3, (n + 1) ** 3
+    coords = fehip.box_coords(n, 0.1, 3)
+    moff = fehip.box_offsets(n, n, n, 2, 1)
+    mrstr = fehip.elem_restriction_create(E, 8, 3, nv, 3 * nv, moff)
+    mbasis = fehip.TensorBasis.create_tensor_H1_lagrange(3, 3, 2, Q)
+    pad = 37
+    q_rstr = fehip.elem_restriction_create_strided(E, Q ** 3, 1, E * Q ** 3 + pad)
+    qf = fehip.QFunction(fehip.QF_BUILD_MASS)
+    qf.add_input("dx", 9, fehip.EVAL_GRAD)
+    qf.add_input("weights", 1, fehip.EVAL_WEIGHT)
+    qf.add_output("qdata", 1, fehip.EVAL_NONE)
+    qf.set_context_int(3)
+    bop = fehip.Operator(qf)
+    bop.set_field("dx", mrstr, mbasis, fehip.VECTOR_ACTIVE)
+    bop.set_field("weights", None, mbasis, fehip.VECTOR_NONE)
+    bop.set_field("qdata", q_rstr, None, fehip.VECTOR_ACTIVE)
+    d_c = fehip.DeviceArray.from_host(coords)
+    d_q = fehip.DeviceArray.from_host(np.full(E * Q ** 3 + pad, np.nan))
+    bop.apply(d_c, d_q)
+    got = d_q.to_host()
+    want = oracle().build_qdata(3, E, 2, Q, 3 * nv, nv, moff, coords)
+    assert got[:E * Q ** 3].tobytes() == want.tobytes()
+    assert np.all(got[E * Q ** 3:] == 0.0)

@@ -112,3 +112,50 @@ def test_serve_invalid_request_fails_alone(gpu):
         srv.wait(bad)
     assert srv.wait(good)[0]["status"] == 0
     srv.close()
+
+
+def _residual(req, x):
+    """||b - M x|| / ||b|| in FP64 on an independently built copy of the request's problem."""
+    p = MassProblem(req.n, req.P, req.Q, req.num_comp, jitter=req.jitter, seed=req.mesh_seed)
+    b = fehip.rng_uniform(req.rhs_seed, p.l_size)
+    d_x, d_Mx = fehip.DeviceArray.from_host(x), fehip.DeviceArray(p.l_size)
+    p.op.apply(d_x, d_Mx)
+    return float(np.linalg.norm(b - d_Mx.to_host()) / np.linalg.norm(b))
+
+
+def test_serve_workload_distribution_with_largest_shape(gpu):
+    """Requests drawn from the serving workload's own distribution (n ~ U{4..40},
+    P ~ U{2..6}, Q = P+1, num_comp ~ U{1..3}, FP64 or mixed), with the largest
+    shape it can draw (n=40, P=6, 3 components: 24.4M DoF) added in both modes:
+    every request reaches 1e-10 (the server's FP64 residual, and for the largest
+    ones an FP64 residual recomputed here on an independently built problem),
+    and a subset solved alone equals its result in the mix bit for bit."""
+    print("serving: workload distribution with (40, 6, 3)", flush=True)
+    R = fehip.serve_request
+    big64 = R(40, 6, 7, 3, fehip.SOLVE_FP64, mesh_seed=77, rhs_seed=77)
+    bigmx = R(40, 6, 7, 3, fehip.SOLVE_MIXED, mesh_seed=78, rhs_seed=78)
+    drawn = make_requests(30, seed=11)
+    mix = drawn[:15] + [big64] + drawn[15:] + [bigmx]
+    summary, results = run_workload(mix, rate_per_s=300.0, keep_x=True, max_active=16, workers=8,
+                                    hbm_fraction=0.5)
+    print({k: summary[k] for k in ("completed", "failed", "throughput_req_per_s", "latency_ms")}, flush=True)
+    assert summary["failed"] == 0, summary["errors"]
+    assert summary["server"]["active_peak"] > 1
+    modes = set()
+    for r, (res, x) in zip(mix, results):
+        assert res["status"] == 0 and res["rel_residual"] <= r.rtol, (r.n, r.P, r.num_comp, res)
+        assert len(x) == res["l_size"]
+        modes.add(r.mode)
+    assert modes == {fehip.SOLVE_FP64, fehip.SOLVE_MIXED}
+    for k in (15, len(mix) - 1):
+        rel = _residual(mix[k], results[k][1])
+        print("largest request", k, "recomputed residual", rel, flush=True)
+        assert rel <= 1.01e-10, (k, rel)
+    # Alone: the two largest and two drawn requests, one at a time.
+    srv = fehip.Server(max_active=1, workers=2, hbm_fraction=0.5)
+    for k in (0, 15, 16, len(mix) - 1):
+        r = mix[k]
+        res, x = srv.wait(srv.submit(r), fehip.serve_l_size(r))
+        assert res["status"] == 0
+        assert x.tobytes() == results[k][1].tobytes(), k
+    srv.close()

@@ -90,3 +90,103 @@ def test_solves_with_box_columns(gpu, n, P, Q, nc):
             assert abs(st["iterations"] - it_cpu) <= 2, (st, it_cpu)
         p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
         assert d_x.to_host().tobytes() == x1.tobytes(), mode
+
+
+def _oracle_solution(p, seed=2):
+    orc = oracle().mass(3, p.num_elem, p.P, p.Q, p.nc, p.l_size, p.num_nodes, p.offsets, p.qdata())
+    b = orc.apply(fehip.rng_uniform(seed, p.l_size))
+    x_cpu, it_cpu, rel_cpu = orc.cg(b, orc.diagonal(), rtol=1e-10)
+    assert rel_cpu <= 1e-10
+    return b, x_cpu, it_cpu
+
+
+def test_solve_graphs_follow_option_changes_and_autotune(gpu):
+    """A solve after an algorithm change, a smaller box layer count, an
+    autotune or a tuning-table change (same operator, same b and x) runs the
+    operator's current kernels: the graphs captured by the previous solve are
+    not replayed (they may hold buffers that autotune or a reallocation freed),
+    and every solve reaches the oracle's solution."""
+    print("solve after option changes", flush=True)
+    p = MassProblem(16, 3, 4, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+    b, x_cpu, _ = _oracle_solution(p)
+    d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
+
+    def solve(mode, want_alg=None):
+        fp32 = mode != fehip.SOLVE_FP64
+        if want_alg is not None:
+            assert p.op.fused_choice(fp32)[0] == want_alg
+        st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+        assert st["rel_residual"] <= 1e-10, (mode, want_alg, st)
+        assert rel_err(d_x.to_host(), x_cpu) < 1e-8, (mode, want_alg)
+
+    fehip.tune_table_clear()
+    try:
+        for mode in (fehip.SOLVE_FP64, fehip.SOLVE_MIXED, fehip.SOLVE_MIXED_RELIABLE):
+            p.op.set_fused_algorithm(fehip.FUSED_BOX_COLUMN)
+            solve(mode, fehip.FUSED_BOX_COLUMN)
+            p.op.set_box_layers(2)  # more segments: the partial area grows (reallocated)
+            solve(mode, fehip.FUSED_BOX_COLUMN)
+            p.op.set_box_layers(0)
+            p.op.set_fused_algorithm(fehip.FUSED_ELEM_TMAP)
+            solve(mode, fehip.FUSED_ELEM_TMAP)
+            p.op.set_fused_algorithm(fehip.FUSED_BLOCK_PARTIAL)
+            solve(mode, fehip.FUSED_BLOCK_PARTIAL)
+            p.op.set_fused_algorithm(fehip.FUSED_AUTO)
+            (alg, ne, _), _ = p.op.autotune(fp32=mode != fehip.SOLVE_FP64, reps=2)  # frees unused plans
+            assert p.op.fused_choice(mode != fehip.SOLVE_FP64) == (alg, ne)
+            solve(mode, alg)
+            # A table row set by hand changes the automatic choice with no option call.
+            fp32 = mode != fehip.SOLVE_FP64
+            other = fehip.FUSED_ELEM_TMAP if alg != fehip.FUSED_ELEM_TMAP else fehip.FUSED_BLOCK_PARTIAL
+            p.op.set_fused_algorithm(other)
+            ne_o = p.op.fused_choice(fp32)[1]  # its default packing
+            p.op.set_fused_algorithm(fehip.FUSED_AUTO)
+            box = int(fehip.box_detect(p.num_elem, 3, p.offsets) is not None)  # the row autotune wrote
+            fehip.tune_table_set(3, 4, 1, int(fp32), 1, other, ne_o, 0.001, p.num_elem, box=box)
+            solve(mode, other)
+            fehip.tune_table_clear()
+    finally:
+        fehip.tune_table_clear()
+
+
+def test_solve_iteration_limit_per_call(gpu):
+    """max_iter belongs to each solve, not to the graphs the first solve
+    captured: a short limit, then a long one, then a short one again, on the
+    same operator, b and x."""
+    print("solve iteration limits", flush=True)
+    p = MassProblem(8, 3, 4, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+    d_b = fehip.DeviceArray.from_host(fehip.rng_uniform(5, p.l_size))
+    d_x = fehip.DeviceArray(p.l_size)
+    s1 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_FP64, rtol=1e-10, max_iter=5)
+    assert s1["iterations"] == 5 and s1["rel_residual"] > 1e-10, s1
+    s2 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_FP64, rtol=1e-10, max_iter=2000)
+    assert s2["rel_residual"] <= 1e-10 and s2["iterations"] > 5, s2
+    s3 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_FP64, rtol=1e-10, max_iter=3)
+    assert s3["iterations"] == 3, s3
+    # Mixed: the limit is per inner FP32 solve; two outer steps of at most 3.
+    m1 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED, rtol=1e-10, max_iter=3, max_outer=2)
+    assert m1["outer"] == 2 and m1["iterations"] <= 6 and m1["rel_residual"] > 1e-10, m1
+    m2 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED, rtol=1e-10)
+    assert m2["rel_residual"] <= 1e-10, m2
+    m3 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED, rtol=1e-10, max_iter=2, max_outer=1)
+    assert m3["outer"] == 1 and m3["iterations"] <= 2, m3
+
+
+@pytest.mark.parametrize("alg", [fehip.FUSED_AUTO, fehip.FUSED_BOX_COLUMN])
+def test_high_order_vector_mass_solves(gpu, alg):
+    """The high-order vector mass (P=8, Q=10, three components) at a reduced
+    mesh: FP64 PCG and the FP32-apply / FP64-refined solve both reach 1e-10 and
+    the oracle CG's solution (the reference has no solver; the oracle CG runs on
+    the restated operator_apply)."""
+    print("P=8 Q=10 nc=3 solves, alg", alg, flush=True)
+    p = MassProblem(4, 8, 10, 3, scatter=fehip.SCATTER_DETERMINISTIC)
+    p.op.set_fused_algorithm(alg)
+    b, x_cpu, it_cpu = _oracle_solution(p)
+    d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
+    s64 = p.op.solve(d_b, d_x, mode=fehip.SOLVE_FP64, rtol=1e-10)
+    assert s64["rel_residual"] <= 1e-10 and abs(s64["iterations"] - it_cpu) <= 2, (s64, it_cpu)
+    assert rel_err(d_x.to_host(), x_cpu) < 1e-8
+    smx = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED, rtol=1e-10)
+    assert smx["rel_residual"] <= 1e-10 and smx["outer"] >= 2, smx
+    assert rel_err(d_x.to_host(), x_cpu) < 1e-8
+    print("fp64", s64, "mixed", smx, "oracle iterations", it_cpu, flush=True)

@@ -39,6 +39,8 @@ struct fem_hip_operator_s {
 
 namespace femhip {
 int probe_launch(int blocks, int threads);
+size_t noalloc_scratch_capacity();
+void* noalloc_scratch_device_ptr();
 }
 
 namespace {
@@ -518,6 +520,24 @@ int fem_hip_operator_apply_f32(fem_hip_operator op, const float* in, size_t in_l
     operator_apply_f32(*op->op, in, in_len, out, out_len, st(s));
   });
 }
+int fem_hip_operator_apply_noalloc(fem_hip_operator h, const double* in, size_t in_len, double* out, size_t out_len,
+                                   void* s) {
+  return guard([&] {
+    need(h, "operator");
+    Operator& op = *h->op;
+    operator_check_ready(op);
+    require(in_len == static_cast<size_t>(op.active_in_l_size),
+            "operator_apply: input vector size does not match active_in_l_size");
+    require(out_len == static_cast<size_t>(op.active_out_l_size),
+            "operator_apply: output vector size does not match active_out_l_size");
+    if (!op.fused_checked || op.fused_kind != 1 || !op.fused)
+      throw Unsupported("apply_noalloc: needs a fused mass operator whose device state exists (apply it once with "
+                        "fem_hip_operator_apply first)");
+    FusedMass& f = operator_fused_mass(op, st(s));  // exists: takes the current options, allocates nothing
+    fused_mass_apply64_scratch(f, in, out, noalloc_scratch_device_ptr(), noalloc_scratch_capacity(), st(s));
+  });
+}
+size_t fem_hip_noalloc_scratch_bytes(void) { return femhip::noalloc_scratch_capacity(); }
 int fem_hip_operator_get_info(fem_hip_operator h, long long* info) {
   return guard([&] {
     need(h, "operator");

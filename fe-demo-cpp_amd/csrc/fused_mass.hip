@@ -960,6 +960,74 @@ void fused_timing_end(FusedMass& f, double* kernel_ms, double* aux_ms, int* coun
   f.events_used = 0;
 }
 
+namespace {
+
+// Whether the element + transpose-map path can run with nothing made: the
+// restriction carries its own map (deterministic restrictions build it at
+// create).
+bool tmap_without_alloc(const FusedMass& f) {
+  const Restriction& r = *f.rstr;
+  return f.tm_built || (r.type == 0 && r.scatter_mode == kScatterDeterministic && !r.t_offsets.empty());
+}
+
+}  // namespace
+
+size_t fused_scratch_bytes64(const FusedMass& f) {
+  const FusedKey k = fused_resolve(f, 0);
+  if (alg_box(k.alg)) return std::max<size_t>(box_part_len(f, k), 1) * f.NC * sizeof(double);
+  if (k.alg == kAlgElemTmap) return static_cast<size_t>(f.num_elem) * f.P * f.P * f.P * f.NC * sizeof(double);
+  return 0;
+}
+
+void fused_mass_apply64_scratch(FusedMass& f, const double* u, double* v, void* scratch, size_t scratch_bytes,
+                                hipStream_t s) {
+  const FusedKey k = fused_resolve(f, 0);
+  FusedLauncher L = find(k);
+  require(L != nullptr, "fused mass: variant not compiled");
+  if (!(alg_box(k.alg) || k.alg == kAlgElemTmap))
+    throw Unsupported("apply_noalloc: the operator's algorithm is neither the box columns nor the element + "
+                      "transpose-map scatter (the algorithms whose only per-apply buffer is the scratch)");
+  const size_t need = fused_scratch_bytes64(f);
+  if (need > scratch_bytes)
+    throw std::invalid_argument("apply_noalloc: the apply needs " + std::to_string(need) +
+                                " bytes of scratch, more than the " + std::to_string(scratch_bytes) + " available");
+  require(scratch != nullptr && reinterpret_cast<uintptr_t>(scratch) % 16 == 0, "apply_noalloc: bad scratch pointer");
+  FusedArgs a;
+  a.comp_stride = f.comp_stride;
+  a.u = u;
+  a.basis = f.B64.data();
+  a.qdata = f.qdata64.data();  // made at create
+  if (alg_box(k.alg)) {
+    require(box_fits(f, k.elems), "fused mass: the box-column algorithm needs a structured box its columns tile");
+    a.box_n[0] = f.box_n[0];
+    a.box_n[1] = f.box_n[1];
+    a.box_n[2] = f.box_n[2];
+    a.box_lz = box_lz_of(f, k);
+    a.v = v;
+    a.partials = scratch;  // every slot is written before the second pass reads it: no clearing
+    a.part_comp_len = std::max<size_t>(box_part_len(f, k), 1);
+    a.box_pass = 0;
+    L(a, s);
+    a.box_pass = 1;
+    L(a, s);
+    return;
+  }
+  if (!tmap_without_alloc(f))
+    throw Unsupported("apply_noalloc: the transpose-map scatter needs a deterministic restriction (its map is made "
+                      "at create)");
+  ensure_tmap(f);  // points at the restriction's map: no device memory
+  a.num_elem = f.num_elem;
+  a.offsets = f.rstr->offsets.data();
+  a.flagged_offsets = f.rstr->offsets.data();
+  a.y_comp_len = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+  a.v = v;
+  a.out = scratch;  // element outputs: every entry is written before the row sums read it
+  if (!f.tm_covers_all) zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(double), s);  // a fill kernel
+  L(a, s);
+  launch_row_sum<double>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows_p, f.tm_ptr_p, f.tm_idx_p,
+                         static_cast<const double*>(scratch), a.y_comp_len, v, s);
+}
+
 uint64_t fused_signature(const FusedMass& f, int fp32) {
   const FusedKey k = fused_resolve(f, fp32);
   const long long parts[] = {k.P, k.Q, k.NC, k.fp32, k.alg, k.exact, k.elems, alg_box(k.alg) ? box_lz_of(f, k) : 0,

@@ -187,6 +187,16 @@ FusedKey fused_resolve(const FusedMass& f, int fp32);
 // equal signatures mean a graph captured at the first still launches the same
 // kernels on live buffers at the second.
 uint64_t fused_signature(const FusedMass& f, int fp32);
+// FP64 apply that allocates nothing: the resolved algorithm's per-apply
+// scratch (the box-column partials, or the transpose-map algorithm's element
+// outputs) lives at `scratch` (scratch_bytes, device memory the caller owns);
+// everything else the kernels read must exist already (made at create, or
+// from the restriction's own transpose map). Throws Unsupported for any other
+// algorithm, and invalid_argument when the scratch is too small.
+// fused_scratch_bytes64 gives the bytes an apply would need.
+size_t fused_scratch_bytes64(const FusedMass& f);
+void fused_mass_apply64_scratch(FusedMass& f, const double* u, double* v, void* scratch, size_t scratch_bytes,
+                                hipStream_t s);
 // Builds lazily made state (block plans, FP32 copies) for the current choice,
 // so later applies allocate nothing (graph capture).
 void fused_prepare(FusedMass& f, int fp32, hipStream_t s);

@@ -143,6 +143,8 @@ SIGNATURES = {
     "fem_hip_operator_check_ready": [_vp],
     "fem_hip_operator_apply": [_vp, _vp, _sz, _vp, _sz, _vp],
     "fem_hip_operator_apply_f32": [_vp, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_operator_apply_noalloc": [_vp, _vp, _sz, _vp, _sz, _vp],
+    "fem_hip_noalloc_scratch_bytes": [],
     "fem_hip_operator_get_info": [_vp, _llp],
     "fem_hip_operator_set_fused_options": [_vp, _i, _i, _i],
     "fem_hip_operator_set_fused_algorithm": [_vp, _i, _i],
@@ -177,7 +179,8 @@ SIGNATURES = {
     "fem_hip_box_column_check": [_i, _i, _i, _i, _i, _i, _llp],
     "fem_hip_mass_solve": [_vp, _vp, _vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveStats), _vp],
 }
-_RESTYPES = {"fem_hip_last_error": ctypes.c_char_p, "fem_hip_version": ctypes.c_char_p}
+_RESTYPES = {"fem_hip_last_error": ctypes.c_char_p, "fem_hip_version": ctypes.c_char_p,
+             "fem_hip_noalloc_scratch_bytes": ctypes.c_size_t}
 
 _lib = None
 
@@ -685,6 +688,11 @@ class Operator:
             call("fem_hip_operator_apply_f32", self.h, in_ptr, in_len, d_out.ptr, d_out.size, _s(stream))
         else:
             call("fem_hip_operator_apply", self.h, in_ptr, in_len, d_out.ptr, d_out.size, _s(stream))
+
+    def apply_noalloc(self, d_in, d_out, stream=None):
+        """FP64 apply that allocates no device memory (scratch in a library-owned
+        device array; the operator's state must exist: apply it once first)."""
+        call("fem_hip_operator_apply_noalloc", self.h, d_in.ptr, d_in.size, d_out.ptr, d_out.size, _s(stream))
 
     def timing_begin(self):
         call("fem_hip_operator_timing_begin", self.h)

@@ -295,6 +295,21 @@ int fem_hip_tune_table_load(const char* path, int* rows_taken);
 /* FP32 fused mass apply on FP32 L-vectors (qdata and basis rounded to FP32 once). */
 int fem_hip_operator_apply_f32(fem_hip_operator op, const float* in, size_t in_len, float* out, size_t out_len,
                                void* stream);
+/* fem_hip_operator_apply with no device allocation. No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). A fused mass operator
+ * whose device state exists (made by its first fem_hip_operator_apply) runs
+ * its FP64 apply with the per-apply scratch (box-column partials, or the
+ * element outputs of the transpose-map scatter) in a library-owned device
+ * array of fem_hip_noalloc_scratch_bytes() per GPU (64 MiB, held until the
+ * process ends); results equal fem_hip_operator_apply's bit for bit. Rejects
+ * an apply needing more scratch (FEM_HIP_ERR_INVALID_ARGUMENT) and any other
+ * algorithm, or a missing device state (FEM_HIP_ERR_UNSUPPORTED). Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race and must be ordered by the caller. */
+int fem_hip_operator_apply_noalloc(fem_hip_operator op, const double* in, size_t in_len, double* out, size_t out_len,
+                                   void* stream);
+size_t fem_hip_noalloc_scratch_bytes(void);
 /* Jacobi diagonal of a fused mass operator (FP64 L-vector of active_out_l_size). */
 int fem_hip_operator_mass_diagonal(fem_hip_operator op, double* diag, size_t len, void* stream);
 /* Kernel-level timing of the fused path with HIP events on the apply's stream:

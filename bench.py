@@ -401,9 +401,12 @@ def main():
         elif alg == fehip.FUSED_ELEM_TMAP_PIPE:
             kernel_names[label] = "fused_mass_pipe_kernel<%d, %d, 1, %s, %d>" % (P, Q, tname, ne)
         elif alg == fehip.FUSED_BOX_COLUMN:
-            bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne]
-            # the last template argument is how u reaches the x-rows: rows (FP64), LDS ring (FP32)
-            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d, %d>" % (P, Q, tname, bx, by, 2 if fp32 else 0)
+            bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne % 100]
+            # the last two template arguments: how u reaches the x-rows (2: the LDS ring,
+            # FP32 and FP64 elems >= 200; 0: rows) and whether qdata is loaded a layer
+            # ahead (1; 0 for elems 100..199)
+            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d, %d, %d>" % (
+                P, Q, tname, bx, by, 2 if fp32 or ne >= 200 else 0, 0 if ne // 100 == 1 else 1)
         else:
             kernel_names[label] = {
                 fehip.FUSED_BLOCK_PARTIAL: "fused_mass_part_kernel<%d, %d, 1, %s, %d, false>",

@@ -797,10 +797,11 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
     g.Nz = nz * (P - 1) + 1;
     g.ncx = nx / BX;
     g.ncy = ny / BY;
-    require(layers >= 0 && (layers == 0 || nz % layers == 0), "box_column_check: layers must divide nz");
-    g.lz = layers > 0 ? layers : box_layers(nz, static_cast<long long>(g.ncx) * g.ncy, 32 * 256 / elems, P);
-    require(g.lz > 0, "box_column_check: no layer count fits");
-    g.nsz = nz / g.lz;
+    require(layers >= 0, "box_column_check: layers must be >= 0");
+    // Automatic: as the apply chooses on 256 CUs (32 elements per column layer per CU).
+    g.nsz = box_segments(nz, static_cast<long long>(g.ncx) * g.ncy, 32 * 256 / (BX * BY), P, layers);
+    require(g.nsz > 0, "box_column_check: no segment count fits");
+    g.lz = (nz + g.nsz - 1) / g.nsz;
     g.cs = g.Nx * g.Ny * g.Nz;
     const int LXn = BX * (P - 1) + 1, LYn = BY * (P - 1) + 1, LZn = g.lz * (P - 1) + 1;
     require(LXn >= 3 && LYn >= 3 && LZn >= 3, "box_column_check: columns too thin");
@@ -816,15 +817,16 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
     for (long long w = 0; w < nseg; ++w) {
       const int cx = static_cast<int>(w % g.ncx), cy = static_cast<int>((w / g.ncx) % g.ncy),
                 sz = static_cast<int>(w / (static_cast<long long>(g.ncx) * g.ncy));
-      for (int lz = 0; lz < LZn; ++lz)
+      const int z0 = box_seg_z0(g, sz), LZs = (box_seg_z0(g, sz + 1) - z0) * (P - 1) + 1;  // this segment's planes
+      for (int lz = 0; lz < LZs; ++lz)
         for (int ly = 0; ly < LYn; ++ly)
           for (int lx = 0; lx < LXn; ++lx) {
             const bool xs = (lx == 0 && cx > 0) || (lx == LXn - 1 && cx < g.ncx - 1);
             const bool ys = (ly == 0 && cy > 0) || (ly == LYn - 1 && cy < g.ncy - 1);
-            const bool zs = (lz == 0 && sz > 0) || (lz == LZn - 1 && sz < g.nsz - 1);
+            const bool zs = (lz == 0 && sz > 0) || (lz == LZs - 1 && sz < g.nsz - 1);
             const long long node = (cx * (LXn - 1) + lx) +
                                    static_cast<long long>(g.Nx) * ((cy * (LYn - 1) + ly) +
-                                                                   static_cast<long long>(g.Ny) * (sz * (LZn - 1) + lz));
+                                                                   static_cast<long long>(g.Ny) * (z0 * (P - 1) + lz));
             if (!(xs || ys || zs)) {
               ++direct[node];
               ++direct_nodes;

@@ -5,8 +5,9 @@
 // about the mesh is read from memory: only qdata, u and v move.
 //
 // The elements are cut into columns of BX x BY elements in x and y, and each
-// column into segments of lz element layers in z; one workgroup owns one
-// segment and marches up its layers. Per layer:
+// column into nsz segments in z, of floor(nz/nsz) or ceil(nz/nsz) element
+// layers (segment s starts at layer s*nz/nsz, rounded down); one workgroup owns
+// one segment and marches up its layers. Per layer:
 //   * each of the BX*BY elements runs the fast kernel's five in-place stages
 //     in its own LDS tile (u gathered straight from its x-rows), each 1D
 //     product in the even-odd form of the symmetric basis (eo_forward);
@@ -23,11 +24,14 @@
 // result is the same bit for bit on every run, and no atomics are used.
 //
 // Partial area of segment w: face_size = 2*LYn*LZn + 2*LXn*LZn + 2*LXn*LYn
-// slots (LXn = BX*(P-1)+1, LYn = BY*(P-1)+1, LZn = lz*(P-1)+1 nodes), slot by
-// the first shared direction of the node (x, else y, else z): see box_slot.
+// slots (LXn = BX*(P-1)+1, LYn = BY*(P-1)+1, LZn = lz*(P-1)+1 nodes with lz
+// the longest segment's layer count), slot by the first shared direction of
+// the node (x, else y, else z): see box_slot.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "fused_mass_kernel.hpp"
 
@@ -38,9 +42,18 @@ struct BoxGeom {
   int nx, ny, nz;     // elements per direction
   int Nx, Ny, Nz;     // nodes per direction
   int ncx, ncy, nsz;  // columns in x, y; segments per column
-  int lz;             // element layers per segment
+  int lz;             // element layers of the longest segment (ceil(nz / nsz))
   int cs;             // component stride of u and v (= Nx*Ny*Nz)
 };
+
+// First element layer of segment s (s = nsz gives nz).
+__host__ __device__ inline int box_seg_z0(const BoxGeom& g, int s) {
+  return static_cast<int>(static_cast<long long>(s) * g.nz / g.nsz);
+}
+// The segment holding element layer l (0 <= l < nz).
+__host__ __device__ inline int box_seg_of_layer(const BoxGeom& g, int l) {
+  return static_cast<int>((static_cast<long long>(l + 1) * g.nsz - 1) / g.nz);
+}
 
 __host__ __device__ inline long long box_face_size(int LXn, int LYn, int LZn) {
   return 2LL * LYn * LZn + 2LL * LXn * LZn + 2LL * LXn * LYn;
@@ -75,13 +88,13 @@ __host__ __device__ inline long long box_shared_count(const BoxGeom& g, int P, i
 // The shared-node pass runs on a grid of (face planes) x (lines) x (points
 // along a line): plane f < ncx-1 is the x-face gx = (f+1)*SX with lines gz and
 // points gy; the next ncy-1 planes are y-faces gy = (f'+1)*SY, lines gz,
-// points gx; the last nsz-1 are segment ends gz = (f''+1)*SZ, lines gy, points
+// points gx; the last nsz-1 are segment ends gz = (P-1)*box_seg_z0(f''+1), lines gy, points
 // gx. A point on a cut of an earlier kind (x before y before z) belongs to
 // that plane and is skipped here. Returns whether (plane, line, point) is a
 // shared node of this plane, and its coordinates.
 __host__ __device__ inline bool box_shared_point(const BoxGeom& g, int P, int BX, int BY, int plane, int line,
                                                  int point, int* gx, int* gy, int* gz) {
-  const int SX = BX * (P - 1), SY = BY * (P - 1), SZ = g.lz * (P - 1);
+  const int SX = BX * (P - 1), SY = BY * (P - 1);
   const int nbx = g.ncx - 1, nby = g.ncy - 1;
   auto xcut = [&](int x) { return x % SX == 0 && x > 0 && x < g.Nx - 1; };
   auto ycut = [&](int y) { return y % SY == 0 && y > 0 && y < g.Ny - 1; };
@@ -104,7 +117,7 @@ __host__ __device__ inline bool box_shared_point(const BoxGeom& g, int P, int BX
   if (line >= g.Ny || point >= g.Nx || ycut(line) || xcut(point)) return false;
   *gx = point;
   *gy = line;
-  *gz = (plane + 1) * SZ;
+  *gz = box_seg_z0(g, plane + 1) * (P - 1);
   return true;
 }
 
@@ -112,22 +125,24 @@ __host__ __device__ inline bool box_shared_point(const BoxGeom& g, int P, int BX
 // node's slot in each; returns how many (1, 2, 4 or 8).
 __host__ __device__ inline int box_owners(const BoxGeom& g, int P, int BX, int BY, int gx, int gy, int gz, int* seg,
                                           int* slot) {
-  const int SX = BX * (P - 1), SY = BY * (P - 1), SZ = g.lz * (P - 1);
-  const int LXn = SX + 1, LYn = SY + 1, LZn = SZ + 1;
+  const int SX = BX * (P - 1), SY = BY * (P - 1);
+  const int LXn = SX + 1, LYn = SY + 1, LZn = g.lz * (P - 1) + 1;
   const bool xs = gx % SX == 0 && gx > 0 && gx < g.Nx - 1;
   const bool ys = gy % SY == 0 && gy > 0 && gy < g.Ny - 1;
-  const bool zs = gz % SZ == 0 && gz > 0 && gz < g.Nz - 1;
-  int cx1 = gx / SX, cy1 = gy / SY, sz1 = gz / SZ;
+  int lyr = gz / (P - 1);
+  if (lyr > g.nz - 1) lyr = g.nz - 1;
+  const int sz1 = box_seg_of_layer(g, lyr);
+  const bool zs = gz % (P - 1) == 0 && sz1 > 0 && gz == box_seg_z0(g, sz1) * (P - 1);
+  int cx1 = gx / SX, cy1 = gy / SY;
   if (cx1 > g.ncx - 1) cx1 = g.ncx - 1;
   if (cy1 > g.ncy - 1) cy1 = g.ncy - 1;
-  if (sz1 > g.nsz - 1) sz1 = g.nsz - 1;
   const int cx0 = xs ? cx1 - 1 : cx1, cy0 = ys ? cy1 - 1 : cy1, sz0 = zs ? sz1 - 1 : sz1;
   int n = 0;
   for (int s = sz0; s <= sz1; ++s)
     for (int cy = cy0; cy <= cy1; ++cy)
       for (int cx = cx0; cx <= cx1; ++cx) {
         seg[n] = (s * g.ncy + cy) * g.ncx + cx;
-        slot[n] = box_slot(gx - cx * SX, gy - cy * SY, gz - s * SZ, xs, ys, LXn, LYn, LZn);
+        slot[n] = box_slot(gx - cx * SX, gy - cy * SY, gz - box_seg_z0(g, s) * (P - 1), xs, ys, LXn, LYn, LZn);
         ++n;
       }
   return n;
@@ -137,13 +152,9 @@ __host__ __device__ inline int box_owners(const BoxGeom& g, int P, int BX, int B
 // (nontemporal) data, which leaves the caches to the u rows that neighbouring
 // elements and columns read again: 64^3, P=5 FP64 box kernel 0.192 -> 0.177 ms
 // on MI355X. FP32 measured the same either way and keeps plain loads.
-template <class T>
-__device__ __forceinline__ T load_qdata(const T* p) {
-  if constexpr (sizeof(T) == 8) {
-    return __builtin_nontemporal_load(p);
-  } else {
-    return *p;
-  }
+template <int Q, class T>
+__device__ __forceinline__ void load_qdata(const T* tile, int t, T (&q)[Q]) {
+  load_qline<Q, T, sizeof(T) == 8>(tile, t, q);
 }
 
 // Even-odd form of a symmetric 1D interpolation matrix B [Q][P]
@@ -249,7 +260,11 @@ struct BoxTile<5, 6> {
 // UM: how u reaches the x-rows. 0: each row thread loads its row when the
 // step starts; 1: the row of the next step is loaded one step ahead; 2: the
 // node planes go through an LDS ring, the next layer's loaded one layer ahead.
-template <int P, int Q, int NC, class T, int BX, int BY, int UM>
+// QA: 1: a layer's qdata is loaded during the layer before it (held in
+// registers for a whole layer); 0: it is loaded in its own layer, right after
+// the first step's u rows, two stages before its use (12 fewer FP64 VGPRs at
+// Q=6, so more workgroups fit on a CU).
+template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA>
 __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_kernel(
     BoxGeom g, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
     size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em) {
@@ -272,8 +287,10 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   int w = blockIdx.x;
   if (nwg % 8 == 0) w = (blockIdx.x % 8) * (nwg / 8) + blockIdx.x / 8;
   const int cx = w % g.ncx, cy = (w / g.ncx) % g.ncy, sz = w / (g.ncx * g.ncy);
-  const int LZn = g.lz * (P - 1) + 1;
-  const int gx0 = cx * BX * (P - 1), gy0 = cy * BY * (P - 1), gz0 = sz * g.lz * (P - 1);
+  const int ez0 = box_seg_z0(g, sz), nl = box_seg_z0(g, sz + 1) - ez0;  // this segment's layers
+  const int LZn = g.lz * (P - 1) + 1;  // slot layout: the longest segment's node planes
+  const int LZs = nl * (P - 1) + 1;    // this segment's node planes
+  const int gx0 = cx * BX * (P - 1), gy0 = cy * BY * (P - 1), gz0 = ez0 * (P - 1);
   const size_t Nxy = static_cast<size_t>(g.Nx) * g.Ny;
   T* my_part = partials + static_cast<size_t>(w) * box_face_size(LXn, LYn, LZn);
 
@@ -336,10 +353,10 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   };
   auto qd_ptr = [&](int ez) -> const T* {
     const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ez);
-    return qdata + e * Q3 + t;
+    return qdata + e * Q3;
   };
   if constexpr (UM == 2) {
-    const int gzb = sz * g.lz * (P - 1);  // all P planes of the first layer
+    const int gzb = gz0;  // all P planes of the first layer
     for (int idx = threadIdx.x; idx < NC * P * PLANE; idx += NT) {
       const int c = idx / (P * PLANE), r = idx - c * (P * PLANE), j = r / PLANE, p = r - j * PLANE;
       s_u[c][(gzb + j) % P][p] = u[static_cast<size_t>(c) * g.cs + node_addr(gzb + j, p)];
@@ -348,17 +365,13 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   T xc[UM == 1 ? P : 1] = {};
   if constexpr (UM == 1) {
     if (row_thread) {
-      const T* r = row_ptr(sz * g.lz, 0);
+      const T* r = row_ptr(ez0, 0);
 #pragma unroll
       for (int ix = 0; ix < P; ++ix) xc[ix] = r[ix];
     }
   }
   T qc[Q] = {};
-  if (line_thread) {
-    const T* r = qd_ptr(sz * g.lz);
-#pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qc[qz] = load_qdata(r + qz * Q2);
-  }
+  if (QA && line_thread) load_qdata<Q, T>(qd_ptr(ez0), t, qc);
   if constexpr (UM == 2) __syncthreads();
   // Nothing loaded above is still in flight when the layer loop starts. (Those
   // loads target registers that later stages reuse; left in flight, the
@@ -367,15 +380,15 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   __builtin_amdgcn_s_waitcnt(0);
 
 #pragma unroll 1
-  for (int k = 0; k < g.lz; ++k) {
-    const int ez = sz * g.lz + k;
+  for (int k = 0; k < nl; ++k) {
+    const int ez = ez0 + k;
     T qn[Q] = {};
     T qd[Q];
 #pragma unroll
     for (int qz = 0; qz < Q; ++qz) qd[qz] = qc[qz];
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {
-      const bool more = k + 1 < g.lz;
+      const bool more = k + 1 < nl;
       const int gzn = (ez + 1) * (P - 1) + 1;  // first new plane of the next layer
       T un[UPT] = {};
       T xn[UM == 1 ? P : 1] = {};
@@ -390,8 +403,8 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
         }
       } else if constexpr (UM == 1) {
         const int cn = c + 1 < NC ? c + 1 : 0, kn = c + 1 < NC ? k : k + 1;
-        if (row_thread && kn < g.lz) {
-          const T* r = row_ptr(sz * g.lz + kn, cn);
+        if (row_thread && kn < nl) {
+          const T* r = row_ptr(ez0 + kn, cn);
 #pragma unroll
           for (int ix = 0; ix < P; ++ix) xn[ix] = r[ix];
         }
@@ -413,17 +426,20 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
           for (int ix = 0; ix < P; ++ix) x[ix] = r[ix];
         }
       }
-      // The next layer's qdata, issued after this step's u loads: outstanding
-      // loads complete in issue order, so S1's wait for u leaves it in flight.
-      // Every lane loads (lanes past the lines and the last layer read a valid
-      // row they do not use), so the number of loads in flight is known where
-      // S1 waits.
+      // The next layer's qdata (QA = 1) or this layer's (QA = 0), issued
+      // after this step's u loads: outstanding loads complete in issue order,
+      // so S1's wait for u leaves it in flight. Every lane loads (lanes past the
+      // lines and the last layer read a valid row they do not use), so the
+      // number of loads in flight is known where S1 waits.
       if (NC == 1 || c == 0) {
-        const int tq = t < Q2 ? t : Q2 - 1, ezq = k + 1 < g.lz ? ez + 1 : ez;
+        const int tq = t < Q2 ? t : Q2 - 1, ezq = QA && k + 1 < nl ? ez + 1 : ez;
         const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ezq);
-        const T* r = qdata + e * Q3 + tq;
-#pragma unroll
-        for (int qz = 0; qz < Q; ++qz) qn[qz] = load_qdata(r + qz * Q2);
+        const T* r = qdata + e * Q3;
+        if constexpr (QA) {
+          load_qdata<Q, T>(r, tq, qn);
+        } else {
+          load_qdata<Q, T>(r, tq, qd);
+        }
       }
       if (row_thread) {
         const int iz = riz, iy = riy;
@@ -496,11 +512,11 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
                   t3 = tb[toff3 + j * ZS];
           T sum = (j == 0 && k > 0) ? s_top[c][pos] + t0 : t0;
           sum = ((sum + t1) + t2) + t3;
-          if (j == P - 1 && k + 1 < g.lz) {
+          if (j == P - 1 && k + 1 < nl) {
             s_top[c][pos] = sum;
           } else {
             const int gz = ez * (P - 1) + j, lzl = gz - gz0;
-            const bool zs = (lzl == 0 && sz > 0) || (lzl == LZn - 1 && sz < g.nsz - 1);
+            const bool zs = (lzl == 0 && sz > 0) || (lzl == LZs - 1 && sz < g.nsz - 1);
             if (!(pxs || pys || zs)) {
               vc[Nxy * gz] = sum;
             } else {
@@ -527,8 +543,11 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
       (void)gzn;
       __syncthreads();  // the next step's S1 overwrites the tiles (and reads the ring)
     }
+    if constexpr (QA) {
 #pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qc[qz] = qn[qz];
+      for (int qz = 0; qz < Q; ++qz) qc[qz] = qn[qz];
+    }
+    (void)qn;
   }
 }
 
@@ -569,18 +588,18 @@ inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
   g.Nz = g.nz * (P - 1) + 1;
   g.ncx = g.nx / BX;
   g.ncy = g.ny / BY;
-  g.lz = a.box_lz;
-  g.nsz = g.lz > 0 ? g.nz / g.lz : 0;
+  g.nsz = a.box_nsz;
+  g.lz = g.nsz > 0 ? (g.nz + g.nsz - 1) / g.nsz : 0;
   g.cs = g.Nx * g.Ny * g.Nz;  // blocked components (checked when the box was detected)
   return g;
 }
 
-template <int P, int Q, int NC, class T, int BX, int BY, int UM>
+template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA>
 void launch_box(const FusedArgs& a, hipStream_t s) {
   using S = Shape<P, Q>;
   const BoxGeom g = box_geom(a, P, BX, BY);
-  require(g.nx % BX == 0 && g.ny % BY == 0 && g.lz > 0 && g.nz % g.lz == 0 && BX * (P - 1) >= 2 &&
-              BY * (P - 1) >= 2 && g.lz * (P - 1) >= 2,
+  require(g.nx % BX == 0 && g.ny % BY == 0 && g.nsz > 0 && g.nsz <= g.nz && BX * (P - 1) >= 2 &&
+              BY * (P - 1) >= 2 && (g.nz / g.nsz) * (P - 1) >= 2,
           "fused mass: box does not divide into the kernel's columns");
   const long long nseg = static_cast<long long>(g.ncx) * g.ncy * g.nsz;
   require(nseg < (1ll << 31) && static_cast<long long>(g.Nx) * g.Ny * g.Nz < (1ll << 31), "fused mass: box too large");
@@ -589,7 +608,7 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
           "fused mass: box partial buffer too small");
   if (a.box_pass == 0) {
     constexpr int threads = round64(BX * BY * S::TPE);
-    fused_box_kernel<P, Q, NC, T, BX, BY, UM><<<static_cast<unsigned>(nseg), threads, 0, s>>>(
+    fused_box_kernel<P, Q, NC, T, BX, BY, UM, QA><<<static_cast<unsigned>(nseg), threads, 0, s>>>(
         g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
         static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
   } else {
@@ -602,23 +621,32 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
   HIP_CHECK_LAUNCH();
 }
 
-// Key elems of a box-column variant: BX * BY elements per layer. FP64 loads
-// its x-rows straight from u, FP32 goes through the LDS ring (the faster of
-// the two for each precision on MI355X at 64^3, P=5).
+// Key elems of a box-column variant: BX * BY elements per layer, plus 100 for
+// the kernel that loads qdata within its own layer (QA = 0), plus 200 for FP64
+// with u through the LDS ring. By default FP64 loads its x-rows straight from
+// u, FP32 goes through the LDS ring.
 template <int P, int Q, int NC, int BX, int BY>
 struct FusedBoxReg {
   FusedBoxReg() {
-    register_fused(FusedKey{P, Q, NC, 0, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, double, BX, BY, 0>, 0);
-    register_fused(FusedKey{P, Q, NC, 1, kAlgBoxColumn, 0, BX * BY}, &launch_box<P, Q, NC, float, BX, BY, 2>, 0);
+    reg<double, 0, 1>(0);
+    reg<double, 0, 0>(100);
+    reg<double, 2, 1>(200);
+    reg<float, 2, 1>(0);
+    reg<float, 2, 0>(100);
+  }
+  template <class T, int UM, int QA>
+  static void reg(int code) {
+    const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, BX * BY + code};
+    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, UM, QA>, 0);
   }
 };
 
-// Column cross-section (BX, BY) of each compiled elems value.
+// Column cross-section (BX, BY) of each compiled elems value (elems % 100).
 inline bool box_columns(int elems, int* BX, int* BY) {
-  switch (elems) {
-    case 16: *BX = 4; *BY = 4; return true;
-    case 8: *BX = 4; *BY = 2; return true;
-    case 4: *BX = 2; *BY = 2; return true;
+  switch (elems % 100) {
+    case 16: *BX = 4; *BY = 4; return elems >= 0;
+    case 8: *BX = 4; *BY = 2; return elems >= 0;
+    case 4: *BX = 2; *BY = 2; return elems >= 0;
     default: return false;
   }
 }

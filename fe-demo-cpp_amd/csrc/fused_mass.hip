@@ -19,18 +19,28 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Reference backend-strided qdata[q*E + e] -> private per-element tiles
-// [e][q]: the fused kernel's z-line threads read qdata[e][qz*Q^2 + t], so lanes
-// of one load instruction read consecutive words.
+// Reference backend-strided qdata[q*E + e] (q = qz*Q^2 + line, line = qy*Q
+// + qx) -> private per-element tiles laid out by qdata_index: the fused
+// kernels' z-line thread t reads its Q values in pairs, and the lanes of one
+// load instruction read consecutive pairs.
 template <class T>
 __global__ __launch_bounds__(kThreads) void qdata_private_kernel(int E, int Q, const double* __restrict__ ref,
                                                                  T* __restrict__ priv) {
   const size_t idx = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
-  const size_t Q3 = static_cast<size_t>(Q) * Q * Q;
+  const size_t Q2 = static_cast<size_t>(Q) * Q, Q3 = Q2 * Q;
   if (idx >= Q3 * E) return;
   // idx walks the private layout (coalesced writes); reads stride E.
-  const size_t e = idx / Q3, q = idx % Q3;
-  priv[idx] = static_cast<T>(ref[q * E + e]);
+  const size_t e = idx / Q3, r = idx % Q3;
+  size_t line, qz;  // the inverse of qdata_index
+  if (Q % 2 == 0) {
+    const size_t pr = r / (2 * Q2), rem = r % (2 * Q2);
+    line = rem / 2;
+    qz = 2 * pr + rem % 2;
+  } else {
+    line = r % Q2;
+    qz = r / Q2;
+  }
+  priv[idx] = static_cast<T>(ref[(qz * Q2 + line) * E + e]);
 }
 
 template <class T>
@@ -52,13 +62,13 @@ __global__ __launch_bounds__(kThreads) void mass_diag_kernel(int E, int P, int Q
   const size_t e = idx / P3;
   const int i = static_cast<int>(idx % P3);
   const int ix = i % P, iy = (i / P) % P, iz = i / (P * P);
-  const double* qe = qpriv + e * Q2 * Q;  // [qz][qy][qx]
+  const double* qe = qpriv + e * Q2 * Q;  // laid out by qdata_index
   double sum = 0.0;
   for (int qy = 0; qy < Q; ++qy) {
     for (int qx = 0; qx < Q; ++qx) {
       const double bxy = B2[qy * P + iy] * B2[qx * P + ix];
       double s = 0.0;
-      for (int qz = 0; qz < Q; ++qz) s += B2[qz * P + iz] * qe[(qz * Q + qy) * Q + qx];
+      for (int qz = 0; qz < Q; ++qz) s += B2[qz * P + iz] * qe[qdata_index(Q, qy * Q + qx, qz)];
       sum += bxy * s;
     }
   }
@@ -320,35 +330,37 @@ bool valid_for(const FusedMass& f, int alg) {
   return (f.scatter != kScatterDeterministic || alg_deterministic(alg)) && (!alg_box(alg) || f.box);
 }
 
-// Segments the box-column apply aims at: 32 elements per column layer per CU.
-long long box_target_segments(int elems) {
+int device_cus() {
   static int cus = 0;
   if (!cus) {
     int dev = 0;
     HIP_CHECK(hipGetDevice(&dev));
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   }
-  return std::max<long long>(1, 32LL * cus / std::max(elems, 1));
+  return cus;
 }
 
-// Element layers per segment: the operator's setting, else box_layers.
-int box_lz_of(const FusedMass& f, const FusedKey& k) {
-  if (f.box_lz > 0) return f.box_lz;
+// Segments per column of the key's box-column kernel on f's box: the
+// operator's layer limit if set, else the count that gives 32 elements per
+// column layer per CU in the fewest layer steps (32 * CUs / elements per layer
+// workgroup slots). Measured on MI355X at 64^3, P=5: more resident segments
+// per CU (3 nine-wave workgroups, whose kernels fit in 72 VGPRs) were 9-15%
+// slower than 2, fewer left CUs idle.
+int box_nsz_of(const FusedMass& f, const FusedKey& k) {
   int BX = 0, BY = 0;
   box_columns(k.elems, &BX, &BY);
-  return box_layers(f.box_n[2], static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY),
-                    box_target_segments(k.elems), f.P);
+  const long long ncols = static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY);
+  if (f.box_lz > 0) return box_segments(f.box_n[2], ncols, 1, f.P, f.box_lz);
+  return box_segments(f.box_n[2], ncols, std::max<long long>(1, 32LL * device_cus() / (BX * BY)), f.P, 0);
 }
 
 // Whether the box-column variant with `elems` elements per layer tiles f's box
-// (without asking the GPU: an automatic layer count exists when some divisor
-// lz of nz has lz*(P-1) >= 2).
+// (without asking the GPU: segments of 1 + 2/(P-1) layers at least exist).
 bool box_fits(const FusedMass& f, int elems) {
   int BX = 0, BY = 0;
   if (!f.box || !f.basis_symmetric || !box_columns(elems, &BX, &BY)) return false;
   if (f.box_n[0] % BX || f.box_n[1] % BY || BX * (f.P - 1) < 2 || BY * (f.P - 1) < 2) return false;
-  if (f.box_lz > 0) return f.box_n[2] % f.box_lz == 0 && f.box_lz * (f.P - 1) >= 2;
-  return box_layers(f.box_n[2], 1, 1, f.P) > 0;
+  return box_segments(f.box_n[2], 1, 1, f.P, f.box_lz) > 0;
 }
 
 bool key_valid(const FusedMass& f, const FusedKey& k) {
@@ -357,8 +369,10 @@ bool key_valid(const FusedMass& f, const FusedKey& k) {
 
 // The first compiled box-column variant that tiles the box (most elements per layer first).
 int box_default_elems(const FusedMass& f, int fp32) {
+  // Most elements per layer first; of one cross-section the layer-ahead qdata
+  // kernel (elems < 100) first.
   std::vector<int> v = fused_elem_variants(f.P, f.Q, f.NC, fp32, kAlgBoxColumn, 0);
-  std::sort(v.rbegin(), v.rend());
+  std::sort(v.begin(), v.end(), [](int a, int b) { return a % 100 != b % 100 ? a % 100 > b % 100 : a < b; });
   for (int ne : v)
     if (box_fits(f, ne)) return ne;
   return 0;
@@ -557,8 +571,10 @@ size_t box_part_len(const FusedMass& f, const FusedKey& k) {
   int BX = 0, BY = 0;
   require(box_columns(k.elems, &BX, &BY), "fused mass: box-column variant not compiled");
   const long long ncols = static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY);
-  const int lz = box_lz_of(f, k);
-  const long long nseg = ncols * (f.box_n[2] / lz);
+  const int nsz = box_nsz_of(f, k);
+  require(nsz > 0, "fused mass: no segment count fits the box");
+  const int lz = (f.box_n[2] + nsz - 1) / nsz;  // the longest segment lays out every segment's slots
+  const long long nseg = ncols * nsz;
   return static_cast<size_t>(nseg * box_face_size(BX * (f.P - 1) + 1, BY * (f.P - 1) + 1, lz * (f.P - 1) + 1));
 }
 
@@ -646,7 +662,7 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
     a.box_n[0] = f.box_n[0];
     a.box_n[1] = f.box_n[1];
     a.box_n[2] = f.box_n[2];
-    a.box_lz = box_lz_of(f, k);
+    a.box_nsz = box_nsz_of(f, k);
     a.v = v;
     a.partials = box_partials<T>(f).data();
     a.part_comp_len = std::max<size_t>(box_part_len(f, k), 1);
@@ -825,14 +841,23 @@ bool detect_box(const int* off, int num_elem, int P, int num_comp, int comp_stri
   return true;
 }
 
-int box_layers(int nz, long long ncols, long long target, int P) {
-  int best = 0, smallest = 0;
-  for (int d = 1; d <= nz; ++d) {
-    if (nz % d != 0 || d * (P - 1) < 2) continue;
-    if (!smallest) smallest = d;
-    if (ncols * (nz / d) >= target) best = d;
+int box_segments(int nz, long long ncols, long long slots, int P, int max_layers) {
+  if (nz <= 0 || ncols <= 0 || slots <= 0) return 0;
+  auto ok = [&](int nsz) { return (nz / nsz) * (P - 1) >= 2; };
+  if (max_layers > 0) {
+    const int nsz = (nz + max_layers - 1) / max_layers;
+    return nsz <= nz && ok(nsz) ? nsz : 0;
   }
-  return best ? best : smallest;
+  int best = 0;
+  long long best_steps = 0;
+  for (int nsz = 1; nsz <= nz && ok(nsz); ++nsz) {
+    const long long rounds = (ncols * nsz + slots - 1) / slots, steps = rounds * ((nz + nsz - 1) / nsz);
+    if (!best || steps < best_steps) {
+      best = nsz;
+      best_steps = steps;
+    }
+  }
+  return best;
 }
 
 FusedKey fused_resolve(const FusedMass& f, int fp32) {
@@ -1002,7 +1027,7 @@ void fused_mass_apply64_scratch(FusedMass& f, const double* u, double* v, void* 
     a.box_n[0] = f.box_n[0];
     a.box_n[1] = f.box_n[1];
     a.box_n[2] = f.box_n[2];
-    a.box_lz = box_lz_of(f, k);
+    a.box_nsz = box_nsz_of(f, k);
     a.v = v;
     a.partials = scratch;  // every slot is written before the second pass reads it: no clearing
     a.part_comp_len = std::max<size_t>(box_part_len(f, k), 1);
@@ -1030,7 +1055,7 @@ void fused_mass_apply64_scratch(FusedMass& f, const double* u, double* v, void* 
 
 uint64_t fused_signature(const FusedMass& f, int fp32) {
   const FusedKey k = fused_resolve(f, fp32);
-  const long long parts[] = {k.P, k.Q, k.NC, k.fp32, k.alg, k.exact, k.elems, alg_box(k.alg) ? box_lz_of(f, k) : 0,
+  const long long parts[] = {k.P, k.Q, k.NC, k.fp32, k.alg, k.exact, k.elems, alg_box(k.alg) ? box_nsz_of(f, k) : 0,
                              f.scatter, f.exact, static_cast<long long>(f.gen)};
   uint64_t h = 1469598103934665603ull;  // FNV-1a over the parts
   for (long long v : parts) {

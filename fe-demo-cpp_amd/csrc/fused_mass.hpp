@@ -85,9 +85,9 @@ struct FusedArgs {
   void* v = nullptr;               // block atomic: pre-zeroed v; block part: v (interior nodes stored)
   void* partials = nullptr;        // block part: [NC][npart]; box column: [NC][segments * face slots]
   size_t part_comp_len = 0;
-  // box column: elements per direction, layers per segment, pass (0 the column kernel, 1 the shared-node sums)
+  // box column: elements per direction, segments per column, pass (0 the column kernel, 1 the shared-node sums)
   int box_n[3] = {0, 0, 0};
-  int box_lz = 0;
+  int box_nsz = 0;
   int box_pass = 0;
 };
 
@@ -147,7 +147,7 @@ struct FusedMass {
   bool box = false;
   bool basis_symmetric = false;  // B(Q-1-q, P-1-p) = B(q, p) to rounding (the box kernel's even-odd form)
   int box_n[3] = {0, 0, 0};
-  int box_lz = 0;             // element layers per box-column segment (0: box_layers)
+  int box_lz = 0;             // most element layers per box-column segment (0: box_segments chooses)
   DevBuf<double> box_part64;  // box-column partials (lazy)
   DevBuf<float> box_part32;
   int scatter = kScatterAtomic;  // requested: atomic (any order) or deterministic (reproducible)
@@ -209,14 +209,13 @@ void ensure_flagged_offsets(FusedMass& f, hipStream_t s);
 // Whether the operator is a structured box with box-column kernels compiled
 // for its shape (its tuning rows are the box = 1 ones).
 bool fused_box_operator(const FusedMass& f);
-// Element layers per segment of the box-column algorithm for nz layers and
-// ncols columns: the largest divisor lz of nz (with lz*(P-1) >= 2) that still
-// leaves at least `target` segments, else the smallest such divisor; 0 if
-// none qualifies. The apply aims at 32 elements per column layer per CU
-// (target = 32 * CUs / elements per layer): measured on MI355X, more
-// segments per CU only share the CU, fewer leave it idle, and longer segments
-// share fewer segment ends.
-int box_layers(int nz, long long ncols, long long target, int P);
+// Segments per column of the box-column algorithm for nz element layers,
+// ncols columns and `slots` workgroups running at once: the count that minimizes ceil(ncols*nsz / slots) * ceil(nz / nsz), the
+// layer steps of the longest-running CU, fewest segments on a tie; segments
+// hold at least 2 node planes beyond their first ((nz / nsz) * (P-1) >= 2).
+// With max_layers > 0 (the operator's setting) the fewest segments of at most
+// max_layers layers. 0 if none qualifies.
+int box_segments(int nz, long long ncols, long long slots, int P, int max_layers);
 // Jacobi diagonal: diag = E^T [ (B o B)^T x3 qdata ] (FP64).
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s);
 

@@ -81,6 +81,47 @@ __device__ __forceinline__ T fma_t(T a, T b, T c) {
   }
 }
 
+// Private qdata layout of one element tile (made by the operator at create):
+// for even Q, [qz/2][line][qz%2] with line = qy*Q + qx, so that z-line thread t
+// reads its values as Q/2 pairs and the lanes of one load instruction read
+// consecutive pairs; for odd Q, [qz][line] (consecutive lanes, single values).
+__host__ __device__ constexpr int qdata_index(int Q, int line, int qz) {
+  return Q % 2 == 0 ? (qz / 2) * 2 * Q * Q + line * 2 + (qz % 2) : qz * Q * Q + line;
+}
+
+// The Q qdata values of z-line t of the element tile at p (p = qdata +
+// e*Q^3): 16-byte pairs for FP64 and 8-byte pairs for FP32 when Q is even
+// (an even Q makes every tile and pair offset even, so the pairs are
+// aligned), single values otherwise. NT: as streaming (read-once) data.
+template <int Q, class T, bool NT = false>
+__device__ __forceinline__ void load_qline(const T* __restrict__ p, int t, T (&q)[Q]) {
+  constexpr int Q2 = Q * Q;
+  if constexpr (Q % 2 == 0) {
+    typedef T t2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int i = 0; i < Q / 2; ++i) {
+      const t2* v = reinterpret_cast<const t2*>(p + i * 2 * Q2 + 2 * t);
+      t2 x;
+      if constexpr (NT) {
+        x = __builtin_nontemporal_load(v);
+      } else {
+        x = *v;
+      }
+      q[2 * i] = x[0];
+      q[2 * i + 1] = x[1];
+    }
+  } else {
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) {
+      if constexpr (NT) {
+        q[qz] = __builtin_nontemporal_load(p + qz * Q2 + t);
+      } else {
+        q[qz] = p[qz * Q2 + t];
+      }
+    }
+  }
+}
+
 template <bool EXACT, class T>
 __device__ __forceinline__ T mac(T a, T b, T acc) {
   if constexpr (EXACT) {
@@ -130,9 +171,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_exac
   // qdata of the thread's z-line (qy,qx) = t, issued first to overlap the gather.
   T qd[Q];
   if (live && t < Q2) {
-    const T* qp = qdata + static_cast<size_t>(e) * Q3 + t;
-#pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qd[qz] = qp[qz * Q2];
+    load_qline<Q, T>(qdata + static_cast<size_t>(e) * Q3, t, qd);
   }
   // Offsets of the thread's x-row (iz,iy) = t, kept in registers for the scatter.
   int o[P];
@@ -284,9 +323,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
 
   T qd[Q];
   if (live && t < Q2) {
-    const T* qp = qdata + static_cast<size_t>(e) * Q3 + t;
-#pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qd[qz] = qp[qz * Q2];
+    load_qline<Q, T>(qdata + static_cast<size_t>(e) * Q3, t, qd);
   }
   int o[P];
   if (live && t < P2) {
@@ -447,8 +484,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe
       for (int ix = 0; ix < P; ++ix) x0[ix] = u[static_cast<size_t>(o[ix])];
     }
     if (live && t < Q2) {
-#pragma unroll
-      for (int qz = 0; qz < Q; ++qz) qd[qz] = qdata[static_cast<size_t>(e) * Q3 + t + qz * Q2];
+      load_qline<Q, T>(qdata + static_cast<size_t>(e) * Q3, t, qd);
     }
   }
 
@@ -464,8 +500,7 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe
       for (int ix = 0; ix < P; ++ix) o_n[ix] = offsets[en * P3 + t * P + ix];
     }
     if (live_n && t < Q2) {
-#pragma unroll
-      for (int qz = 0; qz < Q; ++qz) qd_n[qz] = qdata[static_cast<size_t>(en) * Q3 + t + qz * Q2];
+      load_qline<Q, T>(qdata + static_cast<size_t>(en) * Q3, t, qd_n);
     }
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {
@@ -636,9 +671,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_bloc
 
   T qd[Q];
   if (live && t < Q2) {
-    const T* qp = qdata + slot * Q3 + t;
-#pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qd[qz] = qp[qz * Q2];
+    load_qline<Q, T>(qdata + slot * Q3, t, qd);
   }
   int lo[P];
   if (live && t < P2) {
@@ -794,9 +827,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
 
   T qd[Q];
   if (live && t < Q2) {
-    const T* qp = qdata + slot * Q3 + t;
-#pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qd[qz] = qp[qz * Q2];
+    load_qline<Q, T>(qdata + slot * Q3, t, qd);
   }
   int lo[P];
   if (live && t < P2) {
@@ -991,8 +1022,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
     const bool live = el < ne_here;
     const size_t slot = static_cast<size_t>(blk) * NB + el;
     if (live && t < Q2) {
-#pragma unroll
-      for (int qz = 0; qz < Q; ++qz) qd[qz] = qdata[slot * Q3 + t + qz * Q2];
+      load_qline<Q, T>(qdata + slot * Q3, t, qd);
     }
     if (live && t < P2) {
 #pragma unroll
@@ -1033,8 +1063,7 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
       const bool live_n = el < ne_n;
       const size_t slot_n = static_cast<size_t>(nb) * NB + el;
       if (live_n && t < Q2) {
-#pragma unroll
-        for (int qz = 0; qz < Q; ++qz) qd_n[qz] = qdata[slot_n * Q3 + t + qz * Q2];
+        load_qline<Q, T>(qdata + slot_n * Q3, t, qd_n);
       }
       if (live_n && t < P2) {
 #pragma unroll

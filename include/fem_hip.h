@@ -253,15 +253,19 @@ int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, in
 #define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS_PIPE 6 /* BLOCK_PARTIAL_LDS as a persistent, software-pipelined kernel */
 #define FEM_HIP_FUSED_BOX_COLUMN 7 /* structured boxes (fem_hip_box_detect): element columns marching in z with node
                                       planes in LDS, shared column faces summed by a second pass (deterministic;
-                                      no index arrays read) */
+                                      no index arrays read); elems = elements per column layer (16, 8, 4),
+                                      + 100 for the kernel that loads qdata within its own layer,
+                                      + 200 (FP64) for the one that loads u through an LDS ring of node planes */
 /* Explicit algorithm and its elements per workgroup (element kernels) or per
  * block (block kernels); 0 = that algorithm's default. A deterministic scatter
  * mode accepts only the deterministic algorithms. exact = 1 overrides. */
 int fem_hip_operator_set_fused_algorithm(fem_hip_operator op, int alg, int elems);
-/* Element layers per workgroup of FEM_HIP_FUSED_BOX_COLUMN (a divisor of the
- * box's nz; 0 = automatic: the most that still gives 32 * CUs / (elements per
- * column layer) workgroups). A value that does not divide nz makes the
- * box-column algorithm unavailable. */
+/* Most element layers per workgroup of FEM_HIP_FUSED_BOX_COLUMN: each column
+ * of the box is cut into ceil(nz / layers) segments of floor or ceil of
+ * nz / segments layers (0 = automatic: the segment count that fills every
+ * workgroup slot the kernel has on the device, by its occupancy, in the fewest
+ * layer steps). The box-column algorithm is unavailable when a segment would
+ * hold fewer than 2 node planes beyond its first. */
 int fem_hip_operator_set_box_layers(fem_hip_operator op, int layers);
 /* The algorithm and elems the next apply in this precision (fp32 = 0/1) will run. */
 int fem_hip_operator_get_fused_choice(fem_hip_operator op, int fp32, int* alg, int* elems);

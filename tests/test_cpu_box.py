@@ -40,9 +40,14 @@ def test_box_detect_rejects_everything_else():
     assert fehip.box_detect(E, P, rows.ravel(), l_size=N) is None
 
 
+# (box, P, elems per layer, most layers per segment (0: automatic)); the
+# layer limits that do not divide nz give segments of unequal length
+# (64 layers in 3 segments: 21, 21, 22).
 CHECKS = [((64, 64, 64), 5, 16, 0), ((64, 64, 64), 5, 8, 0), ((8, 8, 8), 5, 16, 2), ((8, 8, 6), 5, 8, 3),
           ((4, 6, 6), 5, 4, 2), ((4, 6, 6), 5, 4, 6), ((4, 6, 4), 5, 8, 1), ((12, 8, 7), 4, 8, 7), ((6, 4, 10), 3, 4, 5),
-          ((4, 4, 1), 5, 16, 0), ((4, 4, 2), 2, 16, 2), ((16, 16, 16), 3, 16, 4), ((2, 2, 9), 8, 4, 3)]
+          ((4, 4, 1), 5, 16, 0), ((4, 4, 2), 2, 16, 2), ((16, 16, 16), 3, 16, 4), ((2, 2, 9), 8, 4, 3),
+          ((64, 64, 64), 5, 16, 22), ((8, 8, 7), 5, 8, 3), ((4, 4, 10), 3, 16, 4), ((4, 2, 11), 4, 8, 2),
+          ((2, 2, 13), 8, 4, 5), ((8, 4, 9), 2, 16, 4), ((12, 12, 5), 6, 16, 2)]
 
 
 @pytest.mark.parametrize("n3,P,elems,layers", CHECKS)
@@ -51,21 +56,28 @@ def test_box_column_index_arithmetic(n3, P, elems, layers):
     N = np.prod([n * (P - 1) + 1 for n in n3])
     assert st["direct_nodes"] + st["shared_nodes"] == N
     bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[elems]
+    ncols = (n3[0] // bx) * (n3[1] // by)
+    assert st["segments"] % ncols == 0
+    nsz = st["segments"] // ncols
     lz = st["layers"]
-    assert st["segments"] == (n3[0] // bx) * (n3[1] // by) * (n3[2] // lz)
+    assert lz == -(-n3[2] // nsz)  # the longest segment: ceil(nz / nsz)
     if layers:
-        assert lz == layers
+        assert nsz == -(-n3[2] // layers) and lz <= layers
     if st["segments"] > 1:
         assert st["max_owners"] in (2, 4, 8)
 
 
 def test_box_column_layers_automatic():
-    # The check assumes 256 CUs and aims at 32 elements per column layer per
-    # CU: 64^3 with 4x4 columns (256 columns) -> 512 segments of 32 layers;
-    # 4x2 columns (512) -> 1024 segments, also of 32 layers.
+    # The check assumes 256 CUs and 32 elements per column layer per CU
+    # (32 * 256 / elems workgroup slots), and picks the segment count with the
+    # fewest layer steps on the busiest CU, fewest segments on a tie: 64^3
+    # with 4x4 columns (256 columns, 512 slots) -> 2 segments of 32 layers; 4x2
+    # (512 columns, 1024 slots) and 2x2 (1024, 2048) -> also 32 layers.
     assert fehip.box_column_check(64, 64, 64, 5, 16)["layers"] == 32
     assert fehip.box_column_check(64, 64, 64, 5, 8)["layers"] == 32
     assert fehip.box_column_check(64, 64, 64, 5, 4)["layers"] == 32
+    # 48 layers over 512 slots: 2 segments of 24 (not a power-of-two divisor).
+    assert fehip.box_column_check(64, 64, 48, 5, 16)["layers"] == 24
     # Too few columns for the target: one layer per segment.
     assert fehip.box_column_check(8, 8, 8, 5, 16)["layers"] == 1
     # P = 2 needs two layers per segment (a segment end must hold 2 node planes).
@@ -78,4 +90,4 @@ def test_box_column_check_rejects_bad_sizes():
     with pytest.raises(ValueError):
         fehip.box_column_check(4, 4, 4, 5, 12)  # no such cross-section
     with pytest.raises(ValueError):
-        fehip.box_column_check(4, 4, 6, 5, 16, 4)  # 4 layers do not divide nz = 6
+        fehip.box_column_check(4, 4, 6, 2, 16, 1)  # P = 2 segments of one layer hold 2 node planes

@@ -316,6 +316,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="Poisson arrival rate, requests/s (default: 0.8 x the measured service capacity)")
     ap.add_argument("--variants", default="", help="comma-separated subset of the apply variants")
+    ap.add_argument("--box-layers", type=int, default=0,
+                    help="most element layers per box-column segment (0: automatic)")
+    ap.add_argument("--elems", type=int, default=0,
+                    help="with --variants: the elems (packing) of the listed variants' algorithm (0: the default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="K",
                     help="also run K plain copies of one FP64 L-vector")
     args = ap.parse_args()
@@ -333,6 +337,8 @@ def main():
     n, P, Q = args.n, 5, 6
     t0 = time.perf_counter()
     p = MassProblem(n, P, Q, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+    if args.box_layers:
+        p.op.set_box_layers(args.box_layers)
     fehip.synchronize()
     log("setup (mesh, build_mass on device): %.2f s" % (time.perf_counter() - t0))
     L = p.l_size
@@ -361,16 +367,13 @@ def main():
     runs = (("fp64_box_column", np.float64, D, 0, fehip.FUSED_BOX_COLUMN),
             ("fp64_block_partial", np.float64, D, 0, fehip.FUSED_BLOCK_PARTIAL),
             ("fp64_elem_tmap", np.float64, D, 0, fehip.FUSED_ELEM_TMAP),
-            ("fp64_elem_tmap_pipe", np.float64, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
             ("fp64_block_partial_lds", np.float64, A, 0, fehip.FUSED_BLOCK_PARTIAL_LDS),
-            ("fp64_block_partial_lds_pipe", np.float64, A, 0, fehip.FUSED_BLOCK_PARTIAL_LDS_PIPE),
             ("fp64_block_atomic", np.float64, A, 0, fehip.FUSED_BLOCK_ATOMIC),
             ("fp64_elem_atomic", np.float64, A, 0, fehip.FUSED_ELEM_ATOMIC),
             ("fp64_exact_tmap", np.float64, D, 1, fehip.FUSED_ELEM_TMAP),
             ("fp32_box_column", np.float32, D, 0, fehip.FUSED_BOX_COLUMN),
             ("fp32_block_partial", np.float32, D, 0, fehip.FUSED_BLOCK_PARTIAL),
             ("fp32_elem_tmap", np.float32, D, 0, fehip.FUSED_ELEM_TMAP),
-            ("fp32_elem_tmap_pipe", np.float32, D, 0, fehip.FUSED_ELEM_TMAP_PIPE),
             ("fp32_block_atomic", np.float32, A, 0, fehip.FUSED_BLOCK_ATOMIC))
     VARIANT_ALG = {r[0]: r[4] for r in runs}
     DET_VARIANTS = {r[0] for r in runs if r[2] == D}
@@ -392,14 +395,12 @@ def main():
         fp32 = dt == np.float32
         p.op.set_fused_options(scatter, exact=exact)
         if not exact:
-            p.op.set_fused_algorithm(alg, tuned_elems("fp32" if fp32 else "fp64", alg))
+            p.op.set_fused_algorithm(alg, args.elems or tuned_elems("fp32" if fp32 else "fp64", alg))
         ne = p.op.fused_choice(fp32)[1]
         tname = "float" if fp32 else "double"
         if alg in (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP):
             kernel_names[label] = "fused_mass_%s_kernel<%d, %d, 1, %s, %d, %s>" % (
                 "exact" if exact else "fast", P, Q, tname, ne, "true" if alg == fehip.FUSED_ELEM_TMAP else "false")
-        elif alg == fehip.FUSED_ELEM_TMAP_PIPE:
-            kernel_names[label] = "fused_mass_pipe_kernel<%d, %d, 1, %s, %d>" % (P, Q, tname, ne)
         elif alg == fehip.FUSED_BOX_COLUMN:
             bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne % 100]
             # the last two template arguments: how u reaches the x-rows (2: the LDS ring,
@@ -411,7 +412,6 @@ def main():
             kernel_names[label] = {
                 fehip.FUSED_BLOCK_PARTIAL: "fused_mass_part_kernel<%d, %d, 1, %s, %d, false>",
                 fehip.FUSED_BLOCK_PARTIAL_LDS: "fused_mass_part_kernel<%d, %d, 1, %s, %d, true>",
-                fehip.FUSED_BLOCK_PARTIAL_LDS_PIPE: "fused_mass_part_pipe_kernel<%d, %d, 1, %s, %d>",
                 fehip.FUSED_BLOCK_ATOMIC: "fused_mass_block_kernel<%d, %d, 1, %s, %d>"}[alg] % (P, Q, tname, ne)
         steps = args.steps if label.startswith("fp64") else max(args.steps // 2, 5)
         m, k, a = time_applies(fehip, p.op, d_u, d_v, steps, args.warmup, stream)
@@ -420,7 +420,7 @@ def main():
         # written, except the element-output (tmap) kernels, which write E*P^3
         # values and leave the sums to the transpose-map pass, and the box-column
         # kernel, which reads no offsets (the box formula gives every node).
-        tmap = alg in (fehip.FUSED_ELEM_TMAP, fehip.FUSED_ELEM_TMAP_PIPE)
+        tmap = alg == fehip.FUSED_ELEM_TMAP
         offs = 0 if alg == fehip.FUSED_BOX_COLUMN else 4 * E * P3
         kbytes = s * L + s * E * Q ** 3 + offs + (s * E * P3 if tmap else s * L)
         variants[label] = {"ms_per_apply": m, "gdofs": L / m / 1e6, "kernel_ms": k, "other_stages_ms": a,

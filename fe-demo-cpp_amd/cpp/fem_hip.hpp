@@ -403,9 +403,8 @@ enum class FusedAlgorithm {
   ElemTmap = FEM_HIP_FUSED_ELEM_TMAP,
   BlockAtomic = FEM_HIP_FUSED_BLOCK_ATOMIC,
   BlockPartial = FEM_HIP_FUSED_BLOCK_PARTIAL,
-  ElemTmapPipe = FEM_HIP_FUSED_ELEM_TMAP_PIPE,
   BlockPartialLds = FEM_HIP_FUSED_BLOCK_PARTIAL_LDS,
-  BlockPartialLdsPipe = FEM_HIP_FUSED_BLOCK_PARTIAL_LDS_PIPE,
+  BoxColumn = FEM_HIP_FUSED_BOX_COLUMN,
 };
 struct FusedChoice {
   FusedAlgorithm alg = FusedAlgorithm::Auto;

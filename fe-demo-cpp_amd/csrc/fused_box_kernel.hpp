@@ -157,6 +157,23 @@ __device__ __forceinline__ void load_qdata(const T* tile, int t, T (&q)[Q]) {
   load_qline<Q, T, sizeof(T) == 8>(tile, t, q);
 }
 
+// One x-row of P consecutive values of u, read as 2-vectors of T (16-byte
+// loads for FP64) and a single value for odd P: fewer memory instructions per
+// row than P single loads. The row starts at any multiple of sizeof(T), so the
+// vectors are declared with T's alignment (the hardware needs dword alignment
+// for these loads).
+template <int P, class T>
+__device__ __forceinline__ void load_row(const T* __restrict__ r, T (&x)[P]) {
+  typedef T t2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+#pragma unroll
+  for (int i = 0; i < P / 2; ++i) {
+    const t2 v = *reinterpret_cast<const t2*>(r + 2 * i);
+    x[2 * i] = v[0];
+    x[2 * i + 1] = v[1];
+  }
+  if constexpr (P % 2 == 1) x[P - 1] = r[P - 1];
+}
+
 // Even-odd form of a symmetric 1D interpolation matrix B [Q][P]
 // (B(Q-1-q, P-1-p) = B(q, p), true for Lagrange bases on symmetric nodes at
 // symmetric points): with x_e = x_j + x_(P-1-j), x_o = x_j - x_(P-1-j),
@@ -365,9 +382,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   T xc[UM == 1 ? P : 1] = {};
   if constexpr (UM == 1) {
     if (row_thread) {
-      const T* r = row_ptr(ez0, 0);
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) xc[ix] = r[ix];
+      load_row<P, T>(row_ptr(ez0, 0), xc);
     }
   }
   T qc[Q] = {};
@@ -404,9 +419,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
       } else if constexpr (UM == 1) {
         const int cn = c + 1 < NC ? c + 1 : 0, kn = c + 1 < NC ? k : k + 1;
         if (row_thread && kn < nl) {
-          const T* r = row_ptr(ez0 + kn, cn);
-#pragma unroll
-          for (int ix = 0; ix < P; ++ix) xn[ix] = r[ix];
+          load_row<P, T>(row_ptr(ez0 + kn, cn), xn);
         }
       }
       // S1: x-rows (iz,iy): contract x.
@@ -421,9 +434,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
 #pragma unroll
           for (int ix = 0; ix < P; ++ix) x[ix] = xc[ix];
         } else {
-          const T* r = row_ptr(ez, c);
-#pragma unroll
-          for (int ix = 0; ix < P; ++ix) x[ix] = r[ix];
+          load_row<P, T>(row_ptr(ez, c), x);
         }
       }
       // The next layer's qdata (QA = 1) or this layer's (QA = 0), issued

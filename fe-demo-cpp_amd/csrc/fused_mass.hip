@@ -902,11 +902,11 @@ FusedKey fused_resolve(const FusedMass& f, int fp32) {
   // Default order: box columns (structured boxes of at least kBoxMinNodes
   // element nodes), partial-sum blocks, then (deterministic) transpose map or
   // (atomic) block atomics, then elements.
-  const int order_det[] = {kAlgBoxColumn, kAlgBlockPart, kAlgElemTmapPipe, kAlgElemTmap};
-  const int order_any[] = {kAlgBoxColumn, kAlgBlockPartLds, kAlgBlockPartLdsPipe, kAlgBlockPart, kAlgBlockAtomic,
-                           kAlgElemAtomic, kAlgElemTmapPipe, kAlgElemTmap};
+  const int order_det[] = {kAlgBoxColumn, kAlgBlockPart, kAlgElemTmap};
+  const int order_any[] = {kAlgBoxColumn, kAlgBlockPartLds, kAlgBlockPart, kAlgBlockAtomic, kAlgElemAtomic,
+                           kAlgElemTmap};
   const int* order = det ? order_det : order_any;
-  const int n = det ? 4 : 8;
+  const int n = det ? 3 : 6;
   const long long elem_nodes = static_cast<long long>(f.num_elem) * f.P * f.P * f.P * f.NC;
   for (int i = 0; i < n; ++i) {
     k.alg = order[i];

@@ -23,15 +23,15 @@ enum FusedAlg : int {
   kAlgElemTmap = 1,    // per element into Y[E*P^3], then the transpose-map pass (deterministic; CPU order)
   kAlgBlockAtomic = 2, // per element block; sums in LDS, boundary nodes one atomic each
   kAlgBlockPart = 3,   // per element block; sums in LDS, boundary nodes one partial each, then a partial-sum pass
-  kAlgElemTmapPipe = 4,  // kAlgElemTmap with a persistent, software-pipelined element kernel
+  // 4 and 6: retired (persistent software-pipelined forms of 1 and 5, which
+  // measured the same as or slower than them on MI355X); no kernels.
   kAlgBlockPartLds = 5,  // kAlgBlockPart with LDS atomic sums in place of the colour rounds (any order)
-  kAlgBlockPartLdsPipe = 6,  // kAlgBlockPartLds as a persistent, software-pipelined kernel
   kAlgBoxColumn = 7,  // structured boxes only: element columns marching in z, node planes in LDS,
                       // shared column faces summed by a second pass (deterministic)
   kAlgCount = 8
 };
 inline bool alg_deterministic(int alg) {
-  return alg == kAlgElemTmap || alg == kAlgBlockPart || alg == kAlgElemTmapPipe || alg == kAlgBoxColumn;
+  return alg == kAlgElemTmap || alg == kAlgBlockPart || alg == kAlgBoxColumn;
 }
 inline bool alg_box(int alg) { return alg == kAlgBoxColumn; }
 // Element nodes (num_elem * P^3 * NC) from which an untuned box operator
@@ -40,12 +40,12 @@ inline bool alg_box(int alg) { return alg == kAlgBoxColumn; }
 // and block algorithms at 0.2-0.4M element nodes and won from about 1M
 // (P = 3..6, one component).
 constexpr long long kBoxMinNodes = 1LL << 20;
-inline bool alg_tmap(int alg) { return alg == kAlgElemTmap || alg == kAlgElemTmapPipe; }
+inline bool alg_tmap(int alg) { return alg == kAlgElemTmap; }
 inline bool alg_blocked(int alg) {
-  return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds || alg == kAlgBlockPartLdsPipe;
+  return alg == kAlgBlockAtomic || alg == kAlgBlockPart || alg == kAlgBlockPartLds;
 }
 inline bool alg_partial(int alg) {
-  return alg == kAlgBlockPart || alg == kAlgBlockPartLds || alg == kAlgBlockPartLdsPipe;
+  return alg == kAlgBlockPart || alg == kAlgBlockPartLds;
 }
 
 // One compiled kernel.

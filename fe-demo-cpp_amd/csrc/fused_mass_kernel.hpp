@@ -443,195 +443,6 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
   }
 }
 
-// Persistent, software-pipelined element kernel with the transpose-map
-// scatter. Each workgroup walks the NE-element batches of its XCD's range with
-// a stride (the grid holds as many workgroups as fit the GPU at once). While it
-// contracts batch k it already has the offsets and qdata of batch k+1 in flight
-// (issued at the top of the iteration), and it issues batch k+1's u gather after
-// the y stage, so HBM loads overlap the contractions instead of alternating with
-// them. The arithmetic per element is the fast kernel's.
-template <int P, int Q, int NC, class T, int NE>
-__global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_pipe_kernel(
-    int num_elem, int comp_stride, const int* __restrict__ offsets, const T* __restrict__ qdata,
-    const T* __restrict__ u, T* __restrict__ out, size_t y_comp_len, MatArg<T, P * Q> Bm) {
-  using S = Shape<P, Q>;
-  constexpr int P2 = S::P2, P3 = S::P3, Q2 = S::Q2, Q3 = S::Q3, TPE = S::TPE, RY = S::RY, ZS = S::ZS;
-  __shared__ T s_b[NE][S::FBUF];
-
-  const long long nbatch = (static_cast<long long>(num_elem) + NE - 1) / NE;
-  // Workgroup g runs on XCD g % 8 (round-robin dispatch); XCD x owns batches
-  // [x*nbatch/8, (x+1)*nbatch/8), walked by its per = gridDim.x/8 workgroups.
-  const int per = gridDim.x / 8;
-  const int xcd = blockIdx.x % 8, lane_wg = blockIdx.x / 8;
-  const long long b_hi = nbatch * (xcd + 1) / 8;
-  long long batch = nbatch * xcd / 8 + lane_wg;
-  if (batch >= b_hi) return;  // the whole workgroup leaves together
-  const int el = threadIdx.x / TPE;
-  const int t = threadIdx.x - el * TPE;
-  T* b = el < NE ? s_b[el] : nullptr;
-  auto B = [&](int q, int p) -> T { return Bm.m[q * P + p]; };
-  auto live_in = [&](long long bt) { return el < NE && bt * NE + el < num_elem; };
-
-  int o[P];
-  T qd[Q], x0[P];
-  {
-    const bool live = live_in(batch);
-    const long long e = batch * NE + el;
-    if (live && t < P2) {
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) o[ix] = offsets[e * P3 + t * P + ix];
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) x0[ix] = u[static_cast<size_t>(o[ix])];
-    }
-    if (live && t < Q2) {
-      load_qline<Q, T>(qdata + static_cast<size_t>(e) * Q3, t, qd);
-    }
-  }
-
-  while (true) {
-    const long long nb = batch + per;
-    const bool more = nb < b_hi;  // uniform over the workgroup
-    const bool live = live_in(batch), live_n = more && live_in(nb);
-    const long long e = batch * NE + el, en = nb * NE + el;
-    int o_n[P];
-    T qd_n[Q], x0_n[P];
-    if (live_n && t < P2) {
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) o_n[ix] = offsets[en * P3 + t * P + ix];
-    }
-    if (live_n && t < Q2) {
-      load_qline<Q, T>(qdata + static_cast<size_t>(en) * Q3, t, qd_n);
-    }
-#pragma unroll 1
-    for (int c = 0; c < NC; ++c) {
-      const size_t coff = static_cast<size_t>(c) * comp_stride;
-      // S1: x-rows (iz,iy): contract x (component 0 was gathered ahead).
-      if (live && t < P2) {
-        const int row = (t / P) * ZS + (t % P) * RY;
-        T x[P];
-#pragma unroll
-        for (int ix = 0; ix < P; ++ix) x[ix] = c == 0 ? x0[ix] : u[static_cast<size_t>(o[ix]) + coff];
-#pragma unroll
-        for (int qx = 0; qx < Q; ++qx) {
-          T acc = T(0);
-#pragma unroll
-          for (int ix = 0; ix < P; ++ix) acc = fma_t(B(qx, ix), x[ix], acc);
-          b[row + qx] = acc;
-        }
-      }
-      __syncthreads();
-      // S2: y-columns (iz,qx), in place.
-      if (live && t < P * Q) {
-        const int col = (t / Q) * ZS + t % Q;
-        T x[P];
-#pragma unroll
-        for (int iy = 0; iy < P; ++iy) x[iy] = b[col + iy * RY];
-#pragma unroll
-        for (int qy = 0; qy < Q; ++qy) {
-          T acc = T(0);
-#pragma unroll
-          for (int iy = 0; iy < P; ++iy) acc = fma_t(B(qy, iy), x[iy], acc);
-          b[col + qy * RY] = acc;
-        }
-      }
-      // The next batch's gather: its offsets were issued an iteration stage ago.
-      if (c == 0 && live_n && t < P2) {
-#pragma unroll
-        for (int ix = 0; ix < P; ++ix) x0_n[ix] = u[static_cast<size_t>(o_n[ix])];
-      }
-      __syncthreads();
-      // S3: z-lines (qy,qx): contract z, times qdata, contract z back.
-      if (live && t < Q2) {
-        const int line = (t / Q) * RY + t % Q;
-        T x[P];
-#pragma unroll
-        for (int iz = 0; iz < P; ++iz) x[iz] = b[iz * ZS + line];
-        T d[Q];
-#pragma unroll
-        for (int qz = 0; qz < Q; ++qz) {
-          T acc = T(0);
-#pragma unroll
-          for (int iz = 0; iz < P; ++iz) acc = fma_t(B(qz, iz), x[iz], acc);
-          d[qz] = qd[qz] * acc;
-        }
-#pragma unroll
-        for (int iz = 0; iz < P; ++iz) {
-          T acc = T(0);
-#pragma unroll
-          for (int qz = 0; qz < Q; ++qz) acc = fma_t(B(qz, iz), d[qz], acc);
-          b[iz * ZS + line] = acc;
-        }
-      }
-      __syncthreads();
-      // S4: y-columns back (iz,qx), in place.
-      if (live && t < P * Q) {
-        const int col = (t / Q) * ZS + t % Q;
-        T x[Q];
-#pragma unroll
-        for (int qy = 0; qy < Q; ++qy) x[qy] = b[col + qy * RY];
-#pragma unroll
-        for (int iy = 0; iy < P; ++iy) {
-          T acc = T(0);
-#pragma unroll
-          for (int qy = 0; qy < Q; ++qy) acc = fma_t(B(qy, iy), x[qy], acc);
-          b[col + iy * RY] = acc;
-        }
-      }
-      __syncthreads();
-      // S5: x-rows back (iz,iy), in place.
-      if (live && t < P2) {
-        const int row = (t / P) * ZS + (t % P) * RY;
-        T x[Q];
-#pragma unroll
-        for (int qx = 0; qx < Q; ++qx) x[qx] = b[row + qx];
-#pragma unroll
-        for (int ix = 0; ix < P; ++ix) {
-          T acc = T(0);
-#pragma unroll
-          for (int qx = 0; qx < Q; ++qx) acc = fma_t(B(qx, ix), x[qx], acc);
-          b[row + ix] = acc;
-        }
-      }
-      __syncthreads();
-      // Element outputs, lanes over consecutive element nodes (coalesced).
-      if (live) {
-        for (int i = t; i < P3; i += TPE) {
-          const int iz = i / P2, iy = (i / P) % P, ix = i % P;
-          out[static_cast<size_t>(c) * y_comp_len + static_cast<size_t>(e) * P3 + i] = b[iz * ZS + iy * RY + ix];
-        }
-      }
-      __syncthreads();  // the buffer is rewritten by the next S1
-    }
-    if (!more) break;
-    batch = nb;
-#pragma unroll
-    for (int ix = 0; ix < P; ++ix) {
-      o[ix] = o_n[ix];
-      x0[ix] = x0_n[ix];
-    }
-#pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qd[qz] = qd_n[qz];
-  }
-}
-
-// Workgroups of a persistent kernel that the GPU holds at once (a multiple of
-// 8, one share per XCD), from the occupancy calculator, cached per kernel.
-template <class K>
-int persistent_grid(K kernel, int threads, size_t lds) {
-  static int grid = 0;
-  if (!grid) {
-    int dev = 0, cus = 0, per_cu = 0;
-    HIP_CHECK(hipGetDevice(&dev));
-    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
-    grid = std::max(8, (cus * std::max(per_cu, 1)) / 8 * 8);
-  }
-  return grid;
-}
-
-template <int P, int Q, int NC, class T, int NE>
-void launch_pipe(const FusedArgs& a, hipStream_t s);
-
 // Block-accumulated variant of the fast path (atomic scatter mode). A workgroup
 // handles one block of NB elements chosen at create time (a compact patch of
 // the mesh, see block_plan.cpp). S5 adds each element output into an LDS
@@ -971,235 +782,6 @@ __global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part
   }
 }
 
-// Persistent, software-pipelined form of the block partial-sum kernel with LDS
-// atomic sums (any order). Each workgroup walks the blocks of its XCD's range
-// with a stride; while it contracts block k, block k+1's header, node ids,
-// qdata and local indices are already in flight (issued at the top of the
-// iteration) and its u values are gathered after the y stage, so each block's
-// dependent load chain (header -> node ids -> u) overlaps the previous block's
-// contractions.
-template <int P, int Q, int NC, class T, int NB>
-__global__ __launch_bounds__(round64(NB* Shape<P, Q>::TPE)) void fused_mass_part_pipe_kernel(
-    int nblk, int comp_stride, const int* __restrict__ node_ptr, const int* __restrict__ nodes,
-    const int* __restrict__ ninterior, const int* __restrict__ nelem, const int* __restrict__ part_ptr,
-    const uint16_t* __restrict__ loc, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v,
-    T* __restrict__ partials, size_t part_comp_len, int ldn, MatArg<T, P * Q> Bm) {
-  using S = Shape<P, Q>;
-  constexpr int P2 = S::P2, P3 = S::P3, Q2 = S::Q2, Q3 = S::Q3, TPE = S::TPE, RY = S::RY, ZS = S::ZS;
-  constexpr int NT = round64(NB * TPE);
-  constexpr int KN = (NB * P3 + NT - 1) / NT;
-  __shared__ T s_b[NB][S::FBUF];
-  extern __shared__ __align__(16) unsigned char s_dyn[];
-  T* s_n = reinterpret_cast<T*>(s_dyn);  // [NC][ldn]
-
-  const int per = gridDim.x / 8;
-  const int xcd = blockIdx.x % 8, lane_wg = blockIdx.x / 8;
-  const int b_hi = static_cast<int>(static_cast<long long>(nblk) * (xcd + 1) / 8);
-  int blk = static_cast<int>(static_cast<long long>(nblk) * xcd / 8) + lane_wg;
-  if (blk >= b_hi) return;  // whole workgroup
-  const int tid = threadIdx.x;
-  const int el = tid / TPE;
-  const int t = tid - el * TPE;
-  auto B = [&](int q, int p) -> T { return Bm.m[q * P + p]; };
-
-  // Current block's state, loaded for the first block here and prefetched after.
-  int n0 = node_ptr[blk], nn = node_ptr[blk + 1] - n0, nint = ninterior[blk], ne_here = nelem[blk];
-  int gn[KN];
-  T un[NC][KN];
-  T qd[Q];
-  int lo[P];
-#pragma unroll
-  for (int k = 0; k < KN; ++k) {
-    const int j = tid + k * NT;
-    gn[k] = j < nn ? nodes[n0 + j] : 0;
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c)
-#pragma unroll
-    for (int k = 0; k < KN; ++k)
-      un[c][k] = tid + k * NT < nn ? u[static_cast<size_t>(gn[k]) + static_cast<size_t>(c) * comp_stride] : T(0);
-  {
-    const bool live = el < ne_here;
-    const size_t slot = static_cast<size_t>(blk) * NB + el;
-    if (live && t < Q2) {
-      load_qline<Q, T>(qdata + slot * Q3, t, qd);
-    }
-    if (live && t < P2) {
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) lo[ix] = loc[slot * P3 + t * P + ix];
-    }
-  }
-
-  while (true) {
-    const bool live = el < ne_here;
-    // Stage the block's u values in LDS.
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int k = 0; k < KN; ++k) {
-        const int j = tid + k * NT;
-        if (j < nn) s_n[c * ldn + j] = un[c][k];
-      }
-    __syncthreads();
-    // Prefetch the next block's header, node ids, qdata and local indices.
-    const int nb = blk + per;
-    const bool more = nb < b_hi;  // uniform
-    int n0_n = 0, nn_n = 0, nint_n = 0, ne_n = 0;
-    if (more) {
-      n0_n = node_ptr[nb];
-      nn_n = node_ptr[nb + 1] - n0_n;
-      nint_n = ninterior[nb];
-      ne_n = nelem[nb];
-    }
-    int gn_n[KN];
-    T qd_n[Q];
-    int lo_n[P];
-    if (more) {
-#pragma unroll
-      for (int k = 0; k < KN; ++k) {
-        const int j = tid + k * NT;
-        gn_n[k] = j < nn_n ? nodes[n0_n + j] : 0;
-      }
-      const bool live_n = el < ne_n;
-      const size_t slot_n = static_cast<size_t>(nb) * NB + el;
-      if (live_n && t < Q2) {
-        load_qline<Q, T>(qdata + slot_n * Q3, t, qd_n);
-      }
-      if (live_n && t < P2) {
-#pragma unroll
-        for (int ix = 0; ix < P; ++ix) lo_n[ix] = loc[slot_n * P3 + t * P + ix];
-      }
-    }
-    T un_n[NC][KN];
-    T* b = live ? s_b[el] : nullptr;
-#pragma unroll 1
-    for (int c = 0; c < NC; ++c) {
-      T* sn = s_n + c * ldn;
-      if (live && t < P2) {
-        const int row = (t / P) * ZS + (t % P) * RY;
-        T x[P];
-#pragma unroll
-        for (int ix = 0; ix < P; ++ix) x[ix] = sn[lo[ix]];
-#pragma unroll
-        for (int qx = 0; qx < Q; ++qx) {
-          T acc = T(0);
-#pragma unroll
-          for (int ix = 0; ix < P; ++ix) acc = fma_t(B(qx, ix), x[ix], acc);
-          b[row + qx] = acc;
-        }
-      }
-      __syncthreads();
-      for (int j = tid; j < nn; j += NT) sn[j] = T(0);
-      if (live && t < P * Q) {
-        const int col = (t / Q) * ZS + t % Q;
-        T x[P];
-#pragma unroll
-        for (int iy = 0; iy < P; ++iy) x[iy] = b[col + iy * RY];
-#pragma unroll
-        for (int qy = 0; qy < Q; ++qy) {
-          T acc = T(0);
-#pragma unroll
-          for (int iy = 0; iy < P; ++iy) acc = fma_t(B(qy, iy), x[iy], acc);
-          b[col + qy * RY] = acc;
-        }
-      }
-      if (c == 0 && more) {  // the next block's u gather, its node ids issued a stage ago
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc)
-#pragma unroll
-          for (int k = 0; k < KN; ++k)
-            un_n[cc][k] = tid + k * NT < nn_n
-                              ? u[static_cast<size_t>(gn_n[k]) + static_cast<size_t>(cc) * comp_stride]
-                              : T(0);
-      }
-      __syncthreads();
-      if (live && t < Q2) {
-        const int line = (t / Q) * RY + t % Q;
-        T x[P];
-#pragma unroll
-        for (int iz = 0; iz < P; ++iz) x[iz] = b[iz * ZS + line];
-        T d[Q];
-#pragma unroll
-        for (int qz = 0; qz < Q; ++qz) {
-          T acc = T(0);
-#pragma unroll
-          for (int iz = 0; iz < P; ++iz) acc = fma_t(B(qz, iz), x[iz], acc);
-          d[qz] = qd[qz] * acc;
-        }
-#pragma unroll
-        for (int iz = 0; iz < P; ++iz) {
-          T acc = T(0);
-#pragma unroll
-          for (int qz = 0; qz < Q; ++qz) acc = fma_t(B(qz, iz), d[qz], acc);
-          b[iz * ZS + line] = acc;
-        }
-      }
-      __syncthreads();
-      if (live && t < P * Q) {
-        const int col = (t / Q) * ZS + t % Q;
-        T x[Q];
-#pragma unroll
-        for (int qy = 0; qy < Q; ++qy) x[qy] = b[col + qy * RY];
-#pragma unroll
-        for (int iy = 0; iy < P; ++iy) {
-          T acc = T(0);
-#pragma unroll
-          for (int qy = 0; qy < Q; ++qy) acc = fma_t(B(qy, iy), x[qy], acc);
-          b[col + iy * RY] = acc;
-        }
-      }
-      __syncthreads();
-      if (live && t < P2) {
-        const int row = (t / P) * ZS + (t % P) * RY;
-        T x[Q];
-#pragma unroll
-        for (int qx = 0; qx < Q; ++qx) x[qx] = b[row + qx];
-#pragma unroll
-        for (int ix = 0; ix < P; ++ix) {
-          T acc = T(0);
-#pragma unroll
-          for (int qx = 0; qx < Q; ++qx) acc = fma_t(B(qx, ix), x[qx], acc);
-          unsafeAtomicAdd(&sn[lo[ix]], acc);  // LDS add
-        }
-      }
-      __syncthreads();
-    }
-    // Flush: interior nodes are complete; each boundary node's block sum is a partial.
-    const size_t pb = static_cast<size_t>(part_ptr[blk]);
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int k = 0; k < KN; ++k) {
-        const int j = tid + k * NT;
-        if (j < nn) {
-          const T val = s_n[c * ldn + j];
-          if (j < nint) {
-            v[static_cast<size_t>(gn[k]) + static_cast<size_t>(c) * comp_stride] = val;
-          } else {
-            partials[static_cast<size_t>(c) * part_comp_len + pb + (j - nint)] = val;
-          }
-        }
-      }
-    if (!more) break;
-    __syncthreads();  // s_n is restaged next
-    blk = nb;
-    n0 = n0_n;
-    nn = nn_n;
-    nint = nint_n;
-    ne_here = ne_n;
-#pragma unroll
-    for (int k = 0; k < KN; ++k) {
-      gn[k] = gn_n[k];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) un[c][k] = un_n[c][k];
-    }
-#pragma unroll
-    for (int qz = 0; qz < Q; ++qz) qd[qz] = qd_n[qz];
-#pragma unroll
-    for (int ix = 0; ix < P; ++ix) lo[ix] = lo_n[ix];
-  }
-}
-
 // Node-sum values a block kernel can keep in LDS (0 = the kernel sizes its own).
 template <int P, int Q, int NC, class T, int NB>
 constexpr int part_node_budget() {
@@ -1247,21 +829,6 @@ void launch_block_part(const FusedArgs& a, hipStream_t s) {
   HIP_CHECK_LAUNCH();
 }
 
-template <int P, int Q, int NC, class T, int NB>
-void launch_block_part_pipe(const FusedArgs& a, hipStream_t s) {
-  using S = Shape<P, Q>;
-  require(a.max_nodes > 0 && a.max_nodes <= part_node_budget<P, Q, NC, T, NB>() && a.max_nodes <= NB * S::P3,
-          "fused mass: block plan exceeds the kernel's LDS node budget");
-  constexpr int threads = round64(NB * S::TPE);
-  const size_t lds = static_cast<size_t>(NC) * part_node_budget<P, Q, NC, T, NB>() * sizeof(T);
-  const int grid = persistent_grid(fused_mass_part_pipe_kernel<P, Q, NC, T, NB>, threads, lds);
-  fused_mass_part_pipe_kernel<P, Q, NC, T, NB><<<static_cast<unsigned>(grid), threads, lds, s>>>(
-      a.nblk, a.comp_stride, a.node_ptr, a.nodes, a.ninterior, a.nelem, a.part_ptr, a.loc,
-      static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v), static_cast<T*>(a.partials),
-      a.part_comp_len, part_node_budget<P, Q, NC, T, NB>(), basis_arg<P, Q, NC, T>(a.basis));
-  HIP_CHECK_LAUNCH();
-}
-
 template <int P, int Q, int NC, class T, int NE, bool EXACT, bool DET>
 void launch_fused(const FusedArgs& a, hipStream_t s) {
   using S = Shape<P, Q>;
@@ -1283,26 +850,6 @@ void launch_fused(const FusedArgs& a, hipStream_t s) {
   HIP_CHECK_LAUNCH();
 }
 
-template <int P, int Q, int NC, class T, int NE>
-void launch_pipe(const FusedArgs& a, hipStream_t s) {
-  using S = Shape<P, Q>;
-  require(static_cast<long long>(a.num_elem) * S::P3 < (1ll << 31), "fused mass: offsets exceed int32 indexing");
-  constexpr int threads = round64(NE * S::TPE);
-  const int grid = persistent_grid(fused_mass_pipe_kernel<P, Q, NC, T, NE>, threads, 0);
-  fused_mass_pipe_kernel<P, Q, NC, T, NE><<<static_cast<unsigned>(grid), threads, 0, s>>>(
-      a.num_elem, a.comp_stride, a.offsets, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u),
-      static_cast<T*>(a.out), a.y_comp_len, basis_arg<P, Q, NC, T>(a.basis));
-  HIP_CHECK_LAUNCH();
-}
-
-template <int P, int Q, int NC, class T, int NE>
-struct FusedPipeReg {
-  FusedPipeReg() {
-    register_fused(FusedKey{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgElemTmapPipe, 0, NE},
-                   &launch_pipe<P, Q, NC, T, NE>, 0);
-  }
-};
-
 template <int P, int Q, int NC, class T, int NE, bool EXACT, bool DET>
 struct FusedReg {
   FusedReg() {
@@ -1312,8 +859,7 @@ struct FusedReg {
   }
 };
 
-// All element-algorithm variants of one shape: FP64 {atomic, tmap} x {fast, exact}, FP32 {atomic, tmap},
-// and the pipelined tmap kernel in both precisions.
+// All element-algorithm variants of one shape: FP64 {atomic, tmap} x {fast, exact}, FP32 {atomic, tmap}.
 template <int P, int Q, int NC, int NE64, int NE32>
 struct FusedShapeReg {
   FusedReg<P, Q, NC, double, NE64, false, false> a;
@@ -1322,8 +868,6 @@ struct FusedShapeReg {
   FusedReg<P, Q, NC, double, NE64, true, false> d;
   FusedReg<P, Q, NC, float, NE32, false, false> e;
   FusedReg<P, Q, NC, float, NE32, false, true> f;
-  FusedPipeReg<P, Q, NC, double, NE64> g;
-  FusedPipeReg<P, Q, NC, float, NE32> h;
 };
 
 template <int P, int Q, int NC, class T, int NB>
@@ -1334,8 +878,6 @@ struct FusedBlockReg {
     register_fused(FusedKey{P, Q, NC, fp32, kAlgBlockPart, 0, NB}, &launch_block_part<P, Q, NC, T, NB, false>,
                    part_node_budget<P, Q, NC, T, NB>());
     register_fused(FusedKey{P, Q, NC, fp32, kAlgBlockPartLds, 0, NB}, &launch_block_part<P, Q, NC, T, NB, true>,
-                   part_node_budget<P, Q, NC, T, NB>());
-    register_fused(FusedKey{P, Q, NC, fp32, kAlgBlockPartLdsPipe, 0, NB}, &launch_block_part_pipe<P, Q, NC, T, NB>,
                    part_node_budget<P, Q, NC, T, NB>());
   }
 };

@@ -21,9 +21,9 @@ EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 3
 QUAD_GAUSS, QUAD_GAUSS_LOBATTO = 0, 1
 SCATTER_ATOMIC, SCATTER_DETERMINISTIC = 0, 1
 FUSED_AUTO, FUSED_ELEM_ATOMIC, FUSED_ELEM_TMAP, FUSED_BLOCK_ATOMIC, FUSED_BLOCK_PARTIAL = -1, 0, 1, 2, 3
-FUSED_ELEM_TMAP_PIPE, FUSED_BLOCK_PARTIAL_LDS, FUSED_BLOCK_PARTIAL_LDS_PIPE, FUSED_BOX_COLUMN = 4, 5, 6, 7
-FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_partial", 4: "elem_tmap_pipe",
-               5: "block_partial_lds", 6: "block_partial_lds_pipe", 7: "box_column"}
+FUSED_BLOCK_PARTIAL_LDS, FUSED_BOX_COLUMN = 5, 7  # 4 and 6 are retired
+FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_partial", 5: "block_partial_lds",
+               7: "box_column"}
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
 QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR, QF_BUILD_DIFFUSION, QF_APPLY_DIFFUSION = 1, 2, 3, 4, 5
 SOLVE_FP64, SOLVE_MIXED, SOLVE_MIXED_RELIABLE = 0, 1, 2

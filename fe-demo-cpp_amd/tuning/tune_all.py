@@ -31,7 +31,7 @@ from fehip.problems import MassProblem  # noqa: E402
 
 SHAPES = [(2, 3, (1, 2, 3)), (3, 4, (1, 2, 3)), (4, 5, (1, 2, 3)), (5, 6, (1, 2, 3)), (6, 7, (1, 2, 3)),
           (8, 10, (1, 3))]
-DET = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BOX_COLUMN)
+DET = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_BOX_COLUMN)
 
 
 TARGETS = (375_000, 1_500_000, 6_000_000, 24_000_000)  # element nodes; the largest last (it sets the alg rows)

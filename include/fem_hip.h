@@ -248,9 +248,9 @@ int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, in
 #define FEM_HIP_FUSED_BLOCK_ATOMIC 2  /* element blocks summed in LDS, boundary nodes one atomic add each */
 #define FEM_HIP_FUSED_BLOCK_PARTIAL 3 /* element blocks summed in LDS, boundary partials summed by a second pass
                                          (deterministic) */
-#define FEM_HIP_FUSED_ELEM_TMAP_PIPE 4 /* ELEM_TMAP with a persistent, software-pipelined element kernel */
+/* 4 and 6 are retired (persistent software-pipelined forms of 1 and 5 that
+ * never measured faster); selecting them reports "not compiled". */
 #define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS 5 /* BLOCK_PARTIAL summing in LDS with atomics (any order) */
-#define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS_PIPE 6 /* BLOCK_PARTIAL_LDS as a persistent, software-pipelined kernel */
 #define FEM_HIP_FUSED_BOX_COLUMN 7 /* structured boxes (fem_hip_box_detect): element columns marching in z with node
                                       planes in LDS, shared column faces summed by a second pass (deterministic;
                                       no index arrays read); elems = elements per column layer (16, 8, 4),

@@ -367,8 +367,7 @@ TEST_CASE(mesh_file_round_trip_and_every_fused_algorithm) {
   DeviceArray<double> d_u(u), d_v;
 
   using A = fem::hip::FusedAlgorithm;
-  for (A alg : {A::ElemAtomic, A::ElemTmap, A::BlockAtomic, A::BlockPartial, A::ElemTmapPipe, A::BlockPartialLds,
-                A::BlockPartialLdsPipe}) {
+  for (A alg : {A::ElemAtomic, A::ElemTmap, A::BlockAtomic, A::BlockPartial, A::BlockPartialLds}) {
     fem::hip::Operator dmass;
     fem::hip::operator_create(mm.host, dmass);
     fem::hip::operator_set_fused_algorithm(dmass, alg);
@@ -411,8 +410,7 @@ TEST_CASE(mass_diagonal_autotune_and_solve) {
 
   const fem::hip::FusedChoice best = fem::hip::operator_autotune(dmass, false, 3);
   REQUIRE(best.ms > 0.0);
-  REQUIRE(best.alg == fem::hip::FusedAlgorithm::ElemTmap || best.alg == fem::hip::FusedAlgorithm::BlockPartial ||
-          best.alg == fem::hip::FusedAlgorithm::ElemTmapPipe);
+  REQUIRE(best.alg == fem::hip::FusedAlgorithm::ElemTmap || best.alg == fem::hip::FusedAlgorithm::BlockPartial);
   REQUIRE(fem::hip::operator_fused_choice(dmass).alg == best.alg);
 
   // Solve M x = M x* and compare with x*.

@@ -20,8 +20,8 @@ from oracle_lib import oracle
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 ALGS = [fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_ATOMIC, fehip.FUSED_BLOCK_PARTIAL,
-        fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BLOCK_PARTIAL_LDS, fehip.FUSED_BLOCK_PARTIAL_LDS_PIPE]
-DET = {fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP_PIPE}
+        fehip.FUSED_BLOCK_PARTIAL_LDS]
+DET = {fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL}
 
 
 @pytest.mark.parametrize("n,P,Q", [(4, 3, 4), (6, 5, 6), (4, 2, 3)])

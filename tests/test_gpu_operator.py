@@ -24,9 +24,9 @@ pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-13
 ALGS = (fehip.FUSED_ELEM_ATOMIC, fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_ATOMIC, fehip.FUSED_BLOCK_PARTIAL,
-        fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BLOCK_PARTIAL_LDS, fehip.FUSED_BLOCK_PARTIAL_LDS_PIPE,
+        fehip.FUSED_BLOCK_PARTIAL_LDS,
         fehip.FUSED_BOX_COLUMN)
-DET_ALGS = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP_PIPE, fehip.FUSED_BOX_COLUMN)
+DET_ALGS = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_BOX_COLUMN)
 
 
 def usable(op, alg):
@@ -356,3 +356,15 @@ def test_build_mass_with_padded_qdata_l_size(gpu):
     want = oracle().build_qdata(3, E, 2, Q, 3 * nv, nv, moff, coords)
     assert got[:E * Q ** 3].tobytes() == want.tobytes()
     assert np.all(got[E * Q ** 3:] == 0.0)
+
+
+def test_retired_algorithms_report_not_compiled(gpu):
+    """Algorithm numbers 4 and 6 (retired pipelined forms) select no kernel."""
+    p = MassProblem(4, 3, 4, 1, scatter=fehip.SCATTER_DETERMINISTIC)
+    for alg in (4, 6):
+        p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)
+        p.op.set_fused_algorithm(alg)
+        with pytest.raises(fehip.InvalidArgument):
+            p.op.fused_choice()
+    p.op.set_fused_algorithm(fehip.FUSED_AUTO)
+    assert p.op.fused_choice()[0] in ALGS

@@ -152,9 +152,9 @@ __host__ __device__ inline int box_owners(const BoxGeom& g, int P, int BX, int B
 // (nontemporal) data, which leaves the caches to the u rows that neighbouring
 // elements and columns read again: 64^3, P=5 FP64 box kernel 0.192 -> 0.177 ms
 // on MI355X. FP32 measured the same either way and keeps plain loads.
-template <int Q, class T>
+template <int Q, class T, bool NT = sizeof(T) == 8>
 __device__ __forceinline__ void load_qdata(const T* tile, int t, T (&q)[Q]) {
-  load_qline<Q, T, sizeof(T) == 8>(tile, t, q);
+  load_qline<Q, T, NT>(tile, t, q);
 }
 
 // One x-row of P consecutive values of u, read as 2-vectors of T (16-byte
@@ -281,7 +281,12 @@ struct BoxTile<5, 6> {
 // registers for a whole layer); 0: it is loaded in its own layer, right after
 // the first step's u rows, two stages before its use (12 fewer FP64 VGPRs at
 // Q=6, so more workgroups fit on a CU).
-template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA>
+// SPLIT > 1 (with NC = 1): the components of a blocked SPLIT-component field
+// are independent scalar masses; workgroup w takes component w % SPLIT of
+// segment w / SPLIT (after the XCD remap, so a segment's components run
+// together on one XCD and share its qdata through that XCD's L2, loaded
+// cached instead of streaming).
+template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA, int SPLIT = 1>
 __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_kernel(
     BoxGeom g, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
     size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em) {
@@ -303,6 +308,15 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   const int nwg = gridDim.x;
   int w = blockIdx.x;
   if (nwg % 8 == 0) w = (blockIdx.x % 8) * (nwg / 8) + blockIdx.x / 8;
+  static_assert(SPLIT == 1 || NC == 1, "a component-split kernel runs one component per workgroup");
+  if constexpr (SPLIT > 1) {
+    const int comp = w % SPLIT;
+    w /= SPLIT;
+    u += static_cast<size_t>(comp) * g.cs;
+    v += static_cast<size_t>(comp) * g.cs;
+    partials += static_cast<size_t>(comp) * part_comp_len;
+  }
+  constexpr bool kQdataNT = sizeof(T) == 8 && SPLIT == 1;
   const int cx = w % g.ncx, cy = (w / g.ncx) % g.ncy, sz = w / (g.ncx * g.ncy);
   const int ez0 = box_seg_z0(g, sz), nl = box_seg_z0(g, sz + 1) - ez0;  // this segment's layers
   const int LZn = g.lz * (P - 1) + 1;  // slot layout: the longest segment's node planes
@@ -386,7 +400,7 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
     }
   }
   T qc[Q] = {};
-  if (QA && line_thread) load_qdata<Q, T>(qd_ptr(ez0), t, qc);
+  if (QA && line_thread) load_qdata<Q, T, kQdataNT>(qd_ptr(ez0), t, qc);
   if constexpr (UM == 2) __syncthreads();
   // Nothing loaded above is still in flight when the layer loop starts. (Those
   // loads target registers that later stages reuse; left in flight, the
@@ -447,9 +461,9 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
         const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ezq);
         const T* r = qdata + e * Q3;
         if constexpr (QA) {
-          load_qdata<Q, T>(r, tq, qn);
+          load_qdata<Q, T, kQdataNT>(r, tq, qn);
         } else {
-          load_qdata<Q, T>(r, tq, qd);
+          load_qdata<Q, T, kQdataNT>(r, tq, qd);
         }
       }
       if (row_thread) {
@@ -605,7 +619,7 @@ inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
   return g;
 }
 
-template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA>
+template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA, bool SPLIT = false>
 void launch_box(const FusedArgs& a, hipStream_t s) {
   using S = Shape<P, Q>;
   const BoxGeom g = box_geom(a, P, BX, BY);
@@ -619,9 +633,16 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
           "fused mass: box partial buffer too small");
   if (a.box_pass == 0) {
     constexpr int threads = round64(BX * BY * S::TPE);
-    fused_box_kernel<P, Q, NC, T, BX, BY, UM, QA><<<static_cast<unsigned>(nseg), threads, 0, s>>>(
-        g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
-        static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
+    if constexpr (SPLIT) {
+      require(nseg * NC < (1ll << 31), "fused mass: box too large");
+      fused_box_kernel<P, Q, 1, T, BX, BY, UM, QA, NC><<<static_cast<unsigned>(nseg * NC), threads, 0, s>>>(
+          g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
+          static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
+    } else {
+      fused_box_kernel<P, Q, NC, T, BX, BY, UM, QA><<<static_cast<unsigned>(nseg), threads, 0, s>>>(
+          g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
+          static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
+    }
   } else {
     const dim3 grid = box_shared_grid(g);
     if (grid.z == 0) return;
@@ -634,21 +655,27 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
 
 // Key elems of a box-column variant: BX * BY elements per layer, plus 100 for
 // the kernel that loads qdata within its own layer (QA = 0), plus 200 for FP64
-// with u through the LDS ring. By default FP64 loads its x-rows straight from
-// u, FP32 goes through the LDS ring.
+// with u through the LDS ring, plus 300 (several components) for the
+// component-split kernel (one component per workgroup, qdata within its own
+// layer). By default FP64 loads its x-rows straight from u, FP32 goes through
+// the LDS ring.
 template <int P, int Q, int NC, int BX, int BY>
 struct FusedBoxReg {
   FusedBoxReg() {
-    reg<double, 0, 1>(0);
-    reg<double, 0, 0>(100);
-    reg<double, 2, 1>(200);
-    reg<float, 2, 1>(0);
-    reg<float, 2, 0>(100);
+    reg<double, 0, 1, false>(0);
+    reg<double, 0, 0, false>(100);
+    reg<double, 2, 1, false>(200);
+    reg<float, 2, 1, false>(0);
+    reg<float, 2, 0, false>(100);
+    if constexpr (NC > 1) {
+      reg<double, 0, 0, true>(300);
+      reg<float, 2, 0, true>(300);
+    }
   }
-  template <class T, int UM, int QA>
+  template <class T, int UM, int QA, bool SPLIT>
   static void reg(int code) {
     const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, BX * BY + code};
-    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, UM, QA>, 0);
+    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, UM, QA, SPLIT>, 0);
   }
 };
 

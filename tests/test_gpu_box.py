@@ -68,13 +68,16 @@ def test_box_column_matches_oracle(gpu, n3, P, Q, nc, elems, layers):
     u = fehip.rng_uniform(5, nc * N)
     d_u, d_u32 = fehip.DeviceArray.from_host(u), fehip.DeviceArray.from_host(u.astype(np.float32))
     # elems + 100: the kernel that loads qdata within its own layer; + 200: FP64
-    # through the LDS ring (FP32 has no such variant: it always takes the ring).
-    for lz, el in [(lz, el) for lz in layers for el in (elems, elems + 100, elems + 200)]:
+    # through the LDS ring (FP32 has no such variant: it always takes the ring);
+    # + 300 (several components): one component per workgroup.
+    variants = (elems, elems + 100, elems + 200) + ((elems + 300,) if nc > 1 else ())
+    for lz, el in [(lz, el) for lz in layers for el in variants]:
+        has32 = el // 100 != 2
         fehip.box_column_check(*n3, P, elems, lz)  # the index arithmetic, on the host
         p.op.set_box_layers(lz)
         p.op.set_fused_algorithm(BOX, el)
         assert p.op.fused_choice() == (BOX, el)
-        if el < 200:
+        if has32:
             assert p.op.fused_choice(fp32=True) == (BOX, el)
         d_v = fehip.DeviceArray.from_host(np.full(nc * N, np.nan))
         p.op.apply(d_u, d_v)
@@ -83,7 +86,7 @@ def test_box_column_matches_oracle(gpu, n3, P, Q, nc, elems, layers):
         d_v = fehip.DeviceArray.from_host(np.full(nc * N, np.nan))
         p.op.apply(d_u, d_v)
         assert d_v.to_host().tobytes() == got.tobytes(), (lz, el)
-        if el >= 200:
+        if not has32:
             continue
         d_v32 = fehip.DeviceArray.from_host(np.full(nc * N, np.nan, np.float32))
         p.op.apply(d_u32, d_v32)

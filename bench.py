@@ -205,6 +205,17 @@ def config5(fehip, args, stream, layers=2):
     d_in[np.float32] = fehip.DeviceArray.from_host(u_host.astype(np.float32))
     d_out = {np.float64: fehip.DeviceArray(L5, np.float64), np.float32: fehip.DeviceArray(L5, np.float32)}
     steps = max(args.steps // 2, 10)
+    # Each algorithm at its fastest packing on this operator (the autotuner's
+    # candidates; for box columns also the component-split kernel).
+    packing = {}
+    for fp32, scatter in ((False, A), (True, D)):
+        p5.op.set_fused_options(scatter, exact=0)
+        for r in p5.op.autotune(fp32=fp32, reps=3)[1]:
+            key = (fp32, r[0])
+            if key not in packing or r[2] < packing[key][1]:
+                packing[key] = (r[1], r[2])
+    big["tuned_packings"] = {("fp32_" if k[0] else "fp64_") + fehip.FUSED_NAMES[k[1]]: v[0]
+                             for k, v in packing.items()}
     for label, dt, scatter, alg in (("fp64_box_column", np.float64, D, fehip.FUSED_BOX_COLUMN),
                                     ("fp32_box_column", np.float32, D, fehip.FUSED_BOX_COLUMN),
                                     ("fp64_block_partial", np.float64, D, fehip.FUSED_BLOCK_PARTIAL),
@@ -212,17 +223,18 @@ def config5(fehip, args, stream, layers=2):
                                     ("fp64_block_atomic", np.float64, A, fehip.FUSED_BLOCK_ATOMIC),
                                     ("fp32_block_partial", np.float32, D, fehip.FUSED_BLOCK_PARTIAL)):
         p5.op.set_fused_options(scatter, exact=0)
-        p5.op.set_fused_algorithm(alg)
+        p5.op.set_fused_algorithm(alg, packing.get((dt == np.float32, alg), (0,))[0])
         m, k, a = time_applies(fehip, p5.op, d_in[dt], d_out[dt], steps, args.warmup, stream)
         sz = 4 if dt == np.float32 else 8
         big[label] = {"ms_per_apply": m, "gdofs": L5 / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
+                      "elems": p5.op.fused_choice(dt == np.float32)[1],
                       "apply_frac_hbm": p5.apply_bytes(sz, sz, sz) / (m * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log("config5", label, json.dumps(big[label]))
     best = min((v["ms_per_apply"], k2) for k2, v in big.items() if k2.startswith("fp64"))[1]
     p5.op.set_fused_options(D, exact=0)
-    p5.op.set_fused_algorithm({"fp64_box_column": fehip.FUSED_BOX_COLUMN, "fp64_block_partial":
-                               fehip.FUSED_BLOCK_PARTIAL, "fp64_elem_tmap": fehip.FUSED_ELEM_TMAP}.get(
-                                   best, fehip.FUSED_AUTO))
+    best_alg = {"fp64_box_column": fehip.FUSED_BOX_COLUMN, "fp64_block_partial": fehip.FUSED_BLOCK_PARTIAL,
+                "fp64_elem_tmap": fehip.FUSED_ELEM_TMAP}.get(best, fehip.FUSED_AUTO)
+    p5.op.set_fused_algorithm(best_alg, packing.get((False, best_alg), (0,))[0])
     # The FP64 apply against the oracle on the slab.
     t1 = time.perf_counter()
     orc, take, l_s, rows = slab_oracle(p5, layers)

@@ -764,6 +764,12 @@ int fem_hip_box_offsets(int nx, int ny, int nz, int P, int node_stride, int* off
     box_offsets(nx, ny, nz, P, node_stride, offsets);
   });
 }
+int fem_hip_box_offsets_device(int nx, int ny, int nz, int P, int node_stride, int* offsets, void* s) {
+  return guard([&] { box_offsets_device(nx, ny, nz, P, node_stride, offsets, st(s)); });
+}
+int fem_hip_box_coords_device(int n, double jitter, uint64_t seed, double* coords, void* s) {
+  return guard([&] { box_coords_device(n, jitter, seed, coords, st(s)); });
+}
 int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords) {
   return guard([&] {
     need(coords, "coords");
@@ -848,7 +854,7 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
     int max_owners = 0;
     long long slots_read = 0, visited = 0;
     for (unsigned pl = 0; pl < grid.z; ++pl) {
-      for (unsigned ln = 0; ln < grid.y; ++ln) {
+      for (unsigned ln = 0; ln < grid.y * kSharedLines; ++ln) {
         for (unsigned pt = 0; pt < grid.x * 64; ++pt) {
           int gx, gy, gz, seg[8], sl[8];
           if (!box_shared_point(g, P, BX, BY, static_cast<int>(pl), static_cast<int>(ln), static_cast<int>(pt), &gx, &gy,

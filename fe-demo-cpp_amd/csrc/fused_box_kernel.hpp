@@ -577,12 +577,16 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
 }
 
 // v at every shared node = its partials summed in ascending segment order
-// (grid: see box_shared_point; blockDim.x = 64 points of a line).
+// (grid: see box_shared_point; a block is kSharedLines lines of 64 points).
+constexpr int kSharedLines = 4;
 template <class T, int P, int BX, int BY>
-__global__ __launch_bounds__(64) void box_shared_sum_kernel(BoxGeom g, int nc, const T* __restrict__ partials,
-                                                            size_t part_comp_len, T* __restrict__ v) {
+__global__ __launch_bounds__(64 * kSharedLines) void box_shared_sum_kernel(BoxGeom g, int nc,
+                                                                           const T* __restrict__ partials,
+                                                                           size_t part_comp_len, T* __restrict__ v) {
   int gx, gy, gz, seg[8], slot[8];
-  if (!box_shared_point(g, P, BX, BY, blockIdx.z, blockIdx.y, blockIdx.x * 64 + threadIdx.x, &gx, &gy, &gz)) return;
+  if (!box_shared_point(g, P, BX, BY, blockIdx.z, blockIdx.y * kSharedLines + threadIdx.y,
+                        blockIdx.x * 64 + threadIdx.x, &gx, &gy, &gz))
+    return;
   const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, slot);
   const int LZn = g.lz * (P - 1) + 1;
   const size_t face = static_cast<size_t>(box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1, LZn));
@@ -600,7 +604,8 @@ inline dim3 box_shared_grid(const BoxGeom& g) {
   const int planes = (g.ncx - 1) + (g.ncy - 1) + (g.nsz - 1);
   const int lines = g.Nz > g.Ny ? g.Nz : g.Ny;
   const int points = g.Nx > g.Ny ? g.Nx : g.Ny;
-  return dim3(static_cast<unsigned>((points + 63) / 64), static_cast<unsigned>(lines), static_cast<unsigned>(planes));
+  return dim3(static_cast<unsigned>((points + 63) / 64), static_cast<unsigned>((lines + kSharedLines - 1) / kSharedLines),
+              static_cast<unsigned>(planes));
 }
 
 inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
@@ -647,8 +652,8 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
     const dim3 grid = box_shared_grid(g);
     if (grid.z == 0) return;
     require(grid.y < 65536 && grid.z < 65536, "fused mass: box too large for the shared-node pass");
-    box_shared_sum_kernel<T, P, BX, BY><<<grid, 64, 0, s>>>(g, NC, static_cast<const T*>(a.partials), a.part_comp_len,
-                                                           static_cast<T*>(a.v));
+    box_shared_sum_kernel<T, P, BX, BY><<<grid, dim3(64, kSharedLines), 0, s>>>(
+        g, NC, static_cast<const T*>(a.partials), a.part_comp_len, static_cast<T*>(a.v));
   }
   HIP_CHECK_LAUNCH();
 }

@@ -383,7 +383,7 @@ DevPlan& ensure_plan(FusedMass& f, int NB, int budget, bool part, hipStream_t s)
   if (!slot) {
     ++f.gen;
     const int P3 = f.P * f.P * f.P, Q3 = f.Q * f.Q * f.Q;
-    BlockPlan plan = build_block_plan(f.rstr->h_offsets.data(), f.num_elem, P3, f.l_size, NB,
+    BlockPlan plan = build_block_plan(f.rstr->host_offsets().data(), f.num_elem, P3, f.l_size, NB,
                                       budget > 0 ? std::min(budget, NB * P3) : 0);
     auto d = std::make_unique<DevPlan>();
     d->NB = NB;
@@ -420,7 +420,7 @@ DevPlan& ensure_plan(FusedMass& f, int NB, int budget, bool part, hipStream_t s)
   DevPlan& d = *slot;
   if (part && d.row_ptr.empty()) {
     // The plan was made for the atomic kernel with the same budget: add the rows.
-    BlockPlan plan = build_block_plan(f.rstr->h_offsets.data(), f.num_elem, f.P * f.P * f.P, f.l_size, NB,
+    BlockPlan plan = build_block_plan(f.rstr->host_offsets().data(), f.num_elem, f.P * f.P * f.P, f.l_size, NB,
                                       budget > 0 ? std::min(budget, NB * f.P * f.P * f.P) : 0);
     PartialRows r = build_partial_rows(plan, f.l_size, f.NC, f.comp_stride);
     d.nrows = static_cast<int>(r.node.size());
@@ -469,13 +469,13 @@ void ensure_tmap(FusedMass& f) {
       f.tm_covers_all = true;  // the nodes are exactly 0..comp_stride-1: the components tile [0, l_size)
     } else {
       std::vector<char> referenced(f.l_size, 0);
-      for (int o : r.h_offsets) referenced[o] = 1;
+      for (int o : r.host_offsets()) referenced[o] = 1;
       f.tm_covers_all = tmap_covers_all(f, referenced);
     }
     f.tm_built = true;
     return;
   }
-  const std::vector<int>& off = r.h_offsets;
+  const std::vector<int>& off = r.host_offsets();
   const size_t nt = off.size();
   std::vector<int> mult(static_cast<size_t>(f.l_size) + 1, 0);
   for (size_t t = 0; t < nt; ++t) ++mult[off[t]];
@@ -770,7 +770,13 @@ __global__ __launch_bounds__(kThreads) void box_check_kernel(size_t nt, int P, i
 // detect_box on the restriction's device copy of the offsets.
 bool detect_box_device(const Restriction& r, int P, int num_comp, int* n3, hipStream_t s) {
   long long n3l[3];
-  if (!box_shape(r.h_offsets.data(), r.num_elem, P, num_comp, r.comp_stride, r.l_size, n3l)) return false;
+  // The shape follows from offsets 0, P and P*P (box_shape); the device check
+  // below tests all of them.
+  std::vector<int> head(static_cast<size_t>(P) * P + 1);
+  if (static_cast<size_t>(r.num_elem) * P * P * P < head.size()) return false;
+  HIP_CHECK(hipMemcpyAsync(head.data(), r.offsets.data(), head.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (!box_shape(head.data(), r.num_elem, P, num_comp, r.comp_stride, r.l_size, n3l)) return false;
   const size_t nt = static_cast<size_t>(r.num_elem) * P * P * P;
   const int Nx = static_cast<int>(n3l[0] * (P - 1) + 1);
   const long long Nxy = static_cast<long long>(Nx) * (n3l[1] * (P - 1) + 1);

@@ -19,6 +19,7 @@
 #include <numeric>
 
 #include "restriction.hpp"
+#include "mesh.hpp"
 
 namespace femhip {
 
@@ -239,7 +240,17 @@ void build_transpose_map_device(Restriction& r, hipStream_t s) {
 
 bool Restriction::same_layout(const Restriction& o) const {
   return type == o.type && num_elem == o.num_elem && elem_size == o.elem_size && num_comp == o.num_comp &&
-         comp_stride == o.comp_stride && l_size == o.l_size && strides == o.strides && h_offsets == o.h_offsets;
+         comp_stride == o.comp_stride && l_size == o.l_size && strides == o.strides &&
+         (type != 0 || host_offsets() == o.host_offsets());
+}
+
+const std::vector<int>& Restriction::host_offsets() const {
+  std::lock_guard<std::mutex> lk(h_mu);
+  if (!h_offsets_valid) {
+    h_offsets = offsets.download();
+    h_offsets_valid = true;
+  }
+  return h_offsets;
 }
 
 void build_transpose_map(int num_elem, int elem_size, int l_size, const int* offsets, TransposeMapHost& map) {
@@ -296,8 +307,42 @@ Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp
     r->scatter_mode = scatter_mode;
     r->max_offset = hi;
     r->h_offsets.assign(offsets, offsets + nt);
+    r->h_offsets_valid = true;
     r->offsets.upload(r->h_offsets);
     if (scatter_mode == kScatterDeterministic) build_transpose_map_device(*r, s);
+  } catch (...) {
+    delete r;
+    throw;
+  }
+  return r;
+}
+
+Restriction* restriction_create_box(int nx, int ny, int nz, int P, int num_comp, int scatter_mode, hipStream_t s) {
+  require(nx > 0 && ny > 0 && nz > 0 && P >= 2 && num_comp > 0, "elem_restriction_create: bad box");
+  require(scatter_mode == kScatterAtomic || scatter_mode == kScatterDeterministic,
+          "elem_restriction_create: unknown scatter mode");
+  const long long N = (static_cast<long long>(nx) * (P - 1) + 1) * (static_cast<long long>(ny) * (P - 1) + 1) *
+                      (static_cast<long long>(nz) * (P - 1) + 1);
+  const long long ne = static_cast<long long>(nx) * ny * nz, nt = ne * P * P * P;
+  require(N * num_comp < (1ll << 31) && nt < (1ll << 31), "elem_restriction_create: box too large for int32");
+  auto* r = new Restriction();
+  try {
+    r->type = 0;
+    r->num_elem = static_cast<int>(ne);
+    r->elem_size = P * P * P;
+    r->num_comp = num_comp;
+    r->comp_stride = static_cast<int>(N);
+    r->l_size = static_cast<int>(N * num_comp);
+    r->e_size = static_cast<size_t>(num_comp) * nt;
+    r->scatter_mode = scatter_mode;
+    r->max_offset = static_cast<int>(N - 1);
+    r->offsets.allocate(static_cast<size_t>(nt));
+    box_offsets_device(nx, ny, nz, P, 1, r->offsets.data(), s);
+    if (scatter_mode == kScatterDeterministic) {
+      build_transpose_map_device(*r, s);  // synchronizes s
+    } else {
+      HIP_CHECK(hipStreamSynchronize(s));
+    }
   } catch (...) {
     delete r;
     throw;
@@ -349,7 +394,8 @@ Restriction* restriction_clone(const Restriction& src) {
     r->strides = src.strides;
     r->backend_strides = src.backend_strides;
     r->scatter_mode = src.scatter_mode;
-    r->h_offsets = src.h_offsets;
+    r->h_offsets = src.host_offsets();
+    r->h_offsets_valid = true;
     r->num_nodes = src.num_nodes;
     r->identity_nodes = src.identity_nodes;
     r->max_offset = src.max_offset;

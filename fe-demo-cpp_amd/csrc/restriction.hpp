@@ -7,6 +7,7 @@
 #pragma once
 
 #include <array>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -31,8 +32,10 @@ struct Restriction {
   bool backend_strides = false;
   int scatter_mode = kScatterAtomic;
 
-  std::vector<int> h_offsets;  // host copy, for operators built on this restriction
-  DevBuf<int> offsets;         // [num_elem][elem_size]
+  DevBuf<int> offsets;  // [num_elem][elem_size]
+  // Host copy of the offsets (block plans, host checks): made at create from
+  // host offsets, or downloaded on first use for restrictions made on the device.
+  const std::vector<int>& host_offsets() const;
 
   int max_offset = 0;  // largest offset (offset restrictions)
 
@@ -43,6 +46,10 @@ struct Restriction {
   DevBuf<int> l_vec_indices, t_offsets, t_indices;
 
   bool same_layout(const Restriction& o) const;
+
+  mutable std::vector<int> h_offsets;
+  mutable bool h_offsets_valid = false;
+  mutable std::mutex h_mu;
 };
 
 // The deterministic map is built on stream s (nullptr: the null stream), which
@@ -50,6 +57,12 @@ struct Restriction {
 Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp, int comp_stride, int l_size,
                                        const int* offsets, int scatter_mode, hipStream_t s = nullptr);
 Restriction* restriction_create_strided(int num_elem, int elem_size, int num_comp, int l_size, const int* strides);
+// The offset restriction of an nx*ny*nz box of P^3-node hexes with
+// lexicographic node numbering and blocked components (comp_stride = node
+// count, l_size = num_comp * node count), its offsets made on the device (valid
+// by construction: no host validation or upload); the deterministic map is
+// built on s, which is synchronized before the call returns.
+Restriction* restriction_create_box(int nx, int ny, int nz, int P, int num_comp, int scatter_mode, hipStream_t s);
 // Deep copy (device arrays duplicated), for operators that store restrictions by value.
 Restriction* restriction_clone(const Restriction& r);
 

@@ -137,17 +137,12 @@ void validate(const ServeRequest& r) {
 std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s) {
   const int n = r.n, P = r.P, Q = r.Q, nc = r.num_comp;
   const int E = n * n * n, P3 = P * P * P, Q3 = Q * Q * Q, nv = (n + 1) * (n + 1) * (n + 1);
-  const int N = static_cast<int>(ipow3(static_cast<long long>(n) * (P - 1) + 1));
-  std::vector<int> off(static_cast<size_t>(E) * P3), moff(static_cast<size_t>(E) * 8);
-  box_offsets(n, n, n, P, 1, off.data());
-  box_offsets(n, n, n, 2, 1, moff.data());
-  std::vector<double> coords(static_cast<size_t>(3) * nv);
-  box_coords(n, r.jitter, r.mesh_seed, coords.data());
-
-  std::shared_ptr<const Restriction> rstr(
-      restriction_create_offset(E, P3, nc, N, nc * N, off.data(), kScatterDeterministic, s));
-  std::shared_ptr<const Restriction> mrstr(
-      restriction_create_offset(E, 8, 3, nv, 3 * nv, moff.data(), kScatterAtomic, s));
+  // Offsets (node and mesh restrictions) and the jittered vertex coordinates
+  // are made on the device: entry for entry what box_offsets and box_coords
+  // make on the host, with no host arrays, validation or uploads.
+  (void)P3;
+  std::shared_ptr<const Restriction> rstr(restriction_create_box(n, n, n, P, nc, kScatterDeterministic, s));
+  std::shared_ptr<const Restriction> mrstr(restriction_create_box(n, n, n, 2, 3, kScatterAtomic, s));
   std::shared_ptr<const Restriction> qrstr(restriction_create_strided(E, Q3, 1, E * Q3, nullptr));
   std::shared_ptr<const Basis> basis(basis_create(make_h1_lagrange(3, nc, P, Q, 0)));
   std::shared_ptr<const Basis> mbasis(basis_create(make_h1_lagrange(3, 3, 2, Q, 0)));
@@ -164,8 +159,8 @@ std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s) {
   operator_set_field(*bop, "dx", mrstr, mbasis, kVectorActive, nullptr, 0, false);
   operator_set_field(*bop, "weights", nullptr, mbasis, kVectorNone, nullptr, 0, false);
   operator_set_field(*bop, "qdata", qrstr, nullptr, kVectorActive, nullptr, 0, false);
-  DevBuf<double> d_coords, qdata(static_cast<size_t>(E) * Q3);
-  d_coords.upload(coords);
+  DevBuf<double> d_coords(static_cast<size_t>(3) * nv), qdata(static_cast<size_t>(E) * Q3);
+  box_coords_device(n, r.jitter, r.mesh_seed, d_coords.data(), s);
   operator_apply(*bop, d_coords.data(), d_coords.size(), qdata.data(), qdata.size(), s);
 
   QFunctionDev mq;
@@ -390,10 +385,21 @@ class MassServer {
         rng_uniform_device(j->req.rhs_seed, L, -1.0, 1.0, j->b.data(), j->stream);
         j->x.allocate(L);
         FusedMass& f = operator_fused_mass(*j->op, j->stream);
-        if (opt_.autotune) maybe_tune(f, j->req.mode != 0, j->stream);
-        if (opt_.cheap_setup || !tuned_shape(f, j->req.mode != 0)) {
-          // The transpose-map algorithm (at its tuned packing, if any): its setup is
-          // a counting sort, where the block algorithms cluster elements on the host.
+        const bool mixed = j->req.mode != 0;
+        if (opt_.autotune) maybe_tune(f, mixed, j->stream);
+        // The automatic choice (tuning table, else the default order) in every
+        // precision the solve applies in; with cheap_setup only where it is an
+        // algorithm whose setup is all on the device (box columns, element +
+        // transpose map), and without it only where the table holds a row for
+        // this size. Otherwise the transpose-map algorithm (its setup is the
+        // device counting sort the restriction already did; the block
+        // algorithms cluster elements on the host).
+        bool use_auto = opt_.cheap_setup || tuned_shape(f, mixed);
+        for (int fp32 = 0; fp32 <= (mixed ? 1 : 0) && use_auto; ++fp32) {
+          const FusedKey k = fused_resolve(f, fp32);
+          if (opt_.cheap_setup && !(alg_box(k.alg) || k.alg == kAlgElemTmap)) use_auto = false;
+        }
+        if (!use_auto) {
           j->op->opt_alg = kAlgElemTmap;
           f.choice.alg = kAlgElemTmap;
         }

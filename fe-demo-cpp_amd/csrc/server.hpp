@@ -50,8 +50,10 @@ struct ServerOptions {
   int batch_iters = 8;        // CG iterations per graph launch
   bool autotune = false;      // tune each new (P, Q, num_comp, precision) on first sight
   bool keep_x = true;         // return the solution vectors
-  // true: every request uses the transpose-map scatter (cheapest setup) at its
-  // tuned packing; false: the tuning table's fastest algorithm where it has one.
+  // true: the tuned (automatic) algorithm where its setup is all on the device
+  // (box columns, element + transpose map), else the transpose map; false: the
+  // tuning table's fastest algorithm where it has a row for the size (block
+  // algorithms cluster elements on the host), else the transpose map.
   bool cheap_setup = true;
 };
 

@@ -174,6 +174,8 @@ SIGNATURES = {
     "fem_hip_stream_copy": [_vp, _vp, _sz, _vp],
     "fem_hip_box_offsets": [_i, _i, _i, _i, _i, _vp],
     "fem_hip_box_coords": [_i, _d, ctypes.c_uint64, _vp],
+    "fem_hip_box_offsets_device": [_i, _i, _i, _i, _i, _vp, _vp],
+    "fem_hip_box_coords_device": [_i, _d, ctypes.c_uint64, _vp, _vp],
     "fem_hip_block_plan_stats": [_i, _i, _i, _vp, _i, _i, _i, _i, _llp],
     "fem_hip_box_detect": [_i, _i, _vp, _i, _i, _i, _vp, _vp],
     "fem_hip_box_column_check": [_i, _i, _i, _i, _i, _i, _llp],
@@ -390,6 +392,20 @@ def rng_uniform_device(seed, n, lo=-1.0, hi=1.0, stream=None):
 def box_offsets(nx, ny, nz, P, node_stride=1):
     out = np.empty(nx * ny * nz * P ** 3, dtype=np.int32)
     call("fem_hip_box_offsets", nx, ny, nz, P, node_stride, _ptr(out))
+    return out
+
+
+def box_offsets_device(nx, ny, nz, P, node_stride=1, stream=None):
+    """box_offsets made on the device (a DeviceArray of int32)."""
+    out = DeviceArray(nx * ny * nz * P ** 3, np.int32)
+    call("fem_hip_box_offsets_device", nx, ny, nz, P, node_stride, out.ptr, _s(stream))
+    return out
+
+
+def box_coords_device(n, jitter=0.1, seed=0, stream=None):
+    """box_coords made on the device (a DeviceArray of 3*(n+1)^3 doubles)."""
+    out = DeviceArray(3 * (n + 1) ** 3)
+    call("fem_hip_box_coords_device", n, float(jitter), ctypes.c_uint64(seed), out.ptr, _s(stream))
     return out
 
 

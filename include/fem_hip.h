@@ -340,10 +340,17 @@ int fem_hip_rng_uniform_device(uint64_t seed, size_t n, double lo, double hi, do
 /* Lexicographic offsets of an nx*ny*nz box of P^3-node hexes on the (nx(P-1)+1)x.. grid,
  * offsets (gx + Nx*(gy + Ny*gz)) * node_stride (tests/cuda/test-cuda-elem-restriction.cu:24-42). */
 int fem_hip_box_offsets(int nx, int ny, int nz, int P, int node_stride, int* offsets);
+/* The same offsets, written to device memory by a kernel enqueued on stream (the
+ * serving loop makes its meshes this way). No counterpart in the reference. */
+int fem_hip_box_offsets_device(int nx, int ny, int nz, int P, int node_stride, int* offsets_device, void* stream);
 /* Vertex coordinates of an n^3 unit-cube mesh, blocked [x...,y...,z...] over the (n+1)^3
  * vertices, interior vertices moved by jitter*h*U[-1,1) per axis, drawn x,y,z per vertex
  * in lexicographic order from splitmix64(seed). */
 int fem_hip_box_coords(int n, double jitter, uint64_t seed, double* coords);
+/* The same coordinates, bit for bit, written to device memory (3*(n+1)^3
+ * doubles) by a kernel enqueued on stream (the splitmix64 stream in counter
+ * form). No counterpart in the reference. */
+int fem_hip_box_coords_device(int n, double jitter, uint64_t seed, double* coords_device, void* stream);
 
 /* Host-only statistics and invariant checks of the element-block plan used by
  * the block algorithms: blocks of at most nb elements and max_nodes unique nodes
@@ -431,8 +438,10 @@ typedef struct {
   int batch_iters;     /* CG iterations per graph launch */
   int autotune;        /* 1: tune each new (P, Q, num_comp, precision) on first sight */
   int keep_x;          /* 1: keep solutions for fem_hip_server_wait */
-  int cheap_setup;     /* 1: transpose-map scatter for every request (no host element clustering);
-                          0: the tuning table's fastest algorithm where it has one */
+  int cheap_setup;     /* 1: the automatic (tuned) algorithm where its setup is all on the device (box
+                          columns, element + transpose map), else the transpose map; 0: the tuning
+                          table's fastest algorithm where it has a row for the size (block algorithms
+                          cluster elements on the host), else the transpose map */
 } fem_hip_server_options;
 typedef struct {
   int n, P, Q, num_comp; /* n^3 hexes of the unit cube, order P, Q Gauss points */

@@ -74,3 +74,18 @@ def test_stream_copy(gpu):
         fehip.stream_copy(odd_dst, odd_src, s)
     with pytest.raises(fehip.InvalidArgument):
         fehip.stream_copy(dst, odd_src, s)
+
+
+@pytest.mark.parametrize("n3,P,stride", [((3, 4, 5), 2, 1), ((4, 4, 4), 5, 1), ((2, 3, 2), 8, 3), ((7, 1, 2), 3, 2)])
+def test_box_offsets_device_equal_host(gpu, n3, P, stride):
+    """The device box offsets (the serving loop's meshes) equal the host generator's entry for entry."""
+    d = fehip.box_offsets_device(*n3, P, stride)
+    assert d.to_host().tobytes() == fehip.box_offsets(*n3, P, stride).tobytes()
+
+
+@pytest.mark.parametrize("n,jitter,seed", [(1, 0.1, 0), (5, 0.1, 3), (16, 0.1, 0), (9, 0.3, 1234567)])
+def test_box_coords_device_equal_host(gpu, n, jitter, seed):
+    """The device vertex coordinates equal the host's bit for bit (the same
+    splitmix64 draws, in counter form, and the same roundings)."""
+    d = fehip.box_coords_device(n, jitter, seed)
+    assert d.to_host().tobytes() == fehip.box_coords(n, jitter, seed).tobytes()

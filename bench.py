@@ -290,13 +290,17 @@ def solve_latency(fehip, reps, n=16, P=3, Q=4):
     xs = x_star.to_host()
     out = {"workload": "%d^3 hexes, P=%d, Q=%d, %d DoF, rtol 1e-10" % (n, P, Q, p.l_size)}
     x64 = None
-    for name, mode in (("fp64", fehip.SOLVE_FP64), ("mixed", fehip.SOLVE_MIXED),
-                       ("mixed_reliable", fehip.SOLVE_MIXED_RELIABLE)):
-        p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+    # fp64 / mixed run the automatic CG form (pipelined at this size); the
+    # *_plain_cg lines the five-launch form for comparison.
+    for name, mode, cgv in (("fp64", fehip.SOLVE_FP64, fehip.CG_AUTO), ("mixed", fehip.SOLVE_MIXED, fehip.CG_AUTO),
+                            ("mixed_reliable", fehip.SOLVE_MIXED_RELIABLE, fehip.CG_AUTO),
+                            ("fp64_plain_cg", fehip.SOLVE_FP64, fehip.CG_PLAIN),
+                            ("mixed_plain_cg", fehip.SOLVE_MIXED, fehip.CG_PLAIN)):
+        p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=cgv)
         lat, st = [], None
         for _ in range(reps):
             t0 = time.perf_counter()
-            st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10)
+            st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=cgv)
             lat.append((time.perf_counter() - t0) * 1e3)
         x = d_x.to_host()
         if x64 is None:
@@ -308,6 +312,7 @@ def solve_latency(fehip, reps, n=16, P=3, Q=4):
                      "err_vs_fp64": float(np.max(np.abs(x - x64)) / scale)}
     out["speedup_mixed_vs_fp64_p50"] = out["fp64"]["p50_ms"] / out["mixed"]["p50_ms"]
     out["speedup_reliable_vs_fp64_p50"] = out["fp64"]["p50_ms"] / out["mixed_reliable"]["p50_ms"]
+    out["speedup_pipelined_vs_plain_fp64_p50"] = out["fp64_plain_cg"]["p50_ms"] / out["fp64"]["p50_ms"]
     return out
 
 

@@ -376,11 +376,13 @@ inline void operator_apply(Operator& op, const DeviceArray<float>& in, DeviceArr
 // _MIXED (FP32 inner PCG, FP64 refinement) or _MIXED_RELIABLE (FP32 PCG with
 // FP64 residual replacement).
 // inner_rtol <= 0: 1e-5 per inner solve (MIXED), 1e-2 between replacements (MIXED_RELIABLE).
+// cg_variant: FEM_HIP_CG_AUTO, _PLAIN or _PIPELINED.
 inline fem_hip_solve_stats mass_solve(Operator& op, const DeviceArray<double>& b, DeviceArray<double>& x, int mode,
-                                      double rtol = 1e-10, void* stream = nullptr, double inner_rtol = 0.0) {
+                                      double rtol = 1e-10, void* stream = nullptr, double inner_rtol = 0.0,
+                                      int cg_variant = FEM_HIP_CG_AUTO) {
   if (inner_rtol <= 0.0) inner_rtol = 0.0;  // the library's default for the mode
   x.resize(b.size());
-  fem_hip_solve_options o{mode, 2000, 30, rtol, inner_rtol};
+  fem_hip_solve_options o{mode, 2000, 30, rtol, inner_rtol, cg_variant};
   fem_hip_solve_stats st{};
   FEM_HIP_CHECK(fem_hip_mass_solve(op.handle.get(), b.data(), x.data(), &o, &st, stream));
   return st;

@@ -64,9 +64,13 @@ int red_blocks(size_t n) {
 // replacement continues from it).
 // kMAXIT (the iteration limit) and kTHRESH are written by init / replace, so
 // a captured graph follows the limits of the run it is replayed in.
-enum { kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kRZ1, kRR1, kDONE1, kITERS1, kMAXIT, kNSCAL };
+// kAP (pipelined CG: the previous step's alpha) has two copies like kRZ.
+enum {
+  kRZ = 0, kPAP, kALPHA, kBETA, kRR, kDONE, kITERS, kTHRESH, kRZ1, kRR1, kDONE1, kITERS1, kMAXIT, kAP, kAP1, kNSCAL
+};
 __host__ __device__ constexpr int slot(int s, int par) {
-  return par == 0 ? s : (s == kRZ ? kRZ1 : s == kRR ? kRR1 : s == kDONE ? kDONE1 : kITERS1);
+  return par == 0 ? s
+                  : (s == kRZ ? kRZ1 : s == kRR ? kRR1 : s == kDONE ? kDONE1 : s == kAP ? kAP1 : kITERS1);
 }
 // Partial-sum regions of `part`: r.z and r.r (update and init), p.Ap (dot).
 constexpr int kPartRZ = 0, kPartRR = kMaxRed, kPartPAP = 2 * kMaxRed;
@@ -408,6 +412,153 @@ __global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, int par, int n
     p[i] = z[i] + beta * p[i];
 }
 
+// ---- Pipelined Jacobi-PCG (Ghysels & Vanroose, "Hiding global
+// synchronization latency in the preconditioned Conjugate Gradient
+// algorithm", Parallel Computing 40(7), 2014, Alg. 3, with M = diag(A)).
+// Vectors: x, r, u = M^-1 r, w = A u, m = M^-1 w, n = A m, and the recurrences
+// p, s = A p, q = M^-1 s, z = A q. An iteration is the apply n = A m and one
+// kernel (pipe_step_kernel), in place of the apply, dot, update and direction
+// kernels of the plain form: fewer dependent launches for problems whose
+// kernels are too short to fill the GPU, at the price of more vector traffic
+// (about 20 vector passes an iteration instead of 13). The dot products of an
+// iteration are formed from partials the previous kernel wrote, double-buffered
+// by parity (regions gamma = r.u, delta = w.u, rho = r.r).
+constexpr int kPipePart = 3 * kMaxRed;
+
+// x = 0, r = b, u = M^-1 r
+template <class T>
+__global__ __launch_bounds__(kThreads) void pipe_init_kernel(size_t n, const T* __restrict__ b,
+                                                             const T* __restrict__ dinv, T* __restrict__ x,
+                                                             T* __restrict__ r, T* __restrict__ u) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    const T ri = b[i];
+    x[i] = T(0);
+    r[i] = ri;
+    u[i] = dinv[i] * ri;
+  }
+}
+
+// m = M^-1 w; the parity-0 partials of r.u, w.u and r.r.
+template <class T>
+__global__ __launch_bounds__(kThreads) void pipe_init2_kernel(size_t n, const T* __restrict__ r,
+                                                              const T* __restrict__ u, const T* __restrict__ w,
+                                                              const T* __restrict__ dinv, T* __restrict__ m,
+                                                              double* __restrict__ part) {
+  __shared__ double sh[3][kThreads / 64];
+  double g = 0.0, d = 0.0, q = 0.0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    const T ri = r[i], ui = u[i], wi = w[i];
+    m[i] = dinv[i] * wi;
+    g += static_cast<double>(ri) * static_cast<double>(ui);
+    d += static_cast<double>(wi) * static_cast<double>(ui);
+    q += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  const double a = block_sum(g, sh[0]), bb = block_sum(d, sh[1]), c = block_sum(q, sh[2]);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kMaxRed + blockIdx.x] = bb;
+    part[2 * kMaxRed + blockIdx.x] = c;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void pipe_init_scalars_kernel(const double* __restrict__ part,
+                                                                     double* __restrict__ sc, double rtol2, int nb,
+                                                                     int max_iter) {
+  __shared__ double sh[kThreads / 64];
+  const double rr = sum_partials(part + 2 * kMaxRed, nb, sh);
+  if (threadIdx.x != 0) return;
+  sc[kRR] = rr;
+  sc[kTHRESH] = rtol2 * rr;
+  sc[kDONE] = (rr <= sc[kTHRESH]) ? 1.0 : 0.0;
+  sc[kITERS] = 0.0;
+  sc[kMAXIT] = static_cast<double>(max_iter);
+  sc[kRZ] = 0.0;  // the previous gamma and alpha (none before the first step)
+  sc[kAP] = 0.0;
+}
+
+// One pipelined step of parity par: gamma, delta, rho from the parity-par
+// partials (every block sums them in one fixed order); the residual test on
+// rho (r of this step) and the iteration limit; then, unless stopped,
+//   beta = gamma / gamma_prev (0 first), alpha = gamma / (delta - beta gamma / alpha_prev)
+//   z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p,
+//   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z, m = M^-1 w,
+// and the parity-(1-par) partials of the new r.u, w.u and r.r. Block 0
+// writes the state copy 1-par (a stopped step copies it on unchanged).
+template <class T>
+__global__ __launch_bounds__(kThreads) void pipe_step_kernel(size_t n, int par, int nb, double* __restrict__ sc,
+                                                             double* __restrict__ part, const T* __restrict__ dinv,
+                                                             const T* __restrict__ nv, T* __restrict__ x,
+                                                             T* __restrict__ r, T* __restrict__ u,
+                                                             T* __restrict__ w, T* __restrict__ m,
+                                                             T* __restrict__ z, T* __restrict__ q,
+                                                             T* __restrict__ s, T* __restrict__ p) {
+  __shared__ double sh[3][kThreads / 64];
+  __shared__ double s_val[3];
+  const double* pin = part + static_cast<size_t>(par) * kPipePart;
+  double* pout = part + static_cast<size_t>(1 - par) * kPipePart;
+  const double gamma = all_partials(pin, nb, sh[0], &s_val[0]);
+  const double delta = all_partials(pin + kMaxRed, nb, sh[1], &s_val[1]);
+  const double rho = all_partials(pin + 2 * kMaxRed, nb, sh[2], &s_val[2]);
+  const bool done0 = sc[slot(kDONE, par)] != 0.0;
+  const double iters = sc[slot(kITERS, par)];
+  const bool stop = done0 || rho <= sc[kTHRESH] || iters >= sc[kMAXIT];
+  const double g_prev = sc[slot(kRZ, par)], a_prev = sc[slot(kAP, par)];
+  double beta = 0.0, alpha = 0.0;
+  if (!stop) {
+    if (iters > 0.0 && g_prev != 0.0) beta = gamma / g_prev;
+    const double den = (iters > 0.0 && a_prev != 0.0) ? delta - beta * gamma / a_prev : delta;
+    alpha = den != 0.0 ? gamma / den : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int o = 1 - par;
+    sc[slot(kDONE, o)] = stop ? 1.0 : 0.0;
+    sc[slot(kITERS, o)] = stop ? iters : iters + 1.0;
+    sc[slot(kRR, o)] = done0 ? sc[slot(kRR, par)] : rho;
+    sc[slot(kRZ, o)] = stop ? g_prev : gamma;
+    sc[slot(kAP, o)] = stop ? a_prev : alpha;
+    sc[kALPHA] = alpha;
+    sc[kBETA] = beta;
+  }
+  if (stop) return;  // the same in every block
+  // The first step of a run starts the recurrences (z = n, q = m, s = w,
+  // p = u) without reading them: they hold whatever an earlier run or the
+  // allocation left, which beta = 0 would not clear if it is Inf or NaN.
+  const bool first = iters == 0.0;
+  const T a = static_cast<T>(alpha), bt = static_cast<T>(beta);
+  double g = 0.0, d = 0.0, rr = 0.0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    T zi = nv[i], qi = m[i], si = w[i], pi = u[i];
+    if (!first) {
+      zi += bt * z[i];
+      qi += bt * q[i];
+      si += bt * s[i];
+      pi += bt * p[i];
+    }
+    z[i] = zi;
+    q[i] = qi;
+    s[i] = si;
+    p[i] = pi;
+    x[i] = x[i] + a * pi;
+    const T ri = r[i] - a * si, ui = u[i] - a * qi, wi = w[i] - a * zi;
+    r[i] = ri;
+    u[i] = ui;
+    w[i] = wi;
+    m[i] = dinv[i] * wi;
+    g += static_cast<double>(ri) * static_cast<double>(ui);
+    d += static_cast<double>(wi) * static_cast<double>(ui);
+    rr += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  const double sg = block_sum(g, sh[0]), sd = block_sum(d, sh[1]), sr = block_sum(rr, sh[2]);
+  if (threadIdx.x == 0) {
+    pout[blockIdx.x] = sg;
+    pout[kMaxRed + blockIdx.x] = sd;
+    pout[2 * kMaxRed + blockIdx.x] = sr;
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void recip_kernel(size_t n, const double* __restrict__ d, T* __restrict__ dinv) {
   const size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
@@ -444,11 +595,15 @@ __global__ __launch_bounds__(kThreads) void sum_to_scalar_kernel(const double* _
   if (threadIdx.x == 0) *out = v;
 }
 
-// One PCG instance (vectors of T) over an apply callback.
+// One PCG instance (vectors of T) over an apply callback: the plain form
+// (five dependent launches an iteration with a two-kernel apply) or, with
+// `pipe`, the pipelined form (three); Ap holds n = A m in the pipelined form.
 template <class T>
 struct Pcg {
   size_t n;
+  bool pipe;
   DevBuf<T> r, z, p, Ap, dinv;
+  DevBuf<T> u, w, m, q, sv;  // pipelined form only (sv: s = A p)
   DevBuf<double> part, sc;
   // Captured batches of iterations: the configured batch size and every power
   // of two below it, so any count of iterations is a few graph launches.
@@ -467,8 +622,17 @@ struct Pcg {
   // Iterations per batch graph: even, at least 2.
   static int even_batch(int b) { return std::max(2, b + (b & 1)); }
 
-  explicit Pcg(size_t n_)
-      : n(n_), r(n_), z(n_), p(n_), Ap(n_), dinv(n_), part(3 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)) {}
+  Pcg(size_t n_, bool pipelined)
+      : n(n_), pipe(pipelined), r(n_), z(n_), p(n_), Ap(n_), dinv(n_),
+        part(pipelined ? 2 * kPipePart : 3 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)) {
+    if (pipe) {
+      u.allocate(n_);
+      w.allocate(n_);
+      m.allocate(n_);
+      q.allocate(n_);
+      sv.allocate(n_);
+    }
+  }
   ~Pcg() { drop_graphs(); }
   void drop_graphs() {
     for (auto& kv : execs) {
@@ -483,6 +647,13 @@ struct Pcg {
   // (beta and the state copy formed in every block).
   template <class Apply>
   void iteration(Apply&& apply, T* x, int par, hipStream_t s) {
+    if (pipe) {
+      apply(m.data(), Ap.data(), s);
+      pipe_step_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), dinv.data(), Ap.data(), x,
+                                                  r.data(), u.data(), w.data(), m.data(), z.data(), q.data(),
+                                                  sv.data(), p.data());
+      return;
+    }
     apply(p.data(), Ap.data(), s);
     double* pap = part.data() + kPartPAP;
     if (aligned16(x)) {  // every other vector is the solver's own allocation
@@ -499,8 +670,20 @@ struct Pcg {
   }
 
   // x = 0 and the first residual, for A x = b with ||r|| <= rtol ||b|| as the
-  // goal within max_iter iterations.
-  void init(const T* b, T* x, double rtol, int max_iter, hipStream_t s) {
+  // goal within max_iter iterations (the pipelined form also applies A twice:
+  // w = A u and the first step's n = A m comes from its first iteration).
+  template <class Apply>
+  void init(Apply&& apply, const T* b, T* x, double rtol, int max_iter, hipStream_t s) {
+    if (pipe) {
+      pipe_init_kernel<T><<<nb, kThreads, 0, s>>>(n, b, dinv.data(), x, r.data(), u.data());
+      HIP_CHECK_LAUNCH();
+      apply(u.data(), w.data(), s);
+      pipe_init2_kernel<T><<<nb, kThreads, 0, s>>>(n, r.data(), u.data(), w.data(), dinv.data(), m.data(),
+                                                   part.data());
+      pipe_init_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb, max_iter);
+      HIP_CHECK_LAUNCH();
+      return;
+    }
     cg_init_kernel<T><<<nb, kThreads, 0, s>>>(n, b, dinv.data(), x, r.data(), z.data(), p.data(), part.data());
     cg_init_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb, max_iter);
     HIP_CHECK_LAUNCH();
@@ -509,6 +692,7 @@ struct Pcg {
   // Residual replacement: r = rnew, x = 0, p kept; the next segment ends when
   // ||r|| <= rtol ||rnew||.
   void replace(const T* rnew, T* x, double rtol, int max_iter, hipStream_t s) {
+    require(!pipe, "mass_solve: residual replacement runs the plain CG form");
     cg_replace_kernel<T><<<nb, kThreads, 0, s>>>(n, rnew, dinv.data(), x, r.data(), z.data(), part.data());
     cg_replace_scalars_kernel<<<1, kThreads, 0, s>>>(part.data(), sc.data(), rtol * rtol, nb, max_iter);
     HIP_CHECK_LAUNCH();
@@ -609,7 +793,7 @@ class PinnedSlots {
 
  private:
   static constexpr int kChunk = 256;
-  static constexpr size_t kSlotBytes = 128;
+  static constexpr size_t kSlotBytes = 256;
   static_assert(kHostN * sizeof(double) <= kSlotBytes, "a slot holds the reported words");
   std::mutex m_;
   std::vector<double*> free_;
@@ -718,7 +902,7 @@ void advance(SolveJob& j) {
       j.t0 = std::chrono::steady_clock::now();
       enqueue_bnorm(j);
       if (j.opt.mode == 0) {
-        st.pcg64->init(j.b, j.x, j.opt.rtol, j.opt.max_iter, j.s);
+        st.pcg64->init(apply64, j.b, j.x, j.opt.rtol, j.opt.max_iter, j.s);
         st.pcg64->launch_iters(apply64, j.b, j.x, fused_signature(f, 0), j.batch, start_count(j, 0.0, 0), j.hs, j.s);
         j.phase = SolveJob::kIter;
       } else {
@@ -738,7 +922,7 @@ void advance(SolveJob& j) {
         j.phase = SolveJob::kDone;
       } else {
         if (!j.started) {
-          st.pcg32->init(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.opt.max_iter, j.s);
+          st.pcg32->init(apply32, st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.opt.max_iter, j.s);
           j.started = true;
         } else {
           st.pcg32->replace(st.r32.data(), st.d32.data(), j.opt.inner_rtol, j.opt.max_iter, j.s);
@@ -814,7 +998,7 @@ void advance(SolveJob& j) {
       } else {
         // No deeper than the target needs: the last step stops at about half of it.
         const double inner = std::max(j.opt.inner_rtol, std::min(0.5 * j.opt.rtol / rel, 0.5));
-        st.pcg32->init(st.r32.data(), st.d32.data(), inner, j.opt.max_iter, j.s);
+        st.pcg32->init(apply32, st.r32.data(), st.d32.data(), inner, j.opt.max_iter, j.s);
         const int count = start_count(j, inner * inner, 0);
         st.pcg32->launch_iters(apply32, st.r32.data(), st.d32.data(), fused_signature(f, 1), j.batch, count, j.hs, j.s);
         j.phase = SolveJob::kInner;
@@ -853,6 +1037,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   require(opt.mode >= 0 && opt.mode <= 2, "mass_solve: unknown mode");
   require(opt.rtol > 0 && opt.max_iter > 0, "mass_solve: rtol and max_iter must be positive");
   require(opt.inner_rtol < 1.0, "mass_solve: inner_rtol must be below 1");
+  require(opt.cg_variant >= 0 && opt.cg_variant <= 2, "mass_solve: cg_variant must be 0 (automatic), 1 or 2");
   require(b != nullptr && x != nullptr, "mass_solve: b and x must be device arrays");
   FusedMass& f = operator_fused_mass(op, s);
   const size_t n = static_cast<size_t>(f.l_size);
@@ -872,13 +1057,19 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   MassSolverState& st = *state;
   // Everything the captured graphs use is made here, outside capture: the PCG
   // vectors, the Jacobi inverse, and the operator's lazily made state (one apply).
+  // The pipelined form below kPipeMaxN entries (where every kernel of an
+  // iteration is a few microseconds and their count sets the time), the plain
+  // one above; residual replacement (MIXED_RELIABLE) always runs the plain form.
+  const bool pipe = opt.mode != 2 && (opt.cg_variant == 2 || (opt.cg_variant == 0 && n <= kPipeMaxN));
+  if (st.pcg64 && st.pcg64->pipe != pipe && opt.mode == 0) st.pcg64.reset();
+  if (st.pcg32 && st.pcg32->pipe != pipe && opt.mode != 0) st.pcg32.reset();
   if (opt.mode == 0 && !st.pcg64) {
-    st.pcg64 = std::make_unique<Pcg<double>>(n);
+    st.pcg64 = std::make_unique<Pcg<double>>(n, pipe);
     recip_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg64->dinv.data());
     HIP_CHECK_LAUNCH();
   }
   if (opt.mode != 0 && !st.pcg32) {
-    st.pcg32 = std::make_unique<Pcg<float>>(n);
+    st.pcg32 = std::make_unique<Pcg<float>>(n, pipe);
     st.r32.allocate(n);
     st.d32.allocate(n);
     recip_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg32->dinv.data());

@@ -11,7 +11,11 @@ struct SolveOptions {
   double rtol = 1e-10;
   double inner_rtol = 0.0;  // <= 0: default_inner_rtol(mode)
   int batch = 8;  // CG iterations per captured graph launch
+  int cg_variant = 0;  // 0: automatic (pipelined up to kPipeMaxN entries), 1: plain CG, 2: pipelined CG
 };
+
+// Vector length up to which the automatic choice takes pipelined CG.
+constexpr size_t kPipeMaxN = size_t{1} << 20;
 
 // Inner tolerance by mode. Refinement: 1e-5 per FP32 solve, so a 1e-10 target
 // takes two refinement steps, and FP32 Jacobi-PCG on the mass matrix keeps the

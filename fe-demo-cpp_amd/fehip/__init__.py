@@ -27,6 +27,7 @@ FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_pa
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
 QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR, QF_BUILD_DIFFUSION, QF_APPLY_DIFFUSION = 1, 2, 3, 4, 5
 SOLVE_FP64, SOLVE_MIXED, SOLVE_MIXED_RELIABLE = 0, 1, 2
+CG_AUTO, CG_PLAIN, CG_PIPELINED = 0, 1, 2  # fem_hip_solve_options.cg_variant
 
 
 ERR_NOT_READY = 6
@@ -73,6 +74,7 @@ class SolveOptions(ctypes.Structure):
         ("max_outer", ctypes.c_int),
         ("rtol", ctypes.c_double),
         ("inner_rtol", ctypes.c_double),
+        ("cg_variant", ctypes.c_int),
     ]
 
 
@@ -722,12 +724,13 @@ class Operator:
         call("fem_hip_operator_mass_diagonal", self.h, d_diag.ptr, d_diag.size, _s(stream))
 
     def solve(self, d_b, d_x, mode=SOLVE_FP64, rtol=1e-10, max_iter=2000, max_outer=30, inner_rtol=None,
-              stream=None):
+              stream=None, cg_variant=CG_AUTO):
         """M x = b. inner_rtol: reduction of each inner FP32 solve (SOLVE_MIXED, default 1e-5)
-        or between residual replacements (SOLVE_MIXED_RELIABLE, default 1e-2)."""
+        or between residual replacements (SOLVE_MIXED_RELIABLE, default 1e-2). cg_variant:
+        CG_AUTO, CG_PLAIN or CG_PIPELINED."""
         if inner_rtol is None:
             inner_rtol = 0.0  # the library's default for the mode
-        opt = SolveOptions(mode, max_iter, max_outer, rtol, inner_rtol)
+        opt = SolveOptions(mode, max_iter, max_outer, rtol, inner_rtol, cg_variant)
         st = SolveStats()
         call("fem_hip_mass_solve", self.h, d_b.ptr, d_x.ptr, ctypes.byref(opt), ctypes.byref(st), _s(stream))
         return {"iterations": st.iterations, "outer": st.outer, "rel_residual": st.rel_residual, "ms": st.ms}

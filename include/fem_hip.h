@@ -386,6 +386,9 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
 #define FEM_HIP_SOLVE_MIXED 1 /* FP32 inner Jacobi-PCG, FP64 iterative refinement */
 #define FEM_HIP_SOLVE_MIXED_RELIABLE 2 /* one FP32 Jacobi-PCG recursion whose residual is replaced by the FP64
                                           b - M x whenever it has dropped by inner_rtol (x accumulated in FP64) */
+#define FEM_HIP_CG_AUTO 0      /* fem_hip_solve_options.cg_variant */
+#define FEM_HIP_CG_PLAIN 1
+#define FEM_HIP_CG_PIPELINED 2
 typedef struct {
   int mode;          /* FEM_HIP_SOLVE_* */
   int max_iter;      /* CG iterations (FP64) / per inner solve (mixed) */
@@ -395,6 +398,10 @@ typedef struct {
                         replacements; <= 0: 1e-5 (MIXED: two steps to 1e-10) / 1e-2 (MIXED_RELIABLE:
                         keeps the FP64 iteration count). A refinement step never goes below half
                         of what the target still needs. */
+  int cg_variant;    /* FEM_HIP_CG_*. 0: automatic (pipelined CG up to 2^20 unknowns, plain CG above; MIXED_RELIABLE
+                        always plain); 1: plain Jacobi-PCG (apply, dot, update, direction: five
+                        dependent launches an iteration); 2: pipelined Jacobi-PCG (Ghysels & Vanroose
+                        2014: apply and one fused kernel, three launches) */
 } fem_hip_solve_options;
 typedef struct {
   int iterations;    /* total CG iterations (inner ones for mixed) */

@@ -190,3 +190,29 @@ def test_high_order_vector_mass_solves(gpu, alg):
     assert smx["rel_residual"] <= 1e-10 and smx["outer"] >= 2, smx
     assert rel_err(d_x.to_host(), x_cpu) < 1e-8
     print("fp64", s64, "mixed", smx, "oracle iterations", it_cpu, flush=True)
+
+
+@pytest.mark.parametrize("n,P,Q,nc", [(16, 3, 4, 1), (6, 5, 6, 3)])
+def test_plain_and_pipelined_cg(gpu, n, P, Q, nc):
+    """Both CG forms (plain: five launches an iteration; pipelined, Ghysels &
+    Vanroose 2014: three) in FP64 and mixed precision reach 1e-10 (true FP64
+    residual), take the oracle CG's iteration count within 2 (FP64), agree with
+    the oracle's solution, and repeat bit for bit."""
+    print("plain vs pipelined CG", n, P, Q, nc, flush=True)
+    p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
+    b, x_cpu, it_cpu = _oracle_solution(p)
+    d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
+    for variant in (fehip.CG_PLAIN, fehip.CG_PIPELINED):
+        for mode in (fehip.SOLVE_FP64, fehip.SOLVE_MIXED):
+            st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=variant)
+            x1 = d_x.to_host()
+            assert st["rel_residual"] <= 1e-10, (variant, mode, st)
+            assert rel_err(x1, x_cpu) < 1e-8, (variant, mode)
+            if mode == fehip.SOLVE_FP64:
+                assert abs(st["iterations"] - it_cpu) <= 2, (variant, st, it_cpu)
+            p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=variant)
+            assert d_x.to_host().tobytes() == x1.tobytes(), (variant, mode)
+            print("variant", variant, "mode", mode, st, flush=True)
+    # Residual replacement runs the plain form whatever is asked.
+    st = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED_RELIABLE, rtol=1e-10, cg_variant=fehip.CG_PIPELINED)
+    assert st["rel_residual"] <= 1e-10

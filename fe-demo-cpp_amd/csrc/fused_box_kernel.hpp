@@ -684,6 +684,16 @@ struct FusedBoxReg {
   }
 };
 
+// Only the component-split kernels (code 300) of a multi-component shape.
+template <int P, int Q, int NC, int BX, int BY>
+struct FusedBoxSplitReg {
+  static_assert(NC > 1, "the split kernels run the components of a multi-component field");
+  FusedBoxSplitReg() {
+    FusedBoxReg<P, Q, NC, BX, BY>::template reg<double, 0, 0, true>(300);
+    FusedBoxReg<P, Q, NC, BX, BY>::template reg<float, 2, 0, true>(300);
+  }
+};
+
 // Column cross-section (BX, BY) of each compiled elems value (elems % 100).
 inline bool box_columns(int elems, int* BX, int* BY) {
   switch (elems % 100) {

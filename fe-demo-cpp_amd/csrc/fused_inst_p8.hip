@@ -15,6 +15,7 @@ const FusedBlockShapeReg<8, 10, 3, 2, 2> blk_8_10_3;
 
 const FusedBoxReg<8, 10, 1, 2, 2> box_8_10_1_4;
 const FusedBoxReg<8, 10, 3, 2, 2> box_8_10_3_4;
+const FusedBoxSplitReg<8, 10, 3, 4, 2> box_8_10_3_8_split;
 
 }  // namespace
 }  // namespace femhip

@@ -420,10 +420,10 @@ def main():
                 "exact" if exact else "fast", P, Q, tname, ne, "true" if alg == fehip.FUSED_ELEM_TMAP else "false")
         elif alg == fehip.FUSED_BOX_COLUMN:
             bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne % 100]
-            # the last two template arguments: how u reaches the x-rows (2: the LDS ring,
-            # FP32 and FP64 elems >= 200; 0: rows) and whether qdata is loaded a layer
-            # ahead (1; 0 for elems 100..199)
-            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d, %d, %d>" % (
+            # the last three template arguments: how u reaches the x-rows (2: the LDS
+            # ring, FP32 and FP64 elems >= 200; 0: rows), whether qdata is loaded a
+            # layer ahead (1; 0 for elems 100..199) and the component split (1)
+            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d, %d, %d, 1>" % (
                 P, Q, tname, bx, by, 2 if fp32 or ne >= 200 else 0, 0 if ne // 100 == 1 else 1)
         else:
             kernel_names[label] = {

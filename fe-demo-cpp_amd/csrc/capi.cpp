@@ -858,8 +858,14 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
         for (unsigned pt = 0; pt < grid.x * 64; ++pt) {
           int gx, gy, gz, seg[8], sl[8];
           if (!box_shared_point(g, P, BX, BY, static_cast<int>(pl), static_cast<int>(ln), static_cast<int>(pt), &gx, &gy,
-                                &gz))
+                                &gz)) {
+            int seg2[8], sl2[8];
+            size_t node2 = 0;
+            if (box_shared_owners(g, P, BX, BY, static_cast<int>(pl), static_cast<int>(ln), static_cast<int>(pt), &node2,
+                                  seg2, sl2) != 0)
+              fail("one-step owners take a point that is not a shared node");
             continue;
+          }
           ++visited;
           if (gx < 0 || gx >= g.Nx || gy < 0 || gy >= g.Ny || gz < 0 || gz >= g.Nz) fail("shared node out of the box");
           const long long node = gx + static_cast<long long>(g.Nx) * (gy + static_cast<long long>(g.Ny) * gz);
@@ -867,6 +873,15 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
           shared[node] = 1;
           const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, sl);
           if (n < 2) fail("a shared node has fewer than 2 segments");
+          {  // the pass's one-step form gives the same node, owners and slots
+            int seg2[8], sl2[8];
+            size_t node2 = 0;
+            const int n2 = box_shared_owners(g, P, BX, BY, static_cast<int>(pl), static_cast<int>(ln),
+                                             static_cast<int>(pt), &node2, seg2, sl2);
+            if (n2 != n || node2 != static_cast<size_t>(node)) fail("one-step owners disagree (count or node)");
+            for (int i = 0; i < n; ++i)
+              if (seg2[i] != seg[i] || sl2[i] != sl[i]) fail("one-step owners disagree (segment or slot)");
+          }
           max_owners = std::max(max_owners, n);
           for (int i = 0; i < n; ++i) {
             if (i > 0 && seg[i] <= seg[i - 1]) fail("segments not ascending");

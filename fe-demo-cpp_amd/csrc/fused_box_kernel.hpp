@@ -148,6 +148,84 @@ __host__ __device__ inline int box_owners(const BoxGeom& g, int P, int BX, int B
   return n;
 }
 
+// The shared-node pass's point (plane, line, point) in one step: whether it is
+// a shared node of that plane (as box_shared_point), its node index, and its
+// owners and slots as box_owners gives them (same order). Each plane kind
+// fixes one coordinate and one pair of owner columns or segments, so only the
+// other two are derived; divisions by P-1, SX and SY become constants when
+// the kernel inlines this with its template arguments, and the segment of a
+// node plane takes 32-bit divisions (nz * nsz < 2^31, checked at launch).
+__host__ __device__ __forceinline__ int box_shared_owners(const BoxGeom& g, int P, int BX, int BY, int plane,
+                                                          int line, int point, size_t* node, int* seg, int* slot) {
+  const int SX = BX * (P - 1), SY = BY * (P - 1);
+  const int LXn = SX + 1, LYn = SY + 1, LZn = g.lz * (P - 1) + 1;
+  const int nbx = g.ncx - 1, nby = g.ncy - 1;
+  const unsigned unz = static_cast<unsigned>(g.nz), unsz = static_cast<unsigned>(g.nsz);
+  auto seg_z0 = [&](int sg) { return static_cast<int>(static_cast<unsigned>(sg) * unz / unsz); };
+  int gx, gy, gz, cx0, cx1, cy0, cy1, sz0, sz1;
+  bool xs = false, ys = false;
+  // The segments holding node plane gz (a shared segment end belongs to two).
+  auto z_owners = [&]() {
+    int lyr = gz / (P - 1);
+    if (lyr > g.nz - 1) lyr = g.nz - 1;
+    sz1 = static_cast<int>((static_cast<unsigned>(lyr + 1) * unsz - 1u) / unz);
+    const bool zs = gz % (P - 1) == 0 && sz1 > 0 && gz == seg_z0(sz1) * (P - 1);
+    sz0 = zs ? sz1 - 1 : sz1;
+  };
+  if (plane < nbx) {  // x-face gx = (plane+1) SX: lines gz, points gy
+    if (line >= g.Nz || point >= g.Ny) return 0;
+    gx = (plane + 1) * SX;
+    gy = point;
+    gz = line;
+    xs = true;
+    cx0 = plane;
+    cx1 = plane + 1;
+    const int qy = gy / SY;
+    ys = gy == qy * SY && gy > 0 && gy < g.Ny - 1;
+    cy1 = qy < g.ncy - 1 ? qy : g.ncy - 1;
+    cy0 = ys ? cy1 - 1 : cy1;
+    z_owners();
+  } else if (plane < nbx + nby) {  // y-face gy = (f+1) SY: lines gz, points gx not on an x cut
+    const int f = plane - nbx;
+    if (line >= g.Nz || point >= g.Nx) return 0;
+    const int qx = point / SX;
+    if (point == qx * SX && point > 0 && point < g.Nx - 1) return 0;
+    gx = point;
+    gy = (f + 1) * SY;
+    gz = line;
+    ys = true;
+    cy0 = f;
+    cy1 = f + 1;
+    cx0 = cx1 = qx < g.ncx - 1 ? qx : g.ncx - 1;
+    z_owners();
+  } else {  // segment end gz = (P-1) z0(f+1): lines gy, points gx, neither on a cut
+    const int f = plane - nbx - nby;
+    if (line >= g.Ny || point >= g.Nx) return 0;
+    const int qx = point / SX, qy = line / SY;
+    if ((point == qx * SX && point > 0 && point < g.Nx - 1) || (line == qy * SY && line > 0 && line < g.Ny - 1))
+      return 0;
+    gx = point;
+    gy = line;
+    gz = seg_z0(f + 1) * (P - 1);
+    cx0 = cx1 = qx < g.ncx - 1 ? qx : g.ncx - 1;
+    cy0 = cy1 = qy < g.ncy - 1 ? qy : g.ncy - 1;
+    sz0 = f;
+    sz1 = f + 1;
+  }
+  *node = static_cast<size_t>(gx) + static_cast<size_t>(g.Nx) * (gy + static_cast<size_t>(g.Ny) * gz);
+  int n = 0;
+  for (int sg = sz0; sg <= sz1; ++sg) {
+    const int lz = gz - seg_z0(sg) * (P - 1);
+    for (int cy = cy0; cy <= cy1; ++cy)
+      for (int cx = cx0; cx <= cx1; ++cx) {
+        seg[n] = (sg * g.ncy + cy) * g.ncx + cx;
+        slot[n] = box_slot(gx - cx * SX, gy - cy * SY, lz, xs, ys, LXn, LYn, LZn);
+        ++n;
+      }
+  }
+  return n;
+}
+
 // qdata is read exactly once per apply. FP64 loads it as streaming
 // (nontemporal) data, which leaves the caches to the u rows that neighbouring
 // elements and columns read again: 64^3, P=5 FP64 box kernel 0.192 -> 0.177 ms
@@ -583,14 +661,13 @@ template <class T, int P, int BX, int BY>
 __global__ __launch_bounds__(64 * kSharedLines) void box_shared_sum_kernel(BoxGeom g, int nc,
                                                                            const T* __restrict__ partials,
                                                                            size_t part_comp_len, T* __restrict__ v) {
-  int gx, gy, gz, seg[8], slot[8];
-  if (!box_shared_point(g, P, BX, BY, blockIdx.z, blockIdx.y * kSharedLines + threadIdx.y,
-                        blockIdx.x * 64 + threadIdx.x, &gx, &gy, &gz))
-    return;
-  const int n = box_owners(g, P, BX, BY, gx, gy, gz, seg, slot);
+  int seg[8], slot[8];
+  size_t node;
+  const int n = box_shared_owners(g, P, BX, BY, blockIdx.z, blockIdx.y * kSharedLines + threadIdx.y,
+                                  blockIdx.x * 64 + threadIdx.x, &node, seg, slot);
+  if (n == 0) return;
   const int LZn = g.lz * (P - 1) + 1;
   const size_t face = static_cast<size_t>(box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1, LZn));
-  const size_t node = static_cast<size_t>(gx) + static_cast<size_t>(g.Nx) * (gy + static_cast<size_t>(g.Ny) * gz);
   for (int c = 0; c < nc; ++c) {
     const T* pc = partials + static_cast<size_t>(c) * part_comp_len;
     T s = pc[seg[0] * face + slot[0]];
@@ -651,7 +728,8 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
   } else {
     const dim3 grid = box_shared_grid(g);
     if (grid.z == 0) return;
-    require(grid.y < 65536 && grid.z < 65536, "fused mass: box too large for the shared-node pass");
+    require(grid.y < 65536 && grid.z < 65536 && static_cast<long long>(g.nz) * g.nsz < (1ll << 31),
+            "fused mass: box too large for the shared-node pass");
     box_shared_sum_kernel<T, P, BX, BY><<<grid, dim3(64, kSharedLines), 0, s>>>(
         g, NC, static_cast<const T*>(a.partials), a.part_comp_len, static_cast<T*>(a.v));
   }

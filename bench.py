@@ -290,12 +290,12 @@ def solve_latency(fehip, reps, n=16, P=3, Q=4):
     xs = x_star.to_host()
     out = {"workload": "%d^3 hexes, P=%d, Q=%d, %d DoF, rtol 1e-10" % (n, P, Q, p.l_size)}
     x64 = None
-    # fp64 / mixed run the automatic CG form (pipelined at this size); the
-    # *_plain_cg lines the five-launch form for comparison.
+    # fp64 / mixed run the automatic CG form (pipelined FP64 up to 2^20 unknowns,
+    # plain FP32 inner solves); the other two lines the other form of each.
     for name, mode, cgv in (("fp64", fehip.SOLVE_FP64, fehip.CG_AUTO), ("mixed", fehip.SOLVE_MIXED, fehip.CG_AUTO),
                             ("mixed_reliable", fehip.SOLVE_MIXED_RELIABLE, fehip.CG_AUTO),
                             ("fp64_plain_cg", fehip.SOLVE_FP64, fehip.CG_PLAIN),
-                            ("mixed_plain_cg", fehip.SOLVE_MIXED, fehip.CG_PLAIN)):
+                            ("mixed_pipelined_cg", fehip.SOLVE_MIXED, fehip.CG_PIPELINED)):
         p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=cgv)
         lat, st = [], None
         for _ in range(reps):

@@ -827,6 +827,7 @@ struct SolveJob {
   // iterations a run still needs instead of whole batches.
   double rate = 0.0, rr_seen = -1.0;
   int it_seen = 0;
+  double run_red2 = 0.0;  // r.r reduction the current run stops at (its threshold over its starting r.r)
   double bnorm = 0.0;
   SolveStats stats;
   std::chrono::steady_clock::time_point t0;
@@ -867,6 +868,9 @@ void enqueue_bnorm(SolveJob& j) {
 int next_count(SolveJob& j) {
   const double rr = j.hs[kRR], thresh = j.hs[kTHRESH];
   const int it = static_cast<int>(j.hs[kITERS]);
+  // The first batch of a run measures from the run's starting r.r, which its
+  // threshold gives (threshold = reduction^2 * r.r at the start).
+  if (j.rr_seen <= 0.0 && j.run_red2 > 0.0 && thresh > 0.0) j.rr_seen = thresh / j.run_red2;
   if (j.rr_seen > 0.0 && rr > 0.0 && it > j.it_seen) j.rate = std::pow(rr / j.rr_seen, 1.0 / (it - j.it_seen));
   j.rr_seen = rr;
   j.it_seen = it;
@@ -876,12 +880,17 @@ int next_count(SolveJob& j) {
 }
 
 // Iterations to launch at the start of a run that must reduce r.r by
-// `reduction2` (the rate of the previous run, else a whole batch). The next
-// observation starts a new baseline.
+// `reduction2` (the rate of the previous run, else kFirstProbe iterations, at
+// most a batch: enough to measure the rate, few enough that a loose tolerance
+// does not run a long batch of finished iterations). The next observation
+// starts a new baseline.
+constexpr int kFirstProbe = 8;
 int start_count(SolveJob& j, double reduction2, int iters_now) {
   j.rr_seen = -1.0;
   j.it_seen = iters_now;
-  if (!(j.rate > 0.0 && j.rate < 1.0) || !(reduction2 > 0.0 && reduction2 < 1.0)) return j.batch;
+  j.run_red2 = reduction2;
+  if (!(j.rate > 0.0 && j.rate < 1.0) || !(reduction2 > 0.0 && reduction2 < 1.0))
+    return std::min(j.batch, kFirstProbe);
   const double need = std::ceil(std::log(reduction2) / std::log(j.rate));
   return static_cast<int>(std::max(1.0, std::min(need, static_cast<double>(j.batch))));
 }
@@ -903,7 +912,8 @@ void advance(SolveJob& j) {
       enqueue_bnorm(j);
       if (j.opt.mode == 0) {
         st.pcg64->init(apply64, j.b, j.x, j.opt.rtol, j.opt.max_iter, j.s);
-        st.pcg64->launch_iters(apply64, j.b, j.x, fused_signature(f, 0), j.batch, start_count(j, 0.0, 0), j.hs, j.s);
+        st.pcg64->launch_iters(apply64, j.b, j.x, fused_signature(f, 0), j.batch,
+                               start_count(j, j.opt.rtol * j.opt.rtol, 0), j.hs, j.s);
         j.phase = SolveJob::kIter;
       } else {
         zero_fill(j.x, st.n * sizeof(double), j.s);
@@ -1057,10 +1067,13 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   MassSolverState& st = *state;
   // Everything the captured graphs use is made here, outside capture: the PCG
   // vectors, the Jacobi inverse, and the operator's lazily made state (one apply).
-  // The pipelined form below kPipeMaxN entries (where every kernel of an
-  // iteration is a few microseconds and their count sets the time), the plain
-  // one above; residual replacement (MIXED_RELIABLE) always runs the plain form.
-  const bool pipe = opt.mode != 2 && (opt.cg_variant == 2 || (opt.cg_variant == 0 && n <= kPipeMaxN));
+  // Automatic: the pipelined form for FP64 solves below kPipeMaxN entries
+  // (where every kernel of an iteration is a few microseconds and their count
+  // sets the time), the plain one otherwise: in FP32 the pipelined recurrences
+  // drift further from the true residual, and an FP32-inner refinement then
+  // needs an extra step (16^3, P=3: 44 iterations in 3 steps against 42 in 2).
+  // Residual replacement (MIXED_RELIABLE) always runs the plain form.
+  const bool pipe = opt.mode != 2 && (opt.cg_variant == 2 || (opt.cg_variant == 0 && opt.mode == 0 && n <= kPipeMaxN));
   if (st.pcg64 && st.pcg64->pipe != pipe && opt.mode == 0) st.pcg64.reset();
   if (st.pcg32 && st.pcg32->pipe != pipe && opt.mode != 0) st.pcg32.reset();
   if (opt.mode == 0 && !st.pcg64) {
@@ -1075,8 +1088,8 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
     recip_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg32->dinv.data());
     HIP_CHECK_LAUNCH();
   }
-  fused_mass_apply64(f, x, st.Mx.data(), s);
-  if (opt.mode != 0) fused_mass_apply32(f, st.d32.data(), st.r32.data(), s);
+  fused_prepare(f, 0, s);  // lazily made operator state, before any capture
+  if (opt.mode != 0) fused_prepare(f, 1, s);
   auto j = std::make_unique<SolveJob>();
   j->op = &op;
   j->f = &f;
@@ -1086,7 +1099,10 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   j->opt = opt;
   if (j->opt.inner_rtol <= 0.0) j->opt.inner_rtol = default_inner_rtol(opt.mode);
   j->s = s;
-  j->batch = opt.batch > 0 ? opt.batch : 8;
+  // Iterations per graph launch: small problems (iterations of a few tens of
+  // microseconds) take longer batches, so a solve needs fewer host round trips
+  // (the launch count follows the measured convergence rate either way).
+  j->batch = opt.batch > 0 ? opt.batch : (n <= kPipeMaxN ? kSmallBatch : 8);
   HIP_CHECK(hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));
   j->hs = pinned_slots().acquire();
   for (int i = 0; i < kHostN; ++i) j->hs[i] = 0.0;
@@ -1134,24 +1150,77 @@ void solve_job_destroy(SolveJob* j) {
   delete j;
 }
 
+namespace {
+
+// Streams for solves asked for on the null stream (graph capture needs another
+// one): taken from a pool per device and returned after the solve, so a solve
+// does not create and destroy a stream (which costs more than a small solve's
+// setup). Each comes with an event that orders it after the null stream's
+// earlier work. Kept for the life of the process (the runtime may be gone at exit).
+class SolveStreamPool {
+ public:
+  struct Entry {
+    int dev;
+    hipStream_t s;
+    hipEvent_t ev;
+  };
+  Entry acquire() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].dev == dev) {
+          Entry e = free_[i];
+          free_.erase(free_.begin() + static_cast<std::ptrdiff_t>(i));
+          return e;
+        }
+    }
+    Entry e{dev, nullptr, nullptr};
+    HIP_CHECK(hipStreamCreateWithFlags(&e.s, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+    return e;
+  }
+  void release(const Entry& e) {
+    std::lock_guard<std::mutex> lk(m_);
+    free_.push_back(e);
+  }
+
+ private:
+  std::mutex m_;
+  std::vector<Entry> free_;
+};
+
+SolveStreamPool& solve_streams() {
+  static SolveStreamPool* pool = new SolveStreamPool();  // never destroyed
+  return *pool;
+}
+
+}  // namespace
+
 void mass_solve(Operator& op, MassSolverState*& state, const double* b, double* x, const SolveOptions& opt,
                 SolveStats& stats, hipStream_t user_stream) {
-  // Graph capture needs a non-default stream.
+  // Graph capture needs a non-default stream: a null stream is served by a
+  // pooled one, ordered after the null stream's earlier work (the caller's
+  // writes of b and x). The solve is synchronous: x is complete on return.
   hipStream_t s = user_stream;
-  hipStream_t own = nullptr;
-  if (s == nullptr) {
-    HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
-    s = own;
-  }
-  struct StreamGuard {
-    hipStream_t s;
-    ~StreamGuard() {
-      if (s) {
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
+  struct PoolGuard {
+    bool held = false;
+    SolveStreamPool::Entry e{};
+    ~PoolGuard() {
+      if (held) {
+        (void)hipStreamSynchronize(e.s);
+        solve_streams().release(e);
       }
     }
-  } guard{own};
+  } guard;
+  if (s == nullptr) {
+    guard.e = solve_streams().acquire();
+    guard.held = true;
+    s = guard.e.s;
+    HIP_CHECK(hipEventRecord(guard.e.ev, nullptr));
+    HIP_CHECK(hipStreamWaitEvent(s, guard.e.ev, 0));
+  }
   std::unique_ptr<SolveJob, void (*)(SolveJob*)> j(solve_job_create(op, state, b, x, opt, s), solve_job_destroy);
   solve_job_wait(*j);
   stats = j->stats;

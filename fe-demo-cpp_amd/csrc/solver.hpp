@@ -10,12 +10,14 @@ struct SolveOptions {
   int max_outer = 20;
   double rtol = 1e-10;
   double inner_rtol = 0.0;  // <= 0: default_inner_rtol(mode)
-  int batch = 8;  // CG iterations per captured graph launch
-  int cg_variant = 0;  // 0: automatic (pipelined up to kPipeMaxN entries), 1: plain CG, 2: pipelined CG
+  int batch = 0;  // CG iterations per captured graph launch (0: kSmallBatch up to kPipeMaxN entries, else 8)
+  int cg_variant = 0;  // 0: automatic (FP64: pipelined up to kPipeMaxN entries; FP32 inner: plain), 1: plain, 2: pipelined
 };
 
-// Vector length up to which the automatic choice takes pipelined CG.
+// Vector length up to which the automatic choice takes pipelined CG (FP64 solves).
 constexpr size_t kPipeMaxN = size_t{1} << 20;
+// Iterations per graph launch for vectors of at most kPipeMaxN entries.
+constexpr int kSmallBatch = 32;
 
 // Inner tolerance by mode. Refinement: 1e-5 per FP32 solve, so a 1e-10 target
 // takes two refinement steps, and FP32 Jacobi-PCG on the mass matrix keeps the

@@ -398,8 +398,8 @@ typedef struct {
                         replacements; <= 0: 1e-5 (MIXED: two steps to 1e-10) / 1e-2 (MIXED_RELIABLE:
                         keeps the FP64 iteration count). A refinement step never goes below half
                         of what the target still needs. */
-  int cg_variant;    /* FEM_HIP_CG_*. 0: automatic (pipelined CG up to 2^20 unknowns, plain CG above; MIXED_RELIABLE
-                        always plain); 1: plain Jacobi-PCG (apply, dot, update, direction: five
+  int cg_variant;    /* FEM_HIP_CG_*. 0: automatic (FP64: pipelined CG up to 2^20 unknowns, plain CG above; the
+                        FP32 inner solves and MIXED_RELIABLE: plain); 1: plain Jacobi-PCG (apply, dot, update, direction: five
                         dependent launches an iteration); 2: pipelined Jacobi-PCG (Ghysels & Vanroose
                         2014: apply and one fused kernel, three launches) */
 } fem_hip_solve_options;

@@ -1006,6 +1006,7 @@ int fem_hip_mass_solve(fem_hip_operator h, const double* b, double* x, const fem
       o.rtol = opt->rtol;
       o.inner_rtol = opt->inner_rtol;
       o.cg_variant = opt->cg_variant;
+      o.separate_second_pass = opt->separate_second_pass;
     }
     SolveStats out;
     mass_solve(*h->op, h->solver, b, x, o, out, st(s));

@@ -643,8 +643,10 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
   }
 }
 
+// second = false (split applies): the element kernel only, without the
+// zero fill (the split requires every entry covered) and the second pass.
 template <class T>
-void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s) {
+void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s, bool second = true) {
   FusedLauncher L = find(k);
   require(L != nullptr, "fused mass: variant not compiled");
   prepare_key<T>(f, k, s);
@@ -694,7 +696,7 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
     if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     L(a, s);
     if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-    if (alg_partial(k.alg))
+    if (alg_partial(k.alg) && second)
       launch_row_sum<T>(d.nrows, f.NC, f.comp_stride, d.rows.data(), d.row_ptr.data(), d.row_part.data(),
                         static_cast<const T*>(a.partials), a.part_comp_len, v, s);
   } else {
@@ -712,8 +714,9 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s)
       if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
       L(a, s);
       if (ev) HIP_CHECK(hipEventRecord(ev->c, s));
-      launch_row_sum<T>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows_p, f.tm_ptr_p, f.tm_idx_p,
-                        elem_outputs<T>(f).data(), a.y_comp_len, v, s);
+      if (second)
+        launch_row_sum<T>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows_p, f.tm_ptr_p, f.tm_idx_p,
+                          elem_outputs<T>(f).data(), a.y_comp_len, v, s);
     } else {
       zero_fill(v, static_cast<size_t>(f.l_size) * sizeof(T), s);
       a.out = v;
@@ -1080,6 +1083,76 @@ void fused_prepare(FusedMass& f, int fp32, hipStream_t s) {
   } else {
     prepare_key<double>(f, k, s);
   }
+}
+
+namespace {
+
+// Interior nodes (stored by the block kernel) are -1; then a row's node gets its row.
+__global__ __launch_bounds__(kThreads) void node_row_kernel(size_t nodes, int* __restrict__ node_row) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < nodes) node_row[i] = -1;
+}
+__global__ __launch_bounds__(kThreads) void node_row_scatter_kernel(int nrows, const int* __restrict__ rows,
+                                                                    int* __restrict__ node_row) {
+  const int m = blockIdx.x * kThreads + threadIdx.x;
+  if (m < nrows) node_row[rows[m]] = m;
+}
+
+template <class T>
+bool split_impl(FusedMass& f, FusedSplit* o, hipStream_t s) {
+  const int fp32 = std::is_same<T, float>::value ? 1 : 0;
+  const FusedKey k = fused_resolve(f, fp32);
+  // Entry i of a vector is node i % nodes of component i / nodes.
+  const long long nodes = f.NC == 1 ? f.l_size : f.comp_stride;
+  if (static_cast<long long>(f.NC) * nodes != f.l_size) return false;
+  if (alg_blocked(k.alg) && alg_partial(k.alg)) {
+    prepare_key<T>(f, k, s);
+    DevPlan& d = ensure_plan(f, k.elems, fused_node_budget(k), true, s);
+    if (!d.covers_all) return false;
+    if (d.node_row.size() != static_cast<size_t>(nodes)) {
+      d.node_row.allocate(static_cast<size_t>(nodes));
+      ++f.gen;
+      node_row_kernel<<<grid_for(static_cast<size_t>(nodes), kThreads), kThreads, 0, s>>>(
+          static_cast<size_t>(nodes), d.node_row.data());
+      if (d.nrows > 0)
+        node_row_scatter_kernel<<<grid_for(static_cast<size_t>(d.nrows), kThreads), kThreads, 0, s>>>(
+            d.nrows, d.rows.data(), d.node_row.data());
+      HIP_CHECK_LAUNCH();
+    }
+    o->node_row = d.node_row.data();
+    o->ptr = d.row_ptr.data();
+    o->idx = d.row_part.data();
+    o->vals = plan_partials<T>(d).data();
+    o->cl = std::max<size_t>(d.npart, 1);
+    o->cs = static_cast<int>(nodes);
+    return true;
+  }
+  if (alg_tmap(k.alg)) {
+    prepare_key<T>(f, k, s);
+    if (!f.tm_covers_all || f.tm_rows_p != nullptr || f.tm_nrows != nodes) return false;
+    o->node_row = nullptr;
+    o->ptr = f.tm_ptr_p;
+    o->idx = f.tm_idx_p;
+    o->vals = elem_outputs<T>(f).data();
+    o->cl = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
+    o->cs = static_cast<int>(nodes);
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+bool fused_split(FusedMass& f, int fp32, FusedSplit* out, hipStream_t s) {
+  return fp32 ? split_impl<float>(f, out, s) : split_impl<double>(f, out, s);
+}
+
+void fused_mass_apply_first64(FusedMass& f, const double* u, double* v, hipStream_t s) {
+  apply_key<double>(f, fused_resolve(f, 0), u, v, s, false);
+}
+
+void fused_mass_apply_first32(FusedMass& f, const float* u, float* v, hipStream_t s) {
+  apply_key<float>(f, fused_resolve(f, 1), u, v, s, false);
 }
 
 void fused_mass_apply64(FusedMass& f, const double* u, double* v, hipStream_t s) {

@@ -113,6 +113,7 @@ struct DevPlan {
   DevBuf<float> q32;   // lazily made
   DevBuf<double> part64;
   DevBuf<float> part32;
+  DevBuf<int> node_row;  // split applies: a node's row, or -1 for a node the block kernel stores (lazy)
 };
 
 // The selected way to apply: algorithm and its elems (0 = the shape's default).
@@ -197,6 +198,29 @@ uint64_t fused_signature(const FusedMass& f, int fp32);
 size_t fused_scratch_bytes64(const FusedMass& f);
 void fused_mass_apply64_scratch(FusedMass& f, const double* u, double* v, void* scratch, size_t scratch_bytes,
                                 hipStream_t s);
+// An apply split for a solver that folds the apply's second pass into its own
+// vector kernel. The first stage (fused_mass_apply_first*) runs only the
+// element kernel, which writes v at the nodes it completes and partial sums
+// or element outputs for the others; then for entry i (component
+// c = i / cs, node i - c cs)
+//   row = node_row ? node_row[node] : node,
+//   (A u)_i = row < 0 ? v_i : sum over k in [ptr[row], ptr[row+1]) of vals[c cl + idx[k]]
+// (added into 0 in ascending k), which is what the second pass computes, in
+// its order.
+struct FusedSplit {
+  const int* node_row = nullptr;  // null: row m is node m
+  const int* ptr = nullptr;
+  const int* idx = nullptr;
+  const void* vals = nullptr;  // partial sums or element outputs, in the apply's precision
+  size_t cl = 0;               // vals per component
+  int cs = 0;                  // vector entries per component
+};
+// Whether the apply in this precision splits so (block partial sums whose rows
+// and interior nodes cover every entry, or the transpose-map algorithm whose
+// rows are all the nodes); fills `out` (buffers and the node-to-row map made now).
+bool fused_split(FusedMass& f, int fp32, FusedSplit* out, hipStream_t s);
+void fused_mass_apply_first64(FusedMass& f, const double* u, double* v, hipStream_t s);
+void fused_mass_apply_first32(FusedMass& f, const float* u, float* v, hipStream_t s);
 // Builds lazily made state (block plans, FP32 copies) for the current choice,
 // so later applies allocate nothing (graph capture).
 void fused_prepare(FusedMass& f, int fp32, hipStream_t s);

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -486,10 +487,12 @@ __global__ __launch_bounds__(kThreads) void pipe_init_scalars_kernel(const doubl
 //   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z, m = M^-1 w,
 // and the parity-(1-par) partials of the new r.u, w.u and r.r. Block 0
 // writes the state copy 1-par (a stopped step copies it on unchanged).
-template <class T>
+// With SPLIT the apply's second pass is folded in: n_i is formed here from
+// what the element kernel left (FusedSplit), in the second pass's order.
+template <class T, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void pipe_step_kernel(size_t n, int par, int nb, double* __restrict__ sc,
                                                              double* __restrict__ part, const T* __restrict__ dinv,
-                                                             const T* __restrict__ nv, T* __restrict__ x,
+                                                             FusedSplit sp, const T* __restrict__ nv, T* __restrict__ x,
                                                              T* __restrict__ r, T* __restrict__ u,
                                                              T* __restrict__ w, T* __restrict__ m,
                                                              T* __restrict__ z, T* __restrict__ q,
@@ -528,9 +531,27 @@ __global__ __launch_bounds__(kThreads) void pipe_step_kernel(size_t n, int par, 
   const bool first = iters == 0.0;
   const T a = static_cast<T>(alpha), bt = static_cast<T>(beta);
   double g = 0.0, d = 0.0, rr = 0.0;
+  const T* __restrict__ spv = static_cast<const T*>(sp.vals);
   for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * kThreads) {
-    T zi = nv[i], qi = m[i], si = w[i], pi = u[i];
+    T ni;
+    if constexpr (SPLIT) {
+      const unsigned ui = static_cast<unsigned>(i), ucs = static_cast<unsigned>(sp.cs);
+      const unsigned c = ui >= ucs ? ui / ucs : 0u;
+      const int node = static_cast<int>(ui - c * ucs);
+      const int row = sp.node_row ? sp.node_row[node] : node;
+      if (row < 0) {
+        ni = nv[i];
+      } else {
+        const T* vc = spv + static_cast<size_t>(c) * sp.cl;
+        T acc = T(0);
+        for (int k = sp.ptr[row], ke = sp.ptr[row + 1]; k < ke; ++k) acc += vc[sp.idx[k]];
+        ni = acc;
+      }
+    } else {
+      ni = nv[i];
+    }
+    T zi = ni, qi = m[i], si = w[i], pi = u[i];
     if (!first) {
       zi += bt * z[i];
       qi += bt * q[i];
@@ -605,6 +626,11 @@ struct Pcg {
   DevBuf<T> r, z, p, Ap, dinv;
   DevBuf<T> u, w, m, q, sv;  // pipelined form only (sv: s = A p)
   DevBuf<double> part, sc;
+  // Pipelined form with the apply split (FusedSplit): the first stage writes Ap
+  // and the step kernel forms the rest of A m itself.
+  bool split = false;
+  FusedSplit sp;
+  std::function<void(const T*, T*, hipStream_t)> first;
   // Captured batches of iterations: the configured batch size and every power
   // of two below it, so any count of iterations is a few graph launches.
   struct Exec {
@@ -648,10 +674,17 @@ struct Pcg {
   template <class Apply>
   void iteration(Apply&& apply, T* x, int par, hipStream_t s) {
     if (pipe) {
-      apply(m.data(), Ap.data(), s);
-      pipe_step_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), dinv.data(), Ap.data(), x,
-                                                  r.data(), u.data(), w.data(), m.data(), z.data(), q.data(),
-                                                  sv.data(), p.data());
+      if (split) {
+        first(m.data(), Ap.data(), s);
+        pipe_step_kernel<T, true><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), dinv.data(), sp,
+                                                          Ap.data(), x, r.data(), u.data(), w.data(), m.data(),
+                                                          z.data(), q.data(), sv.data(), p.data());
+      } else {
+        apply(m.data(), Ap.data(), s);
+        pipe_step_kernel<T, false><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), dinv.data(), sp,
+                                                           Ap.data(), x, r.data(), u.data(), w.data(), m.data(),
+                                                           z.data(), q.data(), sv.data(), p.data());
+      }
       return;
     }
     apply(p.data(), Ap.data(), s);
@@ -705,6 +738,7 @@ struct Pcg {
   template <class Apply>
   void prepare_graph(Apply&& apply, const T* b, T* x, uint64_t sig, int batch_iters, hipStream_t s) {
     batch_iters = even_batch(batch_iters);
+    if (split) sig ^= 0x9e3779b97f4a7c15ull;  // graphs with and without the split differ
     if (!execs.empty() && batch == batch_iters && b_bound == b && x_bound == x && sig_bound == sig) return;
     drop_graphs();
     std::vector<int> sizes{batch_iters};  // even sizes only: the state ends in copy 0
@@ -1087,6 +1121,22 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
     st.d32.allocate(n);
     recip_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg32->dinv.data());
     HIP_CHECK_LAUNCH();
+  }
+  // The pipelined form folds the apply's second pass into its step kernel
+  // where the apply splits (block partial sums, transpose map): one dependent
+  // kernel fewer per iteration, the same arithmetic.
+  FusedMass* fp = &f;
+  if (opt.mode == 0 && st.pcg64->pipe) {
+    FusedSplit fs;
+    st.pcg64->split = !opt.separate_second_pass && fused_split(f, 0, &fs, s);
+    st.pcg64->sp = st.pcg64->split ? fs : FusedSplit{};
+    st.pcg64->first = [fp](const double* in, double* out, hipStream_t ss) { fused_mass_apply_first64(*fp, in, out, ss); };
+  }
+  if (opt.mode != 0 && st.pcg32->pipe) {
+    FusedSplit fs;
+    st.pcg32->split = !opt.separate_second_pass && fused_split(f, 1, &fs, s);
+    st.pcg32->sp = st.pcg32->split ? fs : FusedSplit{};
+    st.pcg32->first = [fp](const float* in, float* out, hipStream_t ss) { fused_mass_apply_first32(*fp, in, out, ss); };
   }
   fused_prepare(f, 0, s);  // lazily made operator state, before any capture
   if (opt.mode != 0) fused_prepare(f, 1, s);

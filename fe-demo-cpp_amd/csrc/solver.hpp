@@ -12,6 +12,7 @@ struct SolveOptions {
   double inner_rtol = 0.0;  // <= 0: default_inner_rtol(mode)
   int batch = 0;  // CG iterations per captured graph launch (0: kSmallBatch up to kPipeMaxN entries, else 8)
   int cg_variant = 0;  // 0: automatic (FP64: pipelined up to kPipeMaxN entries; FP32 inner: plain), 1: plain, 2: pipelined
+  int separate_second_pass = 0;  // pipelined form: 1 keeps the apply's second pass as its own kernel
 };
 
 // Vector length up to which the automatic choice takes pipelined CG (FP64 solves).

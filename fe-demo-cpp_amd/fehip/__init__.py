@@ -75,6 +75,7 @@ class SolveOptions(ctypes.Structure):
         ("rtol", ctypes.c_double),
         ("inner_rtol", ctypes.c_double),
         ("cg_variant", ctypes.c_int),
+        ("separate_second_pass", ctypes.c_int),
     ]
 
 
@@ -724,13 +725,14 @@ class Operator:
         call("fem_hip_operator_mass_diagonal", self.h, d_diag.ptr, d_diag.size, _s(stream))
 
     def solve(self, d_b, d_x, mode=SOLVE_FP64, rtol=1e-10, max_iter=2000, max_outer=30, inner_rtol=None,
-              stream=None, cg_variant=CG_AUTO):
+              stream=None, cg_variant=CG_AUTO, separate_second_pass=False):
         """M x = b. inner_rtol: reduction of each inner FP32 solve (SOLVE_MIXED, default 1e-5)
         or between residual replacements (SOLVE_MIXED_RELIABLE, default 1e-2). cg_variant:
-        CG_AUTO, CG_PLAIN or CG_PIPELINED."""
+        CG_AUTO, CG_PLAIN or CG_PIPELINED; separate_second_pass: the pipelined form keeps the
+        apply's second pass as its own kernel instead of folding it into its step."""
         if inner_rtol is None:
             inner_rtol = 0.0  # the library's default for the mode
-        opt = SolveOptions(mode, max_iter, max_outer, rtol, inner_rtol, cg_variant)
+        opt = SolveOptions(mode, max_iter, max_outer, rtol, inner_rtol, cg_variant, int(separate_second_pass))
         st = SolveStats()
         call("fem_hip_mass_solve", self.h, d_b.ptr, d_x.ptr, ctypes.byref(opt), ctypes.byref(st), _s(stream))
         return {"iterations": st.iterations, "outer": st.outer, "rel_residual": st.rel_residual, "ms": st.ms}

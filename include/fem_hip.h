@@ -402,6 +402,9 @@ typedef struct {
                         FP32 inner solves and MIXED_RELIABLE: plain); 1: plain Jacobi-PCG (apply, dot, update, direction: five
                         dependent launches an iteration); 2: pipelined Jacobi-PCG (Ghysels & Vanroose
                         2014: apply and one fused kernel, three launches) */
+  int separate_second_pass; /* pipelined form: 0 (default) folds the apply's second pass (block partial sums,
+                        transpose map) into the step kernel where the apply splits, one launch fewer an
+                        iteration with the same arithmetic; 1 keeps it as its own kernel */
 } fem_hip_solve_options;
 typedef struct {
   int iterations;    /* total CG iterations (inner ones for mixed) */

@@ -216,3 +216,26 @@ def test_plain_and_pipelined_cg(gpu, n, P, Q, nc):
     # Residual replacement runs the plain form whatever is asked.
     st = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED_RELIABLE, rtol=1e-10, cg_variant=fehip.CG_PIPELINED)
     assert st["rel_residual"] <= 1e-10
+
+
+@pytest.mark.parametrize("n,P,Q,nc", [(16, 3, 4, 1), (6, 5, 6, 3)])
+@pytest.mark.parametrize("alg", [fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_ELEM_TMAP, fehip.FUSED_AUTO])
+def test_pipelined_split_pass_bit_equal(gpu, n, P, Q, nc, alg):
+    """The pipelined form with the apply's second pass (block partial sums or the
+    transpose map) folded into its step kernel gives the bits and iteration
+    counts of the form that runs the pass as its own kernel, in FP64 and mixed
+    precision, and the oracle's solution."""
+    print("split second pass", n, P, Q, nc, alg, flush=True)
+    p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
+    p.op.set_fused_algorithm(alg)
+    b, x_cpu, it_cpu = _oracle_solution(p)
+    d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
+    for mode in (fehip.SOLVE_FP64, fehip.SOLVE_MIXED):
+        s1 = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=fehip.CG_PIPELINED)
+        x1 = d_x.to_host()
+        s2 = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=fehip.CG_PIPELINED, separate_second_pass=True)
+        x2 = d_x.to_host()
+        print("mode", mode, s1, s2, flush=True)
+        assert s1["rel_residual"] <= 1e-10 and rel_err(x1, x_cpu) < 1e-8, (mode, s1)
+        assert (s1["iterations"], s1["outer"]) == (s2["iterations"], s2["outer"]), (mode, s1, s2)
+        assert np.array_equal(x1.view(np.uint64), x2.view(np.uint64)), mode

@@ -3,10 +3,13 @@
 // message for fem_hip_last_error() on the calling thread.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/fem_hip.h"
 #include "basis.hpp"
@@ -15,6 +18,7 @@
 #include "fused_box_kernel.hpp"
 #include "mesh_io.hpp"
 #include "server.hpp"
+#include "batch_solve.hpp"
 #include "common.hpp"
 #include "mesh.hpp"
 #include "operator.hpp"
@@ -1019,6 +1023,68 @@ int fem_hip_mass_solve(fem_hip_operator h, const double* b, double* x, const fem
   });
 }
 
+int fem_hip_mass_solve_batched(int count, const fem_hip_operator* ops, const double* const* b, double* const* x,
+                               const fem_hip_solve_options* opts, int iters_per_launch, fem_hip_solve_stats* stats,
+                               int* status) {
+  return guard([&] {
+    require(count >= 0, "mass_solve_batched: count must be >= 0");
+    if (count == 0) return;
+    need(ops, "ops");
+    need(b, "b");
+    need(x, "x");
+    need(opts, "opts");
+    for (int i = 0; i < count; ++i) {
+      need(ops[i], "ops[i]");
+      need(b[i], "b[i]");
+      need(x[i], "x[i]");
+      for (int k = 0; k < i; ++k)
+        require(ops[k] != ops[i], "mass_solve_batched: an operator may appear once (its element outputs are per solve)");
+    }
+    HIP_CHECK(hipDeviceSynchronize());  // b and x written on any stream are complete
+    std::vector<std::unique_ptr<femhip::BatchJob, void (*)(femhip::BatchJob*)>> jobs;
+    for (int i = 0; i < count; ++i) {
+      SolveOptions o;
+      o.mode = opts[i].mode;
+      o.max_iter = opts[i].max_iter;
+      o.max_outer = opts[i].max_outer;
+      o.rtol = opts[i].rtol;
+      o.inner_rtol = opts[i].inner_rtol;
+      jobs.emplace_back(femhip::batch_job_create(*ops[i]->op, b[i], x[i], o, nullptr), femhip::batch_job_destroy);
+    }
+    std::unique_ptr<femhip::BatchEngine, void (*)(femhip::BatchEngine*)> eng(
+        femhip::batch_engine_create(iters_per_launch > 0 ? iters_per_launch : 8), femhip::batch_engine_destroy);
+    for (auto& j : jobs) femhip::batch_engine_add(*eng, j.get());
+    std::vector<femhip::BatchJob*> done;
+    while (done.size() < jobs.size()) {
+      if (!femhip::batch_engine_poll(*eng, done)) std::this_thread::sleep_for(std::chrono::microseconds(5));
+    }
+    for (int i = 0; i < count; ++i) {
+      const femhip::BatchJob& j = *jobs[i];
+      const SolveStats& s = femhip::batch_job_stats(j);
+      if (stats) {
+        stats[i].iterations = s.iterations;
+        stats[i].outer = s.outer;
+        stats[i].rel_residual = s.rel_residual;
+        stats[i].ms = s.ms;
+      }
+      if (status) status[i] = femhip::batch_job_status(j);
+    }
+    eng.reset();  // waits for the cohorts' streams
+    for (int i = 0; i < count; ++i) {
+      const femhip::BatchJob& j = *jobs[i];
+      const std::string msg = "mass_solve_batched: solve " + std::to_string(i) + ": " + femhip::batch_job_error(j);
+      switch (femhip::batch_job_status(j)) {
+        case 0: continue;
+        case 1: throw std::invalid_argument(msg);
+        case 2: throw femhip::DeviceError(msg);
+        case 4: throw femhip::Unsupported(msg);
+        case 5: throw femhip::OutOfMemory(msg);
+        default: throw std::runtime_error(msg);
+      }
+    }
+  });
+}
+
 }  // extern "C"
 
 /* ---- serving loop ---- */
@@ -1055,6 +1121,7 @@ int fem_hip_server_create(const fem_hip_server_options* o, fem_hip_server* out) 
       so.autotune = o->autotune != 0;
       so.keep_x = o->keep_x != 0;
       so.cheap_setup = o->cheap_setup != 0;
+      so.batched = o->batched != 0;
     }
     auto h = std::make_unique<fem_hip_server_s>();
     h->s = femhip::server_create(so);

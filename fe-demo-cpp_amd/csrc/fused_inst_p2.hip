@@ -1,4 +1,5 @@
 // Compiled fused-mass shapes with P = 2 (registered at library load).
+#include "batch_kernel.hpp"
 #include "fused_box_kernel.hpp"
 #include "fused_mass_kernel.hpp"
 
@@ -6,8 +7,11 @@ namespace femhip {
 namespace {
 
 const FusedShapeReg<2, 3, 1, 64, 64> reg_2_3_1_64;
+const BatchShapeReg<2, 3, 1, 64, 64> batch_2_3_1_64;
 const FusedShapeReg<2, 3, 2, 64, 64> reg_2_3_2_64;
+const BatchShapeReg<2, 3, 2, 64, 64> batch_2_3_2_64;
 const FusedShapeReg<2, 3, 3, 64, 64> reg_2_3_3_64;
+const BatchShapeReg<2, 3, 3, 64, 64> batch_2_3_3_64;
 
 const FusedBlockShapeReg<2, 3, 1, 32, 32> blk_2_3_1;
 const FusedBlockShapeReg<2, 3, 2, 32, 32> blk_2_3_2;

@@ -1,4 +1,5 @@
 // Compiled fused-mass shapes with P = 4 (registered at library load).
+#include "batch_kernel.hpp"
 #include "fused_box_kernel.hpp"
 #include "fused_mass_kernel.hpp"
 
@@ -6,9 +7,13 @@ namespace femhip {
 namespace {
 
 const FusedShapeReg<4, 5, 1, 23, 23> reg_4_5_1_23;
+const BatchShapeReg<4, 5, 1, 23, 23> batch_4_5_1_23;
 const FusedShapeReg<4, 5, 1, 10, 10> reg_4_5_1_10;
+const BatchShapeReg<4, 5, 1, 10, 10> batch_4_5_1_10;
 const FusedShapeReg<4, 5, 2, 23, 23> reg_4_5_2_23;
+const BatchShapeReg<4, 5, 2, 23, 23> batch_4_5_2_23;
 const FusedShapeReg<4, 5, 3, 23, 23> reg_4_5_3_23;
+const BatchShapeReg<4, 5, 3, 23, 23> batch_4_5_3_23;
 
 const FusedBlockShapeReg<4, 5, 1, 16, 16> blk_4_5_1;
 const FusedBlockShapeReg<4, 5, 2, 8, 8> blk_4_5_2;

@@ -1,4 +1,5 @@
 // Compiled fused-mass shapes with P = 5 (registered at library load).
+#include "batch_kernel.hpp"
 #include "fused_box_kernel.hpp"
 #include "fused_mass_kernel.hpp"
 
@@ -6,10 +7,15 @@ namespace femhip {
 namespace {
 
 const FusedShapeReg<5, 6, 1, 16, 16> reg_5_6_1_16;
+const BatchShapeReg<5, 6, 1, 16, 16> batch_5_6_1_16;
 const FusedShapeReg<5, 6, 1, 7, 7> reg_5_6_1_7;
+const BatchShapeReg<5, 6, 1, 7, 7> batch_5_6_1_7;
 const FusedShapeReg<5, 6, 1, 14, 14> reg_5_6_1_14;
+const BatchShapeReg<5, 6, 1, 14, 14> batch_5_6_1_14;
 const FusedShapeReg<5, 6, 2, 16, 16> reg_5_6_2_16;
+const BatchShapeReg<5, 6, 2, 16, 16> batch_5_6_2_16;
 const FusedShapeReg<5, 6, 3, 16, 16> reg_5_6_3_16;
+const BatchShapeReg<5, 6, 3, 16, 16> batch_5_6_3_16;
 
 const FusedBlockShapeReg<5, 6, 1, 16, 16> blk_5_6_1;
 const FusedBlockShapeReg<5, 6, 1, 14, 14> blk_5_6_1_14;

@@ -300,17 +300,17 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_exac
 // scattered with lanes walking consecutive element nodes (coalesced), from an
 // LDS copy of the element block's offsets. In the atomic mode, nodes with a single
 // contributor (bit 31 of their offset set at create time) are plain stores.
+// One workgroup's group of NE elements (elements blk*NE ...), given its LDS:
+// the body of fused_mass_fast_kernel, also run per work item by the batched
+// element kernel (batch_kernel.hpp).
 template <int P, int Q, int NC, class T, int NE, bool DET>
-__global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast_kernel(
-    int num_elem, int comp_stride, const int* __restrict__ offsets, const T* __restrict__ qdata,
-    const T* __restrict__ u, T* __restrict__ out, size_t y_comp_len, MatArg<T, P * Q> Bm) {
+__device__ __forceinline__ void fast_elem_group(int blk, int num_elem, int comp_stride,
+                                                const int* __restrict__ offsets, const T* __restrict__ qdata,
+                                                const T* __restrict__ u, T* __restrict__ out, size_t y_comp_len,
+                                                const MatArg<T, P * Q>& Bm, T (*s_b)[Shape<P, Q>::FBUF],
+                                                int* s_off) {
   using S = Shape<P, Q>;
   constexpr int P2 = S::P2, P3 = S::P3, Q2 = S::Q2, Q3 = S::Q3, TPE = S::TPE, RY = S::RY, ZS = S::ZS;
-  __shared__ T s_b[NE][S::FBUF];
-  __shared__ int s_off[DET ? 1 : NE * P3];
-
-  const int per = gridDim.x / 8;
-  const int blk = (blockIdx.x % 8) * per + blockIdx.x / 8;
   const long long e0 = static_cast<long long>(blk) * NE;
   if (e0 >= num_elem) return;  // the whole workgroup leaves together
   const int n_here = static_cast<int>(min(static_cast<long long>(NE), num_elem - e0));
@@ -441,6 +441,18 @@ __global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast
     }
     if (c + 1 < NC) __syncthreads();
   }
+}
+
+template <int P, int Q, int NC, class T, int NE, bool DET>
+__global__ __launch_bounds__(round64(NE* Shape<P, Q>::TPE)) void fused_mass_fast_kernel(
+    int num_elem, int comp_stride, const int* __restrict__ offsets, const T* __restrict__ qdata,
+    const T* __restrict__ u, T* __restrict__ out, size_t y_comp_len, MatArg<T, P * Q> Bm) {
+  using S = Shape<P, Q>;
+  __shared__ T s_b[NE][S::FBUF];
+  __shared__ int s_off[DET ? 1 : NE * S::P3];
+  const int per = gridDim.x / 8;
+  const int blk = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  fast_elem_group<P, Q, NC, T, NE, DET>(blk, num_elem, comp_stride, offsets, qdata, u, out, y_comp_len, Bm, s_b, s_off);
 }
 
 // Block-accumulated variant of the fast path (atomic scatter mode). A workgroup

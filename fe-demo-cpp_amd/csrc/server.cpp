@@ -17,6 +17,7 @@
 #include <thread>
 
 #include "basis.hpp"
+#include "batch_solve.hpp"
 #include "basis_setup.hpp"
 #include "fused_mass.hpp"
 #include "mesh.hpp"
@@ -109,8 +110,10 @@ struct Job {
   DevBuf<double> b, x;
   MassSolverState* state = nullptr;
   SolveJob* solve = nullptr;
+  BatchJob* batch = nullptr;  // batched solves: driven by the server's BatchEngine
   ServeResult res;
   ~Job() {
+    batch_job_destroy(batch);
     solve_job_destroy(solve);
     mass_solver_state_destroy(state);
     op.reset();
@@ -295,13 +298,30 @@ class MassServer {
     done_cv_.notify_all();
   }
 
+  // Hands a running job to the workers for teardown.
+  void to_teardown(std::unique_ptr<Job> j) {
+    std::unique_lock<std::mutex> lk(mu_);
+    running_.erase(j.get());
+    ++in_teardown_;
+    teardown_q_.push_back(std::move(j));
+    work_cv_.notify_one();
+  }
+
   void scheduler_loop() {
     (void)hipSetDevice(device_);
     std::vector<std::unique_ptr<Job>> mine;  // running jobs owned by this thread
+    std::map<BatchJob*, Job*> batched;       // mine's jobs in the batch engine
     while (true) {
       {
         std::unique_lock<std::mutex> lk(mu_);
-        for (auto& j : ready_q_) mine.push_back(std::move(j));
+        for (auto& j : ready_q_) {
+          if (j->batch) {
+            if (!engine_) engine_.reset(batch_engine_create(opt_.batch_iters));
+            batch_engine_add(*engine_, j->batch);
+            batched[j->batch] = j.get();
+          }
+          mine.push_back(std::move(j));
+        }
         ready_q_.clear();
         admit_locked();
         const bool idle = mine.empty();
@@ -311,9 +331,30 @@ class MassServer {
           continue;
         }
       }
-      // Advance every running solve; hand finished ones to the workers.
       bool progressed = false;
+      // Batched solves: one poll advances every cohort.
+      if (engine_ && batch_engine_jobs(*engine_) > 0) {
+        std::vector<BatchJob*> fin;
+        progressed |= batch_engine_poll(*engine_, fin);
+        for (BatchJob* b : fin) {
+          Job* jp = batched[b];
+          batched.erase(b);
+          const SolveStats& st = batch_job_stats(*b);
+          jp->res.iterations = st.iterations;
+          jp->res.outer = st.outer;
+          jp->res.rel_residual = st.rel_residual;
+          jp->res.solve_ms = ms_between(jp->t_ready, Clock::now());
+          if (batch_job_status(*b) != 0) {
+            jp->res.status = batch_job_status(*b);
+            jp->res.error = batch_job_error(*b);
+          }
+          for (auto& j : mine)
+            if (j.get() == jp) to_teardown(std::move(j));
+        }
+      }
+      // Advance every other running solve; hand finished ones to the workers.
       for (auto& j : mine) {
+        if (!j || j->batch) continue;
         try {
           if (solve_job_step(*j->solve)) {
             progressed = true;
@@ -322,20 +363,12 @@ class MassServer {
             j->res.outer = st.outer;
             j->res.rel_residual = st.rel_residual;
             j->res.solve_ms = ms_between(j->t_ready, Clock::now());
-            std::unique_lock<std::mutex> lk(mu_);
-            running_.erase(j.get());
-            ++in_teardown_;
-            teardown_q_.push_back(std::move(j));
-            work_cv_.notify_one();
+            to_teardown(std::move(j));
           }
         } catch (const std::exception& e) {
           j->res.status = status_of(e);
           j->res.error = e.what();
-          std::unique_lock<std::mutex> lk(mu_);
-          running_.erase(j.get());
-          ++in_teardown_;
-          teardown_q_.push_back(std::move(j));
-          work_cv_.notify_one();
+          to_teardown(std::move(j));
           progressed = true;
         }
       }
@@ -384,6 +417,23 @@ class MassServer {
         j->b.allocate(L);  // the rhs stream generated in place, no host copy
         rng_uniform_device(j->req.rhs_seed, L, -1.0, 1.0, j->b.data(), j->stream);
         j->x.allocate(L);
+        SolveOptions so;
+        so.mode = j->req.mode;
+        so.rtol = j->req.rtol;
+        so.max_iter = 2000;
+        so.max_outer = 30;
+        so.batch = opt_.batch_iters;
+        so.inner_rtol = default_inner_rtol(so.mode);
+        const bool mixed = j->req.mode != 0;
+        if (opt_.batched && batch_supported(j->req.P, j->req.Q, j->req.num_comp, j->req.mode)) {
+          // The tuned packing of the transpose-map kernel (autotuned per shape when asked).
+          if (opt_.autotune) maybe_tune(operator_fused_mass(*j->op, j->stream), mixed, j->stream);
+          j->batch = batch_job_create(*j->op, j->b.data(), j->x.data(), so, j->stream);
+          j->res.build_ms = ms_between(j->t_admit, Clock::now());
+        }
+      }
+      if (!j->batch) {
+        CachedAllocScope cached(cache_.get(), j->stream);
         FusedMass& f = operator_fused_mass(*j->op, j->stream);
         const bool mixed = j->req.mode != 0;
         if (opt_.autotune) maybe_tune(f, mixed, j->stream);
@@ -412,11 +462,11 @@ class MassServer {
         so.inner_rtol = default_inner_rtol(so.mode);
         j->solve = solve_job_create(*j->op, j->state, j->b.data(), j->x.data(), so, j->stream);
         HIP_CHECK(hipStreamSynchronize(j->stream));
+        const Clock::time_point t_built = Clock::now();
+        j->res.build_ms = ms_between(j->t_admit, t_built);
+        solve_job_prepare_graphs(*j->solve);  // exclusive: other threads' allocations and copies wait
+        j->res.graph_ms = ms_between(t_built, Clock::now());
       }
-      const Clock::time_point t_built = Clock::now();
-      j->res.build_ms = ms_between(j->t_admit, t_built);
-      solve_job_prepare_graphs(*j->solve);  // exclusive: other threads' allocations and copies wait
-      j->res.graph_ms = ms_between(t_built, Clock::now());
     } catch (const std::exception& e) {
       ok = false;
       j->res.status = status_of(e);
@@ -519,6 +569,7 @@ class MassServer {
   ServerStats stats_;
   std::thread scheduler_;
   std::vector<std::thread> workers_;
+  std::unique_ptr<BatchEngine, void (*)(BatchEngine*)> engine_{nullptr, batch_engine_destroy};  // scheduler thread only
 };
 
 MassServer* server_create(const ServerOptions& o) { return new MassServer(o); }

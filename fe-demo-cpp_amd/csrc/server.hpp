@@ -7,10 +7,13 @@
 // requests are set up on worker threads (mesh, restrictions, qdata through the
 // build_mass operator, fused mass operator with the tuned algorithm, the
 // captured CG graphs), then one scheduler thread advances every running solve
-// without blocking: each solve runs its graph-captured CG batches on its own
-// stream, so the solves of many requests overlap on the device, and a request
-// that arrives meanwhile starts as soon as a slot and memory free up. Finished
-// requests are copied out and torn down on the workers, releasing their budget.
+// without blocking. Batched (the default): requests of one shape (P, Q,
+// num_comp, precision) join a cohort whose persistent element and step
+// kernels, captured once as a hipGraph, advance all of them per launch; a
+// request that arrives meanwhile joins at the next launch as soon as a slot
+// and memory free up. Otherwise each solve runs its own graph-captured CG
+// batches on its own stream. Finished requests are copied out and torn down on
+// the workers, releasing their budget.
 #pragma once
 
 #include <cstddef>
@@ -55,6 +58,11 @@ struct ServerOptions {
   // tuning table's fastest algorithm where it has a row for the size (block
   // algorithms cluster elements on the host), else the transpose map.
   bool cheap_setup = true;
+  // true: FP64 and mixed-refinement solves of shapes with a batched element
+  // kernel run in cohorts (batch_solve.hpp): concurrent requests of one shape
+  // share each persistent element and step launch of a captured graph. false
+  // (and every other request): one stream and captured graph per request.
+  bool batched = true;
 };
 
 struct ServerStats {

@@ -183,6 +183,8 @@ SIGNATURES = {
     "fem_hip_box_detect": [_i, _i, _vp, _i, _i, _i, _vp, _vp],
     "fem_hip_box_column_check": [_i, _i, _i, _i, _i, _i, _llp],
     "fem_hip_mass_solve": [_vp, _vp, _vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveStats), _vp],
+    "fem_hip_mass_solve_batched": [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                   ctypes.POINTER(SolveOptions), _i, ctypes.POINTER(SolveStats), _ip],
 }
 _RESTYPES = {"fem_hip_last_error": ctypes.c_char_p, "fem_hip_version": ctypes.c_char_p,
              "fem_hip_noalloc_scratch_bytes": ctypes.c_size_t}
@@ -738,6 +740,27 @@ class Operator:
         return {"iterations": st.iterations, "outer": st.outer, "rel_residual": st.rel_residual, "ms": st.ms}
 
 
+def mass_solve_batched(ops, d_bs, d_xs, modes=SOLVE_FP64, rtol=1e-10, max_iter=2000, max_outer=30,
+                       inner_rtol=None, iters_per_launch=8):
+    """Solves M_i x_i = b_i for every operator together (fem_hip_mass_solve_batched):
+    solves of one shape and precision share each persistent launch of their
+    cohort's graph. modes: one mode or one per solve (SOLVE_FP64 / SOLVE_MIXED).
+    Returns a list of stats dicts."""
+    n = len(ops)
+    if isinstance(modes, int):
+        modes = [modes] * n
+    opts = (SolveOptions * n)(*[SolveOptions(m, max_iter, max_outer, rtol, 0.0 if inner_rtol is None else inner_rtol,
+                                             CG_AUTO, 0) for m in modes])
+    sts = (SolveStats * n)()
+    status = (ctypes.c_int * n)()
+    hs = (_vp * n)(*[o.h for o in ops])
+    bs = (_vp * n)(*[b.ptr for b in d_bs])
+    xs = (_vp * n)(*[x.ptr for x in d_xs])
+    call("fem_hip_mass_solve_batched", n, hs, bs, xs, opts, iters_per_launch, sts, status)
+    return [{"iterations": st.iterations, "outer": st.outer, "rel_residual": st.rel_residual, "ms": st.ms}
+            for st in sts]
+
+
 def operator_create_mass(restriction, basis, d_qdata):
     h = ctypes.c_void_p()
     call("fem_hip_operator_create_mass", restriction.h, basis.h, d_qdata.ptr, ctypes.byref(h))
@@ -779,7 +802,7 @@ def mesh_write(path, mesh):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_active", ctypes.c_int), ("workers", ctypes.c_int), ("hbm_fraction", ctypes.c_double),
                 ("batch_iters", ctypes.c_int), ("autotune", ctypes.c_int), ("keep_x", ctypes.c_int),
-                ("cheap_setup", ctypes.c_int)]
+                ("cheap_setup", ctypes.c_int), ("batched", ctypes.c_int)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -813,12 +836,14 @@ def serve_request_bytes(req):
 
 
 class Server:
-    """Continuous-batching mass-solve server (fem_hip_server_*)."""
+    """Continuous-batching mass-solve server (fem_hip_server_*). batched: requests
+    of one shape and precision share the persistent launches of their cohort's
+    captured graph (else one stream and graph per request)."""
 
     def __init__(self, max_active=16, workers=8, hbm_fraction=0.5, batch_iters=8, autotune=False, keep_x=True,
-                 cheap_setup=True):
+                 cheap_setup=True, batched=True):
         o = ServerOptions(max_active, workers, hbm_fraction, batch_iters, int(autotune), int(keep_x),
-                          int(cheap_setup))
+                          int(cheap_setup), int(batched))
         self.h = ctypes.c_void_p()
         call("fem_hip_server_create", ctypes.byref(o), ctypes.byref(self.h))
         self.keep_x = keep_x

@@ -414,6 +414,22 @@ typedef struct {
 } fem_hip_solve_stats;
 int fem_hip_mass_solve(fem_hip_operator op, const double* b, double* x, const fem_hip_solve_options* opt,
                        fem_hip_solve_stats* stats, void* stream);
+/* No reference counterpart (the reference solves nothing): `count` solves of
+ * M_i x_i = b_i on distinct fused mass operators, run together. Solves whose
+ * operators share (P, Q, num_comp, basis) and precision (FEM_HIP_SOLVE_FP64, or
+ * FEM_HIP_SOLVE_MIXED's FP32 inner solves) form a cohort: each launch of its
+ * captured graph advances every one of them by iters_per_launch (<= 0: 8)
+ * pipelined Jacobi-PCG iterations, the element work of all of them in one
+ * persistent element kernel (transpose-map algorithm, deterministic) and their
+ * vector updates and reductions in one persistent step kernel. A solve's result
+ * is the same bits whichever other solves share its launches. Modes FP64 and
+ * MIXED only; the operators are set to the transpose-map algorithm. Synchronous:
+ * the x_i are complete on return. stats (optional) and status (optional) have
+ * count entries; status[i] is 0 or the FEM_HIP_ERR_* of solve i (the call then
+ * returns the first such code). */
+int fem_hip_mass_solve_batched(int count, const fem_hip_operator* ops, const double* const* b, double* const* x,
+                               const fem_hip_solve_options* opts, int iters_per_launch, fem_hip_solve_stats* stats,
+                               int* status);
 
 /* ---- on-disk meshes ------------------------------------------------------------
  * No reference counterpart (its meshes are built in code: tests/test-operator.cpp:90-121,
@@ -452,6 +468,10 @@ typedef struct {
                           columns, element + transpose map), else the transpose map; 0: the tuning
                           table's fastest algorithm where it has a row for the size (block algorithms
                           cluster elements on the host), else the transpose map */
+  int batched;         /* 1: FP64 and MIXED requests whose shape has a batched kernel run in cohorts
+                          (see fem_hip_mass_solve_batched): the element blocks of concurrent requests of
+                          one shape share each persistent launch of one captured graph; 0: one stream and
+                          captured graph per request */
 } fem_hip_server_options;
 typedef struct {
   int n, P, Q, num_comp; /* n^3 hexes of the unit cube, order P, Q Gauss points */

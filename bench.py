@@ -330,6 +330,10 @@ def main():
     ap.add_argument("--no-big", action="store_true", help="skip the 48^3, P=8, Q=10, 3-component line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-max-active", type=int, default=16)
+    ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-unbatched", action="store_true",
+                    help="one stream and captured graph per request instead of shape cohorts")
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="Poisson arrival rate, requests/s (default: 0.8 x the measured service capacity)")
     ap.add_argument("--variants", default="", help="comma-separated subset of the apply variants")
@@ -479,7 +483,8 @@ def main():
         from fehip.serving import make_requests, measure_capacity, run_workload
 
         t1 = time.perf_counter()
-        opts = dict(max_active=16, workers=8, hbm_fraction=0.5)
+        opts = dict(max_active=args.serve_max_active, workers=args.serve_workers, hbm_fraction=0.5,
+                    batched=not args.serve_unbatched)
         capacity = None
         rate = args.serve_rate
         if rate <= 0:
@@ -488,9 +493,21 @@ def main():
             log("serving capacity %.1f requests/s -> arrival rate %.1f/s" % (capacity, rate))
         serve, _ = run_workload(make_requests(args.serve_requests, seed=0), rate, **opts)
         serve["measured_capacity_req_per_s"] = capacity
+        serve["server_options"] = opts
+        # The same capacity measurement with one stream and captured graph per
+        # request (no cohorts), for comparison.
+        other = dict(opts, batched=not opts["batched"])
+        cap2, s2 = measure_capacity(300, **other)
+        serve["capacity_other_mode"] = {"batched": other["batched"], "capacity_req_per_s": cap2,
+                                        "latency_ms": s2["latency_ms"], "setup_ms_p50": s2["setup_ms_p50"],
+                                        "solve_ms_p50": s2["solve_ms_p50"]}
         serve["workload"] = ("%d mass-solve requests, Poisson arrivals at %.1f/s (0.8 x measured capacity), "
                              "n ~ U{4..40}, P ~ U{2..6}, Q = P+1, num_comp ~ U{1..3}, FP64 or mixed (1:1), "
-                             "rtol 1e-10, request r seeded 1000+r" % (args.serve_requests, rate))
+                             "rtol 1e-10, request r seeded 1000+r; %s" % (
+                                 args.serve_requests, rate,
+                                 "shape cohorts: concurrent requests of one (P, Q, num_comp, precision) share each "
+                                 "persistent element/step launch of one captured graph" if opts["batched"] else
+                                 "one stream and captured graph per request"))
         log("serve (%.1f s)" % (time.perf_counter() - t1), json.dumps(serve))
     cpu = None
     if not args.no_cpu:

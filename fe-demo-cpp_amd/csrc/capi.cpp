@@ -1122,6 +1122,7 @@ int fem_hip_server_create(const fem_hip_server_options* o, fem_hip_server* out) 
       so.keep_x = o->keep_x != 0;
       so.cheap_setup = o->cheap_setup != 0;
       so.batched = o->batched != 0;
+      if (o->batch_max_elem_nodes > 0) so.batch_max_nodes = o->batch_max_elem_nodes;
     }
     auto h = std::make_unique<fem_hip_server_s>();
     h->s = femhip::server_create(so);

@@ -425,7 +425,9 @@ class MassServer {
         so.batch = opt_.batch_iters;
         so.inner_rtol = default_inner_rtol(so.mode);
         const bool mixed = j->req.mode != 0;
-        if (opt_.batched && batch_supported(j->req.P, j->req.Q, j->req.num_comp, j->req.mode)) {
+        const long long elem_nodes = ipow3(j->req.n) * ipow3(j->req.P) * j->req.num_comp;
+        if (opt_.batched && elem_nodes <= opt_.batch_max_nodes &&
+            batch_supported(j->req.P, j->req.Q, j->req.num_comp, j->req.mode)) {
           // The tuned packing of the transpose-map kernel (autotuned per shape when asked).
           if (opt_.autotune) maybe_tune(operator_fused_mass(*j->op, j->stream), mixed, j->stream);
           j->batch = batch_job_create(*j->op, j->b.data(), j->x.data(), so, j->stream);

@@ -59,10 +59,15 @@ struct ServerOptions {
   // algorithms cluster elements on the host), else the transpose map.
   bool cheap_setup = true;
   // true: FP64 and mixed-refinement solves of shapes with a batched element
-  // kernel run in cohorts (batch_solve.hpp): concurrent requests of one shape
-  // share each persistent element and step launch of a captured graph. false
-  // (and every other request): one stream and captured graph per request.
+  // kernel and at most batch_max_nodes element nodes (num_elem * P^3 *
+  // num_comp) run in cohorts (batch_solve.hpp): concurrent requests of one
+  // shape share each persistent element and step launch of a captured graph.
+  // Every other request (and all with false): one stream and captured graph
+  // per request, with the tuned algorithm (box columns for large boxes).
   bool batched = true;
+  // Requests this small cannot fill the GPU alone (on MI355X the box columns,
+  // which the batched kernels do not use, win from about this size up).
+  long long batch_max_nodes = 1LL << 20;
 };
 
 struct ServerStats {

@@ -802,7 +802,7 @@ def mesh_write(path, mesh):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_active", ctypes.c_int), ("workers", ctypes.c_int), ("hbm_fraction", ctypes.c_double),
                 ("batch_iters", ctypes.c_int), ("autotune", ctypes.c_int), ("keep_x", ctypes.c_int),
-                ("cheap_setup", ctypes.c_int), ("batched", ctypes.c_int)]
+                ("cheap_setup", ctypes.c_int), ("batched", ctypes.c_int), ("batch_max_elem_nodes", ctypes.c_longlong)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -837,13 +837,14 @@ def serve_request_bytes(req):
 
 class Server:
     """Continuous-batching mass-solve server (fem_hip_server_*). batched: requests
-    of one shape and precision share the persistent launches of their cohort's
-    captured graph (else one stream and graph per request)."""
+    of one shape and precision with at most batch_max_elem_nodes element nodes
+    (0: 2^20) share the persistent launches of their cohort's captured graph;
+    others get one stream and graph per request."""
 
     def __init__(self, max_active=16, workers=8, hbm_fraction=0.5, batch_iters=8, autotune=False, keep_x=True,
-                 cheap_setup=True, batched=True):
+                 cheap_setup=True, batched=True, batch_max_elem_nodes=0):
         o = ServerOptions(max_active, workers, hbm_fraction, batch_iters, int(autotune), int(keep_x),
-                          int(cheap_setup), int(batched))
+                          int(cheap_setup), int(batched), int(batch_max_elem_nodes))
         self.h = ctypes.c_void_p()
         call("fem_hip_server_create", ctypes.byref(o), ctypes.byref(self.h))
         self.keep_x = keep_x

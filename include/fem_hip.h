@@ -468,10 +468,13 @@ typedef struct {
                           columns, element + transpose map), else the transpose map; 0: the tuning
                           table's fastest algorithm where it has a row for the size (block algorithms
                           cluster elements on the host), else the transpose map */
-  int batched;         /* 1: FP64 and MIXED requests whose shape has a batched kernel run in cohorts
-                          (see fem_hip_mass_solve_batched): the element blocks of concurrent requests of
-                          one shape share each persistent launch of one captured graph; 0: one stream and
-                          captured graph per request */
+  int batched;         /* 1: FP64 and MIXED requests whose shape has a batched kernel and that have at
+                          most batch_max_elem_nodes element nodes run in cohorts (see
+                          fem_hip_mass_solve_batched): the element blocks of concurrent requests of one
+                          shape share each persistent launch of one captured graph; larger requests (and
+                          all with 0): one stream and captured graph per request, tuned algorithm */
+  long long batch_max_elem_nodes; /* num_elem * P^3 * num_comp; <= 0: 2^20 (requests too small to fill
+                          the GPU alone; the box columns, which cohorts do not use, win above it) */
 } fem_hip_server_options;
 typedef struct {
   int n, P, Q, num_comp; /* n^3 hexes of the unit cube, order P, Q Gauss points */

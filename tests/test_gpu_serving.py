@@ -2,8 +2,10 @@
 
 * A request's solution is the same bit for bit alone and in a mix of concurrent
   requests (every solve uses the deterministic scatter and fixed reductions; the
-  tuning table is not changed in between), and equals the direct mass_solve of
-  the same problem with the same scatter algorithm.
+  tuning table is not changed in between), both when small requests of one
+  shape share cohort launches (batched, the default) and with one stream and
+  graph per request, and equals the direct mass_solve of the same problem with
+  the same scatter algorithm.
 * Admission follows the HBM budget: with a budget that fits one request, only one
   runs at a time; a request larger than the budget fails with OUT_OF_MEMORY.
 * Every request reaches its residual target; FP64 and mixed solves both do.
@@ -27,11 +29,14 @@ def req_list():
             R(7, 5, 6, 1, fehip.SOLVE_FP64, mesh_seed=5, rhs_seed=15)]
 
 
-def test_serve_alone_equals_in_mix(gpu):
-    print("serving: alone vs mix", flush=True)
+@pytest.mark.parametrize("batched", [True, False])
+def test_serve_alone_equals_in_mix(gpu, batched):
+    """batched: the small requests share cohort launches; else one stream and
+    graph per request."""
+    print("serving: alone vs mix, batched", batched, flush=True)
     reqs = req_list()
     alone = []
-    srv = fehip.Server(max_active=1, workers=2, hbm_fraction=0.3)
+    srv = fehip.Server(max_active=1, workers=2, hbm_fraction=0.3, batched=batched)
     for r in reqs:
         res, x = srv.wait(srv.submit(r), fehip.serve_l_size(r))
         assert res["status"] == 0 and res["rel_residual"] <= r.rtol * 1.0001, res
@@ -41,7 +46,7 @@ def test_serve_alone_equals_in_mix(gpu):
     filler = make_requests(24, seed=5, n_range=(3, 10), p_range=(2, 6))
     mix = filler[:12] + reqs + filler[12:]
     summary, results = run_workload(mix, rate_per_s=400.0, keep_x=True, max_active=12, workers=6,
-                                    hbm_fraction=0.3)
+                                    hbm_fraction=0.3, batched=batched)
     assert summary["failed"] == 0, summary["errors"]
     assert summary["server"]["active_peak"] > 1
     for k, r in enumerate(reqs):

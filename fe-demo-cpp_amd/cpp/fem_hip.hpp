@@ -518,8 +518,10 @@ class Server {
 };
 
 // Defaults: 16 requests at once, 8 setup workers, half the free HBM, 8 CG
-// iterations per graph launch, no autotuning, solutions kept, transpose-map scatter.
-inline fem_hip_server_options server_options() { return fem_hip_server_options{16, 8, 0.5, 8, 0, 1, 1}; }
+// iterations per graph launch, no autotuning, solutions kept, the automatic
+// algorithm where its setup is all on the device, small requests (up to 2^20
+// element nodes) batched in shape cohorts.
+inline fem_hip_server_options server_options() { return fem_hip_server_options{16, 8, 0.5, 8, 0, 1, 1, 1, 0}; }
 
 inline void synchronize() { FEM_HIP_CHECK(fem_hip_device_synchronize()); }
 

@@ -654,6 +654,211 @@ __global__ __launch_bounds__(round64(BX* BY* Shape<P, Q>::TPE)) void fused_box_k
   }
 }
 
+// Orders a wavefront's own LDS accesses: LDS executes one wavefront's
+// instructions in order, so a lane's read after this sees what any lane of
+// the same wavefront wrote before it; the fences keep the compiler from
+// moving memory operations across (the wave_barrier idiom of rocPRIM).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The box-column kernel with one wavefront per element: the five stages of an
+// element run on the 64 lanes of its own wavefront (x-rows: P^2 lanes; y
+// lines: P*Q, z-lines: Q^2, taken 64 at a time), so they are ordered by
+// wave_sync alone, and the workgroup waits twice per layer (before the node
+// sums, which read every element's tile, and before the next layer's first
+// stage overwrites the tiles) instead of six times. Segments, node sums, the
+// partial area and the shared-node pass are those of fused_box_kernel; u rows
+// are loaded straight from u and qdata within its own layer (UM = 0, QA = 0).
+template <int P, int Q, int NC, class T, int BX, int BY, int SPLIT = 1>
+__global__ __launch_bounds__(64 * BX * BY) void fused_box_wave_kernel(
+    BoxGeom g, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
+    size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em) {
+  using S = Shape<P, Q>;
+  constexpr int NE = BX * BY;
+  constexpr int P2 = S::P2, Q2 = S::Q2, Q3 = S::Q3, PQ = P * Q;
+  constexpr int RY = BoxTile<P, Q>::RY, ZS = BoxTile<P, Q>::ZS, FB = BoxTile<P, Q>::FBUF;
+  constexpr int M = P > Q ? P : Q;
+  static_assert(RY >= M && ZS >= (M - 1) * RY + M && FB >= (P - 1) * ZS + (M - 1) * RY + M,
+                "element tile strides hold every (iz, y, x) entry");
+  static_assert(P2 <= 64, "one x-row per lane");
+  constexpr int QL = (Q2 + 63) / 64;  // z-lines per lane
+  constexpr int LXn = BX * (P - 1) + 1, LYn = BY * (P - 1) + 1, PLANE = LXn * LYn;
+  static_assert(PLANE <= NE * 64, "one node position per thread");
+  __shared__ T s_b[NE + 1][FB];   // element tiles, then one tile of zeros (absent contributions)
+  __shared__ T s_top[NC][PLANE];  // top node plane of the last layer, completed by the next one
+
+  const int nwg = gridDim.x;
+  int w = blockIdx.x;
+  if (nwg % 8 == 0) w = (blockIdx.x % 8) * (nwg / 8) + blockIdx.x / 8;
+  static_assert(SPLIT == 1 || NC == 1, "a component-split kernel runs one component per workgroup");
+  if constexpr (SPLIT > 1) {
+    const int comp = w % SPLIT;
+    w /= SPLIT;
+    u += static_cast<size_t>(comp) * g.cs;
+    v += static_cast<size_t>(comp) * g.cs;
+    partials += static_cast<size_t>(comp) * part_comp_len;
+  }
+  constexpr bool kQdataNT = sizeof(T) == 8 && SPLIT == 1;
+  const int cx = w % g.ncx, cy = (w / g.ncx) % g.ncy, sz = w / (g.ncx * g.ncy);
+  const int ez0 = box_seg_z0(g, sz), nl = box_seg_z0(g, sz + 1) - ez0;
+  const int LZn = g.lz * (P - 1) + 1;
+  const int LZs = nl * (P - 1) + 1;
+  const int gx0 = cx * BX * (P - 1), gy0 = cy * BY * (P - 1), gz0 = ez0 * (P - 1);
+  const size_t Nxy = static_cast<size_t>(g.Nx) * g.Ny;
+  T* my_part = partials + static_cast<size_t>(w) * box_face_size(LXn, LYn, LZn);
+
+  const int el = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int lex = el % BX, ley = el / BX;
+  const int ex = cx * BX + lex, ey = cy * BY + ley;
+  T* b = s_b[el];
+  for (int i = threadIdx.x; i < FB; i += blockDim.x) s_b[NE][i] = T(0);
+
+  // Node positions of the column cross-section (as in fused_box_kernel).
+  const int pos = threadIdx.x;
+  const bool pos_thread = pos < PLANE;
+  const int ply = pos_thread ? pos / LXn : 0, plx = pos_thread ? pos - (pos / LXn) * LXn : 0;
+  constexpr int ZT = NE * FB;
+  int toff0 = ZT, toff1 = ZT, toff2 = ZT, toff3 = ZT;
+  if (pos_thread) {
+    const int qx = plx / (P - 1), rx = plx - qx * (P - 1), qy = ply / (P - 1), ry = ply - qy * (P - 1);
+    const bool xlo = rx == 0 && qx > 0, xup = rx != 0 || qx < BX;
+    const bool ylo = ry == 0 && qy > 0, yup = ry != 0 || qy < BY;
+    const int oxl = (qx - 1) * FB + (P - 1), oxu = qx * FB + rx;
+    const int oyl = (qy - 1) * BX * FB + (P - 1) * RY, oyu = qy * BX * FB + ry * RY;
+    if (ylo && xlo) toff0 = oyl + oxl;
+    if (ylo && xup) toff1 = oyl + oxu;
+    if (yup && xlo) toff2 = oyu + oxl;
+    if (yup && xup) toff3 = oyu + oxu;
+  }
+  const bool pxs = (plx == 0 && cx > 0) || (plx == LXn - 1 && cx < g.ncx - 1);
+  const bool pys = (ply == 0 && cy > 0) || (ply == LYn - 1 && cy < g.ncy - 1);
+  T* const pv = v + static_cast<size_t>(gx0 + plx) + static_cast<size_t>(g.Nx) * (gy0 + ply);
+
+  const bool row_lane = lane < P2;
+  const int riz = lane / P, riy = lane - (lane / P) * P;
+  auto row_ptr = [&](int ez, int c) -> const T* {
+    return u + static_cast<size_t>(c) * g.cs + static_cast<size_t>(ex * (P - 1)) +
+           static_cast<size_t>(g.Nx) * (ey * (P - 1) + riy) + Nxy * (ez * (P - 1) + riz);
+  };
+  __syncthreads();  // the zero tile
+
+  T qd[QL][Q];
+#pragma unroll 1
+  for (int k = 0; k < nl; ++k) {
+    const int ez = ez0 + k;
+#pragma unroll 1
+    for (int c = 0; c < NC; ++c) {
+      // S1: x-rows (iz, iy): the row of u, then this layer's qdata (the
+      // first component's step loads it for all), both before S1 waits.
+      T x[P];
+      if (row_lane) load_row<P, T>(row_ptr(ez, c), x);
+      if (NC == 1 || c == 0) {
+        const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ez);
+#pragma unroll
+        for (int j = 0; j < QL; ++j) {
+          const int line = lane + 64 * j;
+          load_qdata<Q, T, kQdataNT>(qdata + e * Q3, line < Q2 ? line : Q2 - 1, qd[j]);
+        }
+      }
+      if (row_lane) {
+        T y[Q];
+        eo_forward<P, Q, T>(x, y, Em);
+        const int row = riz * ZS + riy * RY;
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) b[row + qx] = y[qx];
+      }
+      wave_sync();
+      // S2: y-columns (iz, qx).
+#pragma unroll
+      for (int t0 = 0; t0 < PQ; t0 += 64) {
+        const int t = t0 + lane;
+        if (t < PQ) {
+          const int col = (t / Q) * ZS + t % Q;
+          T xx[P], y[Q];
+#pragma unroll
+          for (int iy = 0; iy < P; ++iy) xx[iy] = b[col + iy * RY];
+          eo_forward<P, Q, T>(xx, y, Em);
+#pragma unroll
+          for (int qy = 0; qy < Q; ++qy) b[col + qy * RY] = y[qy];
+        }
+      }
+      wave_sync();
+      // S3: z-lines (qy, qx): contract z, times qdata, contract z back.
+#pragma unroll
+      for (int j = 0; j < QL; ++j) {
+        const int t = lane + 64 * j;
+        if (t < Q2) {
+          const int line = (t / Q) * RY + t % Q;
+          T xx[P], d[Q];
+#pragma unroll
+          for (int iz = 0; iz < P; ++iz) xx[iz] = b[iz * ZS + line];
+          eo_forward<P, Q, T>(xx, d, Em);
+#pragma unroll
+          for (int qz = 0; qz < Q; ++qz) d[qz] *= qd[j][qz];
+          eo_transpose<P, Q, T>(d, xx, Em);
+#pragma unroll
+          for (int iz = 0; iz < P; ++iz) b[iz * ZS + line] = xx[iz];
+        }
+      }
+      wave_sync();
+      // S4: y-columns back.
+#pragma unroll
+      for (int t0 = 0; t0 < PQ; t0 += 64) {
+        const int t = t0 + lane;
+        if (t < PQ) {
+          const int col = (t / Q) * ZS + t % Q;
+          T y[Q], xx[P];
+#pragma unroll
+          for (int qy = 0; qy < Q; ++qy) y[qy] = b[col + qy * RY];
+          eo_transpose<P, Q, T>(y, xx, Em);
+#pragma unroll
+          for (int iy = 0; iy < P; ++iy) b[col + iy * RY] = xx[iy];
+        }
+      }
+      wave_sync();
+      // S5: x-rows back.
+      if (row_lane) {
+        const int row = riz * ZS + riy * RY;
+        T y[Q], xx[P];
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) y[qx] = b[row + qx];
+        eo_transpose<P, Q, T>(y, xx, Em);
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) b[row + ix] = xx[ix];
+      }
+      __syncthreads();
+      // Node sums (as in fused_box_kernel).
+      if (pos_thread) {
+        const T* tb = &s_b[0][0];
+        T* vc = pv + static_cast<size_t>(c) * g.cs;
+        T* pc = my_part + static_cast<size_t>(c) * part_comp_len;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const T t0 = tb[toff0 + j * ZS], t1 = tb[toff1 + j * ZS], t2 = tb[toff2 + j * ZS],
+                  t3 = tb[toff3 + j * ZS];
+          T sum = (j == 0 && k > 0) ? s_top[c][pos] + t0 : t0;
+          sum = ((sum + t1) + t2) + t3;
+          if (j == P - 1 && k + 1 < nl) {
+            s_top[c][pos] = sum;
+          } else {
+            const int gz = ez * (P - 1) + j, lzl = gz - gz0;
+            const bool zs = (lzl == 0 && sz > 0) || (lzl == LZs - 1 && sz < g.nsz - 1);
+            if (!(pxs || pys || zs)) {
+              vc[Nxy * gz] = sum;
+            } else {
+              pc[box_slot(plx, ply, lzl, pxs, pys, LXn, LYn, LZn)] = sum;
+            }
+          }
+        }
+      }
+      __syncthreads();  // the next step's S1 overwrites the tiles
+    }
+  }
+}
+
 // v at every shared node = its partials summed in ascending segment order
 // (grid: see box_shared_point; a block is kSharedLines lines of 64 points).
 constexpr int kSharedLines = 4;
@@ -701,7 +906,7 @@ inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
   return g;
 }
 
-template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA, bool SPLIT = false>
+template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA, bool SPLIT = false, bool WAVE = false>
 void launch_box(const FusedArgs& a, hipStream_t s) {
   using S = Shape<P, Q>;
   const BoxGeom g = box_geom(a, P, BX, BY);
@@ -715,7 +920,13 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
           "fused mass: box partial buffer too small");
   if (a.box_pass == 0) {
     constexpr int threads = round64(BX * BY * S::TPE);
-    if constexpr (SPLIT) {
+    if constexpr (WAVE) {
+      constexpr int NCK = SPLIT ? 1 : NC, SP = SPLIT ? NC : 1;
+      require(nseg * SP < (1ll << 31), "fused mass: box too large");
+      fused_box_wave_kernel<P, Q, NCK, T, BX, BY, SP><<<static_cast<unsigned>(nseg * SP), 64 * BX * BY, 0, s>>>(
+          g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
+          static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
+    } else if constexpr (SPLIT) {
       require(nseg * NC < (1ll << 31), "fused mass: box too large");
       fused_box_kernel<P, Q, 1, T, BX, BY, UM, QA, NC><<<static_cast<unsigned>(nseg * NC), threads, 0, s>>>(
           g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
@@ -769,6 +980,26 @@ struct FusedBoxSplitReg {
   FusedBoxSplitReg() {
     FusedBoxReg<P, Q, NC, BX, BY>::template reg<double, 0, 0, true>(300);
     FusedBoxReg<P, Q, NC, BX, BY>::template reg<float, 2, 0, true>(300);
+  }
+};
+
+// The wavefront-per-element kernels (fused_box_wave_kernel): code 400, and
+// 500 for the component split of a multi-component field.
+template <int P, int Q, int NC, int BX, int BY>
+struct FusedBoxWaveReg {
+  FusedBoxWaveReg() {
+    if constexpr (NC == 1) {
+      reg<double, false>(400);
+      reg<float, false>(400);
+    } else {
+      reg<double, true>(500);
+      reg<float, true>(500);
+    }
+  }
+  template <class T, bool SPLIT>
+  static void reg(int code) {
+    const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, BX * BY + code};
+    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, 0, 0, SPLIT, true>, 0);
   }
 };
 

@@ -664,29 +664,35 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The box-column kernel with one wavefront per element: the five stages of an
-// element run on the 64 lanes of its own wavefront (x-rows: P^2 lanes; y
-// lines: P*Q, z-lines: Q^2, taken 64 at a time), so they are ordered by
+// The box-column kernel with whole wavefronts per element: the five stages of
+// an element run on the lanes of its own wavefront (EPW = 1: all 64; EPW = 2:
+// two elements per wavefront, 32 lanes each) — x-rows one per lane, y lines
+// (P*Q) and z-lines (Q^2) taken LPE at a time — so they are ordered by
 // wave_sync alone, and the workgroup waits twice per layer (before the node
 // sums, which read every element's tile, and before the next layer's first
 // stage overwrites the tiles) instead of six times. Segments, node sums, the
 // partial area and the shared-node pass are those of fused_box_kernel; u rows
 // are loaded straight from u and qdata within its own layer (UM = 0, QA = 0).
-template <int P, int Q, int NC, class T, int BX, int BY, int SPLIT = 1>
-__global__ __launch_bounds__(64 * BX * BY) void fused_box_wave_kernel(
+// WPE > 0 asks the compiler to fit WPE wavefronts per SIMD (a register budget
+// of 512 / WPE VGPRs).
+template <int P, int Q, int NC, class T, int BX, int BY, int SPLIT = 1, int EPW = 1, int WPE = 0>
+__global__ __launch_bounds__(64 * BX * BY / EPW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void fused_box_wave_kernel(
     BoxGeom g, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
     size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em) {
   using S = Shape<P, Q>;
   constexpr int NE = BX * BY;
+  constexpr int LPE = 64 / EPW;  // lanes per element
+  static_assert((EPW == 1 || EPW == 2) && NE % EPW == 0, "one or two elements per wavefront");
   constexpr int P2 = S::P2, Q2 = S::Q2, Q3 = S::Q3, PQ = P * Q;
   constexpr int RY = BoxTile<P, Q>::RY, ZS = BoxTile<P, Q>::ZS, FB = BoxTile<P, Q>::FBUF;
   constexpr int M = P > Q ? P : Q;
   static_assert(RY >= M && ZS >= (M - 1) * RY + M && FB >= (P - 1) * ZS + (M - 1) * RY + M,
                 "element tile strides hold every (iz, y, x) entry");
-  static_assert(P2 <= 64, "one x-row per lane");
-  constexpr int QL = (Q2 + 63) / 64;  // z-lines per lane
+  static_assert(P2 <= LPE, "one x-row per lane");
+  constexpr int QL = (Q2 + LPE - 1) / LPE;  // z-lines per lane
   constexpr int LXn = BX * (P - 1) + 1, LYn = BY * (P - 1) + 1, PLANE = LXn * LYn;
-  static_assert(PLANE <= NE * 64, "one node position per thread");
+  static_assert(PLANE <= NE * LPE, "one node position per thread");
   __shared__ T s_b[NE + 1][FB];   // element tiles, then one tile of zeros (absent contributions)
   __shared__ T s_top[NC][PLANE];  // top node plane of the last layer, completed by the next one
 
@@ -710,7 +716,7 @@ __global__ __launch_bounds__(64 * BX * BY) void fused_box_wave_kernel(
   const size_t Nxy = static_cast<size_t>(g.Nx) * g.Ny;
   T* my_part = partials + static_cast<size_t>(w) * box_face_size(LXn, LYn, LZn);
 
-  const int el = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int el = threadIdx.x / LPE, lane = threadIdx.x % LPE;
   const int lex = el % BX, ley = el / BX;
   const int ex = cx * BX + lex, ey = cy * BY + ley;
   T* b = s_b[el];
@@ -759,7 +765,7 @@ __global__ __launch_bounds__(64 * BX * BY) void fused_box_wave_kernel(
         const size_t e = static_cast<size_t>(ex) + static_cast<size_t>(g.nx) * (ey + static_cast<size_t>(g.ny) * ez);
 #pragma unroll
         for (int j = 0; j < QL; ++j) {
-          const int line = lane + 64 * j;
+          const int line = lane + LPE * j;
           load_qdata<Q, T, kQdataNT>(qdata + e * Q3, line < Q2 ? line : Q2 - 1, qd[j]);
         }
       }
@@ -773,7 +779,7 @@ __global__ __launch_bounds__(64 * BX * BY) void fused_box_wave_kernel(
       wave_sync();
       // S2: y-columns (iz, qx).
 #pragma unroll
-      for (int t0 = 0; t0 < PQ; t0 += 64) {
+      for (int t0 = 0; t0 < PQ; t0 += LPE) {
         const int t = t0 + lane;
         if (t < PQ) {
           const int col = (t / Q) * ZS + t % Q;
@@ -789,7 +795,7 @@ __global__ __launch_bounds__(64 * BX * BY) void fused_box_wave_kernel(
       // S3: z-lines (qy, qx): contract z, times qdata, contract z back.
 #pragma unroll
       for (int j = 0; j < QL; ++j) {
-        const int t = lane + 64 * j;
+        const int t = lane + LPE * j;
         if (t < Q2) {
           const int line = (t / Q) * RY + t % Q;
           T xx[P], d[Q];
@@ -806,7 +812,7 @@ __global__ __launch_bounds__(64 * BX * BY) void fused_box_wave_kernel(
       wave_sync();
       // S4: y-columns back.
 #pragma unroll
-      for (int t0 = 0; t0 < PQ; t0 += 64) {
+      for (int t0 = 0; t0 < PQ; t0 += LPE) {
         const int t = t0 + lane;
         if (t < PQ) {
           const int col = (t / Q) * ZS + t % Q;
@@ -906,7 +912,8 @@ inline BoxGeom box_geom(const FusedArgs& a, int P, int BX, int BY) {
   return g;
 }
 
-template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA, bool SPLIT = false, bool WAVE = false>
+template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA, bool SPLIT = false, int WAVE = 0,
+          int WPE = 0>
 void launch_box(const FusedArgs& a, hipStream_t s) {
   using S = Shape<P, Q>;
   const BoxGeom g = box_geom(a, P, BX, BY);
@@ -920,10 +927,11 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
           "fused mass: box partial buffer too small");
   if (a.box_pass == 0) {
     constexpr int threads = round64(BX * BY * S::TPE);
-    if constexpr (WAVE) {
+    if constexpr (WAVE > 0) {  // WAVE: elements per wavefront
       constexpr int NCK = SPLIT ? 1 : NC, SP = SPLIT ? NC : 1;
       require(nseg * SP < (1ll << 31), "fused mass: box too large");
-      fused_box_wave_kernel<P, Q, NCK, T, BX, BY, SP><<<static_cast<unsigned>(nseg * SP), 64 * BX * BY, 0, s>>>(
+      fused_box_wave_kernel<P, Q, NCK, T, BX, BY, SP, WAVE, WPE>
+          <<<static_cast<unsigned>(nseg * SP), 64 * BX * BY / WAVE, 0, s>>>(
           g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
           static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
     } else if constexpr (SPLIT) {
@@ -983,23 +991,25 @@ struct FusedBoxSplitReg {
   }
 };
 
-// The wavefront-per-element kernels (fused_box_wave_kernel): code 400, and
-// 500 for the component split of a multi-component field.
-template <int P, int Q, int NC, int BX, int BY>
+// The kernels with whole wavefronts per element (fused_box_wave_kernel), EPW
+// elements per wavefront: code 400 (EPW 1) or 600 (EPW 2), plus 100 for the
+// component split of a multi-component field.
+template <int P, int Q, int NC, int BX, int BY, int EPW = 1>
 struct FusedBoxWaveReg {
   FusedBoxWaveReg() {
+    constexpr int base = EPW == 1 ? 400 : 600;
     if constexpr (NC == 1) {
-      reg<double, false>(400);
-      reg<float, false>(400);
+      reg<double, false>(base);
+      reg<float, false>(base);
     } else {
-      reg<double, true>(500);
-      reg<float, true>(500);
+      reg<double, true>(base + 100);
+      reg<float, true>(base + 100);
     }
   }
   template <class T, bool SPLIT>
   static void reg(int code) {
     const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, BX * BY + code};
-    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, 0, 0, SPLIT, true>, 0);
+    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW>, 0);
   }
 };
 

@@ -21,5 +21,10 @@ const FusedBoxReg<2, 3, 1, 4, 4> box_2_3_1_16;
 const FusedBoxReg<2, 3, 2, 4, 2> box_2_3_2_8;
 const FusedBoxReg<2, 3, 3, 4, 2> box_2_3_3_8;
 
+// One or two elements per wavefront (fused_box_wave_kernel).
+const FusedBoxWaveReg<2, 3, 1, 4, 4, 2> wave2_2_3_1_16;
+const FusedBoxWaveReg<2, 3, 2, 4, 2, 2> wave2_2_3_2_8;
+const FusedBoxWaveReg<2, 3, 3, 4, 2, 2> wave2_2_3_3_8;
+
 }  // namespace
 }  // namespace femhip

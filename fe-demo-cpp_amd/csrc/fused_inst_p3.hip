@@ -24,5 +24,10 @@ const FusedBoxReg<3, 4, 1, 2, 2> box_3_4_1_4;
 const FusedBoxReg<3, 4, 2, 4, 2> box_3_4_2_8;
 const FusedBoxReg<3, 4, 3, 4, 2> box_3_4_3_8;
 
+// One or two elements per wavefront (fused_box_wave_kernel).
+const FusedBoxWaveReg<3, 4, 1, 4, 4, 2> wave2_3_4_1_16;
+const FusedBoxWaveReg<3, 4, 2, 4, 2, 2> wave2_3_4_2_8;
+const FusedBoxWaveReg<3, 4, 3, 4, 2, 2> wave2_3_4_3_8;
+
 }  // namespace
 }  // namespace femhip

@@ -24,5 +24,10 @@ const FusedBoxReg<4, 5, 1, 2, 2> box_4_5_1_4;
 const FusedBoxReg<4, 5, 2, 2, 2> box_4_5_2_4;
 const FusedBoxReg<4, 5, 3, 2, 2> box_4_5_3_4;
 
+// One or two elements per wavefront (fused_box_wave_kernel).
+const FusedBoxWaveReg<4, 5, 1, 4, 4, 2> wave2_4_5_1_16;
+const FusedBoxWaveReg<4, 5, 2, 2, 2, 2> wave2_4_5_2_4;
+const FusedBoxWaveReg<4, 5, 3, 2, 2, 2> wave2_4_5_3_4;
+
 }  // namespace
 }  // namespace femhip

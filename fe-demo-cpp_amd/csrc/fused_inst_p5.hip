@@ -30,8 +30,11 @@ const FusedBoxReg<5, 6, 1, 2, 2> box_5_6_1_4;
 const FusedBoxReg<5, 6, 2, 4, 2> box_5_6_2_8;
 const FusedBoxReg<5, 6, 3, 4, 2> box_5_6_3_8;
 
-const FusedBoxWaveReg<5, 6, 1, 4, 4> wave_5_6_1_16;
-const FusedBoxWaveReg<5, 6, 1, 4, 2> wave_5_6_1_8;
+// One or two elements per wavefront (fused_box_wave_kernel).
+const FusedBoxWaveReg<5, 6, 1, 4, 4, 2> wave2_5_6_1_16;
+const FusedBoxWaveReg<5, 6, 1, 4, 2, 2> wave2_5_6_1_8;
+const FusedBoxWaveReg<5, 6, 2, 4, 2, 2> wave2_5_6_2_8;
+const FusedBoxWaveReg<5, 6, 3, 4, 2, 2> wave2_5_6_3_8;
 
 }  // namespace
 }  // namespace femhip

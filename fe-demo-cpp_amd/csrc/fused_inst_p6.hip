@@ -24,5 +24,11 @@ const FusedBoxReg<6, 7, 1, 2, 2> box_6_7_1_4;
 const FusedBoxReg<6, 7, 2, 2, 2> box_6_7_2_4;
 const FusedBoxReg<6, 7, 3, 2, 2> box_6_7_3_4;
 
+// One or two elements per wavefront (fused_box_wave_kernel).
+const FusedBoxWaveReg<6, 7, 1, 4, 2> wave_6_7_1_8;
+const FusedBoxWaveReg<6, 7, 1, 2, 2> wave_6_7_1_4;
+const FusedBoxWaveReg<6, 7, 2, 2, 2> wave_6_7_2_4;
+const FusedBoxWaveReg<6, 7, 3, 2, 2> wave_6_7_3_4;
+
 }  // namespace
 }  // namespace femhip

@@ -57,9 +57,12 @@ CASES = [((8, 8, 8), 5, 6, 1, 16, (0, 1, 2, 3, 8)),
          ((4, 2, 2), 8, 10, 1, 4, (0,))]
 # Shapes whose cross-section is compiled only as the component-split kernel.
 SPLIT_ONLY = {(8, 10, 3, 8)}
-# The wavefront-per-element kernels compiled (elems + 400; + 500 for the
-# component split of a multi-component field).
-WAVE = {(5, 6, 1, 16): 416, (5, 6, 1, 8): 408, (8, 10, 3, 8): 508, (8, 10, 3, 4): 504}
+# The kernels with whole wavefronts per element compiled (elems + 400: one
+# element per wavefront, + 600: two; + 100 for the component split of a
+# multi-component field).
+WAVE = {(5, 6, 1, 16): (616,), (5, 6, 1, 8): (608,), (5, 6, 3, 8): (708,), (2, 3, 3, 8): (708,),
+        (2, 3, 1, 16): (616,), (3, 4, 2, 8): (708,), (4, 5, 3, 4): (704,), (6, 7, 1, 8): (408,),
+        (8, 10, 3, 8): (508,), (8, 10, 3, 4): (504,), (8, 10, 1, 4): (404,)}
 
 
 @pytest.mark.parametrize("n3,P,Q,nc,elems,layers", CASES)
@@ -75,13 +78,13 @@ def test_box_column_matches_oracle(gpu, n3, P, Q, nc, elems, layers):
     d_u, d_u32 = fehip.DeviceArray.from_host(u), fehip.DeviceArray.from_host(u.astype(np.float32))
     # elems + 100: the kernel that loads qdata within its own layer; + 200: FP64
     # through the LDS ring (FP32 has no such variant: it always takes the ring);
-    # + 300 (several components): one component per workgroup; + 400 / + 500: one
-    # wavefront per element (without / with the component split).
+    # + 300 (several components): one component per workgroup; + 400 / + 600:
+    # one / two elements per wavefront (+ 100: with the component split).
     variants = (elems, elems + 100, elems + 200) + ((elems + 300,) if nc > 1 else ())
     if (P, Q, nc, elems) in SPLIT_ONLY:
         variants = (elems + 300,)
     if (P, Q, nc, elems) in WAVE:
-        variants += (WAVE[(P, Q, nc, elems)],)
+        variants += WAVE[(P, Q, nc, elems)]
     for lz, el in [(lz, el) for lz in layers for el in variants]:
         has32 = el // 100 != 2
         fehip.box_column_check(*n3, P, elems, lz)  # the index arithmetic, on the host

@@ -282,9 +282,18 @@ MatArg<T, EvenOdd<P, Q>::N> even_odd_arg(const void* basis) {
   return M;
 }
 
+// Even-odd coefficients read from memory through scalar loads (a uniform
+// address in the constant address space): for bases with many coefficients,
+// which would not fit the SGPRs as kernel arguments (P = 8, Q = 10: 40
+// doubles), each stage reloads them instead of holding them for the kernel.
+template <class T>
+struct EoLoads {
+  const __attribute__((address_space(4))) T* m;
+};
+
 // out[Q] = B x[P]
-template <int P, int Q, class T>
-__device__ __forceinline__ void eo_forward(const T (&x)[P], T (&out)[Q], const MatArg<T, EvenOdd<P, Q>::N>& M) {
+template <int P, int Q, class T, class CM = MatArg<T, EvenOdd<P, Q>::N>>
+__device__ __forceinline__ void eo_forward(const T (&x)[P], T (&out)[Q], const CM& M) {
   using E = EvenOdd<P, Q>;
   T xe[E::HP > 0 ? E::HP : 1], xo[E::HP > 0 ? E::HP : 1];
 #pragma unroll
@@ -310,8 +319,8 @@ __device__ __forceinline__ void eo_forward(const T (&x)[P], T (&out)[Q], const M
 }
 
 // out[P] = B^T y[Q]
-template <int P, int Q, class T>
-__device__ __forceinline__ void eo_transpose(const T (&y)[Q], T (&out)[P], const MatArg<T, EvenOdd<P, Q>::N>& M) {
+template <int P, int Q, class T, class CM = MatArg<T, EvenOdd<P, Q>::N>>
+__device__ __forceinline__ void eo_transpose(const T (&y)[Q], T (&out)[P], const CM& M) {
   using E = EvenOdd<P, Q>;
   T ye[E::QF > 0 ? E::QF : 1], yo[E::QF > 0 ? E::QF : 1];
 #pragma unroll
@@ -679,10 +688,23 @@ template <int P, int Q, int NC, class T, int BX, int BY, int SPLIT = 1, int EPW 
 __global__ __launch_bounds__(64 * BX * BY / EPW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void fused_box_wave_kernel(
     BoxGeom g, const T* __restrict__ qdata, const T* __restrict__ u, T* __restrict__ v, T* __restrict__ partials,
-    size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em) {
+    size_t part_comp_len, MatArg<T, EvenOdd<P, Q>::N> Em, const T* __restrict__ eo) {
   using S = Shape<P, Q>;
   constexpr int NE = BX * BY;
   constexpr int LPE = 64 / EPW;  // lanes per element
+  // More than 24 coefficients (P = 8): read each stage's from eo (the same
+  // values as Em) with scalar loads; the asm hides the pointer's origin so the
+  // loads stay in their stage.
+  constexpr bool kEoLoads = EvenOdd<P, Q>::N > 24;
+  auto coef = [&]() {
+    if constexpr (kEoLoads) {
+      auto p = (const __attribute__((address_space(4))) T*)eo;
+      asm volatile("" : "+s"(p));
+      return EoLoads<T>{p};
+    } else {
+      return Em;
+    }
+  };
   static_assert((EPW == 1 || EPW == 2) && NE % EPW == 0, "one or two elements per wavefront");
   constexpr int P2 = S::P2, Q2 = S::Q2, Q3 = S::Q3, PQ = P * Q;
   constexpr int RY = BoxTile<P, Q>::RY, ZS = BoxTile<P, Q>::ZS, FB = BoxTile<P, Q>::FBUF;
@@ -771,7 +793,7 @@ void fused_box_wave_kernel(
       }
       if (row_lane) {
         T y[Q];
-        eo_forward<P, Q, T>(x, y, Em);
+        eo_forward<P, Q, T>(x, y, coef());
         const int row = riz * ZS + riy * RY;
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) b[row + qx] = y[qx];
@@ -786,7 +808,7 @@ void fused_box_wave_kernel(
           T xx[P], y[Q];
 #pragma unroll
           for (int iy = 0; iy < P; ++iy) xx[iy] = b[col + iy * RY];
-          eo_forward<P, Q, T>(xx, y, Em);
+          eo_forward<P, Q, T>(xx, y, coef());
 #pragma unroll
           for (int qy = 0; qy < Q; ++qy) b[col + qy * RY] = y[qy];
         }
@@ -801,10 +823,11 @@ void fused_box_wave_kernel(
           T xx[P], d[Q];
 #pragma unroll
           for (int iz = 0; iz < P; ++iz) xx[iz] = b[iz * ZS + line];
-          eo_forward<P, Q, T>(xx, d, Em);
+          const auto cz = coef();
+          eo_forward<P, Q, T>(xx, d, cz);
 #pragma unroll
           for (int qz = 0; qz < Q; ++qz) d[qz] *= qd[j][qz];
-          eo_transpose<P, Q, T>(d, xx, Em);
+          eo_transpose<P, Q, T>(d, xx, cz);
 #pragma unroll
           for (int iz = 0; iz < P; ++iz) b[iz * ZS + line] = xx[iz];
         }
@@ -819,7 +842,7 @@ void fused_box_wave_kernel(
           T y[Q], xx[P];
 #pragma unroll
           for (int qy = 0; qy < Q; ++qy) y[qy] = b[col + qy * RY];
-          eo_transpose<P, Q, T>(y, xx, Em);
+          eo_transpose<P, Q, T>(y, xx, coef());
 #pragma unroll
           for (int iy = 0; iy < P; ++iy) b[col + iy * RY] = xx[iy];
         }
@@ -831,7 +854,7 @@ void fused_box_wave_kernel(
         T y[Q], xx[P];
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) y[qx] = b[row + qx];
-        eo_transpose<P, Q, T>(y, xx, Em);
+        eo_transpose<P, Q, T>(y, xx, coef());
 #pragma unroll
         for (int ix = 0; ix < P; ++ix) b[row + ix] = xx[ix];
       }
@@ -928,12 +951,14 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
   if (a.box_pass == 0) {
     constexpr int threads = round64(BX * BY * S::TPE);
     if constexpr (WAVE > 0) {  // WAVE: elements per wavefront
+      require(a.basis_eo != nullptr, "fused mass: the box kernel's even-odd coefficients are not on the device");
       constexpr int NCK = SPLIT ? 1 : NC, SP = SPLIT ? NC : 1;
       require(nseg * SP < (1ll << 31), "fused mass: box too large");
       fused_box_wave_kernel<P, Q, NCK, T, BX, BY, SP, WAVE, WPE>
           <<<static_cast<unsigned>(nseg * SP), 64 * BX * BY / WAVE, 0, s>>>(
           g, static_cast<const T*>(a.qdata), static_cast<const T*>(a.u), static_cast<T*>(a.v),
-          static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis));
+          static_cast<T*>(a.partials), a.part_comp_len, even_odd_arg<P, Q, T>(a.basis),
+          static_cast<const T*>(a.basis_eo));
     } else if constexpr (SPLIT) {
       require(nseg * NC < (1ll << 31), "fused mass: box too large");
       fused_box_kernel<P, Q, 1, T, BX, BY, UM, QA, NC><<<static_cast<unsigned>(nseg * NC), threads, 0, s>>>(

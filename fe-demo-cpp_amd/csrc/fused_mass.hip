@@ -566,6 +566,50 @@ DevBuf<float>& box_partials<float>(FusedMass& f) {
   return f.box_part32;
 }
 
+template <class T>
+DevBuf<T>& box_eo(FusedMass& f);
+template <>
+DevBuf<double>& box_eo<double>(FusedMass& f) {
+  return f.box_eo64;
+}
+template <>
+DevBuf<float>& box_eo<float>(FusedMass& f) {
+  return f.box_eo32;
+}
+
+// interp_1d B [Q][P] packed as the box kernels' even-odd coefficients
+// (the layout and arithmetic of even_odd_arg in fused_box_kernel.hpp).
+template <class T>
+std::vector<T> even_odd_table(int P, int Q, const T* B) {
+  const int HP = P / 2, HQ = (Q + 1) / 2, QF = Q / 2, PM = P & 1;
+  const int BM = HQ * HP, BO = HQ * HP + PM * HQ;
+  std::vector<T> m(static_cast<size_t>(BO + QF * HP));
+  for (int q = 0; q < HQ; ++q)
+    for (int j = 0; j < HP; ++j) m[q * HP + j] = (B[q * P + j] + B[q * P + P - 1 - j]) / T(2);
+  if (PM)
+    for (int q = 0; q < HQ; ++q) m[BM + q] = B[q * P + P / 2];
+  for (int q = 0; q < QF; ++q)
+    for (int j = 0; j < HP; ++j) m[BO + q * HP + j] = (B[q * P + j] - B[q * P + P - 1 - j]) / T(2);
+  return m;
+}
+
+template <class T>
+const T* box_eo_table(FusedMass& f) {
+  DevBuf<T>& d = box_eo<T>(f);
+  if (d.empty()) {
+    const std::vector<T>& B = [&]() -> const std::vector<T>& {
+      if constexpr (std::is_same<T, float>::value) {
+        return f.B32;
+      } else {
+        return f.B64;
+      }
+    }();
+    d.upload(even_odd_table<T>(f.P, f.Q, B.data()));
+    ++f.gen;
+  }
+  return d.data();
+}
+
 // Per-component length of the box-column partial area for the key's variant.
 size_t box_part_len(const FusedMass& f, const FusedKey& k) {
   int BX = 0, BY = 0;
@@ -608,6 +652,7 @@ void prepare_key(FusedMass& f, const FusedKey& k, hipStream_t s) {
   if (alg_box(k.alg)) {
     require(box_fits(f, k.elems), "fused mass: the box-column algorithm needs a structured box its columns tile");
     (void)elem_qdata<T>(f, s);
+    (void)box_eo_table<T>(f);
     DevBuf<T>& pb = box_partials<T>(f);
     const size_t n = std::max<size_t>(box_part_len(f, k), 1) * f.NC;
     if (pb.size() < n) {
@@ -668,6 +713,7 @@ void apply_key(FusedMass& f, const FusedKey& k, const T* u, T* v, hipStream_t s,
     a.v = v;
     a.partials = box_partials<T>(f).data();
     a.part_comp_len = std::max<size_t>(box_part_len(f, k), 1);
+    a.basis_eo = box_eo<T>(f).data();
     if (ev) HIP_CHECK(hipEventRecord(ev->b, s));
     a.box_pass = 0;
     L(a, s);
@@ -1040,6 +1086,9 @@ void fused_mass_apply64_scratch(FusedMass& f, const double* u, double* v, void* 
     a.v = v;
     a.partials = scratch;  // every slot is written before the second pass reads it: no clearing
     a.part_comp_len = std::max<size_t>(box_part_len(f, k), 1);
+    require(!f.box_eo64.empty(),
+            "apply_noalloc: run one ordinary apply first (it puts the box kernels' coefficients on the device)");
+    a.basis_eo = f.box_eo64.data();
     a.box_pass = 0;
     L(a, s);
     a.box_pass = 1;

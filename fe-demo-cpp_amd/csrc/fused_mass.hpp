@@ -84,6 +84,7 @@ struct FusedArgs {
   const int* part_ptr = nullptr;   // [nblk + 1]
   void* v = nullptr;               // block atomic: pre-zeroed v; block part: v (interior nodes stored)
   void* partials = nullptr;        // block part: [NC][npart]; box column: [NC][segments * face slots]
+  const void* basis_eo = nullptr;  // box column: the even-odd coefficients of interp_1d on the device (T)
   size_t part_comp_len = 0;
   // box column: elements per direction, segments per column, pass (0 the column kernel, 1 the shared-node sums)
   int box_n[3] = {0, 0, 0};
@@ -149,6 +150,8 @@ struct FusedMass {
   bool basis_symmetric = false;  // B(Q-1-q, P-1-p) = B(q, p) to rounding (the box kernel's even-odd form)
   int box_n[3] = {0, 0, 0};
   int box_lz = 0;             // most element layers per box-column segment (0: box_segments chooses)
+  DevBuf<double> box_eo64;    // box columns: interp_1d's even-odd coefficients on the device (lazy)
+  DevBuf<float> box_eo32;
   DevBuf<double> box_part64;  // box-column partials (lazy)
   DevBuf<float> box_part32;
   int scatter = kScatterAtomic;  // requested: atomic (any order) or deterministic (reproducible)

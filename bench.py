@@ -184,6 +184,24 @@ def slab_rel_err(gpu_vec, cpu_slab, l_s, rows, nc, N):
     return float(np.max(np.abs(g - c_)) / np.max(np.abs(c_)))
 
 
+def box_kernel_name(P, Q, nc, fp32, ne):
+    """The demangled name of the box-column kernel a packing (elems) runs: the
+    cross-section (elems % 100) and the variant (elems // 100: 0 qdata a layer
+    ahead, 1 qdata in its own layer, 2 FP64 through the LDS ring, 3 the
+    component split, 4 / 6 one / two elements per wavefront, 5 / 7 the same
+    with the component split)."""
+    bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne % 100]
+    t, code = "float" if fp32 else "double", ne // 100
+    if code >= 4:
+        split = code in (5, 7)
+        return "fused_box_wave_kernel<%d, %d, %d, %s, %d, %d, %d, %d, 0>" % (
+            P, Q, 1 if split else nc, t, bx, by, nc if split else 1, 1 if code < 6 else 2)
+    um = 2 if fp32 or code == 2 else 0
+    if code == 3:
+        return "fused_box_kernel<%d, %d, 1, %s, %d, %d, %d, 0, %d>" % (P, Q, t, bx, by, um, nc)
+    return "fused_box_kernel<%d, %d, %d, %s, %d, %d, %d, %d, 1>" % (P, Q, nc, t, bx, by, um, 0 if code == 1 else 1)
+
+
 def config5(fehip, args, stream, layers=2):
     """48^3 hexes, P=8, Q=10, 3 components (114.8M DoF): the FP64 and FP32 applies
     of each algorithm (GDoF/s, %HBM), the FP64 apply's error against the oracle's
@@ -226,9 +244,14 @@ def config5(fehip, args, stream, layers=2):
         p5.op.set_fused_algorithm(alg, packing.get((dt == np.float32, alg), (0,))[0])
         m, k, a = time_applies(fehip, p5.op, d_in[dt], d_out[dt], steps, args.warmup, stream)
         sz = 4 if dt == np.float32 else 8
+        ne5 = p5.op.fused_choice(dt == np.float32)[1]
         big[label] = {"ms_per_apply": m, "gdofs": L5 / m / 1e6, "kernel_ms": k, "other_stages_ms": a,
-                      "elems": p5.op.fused_choice(dt == np.float32)[1],
+                      "elems": ne5,
                       "apply_frac_hbm": p5.apply_bytes(sz, sz, sz) / (m * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if alg == fehip.FUSED_BOX_COLUMN:  # the kernel reads u and qdata and writes v (no offsets)
+            kb = sz * (2 * L5 + p5.num_elem * p5.Q ** 3)
+            big[label].update({"kernel": box_kernel_name(p5.P, p5.Q, p5.nc, dt == np.float32, ne5),
+                               "kernel_bytes": kb, "kernel_frac_hbm": kb / (k * 1e-3) / 1e9 / HBM_PEAK_GBS})
         log("config5", label, json.dumps(big[label]))
     best = min((v["ms_per_apply"], k2) for k2, v in big.items() if k2.startswith("fp64"))[1]
     p5.op.set_fused_options(D, exact=0)
@@ -423,12 +446,7 @@ def main():
             kernel_names[label] = "fused_mass_%s_kernel<%d, %d, 1, %s, %d, %s>" % (
                 "exact" if exact else "fast", P, Q, tname, ne, "true" if alg == fehip.FUSED_ELEM_TMAP else "false")
         elif alg == fehip.FUSED_BOX_COLUMN:
-            bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne % 100]
-            # the last three template arguments: how u reaches the x-rows (2: the LDS
-            # ring, FP32 and FP64 elems >= 200; 0: rows), whether qdata is loaded a
-            # layer ahead (1; 0 for elems 100..199) and the component split (1)
-            kernel_names[label] = "fused_box_kernel<%d, %d, 1, %s, %d, %d, %d, %d, 1>" % (
-                P, Q, tname, bx, by, 2 if fp32 or ne >= 200 else 0, 0 if ne // 100 == 1 else 1)
+            kernel_names[label] = box_kernel_name(P, Q, 1, fp32, ne)
         else:
             kernel_names[label] = {
                 fehip.FUSED_BLOCK_PARTIAL: "fused_mass_part_kernel<%d, %d, 1, %s, %d, false>",

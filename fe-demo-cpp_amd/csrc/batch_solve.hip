@@ -288,8 +288,9 @@ struct Cohort {
   double* d_sc = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<BatchJob*> members;  // in table order while a replay is in flight
-  std::vector<BatchJob*> joining;  // started, waiting for a table entry
+  std::vector<BatchJob*> members;   // the runs in the cohort
+  std::vector<BatchJob*> launched;  // the table of the replay in flight (h_sc's order)
+  std::vector<BatchJob*> joining;   // started, waiting for a table entry
   bool in_flight = false;
   long long replays = 0;
 
@@ -400,6 +401,7 @@ struct Cohort {
     HIP_CHECK(hipGraphLaunch(exec, s));
     HIP_CHECK(hipMemcpyAsync(h_sc, d_sc, sizeof(double) * kBatchSlots * kNSCAL, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(ev, s));
+    launched = members;
     in_flight = true;
     ++replays;
   }
@@ -635,16 +637,14 @@ class BatchEngine {
       HIP_CHECK(e);
       c.in_flight = false;
       progressed = true;
-      // h_sc holds the scalars of the runs in the order they were launched.
-      const std::vector<BatchJob*> launched = c.members;
-      std::vector<BatchJob*> stay;
+      // h_sc holds the scalars of the runs in the order they were launched
+      // (a run that has failed since has left the members and is skipped).
+      const std::vector<BatchJob*> launched = c.launched;
       for (size_t i = 0; i < launched.size(); ++i) {
         BatchJob& j = *launched[i];
+        if (std::find(c.members.begin(), c.members.end(), &j) == c.members.end()) continue;
         const double* sc = c.h_sc + i * kNSCAL;
-        if (sc[kDONE] == 0.0) {
-          stay.push_back(&j);
-          continue;
-        }
+        if (sc[kDONE] == 0.0) continue;
         c.members.erase(std::remove(c.members.begin(), c.members.end(), &j), c.members.end());
         try {
           on_run_done(j, sc);
@@ -652,7 +652,6 @@ class BatchEngine {
           fail(j, ex);
         }
       }
-      c.members = stay;
     }
     while (!c.joining.empty() && c.members.size() < static_cast<size_t>(kBatchSlots)) {
       BatchJob* j = c.joining.front();

@@ -152,8 +152,9 @@ struct BatchJob {
   Phase phase = kStart;
   bool in_flight = false;
   hipEvent_t ev = nullptr;
-  double* hs = nullptr;  // pinned: [0] ||b - M x||^2
-  double bnorm = 0.0;
+  hipEvent_t ready = nullptr;  // the end of the job's setup on its creation stream
+  double* hs = nullptr;        // pinned: [0] ||b - M x||^2, [1] ||b||^2
+  double bnorm = -1.0;         // ||b||, from hs[1] once a residual has been read back
   int outer = 0, inner_total = 0;
   SolveStats stats;
   int status = 0;
@@ -162,6 +163,7 @@ struct BatchJob {
   std::chrono::steady_clock::time_point t0;
   ~BatchJob() {
     if (ev) (void)hipEventDestroy(ev);
+    if (ready) (void)hipEventDestroy(ready);
     if (hs) pinned_words().release(hs);
   }
 };
@@ -229,12 +231,14 @@ void enqueue_residual(BatchJob& j, float* r32, hipStream_t s) {
   residual_kernel<<<nb, kThreads, 0, s>>>(j.n, j.b, j.Mx.data(), j.r64.data(), r32, j.rpart.data());
   sum_to_scalar_kernel<<<1, kThreads, 0, s>>>(j.rpart.data(), j.rscal.data(), nb);
   HIP_CHECK_LAUNCH();
-  HIP_CHECK(hipMemcpyAsync(j.hs, j.rscal.data(), sizeof(double), hipMemcpyDeviceToHost, s));
+  // With ||b||^2, which setup left in rscal[1].
+  HIP_CHECK(hipMemcpyAsync(j.hs, j.rscal.data(), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipEventRecord(j.ev, s));
   j.in_flight = true;
 }
 
-double rel_of(const BatchJob& j) {
+double rel_of(BatchJob& j) {
+  if (j.bnorm < 0) j.bnorm = std::sqrt(std::max(j.hs[1], 0.0));
   const double rr = std::sqrt(std::max(j.hs[0], 0.0));
   return j.bnorm > 0 ? rr / j.bnorm : rr;
 }
@@ -441,7 +445,7 @@ BatchJob* batch_job_create(Operator& op, const double* b, double* x, const Solve
   }
   if (!fused_split(*f, j->fp32, &j->sp, s) || j->sp.node_row != nullptr)
     throw Unsupported("batch: the operator's transpose map does not cover every node in order");
-  DevBuf<double> diag(n);
+  StreamTmp<double> diag(n, s);  // freed in s's order: nothing here waits for the device
   fused_mass_diagonal(*f, diag.data(), s);
   if (j->fp32) {
     j->v32 = std::make_unique<RunVecs<float>>(n);
@@ -456,16 +460,17 @@ BatchJob* batch_job_create(Operator& op, const double* b, double* x, const Solve
   j->Mx.allocate(n);
   j->r64.allocate(n);
   j->rpart.allocate(kMaxRed);
-  j->rscal.allocate(1);
+  j->rscal.allocate(2);
   HIP_CHECK(hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&j->ready, hipEventDisableTiming));
   j->hs = pinned_words().acquire();
   const int nb = red_blocks(n);
   dot_partial_kernel<double><<<nb, kThreads, 0, s>>>(n, b, b, j->rpart.data());
-  sum_to_scalar_kernel<<<1, kThreads, 0, s>>>(j->rpart.data(), j->rscal.data(), nb);
+  sum_to_scalar_kernel<<<1, kThreads, 0, s>>>(j->rpart.data(), j->rscal.data() + 1, nb);
   HIP_CHECK_LAUNCH();
-  HIP_CHECK(hipMemcpyAsync(j->hs, j->rscal.data(), sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  j->bnorm = std::sqrt(std::max(j->hs[0], 0.0));
+  // No host wait: the engine orders the cohort's stream after this event, and
+  // ||b||^2 comes back with the job's first residual.
+  HIP_CHECK(hipEventRecord(j->ready, s));
   return j.release();
 }
 
@@ -580,6 +585,7 @@ class BatchEngine {
     Cohort& c = cohort_for(j);
     j.cohort = &c;
     j.t0 = std::chrono::steady_clock::now();
+    HIP_CHECK(hipStreamWaitEvent(c.s, j.ready, 0));  // the job's setup has run
     if (!j.fp32) {
       enqueue_run_start<double>(j, j.b, j.x, j.opt.rtol, c.s);
       j.phase = BatchJob::kJoin;

@@ -77,8 +77,10 @@ bool batch_supported(int P, int Q, int NC, int mode);
 // refinement (mode 1).
 struct BatchJob;
 // Prepares everything the solve of M x = b needs (the operator set to the
-// transpose-map algorithm, the Jacobi inverse, the run vectors, ||b||) on `s`
-// and synchronizes it. b and x must stay valid until the job is destroyed.
+// transpose-map algorithm, the Jacobi inverse, the run vectors, ||b||^2) on
+// `s` without waiting for it: the engine orders the job's work after what `s`
+// holds when this returns (b and the operator's data made there included).
+// b and x must stay valid until the job is destroyed.
 BatchJob* batch_job_create(Operator& op, const double* b, double* x, const SolveOptions& opt, hipStream_t s);
 const SolveStats& batch_job_stats(const BatchJob& j);
 // 0, or the C ABI status of the failure (its message in batch_job_error).

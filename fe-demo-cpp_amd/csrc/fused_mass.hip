@@ -818,6 +818,11 @@ __global__ __launch_bounds__(kThreads) void box_check_kernel(size_t nt, int P, i
 
 // detect_box on the restriction's device copy of the offsets.
 bool detect_box_device(const Restriction& r, int P, int num_comp, int* n3, hipStream_t s) {
+  if (r.box_n[0] > 0 && r.type == 0 && r.elem_size == P * P * P && r.num_comp == num_comp) {
+    // Made by the box generator: its offsets are the box formula (nothing to read back).
+    for (int d = 0; d < 3; ++d) n3[d] = r.box_n[d];
+    return true;
+  }
   long long n3l[3];
   // The shape follows from offsets 0, P and P*P (box_shape); the device check
   // below tests all of them.
@@ -1296,18 +1301,20 @@ TuneEntry fused_autotune(FusedMass& f, int fp32, int reps, hipStream_t s, std::v
 void fused_mass_diagonal(FusedMass& f, double* diag, hipStream_t s) {
   ensure_tmap(f);
   if (!f.tm_covers_all) zero_fill(diag, static_cast<size_t>(f.l_size) * sizeof(double), s);
-  StreamTmp<double> B2(f.Bsq.size(), s);
-  HIP_CHECK(hipMemcpyAsync(B2.data(), f.Bsq.data(), f.Bsq.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  if (f.Bsq_dev.size() != f.Bsq.size()) {
+    // From the operator's own host copy, which outlives the copy (nothing waits here).
+    f.Bsq_dev.allocate(f.Bsq.size());
+    HIP_CHECK(hipMemcpyAsync(f.Bsq_dev.data(), f.Bsq.data(), f.Bsq.size() * sizeof(double), hipMemcpyHostToDevice,
+                             s));
+  }
   const size_t n = static_cast<size_t>(f.num_elem) * f.P * f.P * f.P;
   StreamTmp<double> Yd(n, s);
-  mass_diag_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(f.num_elem, f.P, f.Q, f.qdata64.data(), B2.data(),
-                                                               Yd.data());
+  mass_diag_kernel<<<grid_for(n, kThreads), kThreads, 0, s>>>(f.num_elem, f.P, f.Q, f.qdata64.data(),
+                                                               f.Bsq_dev.data(), Yd.data());
   HIP_CHECK_LAUNCH();
   // Every component gets the same diagonal (value stride 0 across components).
   launch_row_sum<double>(f.tm_nrows, f.NC, f.comp_stride, f.tm_rows_p, f.tm_ptr_p, f.tm_idx_p,
                          Yd.data(), 0, diag, s);
-  // The host copy of Bsq must outlive the async copy.
-  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace femhip

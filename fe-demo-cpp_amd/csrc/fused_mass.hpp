@@ -133,6 +133,7 @@ struct FusedMass {
   std::vector<double> B64;            // interp_1d [Q][P]
   std::vector<float> B32;
   std::vector<double> Bsq;            // interp_1d squared entry-wise (Jacobi diagonal)
+  DevBuf<double> Bsq_dev;             // its device copy (made by the first diagonal; after Bsq, so freed first)
   std::map<std::pair<int, int>, std::unique_ptr<DevPlan>> plans;  // by (NB, node budget)
   DevBuf<double> Y64;                 // element outputs of the transpose-map algorithm (lazy)
   DevBuf<float> Y32;

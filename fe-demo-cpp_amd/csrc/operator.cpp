@@ -99,7 +99,7 @@ Operator* operator_create(const QFunctionDev& qf) {
 // src/operator.cpp:44-117
 void operator_set_field(Operator& op, const std::string& name, std::shared_ptr<const Restriction> r,
                         std::shared_ptr<const Basis> b, int vkind, const double* passive, size_t passive_len,
-                        bool passive_on_device) {
+                        bool passive_on_device, hipStream_t copy_stream) {
   const QFField* qf_field = nullptr;
   OpField* slot = nullptr;
   bool is_input = true;
@@ -163,12 +163,17 @@ void operator_set_field(Operator& op, const std::string& name, std::shared_ptr<c
   f.vkind = vkind;
   if (vkind == kVectorPassive) {
     f.passive.allocate(passive_len);
-    SharedCapture lk;
-    HIP_CHECK(hipMemcpy(f.passive.data(), passive, passive_len * sizeof(double),
-                        passive_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
-    // A device-to-device hipMemcpy may return before the copy is done: finish it,
-    // so the caller may reuse the source and other streams may read the copy.
-    if (passive_on_device) HIP_CHECK(hipStreamSynchronize(nullptr));
+    if (passive_on_device && copy_stream) {
+      HIP_CHECK(hipMemcpyAsync(f.passive.data(), passive, passive_len * sizeof(double), hipMemcpyDeviceToDevice,
+                               copy_stream));
+    } else {
+      SharedCapture lk;
+      HIP_CHECK(hipMemcpy(f.passive.data(), passive, passive_len * sizeof(double),
+                          passive_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+      // A device-to-device hipMemcpy may return before the copy is done: finish it,
+      // so the caller may reuse the source and other streams may read the copy.
+      if (passive_on_device) HIP_CHECK(hipStreamSynchronize(nullptr));
+    }
   }
   if (vkind == kVectorActive) (is_input ? op.active_in_l_size : op.active_out_l_size) = f.rstr->l_size;
   op.num_qpts = nq;

@@ -42,9 +42,12 @@ struct Operator {
 };
 
 Operator* operator_create(const QFunctionDev& qf);
+// A passive device array is copied at once (complete on return), or with
+// copy_stream set, in that stream's order (the caller keeps the source alive
+// until the copy has run and orders later users of the operator after it).
 void operator_set_field(Operator& op, const std::string& name, std::shared_ptr<const Restriction> r,
                         std::shared_ptr<const Basis> b, int vkind, const double* passive, size_t passive_len,
-                        bool passive_on_device);
+                        bool passive_on_device, hipStream_t copy_stream = nullptr);
 void operator_check_ready(const Operator& op);
 void operator_apply(Operator& op, const double* in, size_t in_len, double* out, size_t out_len, hipStream_t s);
 void operator_apply_f32(Operator& op, const float* in, size_t in_len, float* out, size_t out_len, hipStream_t s);

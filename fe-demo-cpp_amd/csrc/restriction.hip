@@ -195,7 +195,9 @@ void exclusive_scan(const int* in, int* out, int n, hipStream_t s) {
 }
 
 // Builds r's transpose map from r.offsets (on the device) on stream s.
-void build_transpose_map_device(Restriction& r, hipStream_t s) {
+// known_rows >= 0: the caller knows every L-node is used (rows = l_size), so
+// the row count is not read back; with sync = false nothing waits for s.
+void build_transpose_map_device(Restriction& r, hipStream_t s, int known_rows = -1, bool sync = true) {
   const size_t nt = static_cast<size_t>(r.num_elem) * r.elem_size;
   const int L = r.l_size;
   StreamTmp<int> count(static_cast<size_t>(L), s), used(static_cast<size_t>(L), s),
@@ -206,9 +208,11 @@ void build_transpose_map_device(Restriction& r, hipStream_t s) {
   HIP_CHECK_LAUNCH();
   exclusive_scan(used.data(), row_of.data(), L, s);
   exclusive_scan(count.data(), start.data(), L, s);
-  int rows = 0;
-  HIP_CHECK(hipMemcpyAsync(&rows, row_of.data() + L, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  int rows = known_rows;
+  if (rows < 0) {
+    HIP_CHECK(hipMemcpyAsync(&rows, row_of.data() + L, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
   r.num_nodes = rows;
   r.identity_nodes = (rows == L);
   if (!r.identity_nodes) r.l_vec_indices.allocate(static_cast<size_t>(rows));
@@ -233,7 +237,7 @@ void build_transpose_map_device(Restriction& r, hipStream_t s) {
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.data(), bytes, r.offsets.data(), keys_out.data(), t_in.data(),
                                                  r.t_indices.data(), static_cast<int>(nt), 0, bits, s));
   }
-  HIP_CHECK(hipStreamSynchronize(s));
+  if (sync) HIP_CHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace
@@ -317,7 +321,8 @@ Restriction* restriction_create_offset(int num_elem, int elem_size, int num_comp
   return r;
 }
 
-Restriction* restriction_create_box(int nx, int ny, int nz, int P, int num_comp, int scatter_mode, hipStream_t s) {
+Restriction* restriction_create_box(int nx, int ny, int nz, int P, int num_comp, int scatter_mode, hipStream_t s,
+                                    bool sync) {
   require(nx > 0 && ny > 0 && nz > 0 && P >= 2 && num_comp > 0, "elem_restriction_create: bad box");
   require(scatter_mode == kScatterAtomic || scatter_mode == kScatterDeterministic,
           "elem_restriction_create: unknown scatter mode");
@@ -336,11 +341,14 @@ Restriction* restriction_create_box(int nx, int ny, int nz, int P, int num_comp,
     r->e_size = static_cast<size_t>(num_comp) * nt;
     r->scatter_mode = scatter_mode;
     r->max_offset = static_cast<int>(N - 1);
+    r->box_n[0] = nx;
+    r->box_n[1] = ny;
+    r->box_n[2] = nz;
     r->offsets.allocate(static_cast<size_t>(nt));
     box_offsets_device(nx, ny, nz, P, 1, r->offsets.data(), s);
     if (scatter_mode == kScatterDeterministic) {
-      build_transpose_map_device(*r, s);  // synchronizes s
-    } else {
+      build_transpose_map_device(*r, s, static_cast<int>(N), sync);  // every node of a box is used
+    } else if (sync) {
       HIP_CHECK(hipStreamSynchronize(s));
     }
   } catch (...) {
@@ -399,6 +407,7 @@ Restriction* restriction_clone(const Restriction& src) {
     r->num_nodes = src.num_nodes;
     r->identity_nodes = src.identity_nodes;
     r->max_offset = src.max_offset;
+    for (int d = 0; d < 3; ++d) r->box_n[d] = src.box_n[d];
     auto dup = [](const DevBuf<int>& from, DevBuf<int>& to) {
       to.allocate(from.size());
       SharedCapture lk;

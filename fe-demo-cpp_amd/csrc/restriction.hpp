@@ -38,6 +38,9 @@ struct Restriction {
   const std::vector<int>& host_offsets() const;
 
   int max_offset = 0;  // largest offset (offset restrictions)
+  // Made by restriction_create_box: the box's elements per direction (the
+  // offsets are the box formula by construction); 0 otherwise.
+  int box_n[3] = {0, 0, 0};
 
   // Deterministic offset restrictions only (built on the device: the same map
   // as build_transpose_map).
@@ -61,8 +64,11 @@ Restriction* restriction_create_strided(int num_elem, int elem_size, int num_com
 // lexicographic node numbering and blocked components (comp_stride = node
 // count, l_size = num_comp * node count), its offsets made on the device (valid
 // by construction: no host validation or upload); the deterministic map is
-// built on s, which is synchronized before the call returns.
-Restriction* restriction_create_box(int nx, int ny, int nz, int P, int num_comp, int scatter_mode, hipStream_t s);
+// built on s, which is synchronized before the call returns unless `sync` is
+// false (then the device arrays are complete in s's order only: a caller
+// that uses them on another stream orders that stream after s).
+Restriction* restriction_create_box(int nx, int ny, int nz, int P, int num_comp, int scatter_mode, hipStream_t s,
+                                    bool sync = true);
 // Deep copy (device arrays duplicated), for operators that store restrictions by value.
 Restriction* restriction_clone(const Restriction& r);
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -107,6 +108,10 @@ struct Job {
   hipStream_t stream = nullptr;  // from `streams`
   StreamPool* streams = nullptr;
   std::unique_ptr<Operator> op;
+  // The build_mass operator, the vertex coordinates and the qdata it made:
+  // kept until teardown (freeing a device block waits for its stream).
+  std::unique_ptr<Operator> bop;
+  DevBuf<double> coords, qdata;
   DevBuf<double> b, x;
   MassSolverState* state = nullptr;
   SolveJob* solve = nullptr;
@@ -117,6 +122,9 @@ struct Job {
     solve_job_destroy(solve);
     mass_solver_state_destroy(state);
     op.reset();
+    bop.reset();
+    coords.release();
+    qdata.release();
     b.release();
     x.release();
     if (stream) {
@@ -135,20 +143,45 @@ void validate(const ServeRequest& r) {
   require(ipow3(static_cast<long long>(r.n) * (r.P - 1) + 1) * r.num_comp < (1ll << 31), "serve: L-vector too large");
 }
 
+// The H1 Lagrange basis of a shape (GLL nodes, Gauss points), made once per
+// process and shared by every request of that shape: a basis is immutable,
+// and making one uploads its matrices (each upload waits for the device,
+// which under load means waiting behind other requests' solves). Its device
+// arrays are plain allocations, not blocks of a job's cache.
+std::shared_ptr<const Basis> shape_basis(int dim, int nc, int P, int Q) {
+  static std::mutex* mu = new std::mutex();  // never destroyed: the runtime may be gone at exit
+  static auto* cache = new std::map<std::array<int, 4>, std::shared_ptr<const Basis>>();
+  std::lock_guard<std::mutex> lk(*mu);
+  std::shared_ptr<const Basis>& slot = (*cache)[{dim, nc, P, Q}];
+  if (!slot) {
+    const CacheScope* prev = cache_scope();
+    cache_scope() = nullptr;
+    try {
+      slot.reset(basis_create(make_h1_lagrange(dim, nc, P, Q, 0)));
+    } catch (...) {
+      cache_scope() = prev;
+      throw;
+    }
+    cache_scope() = prev;
+  }
+  return slot;
+}
+
 // Mesh, restrictions, qdata (build_mass on the device) and the fused mass
-// operator of a request, as tests/test-operator.cpp:17-24,75-112 set them up.
-std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s) {
+// operator of a request, as tests/test-operator.cpp:17-24,75-112 set them up,
+// enqueued on s with no wait for the device: later work on the job's data is
+// ordered after s (the build operator, coordinates and qdata go into the job).
+std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s, Job& job) {
   const int n = r.n, P = r.P, Q = r.Q, nc = r.num_comp;
-  const int E = n * n * n, P3 = P * P * P, Q3 = Q * Q * Q, nv = (n + 1) * (n + 1) * (n + 1);
+  const int E = n * n * n, Q3 = Q * Q * Q, nv = (n + 1) * (n + 1) * (n + 1);
   // Offsets (node and mesh restrictions) and the jittered vertex coordinates
   // are made on the device: entry for entry what box_offsets and box_coords
   // make on the host, with no host arrays, validation or uploads.
-  (void)P3;
-  std::shared_ptr<const Restriction> rstr(restriction_create_box(n, n, n, P, nc, kScatterDeterministic, s));
-  std::shared_ptr<const Restriction> mrstr(restriction_create_box(n, n, n, 2, 3, kScatterAtomic, s));
+  std::shared_ptr<const Restriction> rstr(restriction_create_box(n, n, n, P, nc, kScatterDeterministic, s, false));
+  std::shared_ptr<const Restriction> mrstr(restriction_create_box(n, n, n, 2, 3, kScatterAtomic, s, false));
   std::shared_ptr<const Restriction> qrstr(restriction_create_strided(E, Q3, 1, E * Q3, nullptr));
-  std::shared_ptr<const Basis> basis(basis_create(make_h1_lagrange(3, nc, P, Q, 0)));
-  std::shared_ptr<const Basis> mbasis(basis_create(make_h1_lagrange(3, 3, 2, Q, 0)));
+  std::shared_ptr<const Basis> basis = shape_basis(3, nc, P, Q);
+  std::shared_ptr<const Basis> mbasis = shape_basis(3, 3, 2, Q);
 
   QFunctionDev bq;
   bq.kind = kQfBuildMass;
@@ -158,13 +191,15 @@ std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s) {
   const int dim = 3;
   bq.ctx.resize(sizeof(int));
   std::memcpy(bq.ctx.data(), &dim, sizeof(int));
-  std::unique_ptr<Operator> bop(operator_create(bq));
-  operator_set_field(*bop, "dx", mrstr, mbasis, kVectorActive, nullptr, 0, false);
-  operator_set_field(*bop, "weights", nullptr, mbasis, kVectorNone, nullptr, 0, false);
-  operator_set_field(*bop, "qdata", qrstr, nullptr, kVectorActive, nullptr, 0, false);
-  DevBuf<double> d_coords(static_cast<size_t>(3) * nv), qdata(static_cast<size_t>(E) * Q3);
-  box_coords_device(n, r.jitter, r.mesh_seed, d_coords.data(), s);
-  operator_apply(*bop, d_coords.data(), d_coords.size(), qdata.data(), qdata.size(), s);
+  job.bop.reset(operator_create(bq));
+  Operator& bop = *job.bop;
+  operator_set_field(bop, "dx", mrstr, mbasis, kVectorActive, nullptr, 0, false);
+  operator_set_field(bop, "weights", nullptr, mbasis, kVectorNone, nullptr, 0, false);
+  operator_set_field(bop, "qdata", qrstr, nullptr, kVectorActive, nullptr, 0, false);
+  job.coords.allocate(static_cast<size_t>(3) * nv);
+  job.qdata.allocate(static_cast<size_t>(E) * Q3);
+  box_coords_device(n, r.jitter, r.mesh_seed, job.coords.data(), s);
+  operator_apply(bop, job.coords.data(), job.coords.size(), job.qdata.data(), job.qdata.size(), s);
 
   QFunctionDev mq;
   mq.kind = nc == 1 ? kQfApplyMass : kQfApplyMassVector;
@@ -176,9 +211,9 @@ std::unique_ptr<Operator> build_problem(const ServeRequest& r, hipStream_t s) {
   qfunction_add_input(mq, "qdata", 1, kEvalNone);
   qfunction_add_output(mq, "v", nc, kEvalInterp);
   std::unique_ptr<Operator> op(operator_create(mq));
-  HIP_CHECK(hipStreamSynchronize(s));  // qdata is complete before the passive copy
   operator_set_field(*op, "u", rstr, basis, kVectorActive, nullptr, 0, false);
-  operator_set_field(*op, "qdata", qrstr, nullptr, kVectorPassive, qdata.data(), qdata.size(), true);
+  // The passive copy in s's order, after the build that writes qdata.
+  operator_set_field(*op, "qdata", qrstr, nullptr, kVectorPassive, job.qdata.data(), job.qdata.size(), true, s);
   operator_set_field(*op, "v", rstr, basis, kVectorActive, nullptr, 0, false);
   op->opt_scatter = kScatterDeterministic;  // the same bits alone and in any mix
   return op;
@@ -411,7 +446,7 @@ class MassServer {
         j->stream = streams_.acquire();
         j->streams = &streams_;
         CachedAllocScope cached(cache_.get(), j->stream);
-        j->op = build_problem(j->req, j->stream);
+        j->op = build_problem(j->req, j->stream, *j);
         const size_t L = static_cast<size_t>(j->op->active_in_l_size);
         j->res.l_size = static_cast<long long>(L);
         j->b.allocate(L);  // the rhs stream generated in place, no host copy

@@ -517,8 +517,8 @@ class Server {
   std::unique_ptr<fem_hip_server_s, Destroy> handle_;
 };
 
-// Defaults: 16 requests at once, 8 setup workers, half the free HBM, 8 CG
-// iterations per graph launch, no autotuning, solutions kept, the automatic
+// Defaults: 16 requests at once, 8 setup workers, half the free HBM, at most
+// 8 CG iterations per graph launch, no autotuning, solutions kept, the automatic
 // algorithm where its setup is all on the device, small requests (up to 2^20
 // element nodes) batched in shape cohorts.
 inline fem_hip_server_options server_options() { return fem_hip_server_options{16, 8, 0.5, 8, 0, 1, 1, 1, 0}; }

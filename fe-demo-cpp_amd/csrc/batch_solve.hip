@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <iterator>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -156,6 +157,10 @@ struct BatchJob {
   double* hs = nullptr;        // pinned: [0] ||b - M x||^2, [1] ||b||^2
   double bnorm = -1.0;         // ||b||, from hs[1] once a residual has been read back
   int outer = 0, inner_total = 0;
+  // Convergence rate of the run (factor on r.r per iteration) and the last
+  // observation it was measured from; sets how many iterations a replay takes.
+  double rate = 0.0, rr_seen = -1.0, run_red2 = 0.0;
+  int it_seen = 0;
   SolveStats stats;
   int status = 0;
   std::string error;
@@ -290,11 +295,17 @@ struct Cohort {
   BatchTable* d_tab = nullptr;
   double* h_sc = nullptr;  // pinned [kBatchSlots][kNSCAL]
   double* d_sc = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // Graphs of 4, 8, ... up to `iters` iterations (even counts: the state ends
+  // in copy 0), so a replay runs about what its members still need.
+  struct Exec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<int, Exec> execs;
   std::vector<BatchJob*> members;   // the runs in the cohort
   std::vector<BatchJob*> launched;  // the table of the replay in flight (h_sc's order)
   std::vector<BatchJob*> joining;   // started, waiting for a table entry
+  std::map<const BatchJob*, int> need;  // iterations each member still needs (estimated)
   bool in_flight = false;
   long long replays = 0;
 
@@ -333,35 +344,43 @@ struct Cohort {
     HIP_CHECK(hipMemcpyAsync(d_tab, h_tab, sizeof(BatchTable), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipStreamSynchronize(s));
     const void* bptr = fp32 ? static_cast<const void*>(basis32.data()) : static_cast<const void*>(basis.data());
+    std::vector<int> sizes{iters};
+    for (int k = 4; k < iters; k *= 2) sizes.push_back(k);
     ExclusiveCapture lk;
-    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      for (int it = 0; it < iters; ++it) {
-        kern.launch(d_tab, it & 1, bptr, grid_elem, s);
-        if (fp32) {
-          batch_step_kernel<float><<<grid_step, kThreads, 0, s>>>(d_tab, it & 1);
-        } else {
-          batch_step_kernel<double><<<grid_step, kThreads, 0, s>>>(d_tab, it & 1);
+    for (int size : sizes) {
+      Exec& e = execs[size];
+      HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        for (int it = 0; it < size; ++it) {
+          kern.launch(d_tab, it & 1, bptr, grid_elem, s);
+          if (fp32) {
+            batch_step_kernel<float><<<grid_step, kThreads, 0, s>>>(d_tab, it & 1);
+          } else {
+            batch_step_kernel<double><<<grid_step, kThreads, 0, s>>>(d_tab, it & 1);
+          }
+          HIP_CHECK_LAUNCH();
         }
+        batch_gather_kernel<<<kBatchSlots, 64, 0, s>>>(d_tab, d_sc);
         HIP_CHECK_LAUNCH();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        release();
+        throw;
       }
-      batch_gather_kernel<<<kBatchSlots, 64, 0, s>>>(d_tab, d_sc);
-      HIP_CHECK_LAUNCH();
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      release();
-      throw;
+      HIP_CHECK(hipStreamEndCapture(s, &e.graph));
+      HIP_CHECK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
     }
-    HIP_CHECK(hipStreamEndCapture(s, &graph));
-    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
 
   void release() noexcept {
     if (s) (void)hipStreamSynchronize(s);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& kv : execs) {
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+      if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    execs.clear();
     SharedCapture lk;
     if (d_tab) (void)hipFree(d_tab);
     if (d_sc) (void)hipFree(d_sc);
@@ -369,8 +388,6 @@ struct Cohort {
     if (h_sc) (void)hipHostFree(h_sc);
     if (ev) (void)hipEventDestroy(ev);
     if (s) (void)hipStreamDestroy(s);
-    exec = nullptr;
-    graph = nullptr;
     d_tab = nullptr;
     d_sc = nullptr;
     h_tab = nullptr;
@@ -381,8 +398,9 @@ struct Cohort {
   ~Cohort() { release(); }
 
   // Rewrites the table with the members (their item ranges in turn), then
-  // enqueues it, one replay and the read-back of the scalars.
-  void launch() {
+  // enqueues it, one replay of the graph with the fewest iterations at least
+  // `need` (the largest if none) and the read-back of the scalars.
+  void launch(int need) {
     BatchTable& t = *h_tab;
     t.nslots = static_cast<int>(members.size());
     int elem = 0, red = 0;
@@ -402,7 +420,9 @@ struct Cohort {
     t.elem_items = elem;
     t.red_items = red;
     HIP_CHECK(hipMemcpyAsync(d_tab, h_tab, sizeof(BatchTable), hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipGraphLaunch(exec, s));
+    auto it = execs.lower_bound(need);
+    if (it == execs.end()) it = std::prev(execs.end());
+    HIP_CHECK(hipGraphLaunch(it->second.exec, s));
     HIP_CHECK(hipMemcpyAsync(h_sc, d_sc, sizeof(double) * kBatchSlots * kNSCAL, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(ev, s));
     launched = members;
@@ -567,6 +587,7 @@ class BatchEngine {
     if (Cohort* c = j.cohort) {
       if (c->s) (void)hipStreamSynchronize(c->s);
       c->members.erase(std::remove(c->members.begin(), c->members.end(), &j), c->members.end());
+      c->need.erase(&j);
       c->joining.erase(std::remove(c->joining.begin(), c->joining.end(), &j), c->joining.end());
     }
     if (j.in_flight) (void)hipEventSynchronize(j.ev);
@@ -588,6 +609,7 @@ class BatchEngine {
     HIP_CHECK(hipStreamWaitEvent(c.s, j.ready, 0));  // the job's setup has run
     if (!j.fp32) {
       enqueue_run_start<double>(j, j.b, j.x, j.opt.rtol, c.s);
+      j.run_red2 = j.opt.rtol * j.opt.rtol;
       j.phase = BatchJob::kJoin;
       c.joining.push_back(&j);
     } else {
@@ -613,6 +635,7 @@ class BatchEngine {
     // No deeper than the target needs: the last step stops at about half of it.
     const double inner = std::max(j.opt.inner_rtol, std::min(0.5 * j.opt.rtol / rel, 0.5));
     enqueue_run_start<float>(j, j.r32.data(), j.d32.data(), inner, j.cohort->s);
+    j.run_red2 = inner * inner;
     j.phase = BatchJob::kJoin;
     j.cohort->joining.push_back(&j);
   }
@@ -635,6 +658,25 @@ class BatchEngine {
     }
   }
 
+  // Iterations a run still needs, from its measured rate (kFirstProbe if none yet).
+  static constexpr int kFirstProbe = 8;
+  static int need_of(const BatchJob& j, const double* sc) {
+    const double rr = sc[kRR], thresh = sc[kTHRESH];
+    if (!(j.rate > 0.0 && j.rate < 1.0) || !(thresh > 0.0)) return kFirstProbe;
+    if (rr <= thresh) return 1;
+    return static_cast<int>(std::max(1.0, std::ceil(std::log(thresh / rr) / std::log(j.rate))));
+  }
+  // Updates a run's rate from its scalars after a replay. The first
+  // observation of a run measures from its starting r.r (threshold / reduction^2).
+  static void observe(BatchJob& j, const double* sc) {
+    const double rr = sc[kRR], thresh = sc[kTHRESH];
+    const int it = static_cast<int>(sc[kITERS]);
+    if (j.rr_seen <= 0.0 && j.run_red2 > 0.0 && thresh > 0.0) j.rr_seen = thresh / j.run_red2;
+    if (j.rr_seen > 0.0 && rr > 0.0 && it > j.it_seen) j.rate = std::pow(rr / j.rr_seen, 1.0 / (it - j.it_seen));
+    j.rr_seen = rr;
+    j.it_seen = it;
+  }
+
   bool poll_cohort(Cohort& c) {
     bool progressed = false;
     if (c.in_flight) {
@@ -650,8 +692,13 @@ class BatchEngine {
         BatchJob& j = *launched[i];
         if (std::find(c.members.begin(), c.members.end(), &j) == c.members.end()) continue;
         const double* sc = c.h_sc + i * kNSCAL;
-        if (sc[kDONE] == 0.0) continue;
+        observe(j, sc);
+        if (sc[kDONE] == 0.0) {
+          c.need[&j] = need_of(j, sc);
+          continue;
+        }
         c.members.erase(std::remove(c.members.begin(), c.members.end(), &j), c.members.end());
+        c.need.erase(&j);
         try {
           on_run_done(j, sc);
         } catch (const std::exception& ex) {
@@ -663,10 +710,20 @@ class BatchEngine {
       BatchJob* j = c.joining.front();
       c.joining.erase(c.joining.begin());
       j->phase = BatchJob::kRun;
+      // A new run: its first replay follows the previous run's rate (refinement), else probes.
+      j->rr_seen = -1.0;
+      j->it_seen = 0;
+      c.need[j] = (j->rate > 0.0 && j->rate < 1.0 && j->run_red2 > 0.0 && j->run_red2 < 1.0)
+                      ? static_cast<int>(std::max(1.0, std::ceil(std::log(j->run_red2) / std::log(j->rate))))
+                      : kFirstProbe;
       c.members.push_back(j);
     }
     if (!c.members.empty()) {
-      c.launch();
+      // The fewest iterations any member still needs (at least 4: shorter
+      // replays cost more in launches and read-backs than they save).
+      int need = 1 << 30;
+      for (BatchJob* j : c.members) need = std::min(need, c.need.count(j) ? c.need[j] : kFirstProbe);
+      c.launch(std::max(4, need));
       progressed = true;
     }
     return progressed;

@@ -50,7 +50,7 @@ struct ServerOptions {
   int max_active = 16;        // requests set up or solving at once (one stream each)
   int workers = 8;            // host threads for setup and teardown
   double hbm_fraction = 0.5;  // share of the HBM free at start that requests may use
-  int batch_iters = 8;        // CG iterations per graph launch
+  int batch_iters = 8;        // most CG iterations per graph launch (fewer where the rate says so)
   bool autotune = false;      // tune each new (P, Q, num_comp, precision) on first sight
   bool keep_x = true;         // return the solution vectors
   // true: the tuned (automatic) algorithm where its setup is all on the device

@@ -461,7 +461,8 @@ typedef struct {
   int max_active;      /* requests set up or solving at once */
   int workers;         /* host threads for setup and teardown */
   double hbm_fraction; /* (0, 0.95] */
-  int batch_iters;     /* CG iterations per graph launch */
+  int batch_iters;     /* most CG iterations per graph launch (a launch takes fewer when the measured
+                          convergence rate says the runs in it need fewer) */
   int autotune;        /* 1: tune each new (P, Q, num_comp, precision) on first sight */
   int keep_x;          /* 1: keep solutions for fem_hip_server_wait */
   int cheap_setup;     /* 1: the automatic (tuned) algorithm where its setup is all on the device (box

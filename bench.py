@@ -190,7 +190,7 @@ def box_kernel_name(P, Q, nc, fp32, ne):
     ahead, 1 qdata in its own layer, 2 FP64 through the LDS ring, 3 the
     component split, 4 / 6 one / two elements per wavefront, 5 / 7 the same
     with the component split)."""
-    bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2)}[ne % 100]
+    bx, by = {16: (4, 4), 8: (4, 2), 4: (2, 2), 18: (8, 2), 32: (8, 4), 34: (16, 2)}[ne % 100]
     t, code = "float" if fp32 else "double", ne // 100
     if code >= 4:
         split = code in (5, 7)

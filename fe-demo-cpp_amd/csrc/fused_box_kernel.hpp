@@ -890,24 +890,39 @@ void fused_box_wave_kernel(
 
 // v at every shared node = its partials summed in ascending segment order
 // (grid: see box_shared_point; a block is kSharedLines lines of 64 points).
+// The NC components' loads of an owner are issued together (each thread's
+// work is a short chain of dependent loads: several chains in flight hide
+// their latency).
 constexpr int kSharedLines = 4;
-template <class T, int P, int BX, int BY>
-__global__ __launch_bounds__(64 * kSharedLines) void box_shared_sum_kernel(BoxGeom g, int nc,
-                                                                           const T* __restrict__ partials,
+template <class T, int P, int BX, int BY, int NC>
+__global__ __launch_bounds__(64 * kSharedLines) void box_shared_sum_kernel(BoxGeom g, const T* __restrict__ partials,
                                                                            size_t part_comp_len, T* __restrict__ v) {
   int seg[8], slot[8];
   size_t node;
-  const int n = box_shared_owners(g, P, BX, BY, blockIdx.z, blockIdx.y * kSharedLines + threadIdx.y,
-                                  blockIdx.x * 64 + threadIdx.x, &node, seg, slot);
+  // A wavefront is one line (64 points of one threadIdx.y): the line, and
+  // everything derived from it (the segments of a node plane and their first
+  // layers, runtime divisions), is wavefront-uniform and taken on the scalar unit.
+  const int line = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.y * kSharedLines + threadIdx.y));
+  const int n = box_shared_owners(g, P, BX, BY, blockIdx.z, line, blockIdx.x * 64 + threadIdx.x, &node, seg, slot);
   if (n == 0) return;
   const int LZn = g.lz * (P - 1) + 1;
   const size_t face = static_cast<size_t>(box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1, LZn));
-  for (int c = 0; c < nc; ++c) {
-    const T* pc = partials + static_cast<size_t>(c) * part_comp_len;
-    T s = pc[seg[0] * face + slot[0]];
-    for (int i = 1; i < n; ++i) s += pc[seg[i] * face + slot[i]];
-    v[static_cast<size_t>(c) * g.cs + node] = s;
+  T s[NC];
+  {
+    const T* p0 = partials + seg[0] * face + slot[0];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s[c] = p0[static_cast<size_t>(c) * part_comp_len];
   }
+  for (int i = 1; i < n; ++i) {
+    const T* pi = partials + seg[i] * face + slot[i];
+    T t[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) t[c] = pi[static_cast<size_t>(c) * part_comp_len];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s[c] += t[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) v[static_cast<size_t>(c) * g.cs + node] = s[c];
 }
 
 // Grid of the shared-node pass.
@@ -974,13 +989,19 @@ void launch_box(const FusedArgs& a, hipStream_t s) {
     if (grid.z == 0) return;
     require(grid.y < 65536 && grid.z < 65536 && static_cast<long long>(g.nz) * g.nsz < (1ll << 31),
             "fused mass: box too large for the shared-node pass");
-    box_shared_sum_kernel<T, P, BX, BY><<<grid, dim3(64, kSharedLines), 0, s>>>(
-        g, NC, static_cast<const T*>(a.partials), a.part_comp_len, static_cast<T*>(a.v));
+    box_shared_sum_kernel<T, P, BX, BY, NC><<<grid, dim3(64, kSharedLines), 0, s>>>(
+        g, static_cast<const T*>(a.partials), a.part_comp_len, static_cast<T*>(a.v));
   }
   HIP_CHECK_LAUNCH();
 }
 
-// Key elems of a box-column variant: BX * BY elements per layer, plus 100 for
+// elems % 100 of a column cross-section: its element count, except 8 x 2
+// (18; 16 is 4 x 4) and 16 x 2 (34; 32 is 8 x 4).
+constexpr int box_code(int BX, int BY) {
+  return BX == 8 && BY == 2 ? 18 : BX == 16 && BY == 2 ? 34 : BX * BY;
+}
+
+// Key elems of a box-column variant: box_code(BX, BY), plus 100 for
 // the kernel that loads qdata within its own layer (QA = 0), plus 200 for FP64
 // with u through the LDS ring, plus 300 (several components) for the
 // component-split kernel (one component per workgroup, qdata within its own
@@ -1001,7 +1022,7 @@ struct FusedBoxReg {
   }
   template <class T, int UM, int QA, bool SPLIT>
   static void reg(int code) {
-    const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, BX * BY + code};
+    const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, box_code(BX, BY) + code};
     register_fused(k, &launch_box<P, Q, NC, T, BX, BY, UM, QA, SPLIT>, 0);
   }
 };
@@ -1033,7 +1054,7 @@ struct FusedBoxWaveReg {
   }
   template <class T, bool SPLIT>
   static void reg(int code) {
-    const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, BX * BY + code};
+    const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, box_code(BX, BY) + code};
     register_fused(k, &launch_box<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW>, 0);
   }
 };
@@ -1044,6 +1065,9 @@ inline bool box_columns(int elems, int* BX, int* BY) {
     case 16: *BX = 4; *BY = 4; return elems >= 0;
     case 8: *BX = 4; *BY = 2; return elems >= 0;
     case 4: *BX = 2; *BY = 2; return elems >= 0;
+    case 18: *BX = 8; *BY = 2; return elems >= 0;
+    case 32: *BX = 8; *BY = 4; return elems >= 0;
+    case 34: *BX = 16; *BY = 2; return elems >= 0;
     default: return false;
   }
 }

@@ -33,6 +33,9 @@ const FusedBoxReg<5, 6, 3, 4, 2> box_5_6_3_8;
 // One or two elements per wavefront (fused_box_wave_kernel).
 const FusedBoxWaveReg<5, 6, 1, 4, 4, 2> wave2_5_6_1_16;
 const FusedBoxWaveReg<5, 6, 1, 4, 2, 2> wave2_5_6_1_8;
+const FusedBoxWaveReg<5, 6, 1, 8, 2, 2> wave2_5_6_1_18;
+const FusedBoxWaveReg<5, 6, 1, 8, 4, 2> wave2_5_6_1_32;
+const FusedBoxWaveReg<5, 6, 1, 16, 2, 2> wave2_5_6_1_34;
 const FusedBoxWaveReg<5, 6, 2, 4, 2, 2> wave2_5_6_2_8;
 const FusedBoxWaveReg<5, 6, 3, 4, 2, 2> wave2_5_6_3_8;
 

@@ -20,6 +20,7 @@ const FusedBoxSplitReg<8, 10, 3, 4, 2> box_8_10_3_8_split;
 // One or two elements per wavefront (fused_box_wave_kernel).
 const FusedBoxWaveReg<8, 10, 1, 2, 2> wave_8_10_1_4;
 const FusedBoxWaveReg<8, 10, 3, 4, 2> wave_8_10_3_8;
+const FusedBoxWaveReg<8, 10, 3, 8, 2> wave_8_10_3_18;
 const FusedBoxWaveReg<8, 10, 3, 2, 2> wave_8_10_3_4;
 
 }  // namespace

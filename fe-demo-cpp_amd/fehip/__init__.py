@@ -696,10 +696,11 @@ class Operator:
     def autotune(self, fp32=False, reps=10, stream=None):
         """Times every valid algorithm/packing; returns (best, [(alg, elems, ms), ...])."""
         a, e, ms, n = ctypes.c_int(), ctypes.c_int(), _d(), ctypes.c_int()
-        res = (_d * (3 * 64))()
+        cap = 256
+        res = (_d * (3 * cap))()
         call("fem_hip_operator_autotune", self.h, int(fp32), reps, _s(stream), ctypes.byref(a), ctypes.byref(e),
-             ctypes.byref(ms), res, 64, ctypes.byref(n))
-        rows = [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2])) for i in range(min(n.value, 64))]
+             ctypes.byref(ms), res, cap, ctypes.byref(n))
+        rows = [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2])) for i in range(min(n.value, cap))]
         return (a.value, e.value, ms.value), rows
 
     def apply(self, d_in, d_out, stream=None):

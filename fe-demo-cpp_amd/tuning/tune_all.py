@@ -16,7 +16,9 @@ nearest its own. The table is written as text (default: mi355x_gfx950.txt
 beside this script), which fehip.load_default_tuning() and
 fem_hip_tune_table_load() read.
 
-Usage: python fe-demo-cpp_amd/tuning/tune_all.py [--out PATH] [--reps N]
+Usage: python fe-demo-cpp_amd/tuning/tune_all.py [--out PATH] [--reps N] [--shapes P,Q,NC;...]
+(--shapes: re-measure only those shapes; the other rows are kept from the
+table at --out.)
 """
 import argparse
 import os
@@ -50,11 +52,17 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "mi355x_gfx950.txt"))
     ap.add_argument("--reps", type=int, default=8)
+    ap.add_argument("--shapes", default="", help="P,Q,NC;... to re-measure (others kept from --out)")
     args = ap.parse_args()
     info = fehip.device_info(0)
     fehip.tune_table_clear()
+    only = {tuple(int(x) for x in t.split(",")) for t in args.shapes.split(";") if t}
+    if only:
+        fehip.tune_table_load(args.out)
     for P, Q, ncs in SHAPES:
         for nc, n in sorted({(nc, mesh_size(P, nc, t)) for nc in ncs for t in TARGETS}):
+            if only and (P, Q, nc) not in only:
+                continue
             t0 = time.perf_counter()
             p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
             p.op.set_fused_options(fehip.SCATTER_ATOMIC, exact=0)

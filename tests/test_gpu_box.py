@@ -54,9 +54,16 @@ CASES = [((8, 8, 8), 5, 6, 1, 16, (0, 1, 2, 3, 8)),
          ((4, 4, 3), 6, 7, 1, 8, (0, 2)),
          ((2, 2, 5), 8, 10, 3, 4, (0, 2)),
          ((4, 4, 3), 8, 10, 3, 8, (0, 2)),
-         ((4, 2, 2), 8, 10, 1, 4, (0,))]
+         ((4, 2, 2), 8, 10, 1, 4, (0,)),
+         ((16, 8, 4), 5, 6, 1, 32, (0, 2, 4)),
+         ((16, 4, 5), 5, 6, 1, 18, (0, 2, 3)),
+         ((32, 4, 3), 5, 6, 1, 34, (0, 2)),
+         ((16, 4, 3), 8, 10, 3, 18, (0, 2))]
 # Shapes whose cross-section is compiled only as the component-split kernel.
 SPLIT_ONLY = {(8, 10, 3, 8)}
+# Cross-sections compiled only as wavefront kernels (codes: 8 x 4 is 32, 8 x 2
+# is 18, 16 x 2 is 34).
+WAVE_ONLY = {(5, 6, 1, 32): (632,), (5, 6, 1, 18): (618,), (5, 6, 1, 34): (634,), (8, 10, 3, 18): (518,)}
 # The kernels with whole wavefronts per element compiled (elems + 400: one
 # element per wavefront, + 600: two; + 100 for the component split of a
 # multi-component field).
@@ -85,6 +92,8 @@ def test_box_column_matches_oracle(gpu, n3, P, Q, nc, elems, layers):
         variants = (elems + 300,)
     if (P, Q, nc, elems) in WAVE:
         variants += WAVE[(P, Q, nc, elems)]
+    if (P, Q, nc, elems) in WAVE_ONLY:
+        variants = WAVE_ONLY[(P, Q, nc, elems)]
     for lz, el in [(lz, el) for lz in layers for el in variants]:
         has32 = el // 100 != 2
         fehip.box_column_check(*n3, P, elems, lz)  # the index arithmetic, on the host

@@ -660,11 +660,18 @@ class BatchEngine {
 
   // Iterations a run still needs, from its measured rate (kFirstProbe if none yet).
   static constexpr int kFirstProbe = 8;
+  // ceil(log(reduction) / log(rate)) iterations, at least 1 and at most
+  // kMostNeed (a rate within rounding of 1 would overflow an int).
+  static constexpr double kMostNeed = 1 << 20;
+  static int iterations_for(double reduction, double rate) {
+    const double k = std::ceil(std::log(reduction) / std::log(rate));
+    return static_cast<int>(std::max(1.0, std::min(k, kMostNeed)));
+  }
   static int need_of(const BatchJob& j, const double* sc) {
     const double rr = sc[kRR], thresh = sc[kTHRESH];
     if (!(j.rate > 0.0 && j.rate < 1.0) || !(thresh > 0.0)) return kFirstProbe;
     if (rr <= thresh) return 1;
-    return static_cast<int>(std::max(1.0, std::ceil(std::log(thresh / rr) / std::log(j.rate))));
+    return iterations_for(thresh / rr, j.rate);
   }
   // Updates a run's rate from its scalars after a replay. The first
   // observation of a run measures from its starting r.r (threshold / reduction^2).
@@ -714,7 +721,7 @@ class BatchEngine {
       j->rr_seen = -1.0;
       j->it_seen = 0;
       c.need[j] = (j->rate > 0.0 && j->rate < 1.0 && j->run_red2 > 0.0 && j->run_red2 < 1.0)
-                      ? static_cast<int>(std::max(1.0, std::ceil(std::log(j->run_red2) / std::log(j->rate))))
+                      ? iterations_for(j->run_red2, j->rate)
                       : kFirstProbe;
       c.members.push_back(j);
     }

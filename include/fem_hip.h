@@ -253,9 +253,13 @@ int fem_hip_operator_set_fused_options(fem_hip_operator op, int scatter_mode, in
 #define FEM_HIP_FUSED_BLOCK_PARTIAL_LDS 5 /* BLOCK_PARTIAL summing in LDS with atomics (any order) */
 #define FEM_HIP_FUSED_BOX_COLUMN 7 /* structured boxes (fem_hip_box_detect): element columns marching in z with node
                                       planes in LDS, shared column faces summed by a second pass (deterministic;
-                                      no index arrays read); elems = elements per column layer (16, 8, 4),
-                                      + 100 for the kernel that loads qdata within its own layer,
-                                      + 200 (FP64) for the one that loads u through an LDS ring of node planes */
+                                      no index arrays read). elems = the column cross-section (16: 4x4 elements,
+                                      8: 4x2, 4: 2x2; for the wavefront kernels also 18: 8x2, 32: 8x4, 34: 16x2)
+                                      plus the kernel: + 0 qdata loaded a layer ahead, + 100 qdata within its own
+                                      layer, + 200 (FP64) u through an LDS ring of node planes, + 300 one component
+                                      per workgroup (several components), + 400 / + 600 one / two elements per
+                                      wavefront (+ 500 / + 700 with one component per workgroup). Which are
+                                      compiled depends on the shape: fem_hip_operator_autotune lists them. */
 /* Explicit algorithm and its elements per workgroup (element kernels) or per
  * block (block kernels); 0 = that algorithm's default. A deterministic scatter
  * mode accepts only the deterministic algorithms. exact = 1 overrides. */

@@ -2,7 +2,9 @@
 // engine (batch_solve.hip): reduction grids and the fixed-order reductions,
 // the scalar slots, the plain and pipelined Jacobi-PCG kernels, and the
 // residual / correction kernels of mixed-precision refinement. Included by
-// those two translation units only (everything here has internal linkage).
+// those two translation units only (everything here has internal linkage; the
+// plain form's scalar kernels, which only the solver launches, are marked
+// maybe_unused).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(kThreads) void cg_init_kernel(size_t n, const T* __
   }
 }
 
-__global__ __launch_bounds__(kThreads) void cg_init_scalars_kernel(const double* __restrict__ part,
+[[maybe_unused]] __global__ __launch_bounds__(kThreads) void cg_init_scalars_kernel(const double* __restrict__ part,
                                                                    double* __restrict__ sc, double rtol2, int nb,
                                                                    int max_iter) {
   __shared__ double sh[2][kThreads / 64];
@@ -169,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void cg_replace_kernel(size_t n, const T*
 
 // After a replacement: new r.z and r.r, the next segment's threshold; the
 // iteration count runs on.
-__global__ __launch_bounds__(kThreads) void cg_replace_scalars_kernel(const double* __restrict__ part,
+[[maybe_unused]] __global__ __launch_bounds__(kThreads) void cg_replace_scalars_kernel(const double* __restrict__ part,
                                                                       double* __restrict__ sc, double rtol2, int nb,
                                                                       int max_iter) {
   __shared__ double sh[2][kThreads / 64];
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int p
 // 1-par: the iteration count, r.r, r.z and the done test r.r <= threshold or
 // the iteration limit (both read from the scalar slots). An iteration that started done copies the state on and
 // leaves p alone (*frozen). Returns beta.
-__device__ double cg_step_end(int par, int nb, double* sc, const double* part, double (*sh)[kThreads / 64],
+[[maybe_unused]] __device__ double cg_step_end(int par, int nb, double* sc, const double* part, double (*sh)[kThreads / 64],
                               double* s_val, bool* frozen) {
   const bool done0 = sc[slot(kDONE, par)] != 0.0;
   const double rz_new = all_partials(part + kPartRZ, nb, sh[0], &s_val[0]);

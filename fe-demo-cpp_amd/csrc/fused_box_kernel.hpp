@@ -150,13 +150,18 @@ __host__ __device__ inline int box_owners(const BoxGeom& g, int P, int BX, int B
 
 // The shared-node pass's point (plane, line, point) in one step: whether it is
 // a shared node of that plane (as box_shared_point), its node index, and its
-// owners and slots as box_owners gives them (same order). Each plane kind
-// fixes one coordinate and one pair of owner columns or segments, so only the
-// other two are derived; divisions by P-1, SX and SY become constants when
-// the kernel inlines this with its template arguments, and the segment of a
-// node plane takes 32-bit divisions (nz * nsz < 2^31, checked at launch).
-__host__ __device__ __forceinline__ int box_shared_owners(const BoxGeom& g, int P, int BX, int BY, int plane,
-                                                          int line, int point, size_t* node, int* seg, int* slot) {
+// owners and slots as box_owners gives them (same order), each passed to
+// f(i, segment, slot) with i = 0, 1, ... Each plane kind fixes one coordinate
+// and one pair of owner columns or segments, so only the other two are
+// derived; divisions by P-1, SX and SY become constants when the kernel
+// inlines this with its template arguments, and the segment of a node plane
+// takes 32-bit divisions (nz * nsz < 2^31, checked at launch). The owners are
+// at most 2 x 2 x 2 (segments x column rows x columns), visited with fixed
+// loop bounds so the pass compiles to straight-line code. Returns how many
+// owners (0: not a shared node of this plane).
+template <class F>
+__host__ __device__ __forceinline__ int box_shared_visit(const BoxGeom& g, int P, int BX, int BY, int plane, int line,
+                                                         int point, size_t* node, F&& f) {
   const int SX = BX * (P - 1), SY = BY * (P - 1);
   const int LXn = SX + 1, LYn = SY + 1, LZn = g.lz * (P - 1) + 1;
   const int nbx = g.ncx - 1, nby = g.ncy - 1;
@@ -214,16 +219,34 @@ __host__ __device__ __forceinline__ int box_shared_owners(const BoxGeom& g, int 
   }
   *node = static_cast<size_t>(gx) + static_cast<size_t>(g.Nx) * (gy + static_cast<size_t>(g.Ny) * gz);
   int n = 0;
-  for (int sg = sz0; sg <= sz1; ++sg) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    if (a > sz1 - sz0) continue;
+    const int sg = sz0 + a;
     const int lz = gz - seg_z0(sg) * (P - 1);
-    for (int cy = cy0; cy <= cy1; ++cy)
-      for (int cx = cx0; cx <= cx1; ++cx) {
-        seg[n] = (sg * g.ncy + cy) * g.ncx + cx;
-        slot[n] = box_slot(gx - cx * SX, gy - cy * SY, lz, xs, ys, LXn, LYn, LZn);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      if (b > cy1 - cy0) continue;
+      const int cy = cy0 + b;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (c > cx1 - cx0) continue;
+        const int cx = cx0 + c;
+        f(n, (sg * g.ncy + cy) * g.ncx + cx, box_slot(gx - cx * SX, gy - cy * SY, lz, xs, ys, LXn, LYn, LZn));
         ++n;
       }
+    }
   }
   return n;
+}
+
+// box_shared_visit's owners collected into arrays (the host check's form).
+__host__ __device__ inline int box_shared_owners(const BoxGeom& g, int P, int BX, int BY, int plane, int line,
+                                                 int point, size_t* node, int* seg, int* slot) {
+  return box_shared_visit(g, P, BX, BY, plane, line, point, node, [&](int i, int sg, int sl) {
+    seg[i] = sg;
+    slot[i] = sl;
+  });
 }
 
 // qdata is read exactly once per apply. FP64 loads it as streaming
@@ -897,30 +920,24 @@ constexpr int kSharedLines = 4;
 template <class T, int P, int BX, int BY, int NC>
 __global__ __launch_bounds__(64 * kSharedLines) void box_shared_sum_kernel(BoxGeom g, const T* __restrict__ partials,
                                                                            size_t part_comp_len, T* __restrict__ v) {
-  int seg[8], slot[8];
-  size_t node;
+  const int LZn = g.lz * (P - 1) + 1;
+  const size_t face = static_cast<size_t>(box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1, LZn));
   // A wavefront is one line (64 points of one threadIdx.y): the line, and
   // everything derived from it (the segments of a node plane and their first
   // layers, runtime divisions), is wavefront-uniform and taken on the scalar unit.
   const int line = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.y * kSharedLines + threadIdx.y));
-  const int n = box_shared_owners(g, P, BX, BY, blockIdx.z, line, blockIdx.x * 64 + threadIdx.x, &node, seg, slot);
-  if (n == 0) return;
-  const int LZn = g.lz * (P - 1) + 1;
-  const size_t face = static_cast<size_t>(box_face_size(BX * (P - 1) + 1, BY * (P - 1) + 1, LZn));
   T s[NC];
-  {
-    const T* p0 = partials + seg[0] * face + slot[0];
+  size_t node;
+  const int n = box_shared_visit(g, P, BX, BY, blockIdx.z, line, blockIdx.x * 64 + threadIdx.x, &node,
+                                 [&](int i, int sg, int sl) {
+                                   const T* pc = partials + static_cast<size_t>(sg) * face + sl;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) s[c] = p0[static_cast<size_t>(c) * part_comp_len];
-  }
-  for (int i = 1; i < n; ++i) {
-    const T* pi = partials + seg[i] * face + slot[i];
-    T t[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) t[c] = pi[static_cast<size_t>(c) * part_comp_len];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) s[c] += t[c];
-  }
+                                   for (int c = 0; c < NC; ++c) {
+                                     const T t = pc[static_cast<size_t>(c) * part_comp_len];
+                                     s[c] = i == 0 ? t : s[c] + t;
+                                   }
+                                 });
+  if (n == 0) return;
 #pragma unroll
   for (int c = 0; c < NC; ++c) v[static_cast<size_t>(c) * g.cs + node] = s[c];
 }

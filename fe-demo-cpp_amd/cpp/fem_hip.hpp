@@ -376,7 +376,7 @@ inline void operator_apply(Operator& op, const DeviceArray<float>& in, DeviceArr
 // _MIXED (FP32 inner PCG, FP64 refinement) or _MIXED_RELIABLE (FP32 PCG with
 // FP64 residual replacement).
 // inner_rtol <= 0: 1e-5 per inner solve (MIXED), 1e-2 between replacements (MIXED_RELIABLE).
-// cg_variant: FEM_HIP_CG_AUTO, _PLAIN or _PIPELINED.
+// cg_variant: FEM_HIP_CG_AUTO, _PLAIN, _PIPELINED or _PLAIN_WIDE.
 inline fem_hip_solve_stats mass_solve(Operator& op, const DeviceArray<double>& b, DeviceArray<double>& x, int mode,
                                       double rtol = 1e-10, void* stream = nullptr, double inner_rtol = 0.0,
                                       int cg_variant = FEM_HIP_CG_AUTO) {

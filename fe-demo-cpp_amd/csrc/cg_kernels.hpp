@@ -3,8 +3,8 @@
 // the scalar slots, the plain and pipelined Jacobi-PCG kernels, and the
 // residual / correction kernels of mixed-precision refinement. Included by
 // those two translation units only (everything here has internal linkage; the
-// plain form's scalar kernels, which only the solver launches, are marked
-// maybe_unused).
+// plain form's scalar and fold kernels, which only the solver launches, are
+// marked maybe_unused).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -367,6 +367,161 @@ __global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, int par, int n
   for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * kThreads)
     p[i] = z[i] + beta * p[i];
+}
+
+// ---- The plain form on long vectors ("wide"): one 16-byte pack per thread
+// instead of a grid stride over at most kMaxRed blocks, so that every CU has
+// many independent loads in flight. Each block writes one partial; a
+// fold kernel sums them into the red_blocks(n) partials of the standard form
+// (block b of nb takes a fixed contiguous range, so the order is fixed), and
+// one-block kernels form alpha and beta (and the step-end state) into the
+// scalar slots, which every block of the next wide kernel reads. An iteration
+// is the apply and seven kernels instead of three: the form for vectors long
+// enough that streaming, not launches, sets the time.
+
+// Wide blocks of a vector of n entries (one pack per thread).
+template <class T>
+int wide_blocks(size_t n) {
+  const size_t nv = std::max<size_t>(1, n / Pack<T>::W);
+  return static_cast<int>((nv + kThreads - 1) / kThreads);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void dot_wide_kernel(size_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                            double* __restrict__ part) {
+  constexpr int W = Pack<T>::W;
+  __shared__ double sh[kThreads / 64];
+  const size_t nv = n / W, iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  double s = 0.0;
+  if (iv < nv) {
+    const Pack<T> x = ldp(a, iv), y = ldp(b, iv);
+#pragma unroll
+    for (int k = 0; k < W; ++k) s += static_cast<double>(x.v[k]) * static_cast<double>(y.v[k]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+    const size_t i = nv * W + threadIdx.x;
+    s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[y * out_stride + b] = the sum of in[y * in_stride + i] over block b's
+// contiguous share of i < nin (nout = gridDim.x blocks), in ascending order
+// per thread and then the fixed tree of block_sum.
+[[maybe_unused]] __global__ __launch_bounds__(kThreads) void fold_partials_kernel(const double* __restrict__ in, int nin,
+                                                                 size_t in_stride, double* __restrict__ out,
+                                                                 size_t out_stride) {
+  __shared__ double sh[kThreads / 64];
+  const int nout = static_cast<int>(gridDim.x);
+  const int per = (nin + nout - 1) / nout;
+  const int lo = static_cast<int>(blockIdx.x) * per, hi = min(nin, lo + per);
+  const double* src = in + blockIdx.y * in_stride;
+  double s = 0.0;
+  for (int i = lo + static_cast<int>(threadIdx.x); i < hi; i += kThreads) s += src[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.y * out_stride + blockIdx.x] = s;
+}
+
+// One block: alpha = r.z / p.Ap from the folded p.Ap partials (0 once done).
+[[maybe_unused]] __global__ __launch_bounds__(kThreads) void cg_alpha_kernel(int par, int nb, double* __restrict__ sc,
+                                                            const double* __restrict__ part) {
+  __shared__ double sh[kThreads / 64];
+  __shared__ double s_pap;
+  const double pap = all_partials(part + kPartPAP, nb, sh, &s_pap);
+  if (threadIdx.x == 0) {
+    const bool done = sc[slot(kDONE, par)] != 0.0;
+    sc[kPAP] = pap;
+    sc[kALPHA] = (done || pap == 0.0) ? 0.0 : sc[slot(kRZ, par)] / pap;
+  }
+}
+
+// x += alpha p, r -= alpha Ap, z = dinv r, the block's r.z and r.r partials
+// (alpha from its slot; an iteration that starts done only sums r.r).
+template <class T>
+__global__ __launch_bounds__(kThreads) void cg_update_wide_kernel(size_t n, int par, const double* __restrict__ sc,
+                                                                  const T* __restrict__ p, const T* __restrict__ Ap,
+                                                                  const T* __restrict__ dinv, T* __restrict__ x,
+                                                                  T* __restrict__ r, T* __restrict__ z,
+                                                                  double* __restrict__ part_rz,
+                                                                  double* __restrict__ part_rr) {
+  constexpr int W = Pack<T>::W;
+  __shared__ double sh[2][kThreads / 64];
+  const bool done = sc[slot(kDONE, par)] != 0.0;
+  const T alpha = static_cast<T>(sc[kALPHA]);
+  double srz = 0.0, srr = 0.0;
+  auto one = [&](T& xi, T& ri, T pi, T api, T di, T& zi) {
+    if (!done) {
+      xi = xi + alpha * pi;
+      ri = ri - alpha * api;
+      zi = di * ri;
+      srz += static_cast<double>(ri) * static_cast<double>(zi);
+    }
+    srr += static_cast<double>(ri) * static_cast<double>(ri);
+  };
+  const size_t nv = n / W, iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (iv < nv) {
+    Pack<T> rv = ldp(r, iv);
+    if (!done) {
+      Pack<T> xv = ldp(x, iv), zv;
+      const Pack<T> pv = ldp(p, iv), av = ldp(Ap, iv), dv = ldp(dinv, iv);
+#pragma unroll
+      for (int k = 0; k < W; ++k) one(xv.v[k], rv.v[k], pv.v[k], av.v[k], dv.v[k], zv.v[k]);
+      stp(x, iv, xv);
+      stp(r, iv, rv);
+      stp(z, iv, zv);
+    } else {
+#pragma unroll
+      for (int k = 0; k < W; ++k) srr += static_cast<double>(rv.v[k]) * static_cast<double>(rv.v[k]);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+    const size_t i = nv * W + threadIdx.x;
+    T xi = x[i], ri = r[i], zi = T(0);
+    one(xi, ri, p[i], Ap[i], dinv[i], zi);
+    if (!done) {
+      x[i] = xi;
+      r[i] = ri;
+      z[i] = zi;
+    }
+  }
+  const double a = block_sum(srz, sh[0]);
+  const double c = block_sum(srr, sh[1]);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = a;
+    part_rr[blockIdx.x] = c;
+  }
+}
+
+// One block: the step end (beta and the state copy 1-par) from the folded
+// r.z and r.r partials.
+[[maybe_unused]] __global__ __launch_bounds__(kThreads) void cg_beta_kernel(int par, int nb, double* __restrict__ sc,
+                                                           const double* __restrict__ part) {
+  __shared__ double sh[2][kThreads / 64];
+  __shared__ double s_val[2];
+  bool frozen = false;
+  (void)cg_step_end(par, nb, sc, part, sh, s_val, &frozen);
+}
+
+// p = z + beta p (beta from its slot); an iteration that started done leaves p.
+template <class T>
+__global__ __launch_bounds__(kThreads) void cg_p_wide_kernel(size_t n, int par, const double* __restrict__ sc,
+                                                             const T* __restrict__ z, T* __restrict__ p) {
+  constexpr int W = Pack<T>::W;
+  if (sc[slot(kDONE, par)] != 0.0) return;
+  const T beta = static_cast<T>(sc[kBETA]);
+  const size_t nv = n / W, iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (iv < nv) {
+    const Pack<T> zv = ldp(z, iv);
+    Pack<T> pv = ldp(p, iv);
+#pragma unroll
+    for (int k = 0; k < W; ++k) pv.v[k] = zv.v[k] + beta * pv.v[k];
+    stp(p, iv, pv);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+    const size_t i = nv * W + threadIdx.x;
+    p[i] = z[i] + beta * p[i];
+  }
 }
 
 // ---- Pipelined Jacobi-PCG (Ghysels & Vanroose, "Hiding global

@@ -51,9 +51,11 @@ template <class T>
 struct Pcg {
   size_t n;
   bool pipe;
+  bool wide;  // plain form: the one-pack-per-thread kernels (long vectors)
   DevBuf<T> r, z, p, Ap, dinv;
   DevBuf<T> u, w, m, q, sv;  // pipelined form only (sv: s = A p)
   DevBuf<double> part, sc;
+  DevBuf<double> partw;  // wide form: per-block partials (p.Ap, then r.z and r.r)
   // Pipelined form with the apply split (FusedSplit): the first stage writes Ap
   // and the step kernel forms the rest of A m itself.
   bool split = false;
@@ -71,14 +73,16 @@ struct Pcg {
   T* x_bound = nullptr;
   uint64_t sig_bound = 0;      // and the apply's fused_signature
 
-  int nb;  // reduction blocks
+  int nb;   // reduction blocks
+  int nbw;  // wide form: one block per kThreads packs
 
   // Iterations per batch graph: even, at least 2.
   static int even_batch(int b) { return std::max(2, b + (b & 1)); }
 
-  Pcg(size_t n_, bool pipelined)
-      : n(n_), pipe(pipelined), r(n_), z(n_), p(n_), Ap(n_), dinv(n_),
-        part(pipelined ? 2 * kPipePart : 3 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)) {
+  Pcg(size_t n_, bool pipelined, bool wide_)
+      : n(n_), pipe(pipelined), wide(wide_ && !pipelined), r(n_), z(n_), p(n_), Ap(n_), dinv(n_),
+        part(pipelined ? 2 * kPipePart : 3 * kMaxRed), sc(kNSCAL), nb(red_blocks(n_)), nbw(wide_blocks<T>(n_)) {
+    if (wide) partw.allocate(2 * static_cast<size_t>(nbw));
     if (pipe) {
       u.allocate(n_);
       w.allocate(n_);
@@ -98,7 +102,8 @@ struct Pcg {
 
   // One iteration of parity par (0 or 1; see the scalar slots): the apply,
   // p.Ap partials, the update (alpha formed in every block) and the step end
-  // (beta and the state copy formed in every block).
+  // (beta and the state copy formed in every block). The wide form forms
+  // alpha and beta once, in one-block kernels between the streaming ones.
   template <class Apply>
   void iteration(Apply&& apply, T* x, int par, hipStream_t s) {
     if (pipe) {
@@ -117,7 +122,17 @@ struct Pcg {
     }
     apply(p.data(), Ap.data(), s);
     double* pap = part.data() + kPartPAP;
-    if (aligned16(x)) {  // every other vector is the solver's own allocation
+    if (wide && aligned16(x)) {
+      double* pw = partw.data();
+      dot_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, p.data(), Ap.data(), pw);
+      fold_partials_kernel<<<dim3(nb, 1), kThreads, 0, s>>>(pw, nbw, 0, pap, 0);
+      cg_alpha_kernel<<<1, kThreads, 0, s>>>(par, nb, sc.data(), part.data());
+      cg_update_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, par, sc.data(), p.data(), Ap.data(), dinv.data(), x,
+                                                        r.data(), z.data(), pw, pw + nbw);
+      fold_partials_kernel<<<dim3(nb, 2), kThreads, 0, s>>>(pw, nbw, nbw, part.data() + kPartRZ, kPartRR - kPartRZ);
+      cg_beta_kernel<<<1, kThreads, 0, s>>>(par, nb, sc.data(), part.data());
+      cg_p_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, par, sc.data(), z.data(), p.data());
+    } else if (aligned16(x)) {  // every other vector is the solver's own allocation
       dot_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), pap);
       cg_update_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), p.data(), Ap.data(), dinv.data(), x,
                                                       r.data(), z.data(), part.data());
@@ -509,7 +524,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   require(opt.mode >= 0 && opt.mode <= 2, "mass_solve: unknown mode");
   require(opt.rtol > 0 && opt.max_iter > 0, "mass_solve: rtol and max_iter must be positive");
   require(opt.inner_rtol < 1.0, "mass_solve: inner_rtol must be below 1");
-  require(opt.cg_variant >= 0 && opt.cg_variant <= 2, "mass_solve: cg_variant must be 0 (automatic), 1 or 2");
+  require(opt.cg_variant >= 0 && opt.cg_variant <= 3, "mass_solve: cg_variant must be 0 (automatic), 1, 2 or 3");
   require(b != nullptr && x != nullptr, "mass_solve: b and x must be device arrays");
   FusedMass& f = operator_fused_mass(op, s);
   const size_t n = static_cast<size_t>(f.l_size);
@@ -534,17 +549,19 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   // sets the time), the plain one otherwise: in FP32 the pipelined recurrences
   // drift further from the true residual, and an FP32-inner refinement then
   // needs an extra step (16^3, P=3: 44 iterations in 3 steps against 42 in 2).
-  // Residual replacement (MIXED_RELIABLE) always runs the plain form.
+  // Residual replacement (MIXED_RELIABLE) always runs the plain form. The
+  // plain form runs its wide kernels from kWideMinN entries up.
   const bool pipe = opt.mode != 2 && (opt.cg_variant == 2 || (opt.cg_variant == 0 && opt.mode == 0 && n <= kPipeMaxN));
-  if (st.pcg64 && st.pcg64->pipe != pipe && opt.mode == 0) st.pcg64.reset();
-  if (st.pcg32 && st.pcg32->pipe != pipe && opt.mode != 0) st.pcg32.reset();
+  const bool wide = !pipe && (opt.cg_variant == 3 || (opt.cg_variant == 0 && n >= kWideMinN));
+  if (st.pcg64 && (st.pcg64->pipe != pipe || st.pcg64->wide != wide) && opt.mode == 0) st.pcg64.reset();
+  if (st.pcg32 && (st.pcg32->pipe != pipe || st.pcg32->wide != wide) && opt.mode != 0) st.pcg32.reset();
   if (opt.mode == 0 && !st.pcg64) {
-    st.pcg64 = std::make_unique<Pcg<double>>(n, pipe);
+    st.pcg64 = std::make_unique<Pcg<double>>(n, pipe, wide);
     recip_kernel<double><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg64->dinv.data());
     HIP_CHECK_LAUNCH();
   }
   if (opt.mode != 0 && !st.pcg32) {
-    st.pcg32 = std::make_unique<Pcg<float>>(n, pipe);
+    st.pcg32 = std::make_unique<Pcg<float>>(n, pipe, wide);
     st.r32.allocate(n);
     st.d32.allocate(n);
     recip_kernel<float><<<grid_for(n, kThreads), kThreads, 0, s>>>(n, st.diag.data(), st.pcg32->dinv.data());

@@ -27,7 +27,7 @@ FUSED_NAMES = {0: "elem_atomic", 1: "elem_tmap", 2: "block_atomic", 3: "block_pa
 VECTOR_ACTIVE, VECTOR_PASSIVE, VECTOR_NONE = 0, 1, 2
 QF_BUILD_MASS, QF_APPLY_MASS, QF_APPLY_MASS_VECTOR, QF_BUILD_DIFFUSION, QF_APPLY_DIFFUSION = 1, 2, 3, 4, 5
 SOLVE_FP64, SOLVE_MIXED, SOLVE_MIXED_RELIABLE = 0, 1, 2
-CG_AUTO, CG_PLAIN, CG_PIPELINED = 0, 1, 2  # fem_hip_solve_options.cg_variant
+CG_AUTO, CG_PLAIN, CG_PIPELINED, CG_PLAIN_WIDE = 0, 1, 2, 3  # fem_hip_solve_options.cg_variant
 
 
 ERR_NOT_READY = 6
@@ -731,7 +731,7 @@ class Operator:
               stream=None, cg_variant=CG_AUTO, separate_second_pass=False):
         """M x = b. inner_rtol: reduction of each inner FP32 solve (SOLVE_MIXED, default 1e-5)
         or between residual replacements (SOLVE_MIXED_RELIABLE, default 1e-2). cg_variant:
-        CG_AUTO, CG_PLAIN or CG_PIPELINED; separate_second_pass: the pipelined form keeps the
+        CG_AUTO, CG_PLAIN, CG_PIPELINED or CG_PLAIN_WIDE; separate_second_pass: the pipelined form keeps the
         apply's second pass as its own kernel instead of folding it into its step."""
         if inner_rtol is None:
             inner_rtol = 0.0  # the library's default for the mode

@@ -393,6 +393,7 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
 #define FEM_HIP_CG_AUTO 0      /* fem_hip_solve_options.cg_variant */
 #define FEM_HIP_CG_PLAIN 1
 #define FEM_HIP_CG_PIPELINED 2
+#define FEM_HIP_CG_PLAIN_WIDE 3
 typedef struct {
   int mode;          /* FEM_HIP_SOLVE_* */
   int max_iter;      /* CG iterations (FP64) / per inner solve (mixed) */
@@ -403,9 +404,12 @@ typedef struct {
                         keeps the FP64 iteration count). A refinement step never goes below half
                         of what the target still needs. */
   int cg_variant;    /* FEM_HIP_CG_*. 0: automatic (FP64: pipelined CG up to 2^20 unknowns, plain CG above; the
-                        FP32 inner solves and MIXED_RELIABLE: plain); 1: plain Jacobi-PCG (apply, dot, update, direction: five
-                        dependent launches an iteration); 2: pipelined Jacobi-PCG (Ghysels & Vanroose
-                        2014: apply and one fused kernel, three launches) */
+                        FP32 inner solves and MIXED_RELIABLE: plain; the plain form wide from 2^25 unknowns);
+                        1: plain Jacobi-PCG (apply, dot, update, direction: five dependent launches an
+                        iteration); 2: pipelined Jacobi-PCG (Ghysels & Vanroose 2014: apply and one fused
+                        kernel, three launches); 3: plain Jacobi-PCG with one 16-byte pack per thread and
+                        alpha, beta formed by one-block kernels between the streaming ones (the same
+                        arithmetic and summation order as 1, for long vectors) */
   int separate_second_pass; /* pipelined form: 0 (default) folds the apply's second pass (block partial sums,
                         transpose map) into the step kernel where the apply splits, one launch fewer an
                         iteration with the same arithmetic; 1 keeps it as its own kernel */

@@ -194,15 +194,18 @@ def test_high_order_vector_mass_solves(gpu, alg):
 
 @pytest.mark.parametrize("n,P,Q,nc", [(16, 3, 4, 1), (6, 5, 6, 3)])
 def test_plain_and_pipelined_cg(gpu, n, P, Q, nc):
-    """Both CG forms (plain: five launches an iteration; pipelined, Ghysels &
-    Vanroose 2014: three) in FP64 and mixed precision reach 1e-10 (true FP64
-    residual), take the oracle CG's iteration count within 2 (FP64), agree with
-    the oracle's solution, and repeat bit for bit."""
+    """Every CG form (plain: five launches an iteration; pipelined, Ghysels &
+    Vanroose 2014: three; plain with the wide kernels: eight) in FP64 and mixed
+    precision reaches 1e-10 (true FP64 residual), takes the oracle CG's
+    iteration count within 2 (FP64), agrees with the oracle's solution, and
+    repeats bit for bit. Both sizes leave a tail of n % 2 (FP64) and n % 4
+    (FP32) entries after the 16-byte packs."""
     print("plain vs pipelined CG", n, P, Q, nc, flush=True)
     p = MassProblem(n, P, Q, nc, scatter=fehip.SCATTER_DETERMINISTIC)
     b, x_cpu, it_cpu = _oracle_solution(p)
     d_b, d_x = fehip.DeviceArray.from_host(b), fehip.DeviceArray(p.l_size)
-    for variant in (fehip.CG_PLAIN, fehip.CG_PIPELINED):
+    assert p.l_size % 4 != 0
+    for variant in (fehip.CG_PLAIN, fehip.CG_PIPELINED, fehip.CG_PLAIN_WIDE):
         for mode in (fehip.SOLVE_FP64, fehip.SOLVE_MIXED):
             st = p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=variant)
             x1 = d_x.to_host()
@@ -213,9 +216,12 @@ def test_plain_and_pipelined_cg(gpu, n, P, Q, nc):
             p.op.solve(d_b, d_x, mode=mode, rtol=1e-10, cg_variant=variant)
             assert d_x.to_host().tobytes() == x1.tobytes(), (variant, mode)
             print("variant", variant, "mode", mode, st, flush=True)
-    # Residual replacement runs the plain form whatever is asked.
+    # Residual replacement runs the plain form whatever is asked, wide if asked.
     st = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED_RELIABLE, rtol=1e-10, cg_variant=fehip.CG_PIPELINED)
     assert st["rel_residual"] <= 1e-10
+    sw = p.op.solve(d_b, d_x, mode=fehip.SOLVE_MIXED_RELIABLE, rtol=1e-10, cg_variant=fehip.CG_PLAIN_WIDE)
+    assert sw["rel_residual"] <= 1e-10 and rel_err(d_x.to_host(), x_cpu) < 1e-8, sw
+    assert abs(sw["iterations"] - st["iterations"]) <= 2, (sw, st)
 
 
 @pytest.mark.parametrize("n,P,Q,nc", [(16, 3, 4, 1), (6, 5, 6, 3)])

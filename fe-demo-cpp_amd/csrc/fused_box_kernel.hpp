@@ -384,6 +384,28 @@ struct BoxTile<5, 6> {
   static constexpr int RY = 11, ZS = 71, FBUF = 357;
 };
 
+// Element tile strides of the wavefront kernels, per value type: a model of
+// LDS bank conflicts (32 banks of 4 bytes; 16 lanes a cycle for 64-bit
+// accesses, 32 for 32-bit ones) over every stage's accesses and the node sums
+// agrees with the measured conflict cycles of BoxTile's strides (P=5: 28% /
+// 43% modelled against 26% / 41% measured in FP64 / FP32; P=8: 48% / 44%
+// against 42% / 41%). These strides are the model's best; the rest keep
+// BoxTile's.
+template <int P, int Q, class T>
+struct WaveTile : BoxTile<P, Q> {};
+template <>
+struct WaveTile<5, 6, float> {
+  static constexpr int RY = 9, ZS = 51, FBUF = 279;
+};
+template <>
+struct WaveTile<8, 10, double> {
+  static constexpr int RY = 10, ZS = 106, FBUF = 855;
+};
+template <>
+struct WaveTile<8, 10, float> {
+  static constexpr int RY = 12, ZS = 138, FBUF = 1095;
+};
+
 // UM: how u reaches the x-rows. 0: each row thread loads its row when the
 // step starts; 1: the row of the next step is loaded one step ahead; 2: the
 // node planes go through an LDS ring, the next layer's loaded one layer ahead.
@@ -730,7 +752,7 @@ void fused_box_wave_kernel(
   };
   static_assert((EPW == 1 || EPW == 2) && NE % EPW == 0, "one or two elements per wavefront");
   constexpr int P2 = S::P2, Q2 = S::Q2, Q3 = S::Q3, PQ = P * Q;
-  constexpr int RY = BoxTile<P, Q>::RY, ZS = BoxTile<P, Q>::ZS, FB = BoxTile<P, Q>::FBUF;
+  constexpr int RY = WaveTile<P, Q, T>::RY, ZS = WaveTile<P, Q, T>::ZS, FB = WaveTile<P, Q, T>::FBUF;
   constexpr int M = P > Q ? P : Q;
   static_assert(RY >= M && ZS >= (M - 1) * RY + M && FB >= (P - 1) * ZS + (M - 1) * RY + M,
                 "element tile strides hold every (iz, y, x) entry");

@@ -194,13 +194,13 @@ __device__ double all_partials(const double* part, int nb, double* sh, double* o
 }
 
 // One CG step's update, with alpha = r.z / p.Ap formed by every block from the
-// p.Ap partials: x += alpha p, r -= alpha Ap, z = dinv r, and the partials of
-// the new r.z and r.r. An iteration that starts done changes nothing (alpha 0)
-// and only sums r.r.
+// p.Ap partials: r -= alpha Ap, z = dinv r, and the partials of the new r.z and
+// r.r. x += alpha p is left to the direction kernel, which reads p anyway (one
+// pass over x and p fewer an iteration, the same arithmetic). An iteration
+// that starts done changes nothing (alpha 0) and only sums r.r.
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_update_kernel(size_t n, int par, int nb, double* __restrict__ sc,
-                                                             const T* __restrict__ p, const T* __restrict__ Ap,
-                                                             const T* __restrict__ dinv, T* __restrict__ x,
+                                                             const T* __restrict__ Ap, const T* __restrict__ dinv,
                                                              T* __restrict__ r, T* __restrict__ z,
                                                              double* __restrict__ part) {
   __shared__ double sh[3][kThreads / 64];
@@ -218,7 +218,6 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(size_t n, int par, 
        i += static_cast<size_t>(gridDim.x) * kThreads) {
     T ri = r[i];
     if (!done) {
-      x[i] = x[i] + alpha * p[i];
       ri = ri - alpha * Ap[i];
       r[i] = ri;
       const T zi = dinv[i] * ri;
@@ -238,8 +237,7 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(size_t n, int par, 
 // The update kernel on 16-byte packs (same arithmetic per entry).
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int par, int nb, double* __restrict__ sc,
-                                                                 const T* __restrict__ p, const T* __restrict__ Ap,
-                                                                 const T* __restrict__ dinv, T* __restrict__ x,
+                                                                 const T* __restrict__ Ap, const T* __restrict__ dinv,
                                                                  T* __restrict__ r, T* __restrict__ z,
                                                                  double* __restrict__ part) {
   constexpr int W = Pack<T>::W;
@@ -254,9 +252,8 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int p
   }
   const T alpha = static_cast<T>(alpha_d);
   double srz = 0.0, srr = 0.0;
-  auto one = [&](T& xi, T& ri, T pi, T api, T di, T& zi) {
+  auto one = [&](T& ri, T api, T di, T& zi) {
     if (!done) {
-      xi = xi + alpha * pi;
       ri = ri - alpha * api;
       zi = di * ri;
       srz += static_cast<double>(ri) * static_cast<double>(zi);
@@ -268,11 +265,10 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int p
        iv += static_cast<size_t>(gridDim.x) * kThreads) {
     Pack<T> rv = ldp(r, iv);
     if (!done) {
-      Pack<T> xv = ldp(x, iv), zv;
-      const Pack<T> pv = ldp(p, iv), av = ldp(Ap, iv), dv = ldp(dinv, iv);
+      Pack<T> zv;
+      const Pack<T> av = ldp(Ap, iv), dv = ldp(dinv, iv);
 #pragma unroll
-      for (int k = 0; k < W; ++k) one(xv.v[k], rv.v[k], pv.v[k], av.v[k], dv.v[k], zv.v[k]);
-      stp(x, iv, xv);
+      for (int k = 0; k < W; ++k) one(rv.v[k], av.v[k], dv.v[k], zv.v[k]);
       stp(r, iv, rv);
       stp(z, iv, zv);
     } else {
@@ -282,10 +278,9 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int p
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
     const size_t i = nv * W + threadIdx.x;
-    T xi = x[i], ri = r[i], zi = T(0);
-    one(xi, ri, p[i], Ap[i], dinv[i], zi);
+    T ri = r[i], zi = T(0);
+    one(ri, Ap[i], dinv[i], zi);
     if (!done) {
-      x[i] = xi;
       r[i] = ri;
       z[i] = zi;
     }
@@ -299,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int p
 }
 
 // The end of a CG step, with beta = r.z(new) / r.z(old) formed by every block
-// from the update's partials: p = z + beta p, and (block 0) the state copy
+// from the update's partials: x += alpha p, p = z + beta p, and (block 0) the state copy
 // 1-par: the iteration count, r.r, r.z and the done test r.r <= threshold or
 // the iteration limit (both read from the scalar slots). An iteration that started done copies the state on and
 // leaves p alone (*frozen). Returns beta.
@@ -330,43 +325,57 @@ __global__ __launch_bounds__(kThreads) void cg_update_vec_kernel(size_t n, int p
   return beta;
 }
 
+// x += alpha p (alpha from its slot, as the update used it), then p = z + beta p.
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_p_vec_kernel(size_t n, int par, int nb,
                                                             double* __restrict__ sc, const double* __restrict__ part,
-                                                            const T* __restrict__ z, T* __restrict__ p) {
+                                                            const T* __restrict__ z, T* __restrict__ p,
+                                                            T* __restrict__ x) {
   constexpr int W = Pack<T>::W;
   __shared__ double sh[2][kThreads / 64];
   __shared__ double s_val[2];
   bool frozen = false;
   const T beta = static_cast<T>(cg_step_end(par, nb, sc, part, sh, s_val, &frozen));
-  if (frozen) return;  // p stays
+  if (frozen) return;  // x and p stay
+  const T alpha = static_cast<T>(sc[kALPHA]);
   const size_t nv = n / W;
   for (size_t iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; iv < nv;
        iv += static_cast<size_t>(gridDim.x) * kThreads) {
     const Pack<T> zv = ldp(z, iv);
-    Pack<T> pv = ldp(p, iv);
+    Pack<T> pv = ldp(p, iv), xv = ldp(x, iv);
 #pragma unroll
-    for (int k = 0; k < W; ++k) pv.v[k] = zv.v[k] + beta * pv.v[k];
+    for (int k = 0; k < W; ++k) {
+      xv.v[k] = xv.v[k] + alpha * pv.v[k];
+      pv.v[k] = zv.v[k] + beta * pv.v[k];
+    }
+    stp(x, iv, xv);
     stp(p, iv, pv);
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
     const size_t i = nv * W + threadIdx.x;
-    p[i] = z[i] + beta * p[i];
+    const T pi = p[i];
+    x[i] = x[i] + alpha * pi;
+    p[i] = z[i] + beta * pi;
   }
 }
 
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_p_kernel(size_t n, int par, int nb,
                                                         double* __restrict__ sc, const double* __restrict__ part,
-                                                        const T* __restrict__ z, T* __restrict__ p) {
+                                                        const T* __restrict__ z, T* __restrict__ p,
+                                                        T* __restrict__ x) {
   __shared__ double sh[2][kThreads / 64];
   __shared__ double s_val[2];
   bool frozen = false;
   const T beta = static_cast<T>(cg_step_end(par, nb, sc, part, sh, s_val, &frozen));
-  if (frozen) return;  // p stays
+  if (frozen) return;  // x and p stay
+  const T alpha = static_cast<T>(sc[kALPHA]);
   for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += static_cast<size_t>(gridDim.x) * kThreads)
-    p[i] = z[i] + beta * p[i];
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    const T pi = p[i];
+    x[i] = x[i] + alpha * pi;
+    p[i] = z[i] + beta * pi;
+  }
 }
 
 // ---- The plain form on long vectors ("wide"): one 16-byte pack per thread
@@ -436,12 +445,12 @@ __global__ __launch_bounds__(kThreads) void dot_wide_kernel(size_t n, const T* _
   }
 }
 
-// x += alpha p, r -= alpha Ap, z = dinv r, the block's r.z and r.r partials
-// (alpha from its slot; an iteration that starts done only sums r.r).
+// r -= alpha Ap, z = dinv r, the block's r.z and r.r partials (alpha from its
+// slot; x += alpha p is the direction kernel's; an iteration that starts done
+// only sums r.r).
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_update_wide_kernel(size_t n, int par, const double* __restrict__ sc,
-                                                                  const T* __restrict__ p, const T* __restrict__ Ap,
-                                                                  const T* __restrict__ dinv, T* __restrict__ x,
+                                                                  const T* __restrict__ Ap, const T* __restrict__ dinv,
                                                                   T* __restrict__ r, T* __restrict__ z,
                                                                   double* __restrict__ part_rz,
                                                                   double* __restrict__ part_rr) {
@@ -450,9 +459,8 @@ __global__ __launch_bounds__(kThreads) void cg_update_wide_kernel(size_t n, int 
   const bool done = sc[slot(kDONE, par)] != 0.0;
   const T alpha = static_cast<T>(sc[kALPHA]);
   double srz = 0.0, srr = 0.0;
-  auto one = [&](T& xi, T& ri, T pi, T api, T di, T& zi) {
+  auto one = [&](T& ri, T api, T di, T& zi) {
     if (!done) {
-      xi = xi + alpha * pi;
       ri = ri - alpha * api;
       zi = di * ri;
       srz += static_cast<double>(ri) * static_cast<double>(zi);
@@ -463,11 +471,10 @@ __global__ __launch_bounds__(kThreads) void cg_update_wide_kernel(size_t n, int 
   if (iv < nv) {
     Pack<T> rv = ldp(r, iv);
     if (!done) {
-      Pack<T> xv = ldp(x, iv), zv;
-      const Pack<T> pv = ldp(p, iv), av = ldp(Ap, iv), dv = ldp(dinv, iv);
+      Pack<T> zv;
+      const Pack<T> av = ldp(Ap, iv), dv = ldp(dinv, iv);
 #pragma unroll
-      for (int k = 0; k < W; ++k) one(xv.v[k], rv.v[k], pv.v[k], av.v[k], dv.v[k], zv.v[k]);
-      stp(x, iv, xv);
+      for (int k = 0; k < W; ++k) one(rv.v[k], av.v[k], dv.v[k], zv.v[k]);
       stp(r, iv, rv);
       stp(z, iv, zv);
     } else {
@@ -477,10 +484,9 @@ __global__ __launch_bounds__(kThreads) void cg_update_wide_kernel(size_t n, int 
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
     const size_t i = nv * W + threadIdx.x;
-    T xi = x[i], ri = r[i], zi = T(0);
-    one(xi, ri, p[i], Ap[i], dinv[i], zi);
+    T ri = r[i], zi = T(0);
+    one(ri, Ap[i], dinv[i], zi);
     if (!done) {
-      x[i] = xi;
       r[i] = ri;
       z[i] = zi;
     }
@@ -503,24 +509,32 @@ __global__ __launch_bounds__(kThreads) void cg_update_wide_kernel(size_t n, int 
   (void)cg_step_end(par, nb, sc, part, sh, s_val, &frozen);
 }
 
-// p = z + beta p (beta from its slot); an iteration that started done leaves p.
+// x += alpha p, then p = z + beta p (alpha and beta from their slots); an
+// iteration that started done leaves both.
 template <class T>
 __global__ __launch_bounds__(kThreads) void cg_p_wide_kernel(size_t n, int par, const double* __restrict__ sc,
-                                                             const T* __restrict__ z, T* __restrict__ p) {
+                                                             const T* __restrict__ z, T* __restrict__ p,
+                                                             T* __restrict__ x) {
   constexpr int W = Pack<T>::W;
   if (sc[slot(kDONE, par)] != 0.0) return;
-  const T beta = static_cast<T>(sc[kBETA]);
+  const T alpha = static_cast<T>(sc[kALPHA]), beta = static_cast<T>(sc[kBETA]);
   const size_t nv = n / W, iv = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (iv < nv) {
     const Pack<T> zv = ldp(z, iv);
-    Pack<T> pv = ldp(p, iv);
+    Pack<T> pv = ldp(p, iv), xv = ldp(x, iv);
 #pragma unroll
-    for (int k = 0; k < W; ++k) pv.v[k] = zv.v[k] + beta * pv.v[k];
+    for (int k = 0; k < W; ++k) {
+      xv.v[k] = xv.v[k] + alpha * pv.v[k];
+      pv.v[k] = zv.v[k] + beta * pv.v[k];
+    }
+    stp(x, iv, xv);
     stp(p, iv, pv);
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
     const size_t i = nv * W + threadIdx.x;
-    p[i] = z[i] + beta * p[i];
+    const T pi = p[i];
+    x[i] = x[i] + alpha * pi;
+    p[i] = z[i] + beta * pi;
   }
 }
 

@@ -127,21 +127,21 @@ struct Pcg {
       dot_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, p.data(), Ap.data(), pw);
       fold_partials_kernel<<<dim3(nb, 1), kThreads, 0, s>>>(pw, nbw, 0, pap, 0);
       cg_alpha_kernel<<<1, kThreads, 0, s>>>(par, nb, sc.data(), part.data());
-      cg_update_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, par, sc.data(), p.data(), Ap.data(), dinv.data(), x,
-                                                        r.data(), z.data(), pw, pw + nbw);
+      cg_update_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, par, sc.data(), Ap.data(), dinv.data(), r.data(), z.data(),
+                                                        pw, pw + nbw);
       fold_partials_kernel<<<dim3(nb, 2), kThreads, 0, s>>>(pw, nbw, nbw, part.data() + kPartRZ, kPartRR - kPartRZ);
       cg_beta_kernel<<<1, kThreads, 0, s>>>(par, nb, sc.data(), part.data());
-      cg_p_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, par, sc.data(), z.data(), p.data());
+      cg_p_wide_kernel<T><<<nbw, kThreads, 0, s>>>(n, par, sc.data(), z.data(), p.data(), x);
     } else if (aligned16(x)) {  // every other vector is the solver's own allocation
       dot_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), pap);
-      cg_update_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), p.data(), Ap.data(), dinv.data(), x,
-                                                      r.data(), z.data(), part.data());
-      cg_p_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), z.data(), p.data());
+      cg_update_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), Ap.data(), dinv.data(), r.data(),
+                                                      z.data(), part.data());
+      cg_p_vec_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), z.data(), p.data(), x);
     } else {
       dot_partial_kernel<T><<<nb, kThreads, 0, s>>>(n, p.data(), Ap.data(), pap);
-      cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), p.data(), Ap.data(), dinv.data(), x,
-                                                  r.data(), z.data(), part.data());
-      cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), z.data(), p.data());
+      cg_update_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), Ap.data(), dinv.data(), r.data(), z.data(),
+                                                  part.data());
+      cg_p_kernel<T><<<nb, kThreads, 0, s>>>(n, par, nb, sc.data(), part.data(), z.data(), p.data(), x);
     }
   }
 
@@ -550,9 +550,10 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   // drift further from the true residual, and an FP32-inner refinement then
   // needs an extra step (16^3, P=3: 44 iterations in 3 steps against 42 in 2).
   // Residual replacement (MIXED_RELIABLE) always runs the plain form. The
-  // plain form runs its wide kernels from kWideMinN entries up.
+  // plain form runs its wide kernels on vectors of kWideMinBytes and more.
   const bool pipe = opt.mode != 2 && (opt.cg_variant == 2 || (opt.cg_variant == 0 && opt.mode == 0 && n <= kPipeMaxN));
-  const bool wide = !pipe && (opt.cg_variant == 3 || (opt.cg_variant == 0 && n >= kWideMinN));
+  const size_t vec_bytes = n * (opt.mode == 0 ? sizeof(double) : sizeof(float));
+  const bool wide = !pipe && (opt.cg_variant == 3 || (opt.cg_variant == 0 && vec_bytes >= kWideMinBytes));
   if (st.pcg64 && (st.pcg64->pipe != pipe || st.pcg64->wide != wide) && opt.mode == 0) st.pcg64.reset();
   if (st.pcg32 && (st.pcg32->pipe != pipe || st.pcg32->wide != wide) && opt.mode != 0) st.pcg32.reset();
   if (opt.mode == 0 && !st.pcg64) {

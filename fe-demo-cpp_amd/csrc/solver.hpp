@@ -12,15 +12,16 @@ struct SolveOptions {
   double inner_rtol = 0.0;  // <= 0: default_inner_rtol(mode)
   int batch = 0;  // CG iterations per captured graph launch (0: kSmallBatch up to kPipeMaxN entries, else 8)
   int cg_variant = 0;  // 0: automatic (FP64: pipelined up to kPipeMaxN entries; FP32 inner: plain; plain: wide
-                       // from kWideMinN), 1: plain, 2: pipelined, 3: plain, wide kernels
+                       // from kWideMinBytes per vector), 1: plain, 2: pipelined, 3: plain, wide kernels
   int separate_second_pass = 0;  // pipelined form: 1 keeps the apply's second pass as its own kernel
 };
 
 // Vector length up to which the automatic choice takes pipelined CG (FP64 solves).
 constexpr size_t kPipeMaxN = size_t{1} << 20;
-// Vector length from which the automatic choice runs the plain form's wide
-// (one pack per thread) kernels.
-constexpr size_t kWideMinN = size_t{1} << 25;
+// Vector size in bytes from which the automatic choice runs the plain form's
+// wide (one pack per thread) kernels: on MI355X the wide form was 2.5% faster
+// on 136 MB FP64 vectors (64^3, P=5) and 1% slower on their 68 MB FP32 copies.
+constexpr size_t kWideMinBytes = size_t{96} << 20;
 // Iterations per graph launch for vectors of at most kPipeMaxN entries.
 constexpr int kSmallBatch = 32;
 

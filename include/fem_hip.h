@@ -404,12 +404,13 @@ typedef struct {
                         keeps the FP64 iteration count). A refinement step never goes below half
                         of what the target still needs. */
   int cg_variant;    /* FEM_HIP_CG_*. 0: automatic (FP64: pipelined CG up to 2^20 unknowns, plain CG above; the
-                        FP32 inner solves and MIXED_RELIABLE: plain; the plain form wide from 2^25 unknowns);
+                        FP32 inner solves and MIXED_RELIABLE: plain; the plain form wide on vectors of 96 MiB
+                        and more: FP64 from 12.6M unknowns, FP32 from 25.2M);
                         1: plain Jacobi-PCG (apply, dot, update, direction: five dependent launches an
                         iteration); 2: pipelined Jacobi-PCG (Ghysels & Vanroose 2014: apply and one fused
                         kernel, three launches); 3: plain Jacobi-PCG with one 16-byte pack per thread and
-                        alpha, beta formed by one-block kernels between the streaming ones (the same
-                        arithmetic and summation order as 1, for long vectors) */
+                        alpha, beta formed by one-block kernels between the streaming ones (the per-entry
+                        arithmetic of 1, another fixed summation order; for long vectors) */
   int separate_second_pass; /* pipelined form: 0 (default) folds the apply's second pass (block partial sums,
                         transpose map) into the step kernel where the apply splits, one launch fewer an
                         iteration with the same arithmetic; 1 keeps it as its own kernel */

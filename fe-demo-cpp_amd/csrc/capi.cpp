@@ -808,7 +808,8 @@ int fem_hip_box_column_check(int nx, int ny, int nz, int P, int elems, int layer
     g.ncx = nx / BX;
     g.ncy = ny / BY;
     require(layers >= 0, "box_column_check: layers must be >= 0");
-    // Automatic: as the apply chooses on 256 CUs (32 elements per column layer per CU).
+    // Automatic: a count box_segments picks for 32 elements per column layer per CU on 256 CUs (the
+    // apply's own choice also counts its kernel's resident workgroups; the check holds for any count).
     g.nsz = box_segments(nz, static_cast<long long>(g.ncx) * g.ncy, 32 * 256 / (BX * BY), P, layers);
     require(g.nsz > 0, "box_column_check: no segment count fits");
     g.lz = (nz + g.nsz - 1) / g.nsz;

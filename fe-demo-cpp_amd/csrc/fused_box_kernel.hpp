@@ -1040,6 +1040,25 @@ constexpr int box_code(int BX, int BY) {
   return BX == 8 && BY == 2 ? 18 : BX == 16 && BY == 2 ? 34 : BX * BY;
 }
 
+// Workgroups of a box-column variant's column kernel resident on one CU.
+template <int P, int Q, int NC, class T, int BX, int BY, int UM, int QA, bool SPLIT = false, int WAVE = 0,
+          int WPE = 0>
+int box_occupancy() {
+  int n = 0;
+  if constexpr (WAVE > 0) {
+    constexpr int NCK = SPLIT ? 1 : NC, SP = SPLIT ? NC : 1;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, fused_box_wave_kernel<P, Q, NCK, T, BX, BY, SP, WAVE, WPE>, 64 * BX * BY / WAVE, 0));
+  } else if constexpr (SPLIT) {
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_box_kernel<P, Q, 1, T, BX, BY, UM, QA, NC>,
+                                                           round64(BX * BY * Shape<P, Q>::TPE), 0));
+  } else {
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_box_kernel<P, Q, NC, T, BX, BY, UM, QA>,
+                                                           round64(BX * BY * Shape<P, Q>::TPE), 0));
+  }
+  return n;
+}
+
 // Key elems of a box-column variant: box_code(BX, BY), plus 100 for
 // the kernel that loads qdata within its own layer (QA = 0), plus 200 for FP64
 // with u through the LDS ring, plus 300 (several components) for the
@@ -1063,6 +1082,7 @@ struct FusedBoxReg {
   static void reg(int code) {
     const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, box_code(BX, BY) + code};
     register_fused(k, &launch_box<P, Q, NC, T, BX, BY, UM, QA, SPLIT>, 0);
+    register_fused_occupancy(k, &box_occupancy<P, Q, NC, T, BX, BY, UM, QA, SPLIT>);
   }
 };
 
@@ -1095,6 +1115,7 @@ struct FusedBoxWaveReg {
   static void reg(int code) {
     const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, box_code(BX, BY) + code};
     register_fused(k, &launch_box<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW>, 0);
+    register_fused_occupancy(k, &box_occupancy<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW>);
   }
 };
 

@@ -157,6 +157,11 @@ std::map<FusedKey, int>& budget_registry() {
   return reg;
 }
 
+std::map<FusedKey, FusedOccupancy>& occupancy_registry() {
+  static std::map<FusedKey, FusedOccupancy> reg;
+  return reg;
+}
+
 FusedLauncher find(const FusedKey& k) {
   auto& reg = fused_registry();
   auto it = reg.find(k);
@@ -173,6 +178,20 @@ std::map<FusedKey, FusedLauncher>& fused_registry() {
 void register_fused(const FusedKey& k, FusedLauncher f, int lds_node_budget) {
   fused_registry()[k] = f;
   budget_registry()[k] = lds_node_budget;
+}
+
+void register_fused_occupancy(const FusedKey& k, FusedOccupancy f) { occupancy_registry()[k] = f; }
+
+int fused_wgs_per_cu(const FusedKey& k) {
+  static std::mutex m;
+  static std::map<FusedKey, int> cache;
+  std::lock_guard<std::mutex> lk(m);
+  auto c = cache.find(k);
+  if (c != cache.end()) return c->second;
+  auto it = occupancy_registry().find(k);
+  const int n = it == occupancy_registry().end() ? 0 : std::max(0, it->second());
+  cache[k] = n;
+  return n;
 }
 
 int fused_node_budget(const FusedKey& k) {
@@ -341,17 +360,24 @@ int device_cus() {
 }
 
 // Segments per column of the key's box-column kernel on f's box: the
-// operator's layer limit if set, else the count that gives 32 elements per
-// column layer per CU in the fewest layer steps (32 * CUs / elements per layer
-// workgroup slots). Measured on MI355X at 64^3, P=5: more resident segments
-// per CU (3 nine-wave workgroups, whose kernels fit in 72 VGPRs) were 9-15%
-// slower than 2, fewer left CUs idle.
+// operator's layer limit if set, else box_segments' choice for the kernel's
+// resident workgroups (the occupancy query times the CUs; the component-split
+// kernels run one workgroup per component of a column segment). Measured on
+// MI355X, same box (apply ms, automatic -> this rule): 64^3, P=5 FP32 8x4
+// columns 0.137 -> 0.115 (two resident 16-wave workgroups per CU, not one),
+// 8x2 0.127 -> 0.118; 48^3, P=8, 3 components FP64 8x2 1.065 -> 1.049 (12
+// layers, not 16); 64^3, P=5 FP64 unchanged (one 16-wave workgroup per CU).
 int box_nsz_of(const FusedMass& f, const FusedKey& k) {
   int BX = 0, BY = 0;
   box_columns(k.elems, &BX, &BY);
   const long long ncols = static_cast<long long>(f.box_n[0] / BX) * (f.box_n[1] / BY);
   if (f.box_lz > 0) return box_segments(f.box_n[2], ncols, 1, f.P, f.box_lz);
-  return box_segments(f.box_n[2], ncols, std::max<long long>(1, 32LL * device_cus() / (BX * BY)), f.P, 0);
+  const int kind = k.elems / 100;  // 3, 5, 7: the component-split kernels
+  const long long split = (kind == 3 || kind == 5 || kind == 7) ? f.NC : 1;
+  const int wgs = fused_wgs_per_cu(k);
+  const long long slots = wgs > 0 ? static_cast<long long>(device_cus()) * wgs
+                                  : std::max<long long>(1, 32LL * device_cus() / (BX * BY));
+  return box_segments(f.box_n[2], ncols * split, slots, f.P, 0);
 }
 
 // Whether the box-column variant with `elems` elements per layer tiles f's box
@@ -908,16 +934,13 @@ int box_segments(int nz, long long ncols, long long slots, int P, int max_layers
     const int nsz = (nz + max_layers - 1) / max_layers;
     return nsz <= nz && ok(nsz) ? nsz : 0;
   }
-  int best = 0;
-  long long best_steps = 0;
-  for (int nsz = 1; nsz <= nz && ok(nsz); ++nsz) {
-    const long long rounds = (ncols * nsz + slots - 1) / slots, steps = rounds * ((nz + nsz - 1) / nsz);
-    if (!best || steps < best_steps) {
-      best = nsz;
-      best_steps = steps;
-    }
-  }
-  return best;
+  auto steps = [&](int nsz) { return ((ncols * nsz + slots - 1) / slots) * ((nz + nsz - 1) / nsz); };
+  long long least = 0;
+  for (int nsz = 1; nsz <= nz && ok(nsz); ++nsz)
+    if (!least || steps(nsz) < least) least = steps(nsz);
+  for (int nsz = 1; nsz <= nz && ok(nsz); ++nsz)
+    if (steps(nsz) * 20 <= least * 21) return nsz;
+  return 0;
 }
 
 FusedKey fused_resolve(const FusedMass& f, int fp32) {

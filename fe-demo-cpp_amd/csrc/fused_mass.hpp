@@ -96,6 +96,12 @@ using FusedLauncher = void (*)(const FusedArgs&, hipStream_t);
 // Registry of compiled instantiations (filled by the fused_inst_*.hip units).
 std::map<FusedKey, FusedLauncher>& fused_registry();
 void register_fused(const FusedKey& k, FusedLauncher f, int lds_node_budget);
+// Box-column variants: workgroups of the variant's column kernel resident on
+// one CU (the HIP occupancy query on the kernel; needs the current device).
+using FusedOccupancy = int (*)();
+void register_fused_occupancy(const FusedKey& k, FusedOccupancy f);
+// The registered query's answer (cached per key), 0 when none is registered.
+int fused_wgs_per_cu(const FusedKey& k);
 // For block kernels: the most unique nodes per block the kernel's LDS is sized for.
 int fused_node_budget(const FusedKey& k);
 // Every compiled elems variant of one algorithm for a shape.
@@ -238,9 +244,13 @@ void ensure_flagged_offsets(FusedMass& f, hipStream_t s);
 // for its shape (its tuning rows are the box = 1 ones).
 bool fused_box_operator(const FusedMass& f);
 // Segments per column of the box-column algorithm for nz element layers,
-// ncols columns and `slots` workgroups running at once: the count that minimizes ceil(ncols*nsz / slots) * ceil(nz / nsz), the
-// layer steps of the longest-running CU, fewest segments on a tie; segments
-// hold at least 2 node planes beyond their first ((nz / nsz) * (P-1) >= 2).
+// ncols columns (times the workgroups per column segment) and `slots`
+// workgroups running at once: the fewest segments whose layer steps of the
+// longest-running CU, ceil(ncols*nsz / slots) * ceil(nz / nsz), are within
+// 5% of the least (the pass over shared nodes grows with every segment end,
+// and a last partial round of workgroups costs less than a full one, since
+// the few CUs still running then share the memory); segments hold at least
+// 2 node planes beyond their first ((nz / nsz) * (P-1) >= 2).
 // With max_layers > 0 (the operator's setting) the fewest segments of at most
 // max_layers layers. 0 if none qualifies.
 int box_segments(int nz, long long ncols, long long slots, int P, int max_layers);

@@ -1114,8 +1114,11 @@ struct FusedBoxWaveReg {
   template <class T, bool SPLIT>
   static void reg(int code) {
     const FusedKey k{P, Q, NC, std::is_same<T, float>::value ? 1 : 0, kAlgBoxColumn, 0, box_code(BX, BY) + code};
-    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW>, 0);
-    register_fused_occupancy(k, &box_occupancy<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW>);
+    // FP32, P = 8: held to 64 VGPRs (it needs 65-68 otherwise), so that two
+    // 16-wave workgroups fit a CU instead of one.
+    constexpr int W = std::is_same<T, float>::value && P == 8 ? 8 : 0;
+    register_fused(k, &launch_box<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW, W>, 0);
+    register_fused_occupancy(k, &box_occupancy<P, Q, NC, T, BX, BY, 0, 0, SPLIT, EPW, W>);
   }
 };
 

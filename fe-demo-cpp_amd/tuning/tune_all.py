@@ -1,10 +1,11 @@
 #!/usr/bin/env python3
 """Autotunes every compiled fused-mass shape on this GPU and writes the table.
 
-For each (P, Q, num_comp) with a compiled fused kernel and each of five mesh
-sizes (jittered structured hex meshes of about 34M, 24M, 6M, 1.5M and 0.375M
-element nodes, at most 96^3 hexes: the largest are far larger than the caches
-and 34M is 64^3 at P=5, the smallest launch-latency-bound), the autotuner times every compiled algorithm
+For each (P, Q, num_comp) with a compiled fused kernel and each of six mesh
+sizes (jittered structured hex meshes of about 180M, 34M, 24M, 6M, 1.5M and
+0.375M element nodes, at most 96^3 hexes: the largest are far larger than the
+caches, 180M is 48^3 at P=8 with 3 components and 34M is 64^3 at P=5, the
+smallest launch-latency-bound), the autotuner times every compiled algorithm
 and packing: FP64 with the atomic scatter allowed (every
 algorithm; the fastest deterministic one is recorded as the FP64
 deterministic choice) and FP32 with the deterministic scatter (the mixed
@@ -36,7 +37,7 @@ SHAPES = [(2, 3, (1, 2, 3)), (3, 4, (1, 2, 3)), (4, 5, (1, 2, 3)), (5, 6, (1, 2,
 DET = (fehip.FUSED_ELEM_TMAP, fehip.FUSED_BLOCK_PARTIAL, fehip.FUSED_BOX_COLUMN)
 
 
-TARGETS = (375_000, 1_500_000, 6_000_000, 24_000_000, 34_000_000)  # element nodes; the largest last (it sets the alg rows)
+TARGETS = (375_000, 1_500_000, 6_000_000, 24_000_000, 34_000_000, 180_000_000)  # element nodes; the largest last (it sets the alg rows)
 
 
 def mesh_size(P, nc, target, n_max=96):

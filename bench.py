@@ -313,8 +313,10 @@ def solve_latency(fehip, reps, n=16, P=3, Q=4):
     xs = x_star.to_host()
     out = {"workload": "%d^3 hexes, P=%d, Q=%d, %d DoF, rtol 1e-10" % (n, P, Q, p.l_size)}
     x64 = None
-    # fp64 / mixed run the automatic CG form (pipelined FP64 up to 2^20 unknowns,
-    # plain FP32 inner solves); the other two lines the other form of each.
+    # fp64 / mixed run the automatic CG form (FP64: pipelined up to 2^20 unknowns,
+    # plain above; FP32 inner solves plain; the plain form with its wide kernels
+    # on vectors of 96 MiB and more); the last two lines force another form:
+    # plain CG with the grid-stride kernels (FP64), pipelined (FP32 inner).
     for name, mode, cgv in (("fp64", fehip.SOLVE_FP64, fehip.CG_AUTO), ("mixed", fehip.SOLVE_MIXED, fehip.CG_AUTO),
                             ("mixed_reliable", fehip.SOLVE_MIXED_RELIABLE, fehip.CG_AUTO),
                             ("fp64_plain_cg", fehip.SOLVE_FP64, fehip.CG_PLAIN),
@@ -335,7 +337,7 @@ def solve_latency(fehip, reps, n=16, P=3, Q=4):
                      "err_vs_fp64": float(np.max(np.abs(x - x64)) / scale)}
     out["speedup_mixed_vs_fp64_p50"] = out["fp64"]["p50_ms"] / out["mixed"]["p50_ms"]
     out["speedup_reliable_vs_fp64_p50"] = out["fp64"]["p50_ms"] / out["mixed_reliable"]["p50_ms"]
-    out["speedup_pipelined_vs_plain_fp64_p50"] = out["fp64_plain_cg"]["p50_ms"] / out["fp64"]["p50_ms"]
+    out["speedup_auto_vs_plain_fp64_p50"] = out["fp64_plain_cg"]["p50_ms"] / out["fp64"]["p50_ms"]
     return out
 
 

@@ -313,7 +313,7 @@ def solve_latency(fehip, reps, n=16, P=3, Q=4):
     xs = x_star.to_host()
     out = {"workload": "%d^3 hexes, P=%d, Q=%d, %d DoF, rtol 1e-10" % (n, P, Q, p.l_size)}
     x64 = None
-    # fp64 / mixed run the automatic CG form (FP64: pipelined up to 2^20 unknowns,
+    # fp64 / mixed run the automatic CG form (FP64: pipelined up to 2^17 unknowns,
     # plain above; FP32 inner solves plain; the plain form with its wide kernels
     # on vectors of 96 MiB and more); the last two lines force another form:
     # plain CG with the grid-stride kernels (FP64), pipelined (FP32 inner).

@@ -598,7 +598,7 @@ SolveJob* solve_job_create(Operator& op, MassSolverState*& state, const double* 
   // Iterations per graph launch: small problems (iterations of a few tens of
   // microseconds) take longer batches, so a solve needs fewer host round trips
   // (the launch count follows the measured convergence rate either way).
-  j->batch = opt.batch > 0 ? opt.batch : (n <= kPipeMaxN ? kSmallBatch : 8);
+  j->batch = opt.batch > 0 ? opt.batch : (n <= kSmallBatchMaxN ? kSmallBatch : 8);
   HIP_CHECK(hipEventCreateWithFlags(&j->ev, hipEventDisableTiming));
   j->hs = pinned_slots().acquire();
   for (int i = 0; i < kHostN; ++i) j->hs[i] = 0.0;

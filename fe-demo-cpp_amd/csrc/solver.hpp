@@ -10,19 +10,25 @@ struct SolveOptions {
   int max_outer = 20;
   double rtol = 1e-10;
   double inner_rtol = 0.0;  // <= 0: default_inner_rtol(mode)
-  int batch = 0;  // CG iterations per captured graph launch (0: kSmallBatch up to kPipeMaxN entries, else 8)
+  int batch = 0;  // CG iterations per captured graph launch (0: kSmallBatch up to kSmallBatchMaxN entries, else 8)
   int cg_variant = 0;  // 0: automatic (FP64: pipelined up to kPipeMaxN entries; FP32 inner: plain; plain: wide
                        // from kWideMinBytes per vector), 1: plain, 2: pipelined, 3: plain, wide kernels
   int separate_second_pass = 0;  // pipelined form: 1 keeps the apply's second pass as its own kernel
 };
 
-// Vector length up to which the automatic choice takes pipelined CG (FP64 solves).
-constexpr size_t kPipeMaxN = size_t{1} << 20;
+// Vector length up to which the automatic choice takes pipelined CG (FP64
+// solves). On MI355X (plain against pipelined, FP64 solves to 1e-10): 36k
+// unknowns 0.83 / 0.75 ms, 118k 0.79 / 0.80 ms, 531k 1.23 / 1.41 ms, 913k
+// 1.57 / 1.82 ms: the pipelined form's extra vectors cost more than its
+// launches save from a few 100k unknowns.
+constexpr size_t kPipeMaxN = size_t{1} << 17;
+// Vector length up to which a solve takes kSmallBatch iterations per graph launch.
+constexpr size_t kSmallBatchMaxN = size_t{1} << 20;
 // Vector size in bytes from which the automatic choice runs the plain form's
 // wide (one pack per thread) kernels: on MI355X the wide form was 2.5% faster
 // on 136 MB FP64 vectors (64^3, P=5) and 1% slower on their 68 MB FP32 copies.
 constexpr size_t kWideMinBytes = size_t{96} << 20;
-// Iterations per graph launch for vectors of at most kPipeMaxN entries.
+// Iterations per graph launch for vectors of at most kSmallBatchMaxN entries.
 constexpr int kSmallBatch = 32;
 
 // Inner tolerance by mode. Refinement: 1e-5 per FP32 solve, so a 1e-10 target

@@ -403,7 +403,7 @@ typedef struct {
                         replacements; <= 0: 1e-5 (MIXED: two steps to 1e-10) / 1e-2 (MIXED_RELIABLE:
                         keeps the FP64 iteration count). A refinement step never goes below half
                         of what the target still needs. */
-  int cg_variant;    /* FEM_HIP_CG_*. 0: automatic (FP64: pipelined CG up to 2^20 unknowns, plain CG above; the
+  int cg_variant;    /* FEM_HIP_CG_*. 0: automatic (FP64: pipelined CG up to 2^17 unknowns, plain CG above; the
                         FP32 inner solves and MIXED_RELIABLE: plain; the plain form wide on vectors of 96 MiB
                         and more: FP64 from 12.6M unknowns, FP32 from 25.2M);
                         1: plain Jacobi-PCG (apply, dot, update, direction: five dependent launches an
